@@ -1,0 +1,1230 @@
+// TEST INFRASTRUCTURE — CPU oracle for the Jet multilevel k-way hot path.
+//
+// This file restates the reference algorithm (euphoricpoptarts/Jet-Partitioner
+// v1.1.0, files under src/) as plain serial C++ so that the GPU path can be
+// checked against it. Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it; the product never does.
+//
+// Where the reference is order-dependent on parallel backends (CAS races,
+// atomic_fetch_add positions, hash-slot layouts), this oracle executes one
+// specific, documented legal schedule of the reference's parallel code; the
+// GPU kernels compute the outcome of the same schedule in parallel. Each
+// function names the reference lines it follows and the schedule it fixes.
+//
+// Pinning (see tests/test_oracle_*.py): murmur3 and splitmix64 against their
+// published vectors; contraction against scipy's P^T A P; connectivity tables
+// against numpy bincount; cut / part sizes / projection against brute force;
+// coarse maps against the structural invariants of each heuristic plus a
+// brute-force re-execution of the sequential rules on tiny graphs; the LP
+// selection, afterburner and rebalancers against closed-form recomputation on
+// small inputs; refinement against the cut/size invariants after every move.
+#include "jet_oracle_prims.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <unordered_map>
+#include <vector>
+
+extern "C" int jet_host_initpart(int32_t n, const int64_t* xadj, const int32_t* adj, const int32_t* adjw,
+                                 const int32_t* vw, int32_t k, double imb, uint64_t seed, int32_t* part);
+
+namespace orc {
+
+struct Graph {
+    int32_t n = 0;
+    std::vector<int64_t> rowptr;
+    std::vector<int32_t> col;
+    std::vector<int32_t> ew;
+    int64_t m() const { return (int64_t)col.size(); }
+    int64_t deg(int32_t i) const { return rowptr[i + 1] - rowptr[i]; }
+};
+
+static Graph make_graph(int32_t n, const int64_t* rowptr, const int32_t* col, const int32_t* ew) {
+    Graph g;
+    g.n = n;
+    g.rowptr.assign(rowptr, rowptr + n + 1);
+    int64_t m = rowptr[n];
+    g.col.assign(col, col + m);
+    if (ew) g.ew.assign(ew, ew + m);
+    else g.ew.assign(m, 1);
+    return g;
+}
+
+// ============================================================== coarsening
+
+// heuristics.hpp:195-248 (coarsen_HEC heavy-neighbour choice).
+// Weighted case: the reference's team MaxLoc reduction keeps the LAST maximal
+// eligible edge on the Serial/OpenMP backends (`wgt >= local.val`); that
+// tie-break is the canonical one here.
+static std::vector<int32_t> hec_heavy(const Graph& g, const int32_t* vw, bool uniform, uint64_t seed, int32_t level) {
+    int32_t n = g.n;
+    std::vector<int32_t> hn(n);
+    if (uniform) {
+        for (int32_t i = 0; i < n; i++) {
+            uint64_t r = rng(seed, (uint32_t)level, SITE_HEC, 0, (uint32_t)i);
+            int64_t deg = g.deg(i);
+            if (deg > 0) hn[i] = g.col[g.rowptr[i] + (int64_t)(r % (uint64_t)deg)];
+            else hn[i] = (int32_t)(r % (uint64_t)n);
+        }
+    } else {
+        int64_t sum = 0;
+        for (int32_t i = 0; i < n; i++) sum += vw[i];
+        int64_t max_allowed = 6 * sum / n;
+        for (int32_t i = 0; i < n; i++) {
+            int64_t start = g.rowptr[i], end = g.rowptr[i + 1];
+            if (end > start && vw[i] < max_allowed) {
+                int64_t best_w = std::numeric_limits<int64_t>::min();
+                int64_t loc = -1;
+                for (int64_t j = start; j < end; j++) {
+                    int32_t v = g.col[j];
+                    if (g.ew[j] >= best_w && vw[v] < max_allowed) {
+                        best_w = g.ew[j];
+                        loc = j;
+                    }
+                }
+                hn[i] = loc >= 0 ? g.col[loc] : i;
+            } else {
+                hn[i] = i;
+            }
+        }
+    }
+    return hn;
+}
+
+// heuristics.hpp:89-169 (parallel_map_construct).
+// Schedule fixed for each claim round: every acting thread performs its first
+// CAS (vcmap(u): MAX -> MAX-1) before any thread performs its second step, and
+// second steps run in decreasing u when swap==1 and increasing u when swap==0,
+// i.e. every acting u runs before the acting vertex it points at. Under that
+// schedule an acting u whose target is acting always re-queues, and a
+// non-acting unclaimed target is claimed by the first acting u in that order.
+// Final ids: roots (vcmap(u)==u) numbered in increasing u (:147-165).
+static int32_t map_construct(const std::vector<int32_t>& hn, std::vector<int32_t>& vcmap) {
+    int32_t n = (int32_t)hn.size();
+    vcmap.assign(n, ORD_MAX);
+    std::vector<int32_t> wl(n);
+    std::iota(wl.begin(), wl.end(), 0);
+    std::vector<uint8_t> inwl(n, 1);
+    int swap = 1;
+    while (!wl.empty()) {
+        auto acting = [&](int32_t u) { return (((u < hn[u]) ? 1 : 0) ^ swap) != 0; };
+        auto step = [&](int32_t u) {
+            int32_t v = hn[u];
+            if (v == u) {
+                vcmap[u] = u;
+                return;
+            }
+            if (inwl[v] && acting(v)) {  // v still holds MAX-1: CAS fails, re-queue
+                vcmap[u] = ORD_MAX;
+                return;
+            }
+            if (vcmap[v] < n) {  // v already aggregated: join it
+                vcmap[u] = vcmap[v];
+                return;
+            }
+            int32_t cv = u < v ? u : v;
+            vcmap[u] = cv;
+            vcmap[v] = cv;
+        };
+        if (swap) {
+            for (size_t x = wl.size(); x-- > 0;)
+                if (acting(wl[x])) step(wl[x]);
+        } else {
+            for (size_t x = 0; x < wl.size(); x++)
+                if (acting(wl[x])) step(wl[x]);
+        }
+        std::vector<int32_t> next;
+        for (int32_t u : wl)
+            if (vcmap[u] >= n) next.push_back(u);
+        for (int32_t u : wl) inwl[u] = 0;
+        for (int32_t u : next) inwl[u] = 1;
+        wl.swap(next);
+        swap ^= 1;
+    }
+    std::vector<int32_t> id(n, -1);
+    int32_t nc = 0;
+    for (int32_t u = 0; u < n; u++)
+        if (vcmap[u] == u) id[u] = nc++;
+    for (int32_t u = 0; u < n; u++) vcmap[u] = id[vcmap[u]];
+    return nc;
+}
+
+// heuristics.hpp:321-425 (pickMatch). With distinct neighbours the xorshift32
+// tie-breaker is injective, so the Serial (last >=) and team (MaxFirstLoc)
+// variants choose the same vertex.
+static void pick_match(const Graph& g, std::vector<int32_t>& hn, const std::vector<int32_t>& vcmap, int32_t u,
+                       bool initial, bool uniform, uint32_t r) {
+    if (!initial && (vcmap[u] != ORD_MAX || hn[u] == ORD_MAX || vcmap[hn[u]] == ORD_MAX)) return;
+    int32_t h = ORD_MAX;
+    int32_t max_ewt = 0;
+    uint32_t tb = 0;
+    for (int64_t j = g.rowptr[u]; j < g.rowptr[u + 1]; j++) {
+        int32_t v = g.col[j];
+        if (initial || vcmap[v] == ORD_MAX) {
+            if (!uniform && max_ewt < g.ew[j]) {
+                max_ewt = g.ew[j];
+                h = v;
+                tb = xorshift32((uint32_t)v + r);
+            } else if (uniform || max_ewt == g.ew[j]) {
+                uint32_t s = xorshift32((uint32_t)v + r);
+                if (s >= tb) {
+                    h = v;
+                    tb = s;
+                }
+            }
+        }
+    }
+    hn[u] = h;
+}
+
+// heuristics.hpp:283-318 (matchHash).
+// Schedule fixed: candidates with a non-null digest run first in candidate
+// order, then null-digest candidates in candidate order. Under it every
+// digest value owns one slot, null-digest candidates all land on the same free
+// slot, and each slot pairs its candidates two at a time in candidate order.
+template <class H>
+static void match_hash(const std::vector<int32_t>& cands, const std::vector<H>& hashes, std::vector<int32_t>& vcmap) {
+    std::unordered_map<H, int32_t> pending;
+    pending.reserve(cands.size() * 2 + 1);
+    for (size_t x = 0; x < cands.size(); x++) {
+        int32_t i = cands[x];
+        H h = hashes[x];
+        auto it = pending.find(h);
+        if (it == pending.end()) {
+            pending.emplace(h, i);
+        } else {
+            int32_t t = it->second;
+            pending.erase(it);
+            int32_t cv = t < i ? t : i;
+            vcmap[t] = cv;
+            vcmap[i] = cv;
+        }
+    }
+}
+
+static int32_t dense_ids(std::vector<int32_t>& vcmap) {
+    int32_t n = (int32_t)vcmap.size();
+    std::vector<int32_t> id(n, -1);
+    int32_t nc = 0;
+    for (int32_t u = 0; u < n; u++)
+        if (vcmap[u] == u) id[u] = nc++;
+    for (int32_t u = 0; u < n; u++) vcmap[u] = id[vcmap[u]];
+    return nc;
+}
+
+// heuristics.hpp:427-671 (coarsen_match, plus 2-hop stages when
+// match_choice == 1). Commit-round part 2 (:483-491) races several active u
+// on one inactive v; the schedule fixed is increasing u, so the smallest
+// active u pointing at v wins.
+static int32_t coarsen_match(const Graph& g, bool uniform, uint64_t seed, int32_t level, int match_choice,
+                             std::vector<int32_t>& vcmap) {
+    int32_t n = g.n;
+    std::vector<int32_t> hn(n, ORD_MAX);
+    vcmap.assign(n, ORD_MAX);
+    if (uniform) {
+        for (int32_t i = 0; i < n; i++) {
+            int64_t deg = g.deg(i);
+            if (deg == 0) continue;
+            uint64_t r = rng(seed, (uint32_t)level, SITE_MATCH_INIT, 0, (uint32_t)i);
+            hn[i] = g.col[g.rowptr[i] + (int64_t)(r % (uint64_t)deg)];
+        }
+    } else {
+        for (int32_t u = 0; u < n; u++)
+            pick_match(g, hn, vcmap, u, true, false, (uint32_t)rng(seed, (uint32_t)level, SITE_PICK, 0, (uint32_t)u));
+    }
+    std::vector<int32_t> wl(n);
+    std::iota(wl.begin(), wl.end(), 0);
+    uint32_t pass = 0;
+    while (!wl.empty()) {
+        for (int r = 0; r < 4; r++) {
+            for (int32_t u : wl) {
+                int32_t v = hn[u];
+                if (v == ORD_MAX || vcmap[u] != ORD_MAX) continue;
+                bool cond = (r > 0) ? (pod_hash_i32(u + r) < pod_hash_i32(v + r)) : (u < v);
+                if (!cond) vcmap[u] = ORD_MAX - 1;
+            }
+            for (int32_t u : wl) {
+                int32_t v = hn[u];
+                if (v == ORD_MAX || vcmap[u] != ORD_MAX) continue;
+                int32_t cv = u < v ? u : v;
+                if (vcmap[v] == ORD_MAX - 1) {
+                    vcmap[v] = cv;
+                    vcmap[u] = cv;
+                }
+            }
+            for (int32_t u : wl)
+                if (vcmap[u] == ORD_MAX - 1) vcmap[u] = ORD_MAX;
+        }
+        pass++;
+        for (int32_t u : wl)
+            pick_match(g, hn, vcmap, u, false, uniform, (uint32_t)rng(seed, (uint32_t)level, SITE_PICK, pass, (uint32_t)u));
+        std::vector<int32_t> next;
+        for (int32_t u : wl)
+            if (vcmap[u] == ORD_MAX && hn[u] != ORD_MAX) next.push_back(u);
+        wl.swap(next);
+    }
+
+    if (match_choice == 1) {
+        auto count_unmatched = [&]() {
+            int64_t c = 0;
+            for (int32_t i = 0; i < n; i++)
+                if (vcmap[i] == ORD_MAX) c++;
+            return c;
+        };
+        int64_t unmapped = count_unmatched();
+        double ratio = (double)unmapped / (double)n;
+        if (ratio > 0.25) {  // leaf matches (:543-559)
+            std::vector<int32_t> cands;
+            std::vector<int32_t> hashes;
+            for (int32_t i = 0; i < n; i++)
+                if (vcmap[i] == ORD_MAX && g.deg(i) == 1) {
+                    cands.push_back(i);
+                    hashes.push_back(g.col[g.rowptr[i]]);
+                }
+            match_hash<int32_t>(cands, hashes, vcmap);
+        }
+        unmapped = count_unmatched();
+        ratio = (double)unmapped / (double)n;
+        if (ratio > 0.25) {  // twin matches (:566-597)
+            std::vector<int32_t> cands;
+            std::vector<uint64_t> hashes;
+            for (int32_t i = 0; i < n; i++)
+                if (vcmap[i] == ORD_MAX) {
+                    uint64_t h = 0;
+                    for (int64_t j = g.rowptr[i]; j < g.rowptr[i + 1]; j++) {
+                        uint64_t y = pod_hash_i32(g.col[j]);
+                        y = y * y + y;
+                        h += y;
+                    }
+                    cands.push_back(i);
+                    hashes.push_back(h);
+                }
+            match_hash<uint64_t>(cands, hashes, vcmap);
+        }
+        unmapped = count_unmatched();
+        ratio = (double)unmapped / (double)n;
+        if (ratio > 0.25) {  // relative matches (:603-640)
+            std::vector<int32_t> cands;
+            std::vector<int32_t> hashes;
+            for (int32_t u = 0; u < n; u++)
+                if (vcmap[u] == ORD_MAX) {
+                    int32_t h = ORD_MAX;
+                    int32_t max_wgt = 0;
+                    int32_t min_deg = ORD_MAX;
+                    for (int64_t j = g.rowptr[u]; j < g.rowptr[u + 1]; j++) {
+                        int32_t v = g.col[j];
+                        int32_t vdeg = (int32_t)g.deg(v);
+                        if (min_deg > vdeg) {
+                            min_deg = vdeg;
+                            max_wgt = g.ew[j];
+                            h = v;
+                        } else if (min_deg == vdeg) {
+                            if (max_wgt < g.ew[j]) {
+                                h = v;
+                                max_wgt = g.ew[j];
+                            }
+                        }
+                    }
+                    cands.push_back(u);
+                    hashes.push_back(h);
+                }
+            match_hash<int32_t>(cands, hashes, vcmap);
+        }
+    }
+    for (int32_t i = 0; i < n; i++)
+        if (vcmap[i] == ORD_MAX) vcmap[i] = i;  // singletons (:644-648)
+    return dense_ids(vcmap);                    // (:650-665)
+}
+
+// contract.hpp:448-491 (generate_coarse_mapping): alg 0 / default -> MtMetis
+// (match + 2-hop), 1 -> HECv1, 2 -> Match (partitioner.hpp:75-87).
+static int32_t coarsen_map(const Graph& g, const int32_t* vw, bool uniform, int32_t alg, uint64_t seed, int32_t level,
+                           std::vector<int32_t>& map) {
+    if (alg == 1) {
+        std::vector<int32_t> hn = hec_heavy(g, vw, uniform, seed, level);
+        return map_construct(hn, map);
+    }
+    return coarsen_match(g, uniform, seed, level, alg == 2 ? 0 : 1, map);
+}
+
+// contract.hpp:347-446 (build_coarse_graph). The coarse adjacency is P^T A P
+// without its diagonal; each coarse row is emitted in increasing column order
+// (one of the orders the reference's unordered consolidate can produce).
+static void contract(const Graph& g, const int32_t* vw, const int32_t* map, int32_t nc, Graph& gc,
+                     std::vector<int32_t>& cvw) {
+    int32_t n = g.n;
+    cvw.assign(nc, 0);
+    std::vector<int64_t> ms(nc + 1, 0);
+    for (int32_t i = 0; i < n; i++) ms[map[i] + 1]++;
+    for (int32_t c = 0; c < nc; c++) ms[c + 1] += ms[c];
+    std::vector<int32_t> members(n);
+    std::vector<int64_t> fill(ms.begin(), ms.end() - 1);
+    for (int32_t i = 0; i < n; i++) members[fill[map[i]]++] = i;
+    for (int32_t i = 0; i < n; i++) cvw[map[i]] += vw[i];
+    gc.n = nc;
+    gc.rowptr.assign(nc + 1, 0);
+    gc.col.clear();
+    gc.ew.clear();
+    std::vector<int64_t> acc(nc, 0);
+    std::vector<uint8_t> seen(nc, 0);
+    std::vector<int32_t> touched;
+    for (int32_t c = 0; c < nc; c++) {
+        touched.clear();
+        for (int64_t x = ms[c]; x < ms[c + 1]; x++) {
+            int32_t i = members[x];
+            for (int64_t j = g.rowptr[i]; j < g.rowptr[i + 1]; j++) {
+                int32_t u = map[g.col[j]];
+                if (u == c) continue;
+                if (!seen[u]) {
+                    seen[u] = 1;
+                    touched.push_back(u);
+                }
+                acc[u] += g.ew[j];
+            }
+        }
+        std::sort(touched.begin(), touched.end());
+        for (int32_t u : touched) {
+            gc.col.push_back(u);
+            gc.ew.push_back((int32_t)acc[u]);
+            acc[u] = 0;
+            seen[u] = 0;
+        }
+        gc.rowptr[c + 1] = (int64_t)gc.col.size();
+    }
+}
+
+// ============================================================== statistics
+
+// part_stat.hpp:77-108: sum over directed edges (= 2 x cut).
+static int64_t total_cut(const Graph& g, const int32_t* part) {
+    int64_t cut = 0;
+    for (int32_t i = 0; i < g.n; i++)
+        for (int64_t j = g.rowptr[i]; j < g.rowptr[i + 1]; j++)
+            if (part[i] != part[g.col[j]]) cut += g.ew[j];
+    return cut;
+}
+
+// part_stat.hpp:146-153.
+static std::vector<int64_t> part_sizes(int32_t n, const int32_t* vw, const int32_t* part, int32_t k) {
+    std::vector<int64_t> s(k, 0);
+    for (int32_t i = 0; i < n; i++) s[part[i]] += vw[i];
+    return s;
+}
+
+// part_stat.hpp:111-114 (ordinal_t arithmetic).
+static int32_t optimal_size(int32_t total, int32_t k) { return (total + k - 1) / k; }
+
+// ============================================================== refinement
+
+struct RefineData {  // jet_refiner.hpp:100-106
+    std::vector<int64_t> part_sizes;
+    int64_t total_size = 0;
+    int64_t cut = 0;
+    int64_t total_imb = 0;
+    bool init = false;
+};
+
+struct LevelRecord {
+    int64_t cut2 = 0;       // best_state.cut after the level (2 x cut)
+    int64_t total_imb = 0;  // best_state.total_imb after the level
+    int32_t n = 0;
+    int64_t m = 0;
+    int32_t iterations = 0;
+    int32_t lp_iterations = 0;
+};
+
+class Refiner {
+  public:
+    int32_t k;
+    // connectivity tables (jet_refiner.hpp:116-123)
+    std::vector<int64_t> conn_off;
+    std::vector<int32_t> conn_entry;
+    std::vector<int64_t> conn_val;
+    std::vector<int32_t> conn_size, dest_cache, lock_bit;
+    // iteration scratch (jet_refiner.hpp:126-161)
+    std::vector<int64_t> pregain, save_gp, save, pos, evict_start, evict_adjust, buckets;
+    std::vector<int32_t> dest, bid, undersized, swap_bit;
+
+    const Graph* g = nullptr;
+    const int32_t* vw = nullptr;
+    int32_t opt = 0, size_max = 0;
+
+    Refiner(int32_t finest_n, int64_t finest_cap, int32_t k_) : k(k_) {
+        conn_off.assign(finest_n + 1, 0);
+        conn_entry.assign(finest_cap, NULL_PART);
+        conn_val.assign(finest_cap, 0);
+        conn_size.assign(finest_n, 0);
+        dest_cache.assign(finest_n, NULL_PART);
+        lock_bit.assign(finest_n, 0);
+        pregain.assign(finest_n, 0);
+        save_gp.assign(finest_n, 0);
+        save.assign(finest_n, 0);
+        pos.assign(finest_n, 0);
+        dest.assign(finest_n, 0);
+        bid.assign(finest_n, -1);
+        swap_bit.assign(finest_n, 0);
+        evict_start.assign(k + 1, 0);
+        evict_adjust.assign(k, 0);
+        undersized.assign(k, 0);
+    }
+
+    // jet_refiner.hpp:987-1038 (row build, serial insertion in edge order,
+    // shrink to used + max(used/4, 3) and rebuild when smaller).
+    void build_row(const int32_t* part, int32_t i) {
+        int64_t gs = conn_off[i], ge = conn_off[i + 1];
+        int32_t size = (int32_t)(ge - gs);
+        int32_t used = 0;
+        auto insert_all = [&](int32_t sz, bool count) {
+            for (int64_t j = g->rowptr[i]; j < g->rowptr[i + 1]; j++) {
+                int32_t v = g->col[j];
+                int64_t w = g->ew[j];
+                int32_t p = part[v];
+                int32_t po = p % sz;
+                if (sz < k) {
+                    while (conn_entry[gs + po] != NULL_PART && conn_entry[gs + po] != p) po = (po + 1) % sz;
+                }
+                conn_val[gs + po] += w;
+                if (conn_entry[gs + po] == NULL_PART) {
+                    conn_entry[gs + po] = p;
+                    if (count) used++;
+                }
+            }
+        };
+        if (size > 0) insert_all(size, true);
+        int32_t old = size;
+        size = used;
+        int32_t q = size / 4;
+        if (q < 3) q = 3;
+        size += q;
+        if (size < old) {
+            for (int64_t j = gs; j < gs + old; j++) {
+                conn_entry[j] = NULL_PART;
+                conn_val[j] = 0;
+            }
+            insert_all(size, false);
+        } else {
+            size = old;
+        }
+        conn_size[i] = size;
+    }
+
+    // jet_refiner.hpp:958-1048 (init_conn_data).
+    void init_conn(const int32_t* part) {
+        int32_t n = g->n;
+        conn_off[0] = 0;
+        for (int32_t i = 0; i < n; i++) {
+            int64_t deg = g->deg(i);
+            if (deg > k) deg = k;
+            conn_off[i + 1] = conn_off[i] + deg;
+        }
+        int64_t cap = conn_off[n];
+        std::fill(conn_entry.begin(), conn_entry.begin() + cap, NULL_PART);
+        std::fill(conn_val.begin(), conn_val.begin() + cap, 0);
+        std::fill(dest_cache.begin(), dest_cache.begin() + n, NULL_PART);
+        std::fill(lock_bit.begin(), lock_bit.begin() + n, 0);
+        for (int32_t i = 0; i < n; i++) build_row(part, i);
+    }
+
+    // Canonical, slot-order-free argmax: highest connectivity, lowest part id
+    // among equals. The reference keeps the first maximum in table-slot order
+    // (jet_refiner.hpp:230-237, :603-611); for direct-mapped rows (table size
+    // == k) slot order is part order, so the two agree there.
+    // jet_refiner.hpp:200-323 (jet_lp).
+    std::vector<int32_t> jet_lp(const int32_t* part, double filter_ratio) {
+        int32_t n = g->n;
+        for (int32_t i = 0; i < n; i++) {
+            int32_t best = dest_cache[i];
+            if (best != NULL_PART) {
+                dest[i] = best;
+                continue;
+            }
+            int32_t p = part[i];
+            best = NO_MOVE;
+            int64_t b_conn = 0, p_conn = 0;
+            int64_t start = conn_off[i];
+            int64_t end = start + conn_size[i];
+            for (int64_t j = start; j < end; j++) {
+                int64_t jc = conn_val[j];
+                int32_t e = conn_entry[j];
+                if (jc > b_conn && e != p) {
+                    best = e;
+                    b_conn = jc;
+                } else if (jc == b_conn && b_conn > 0 && e != p && e < best) {
+                    best = e;
+                } else if (jc > 0 && e == p) {
+                    p_conn = jc;
+                }
+            }
+            save_gp[i] = 0;
+            if (best != NO_MOVE) {
+                if (b_conn >= p_conn || ((double)(p_conn - b_conn) < std::floor(filter_ratio * (double)p_conn))) {
+                    save_gp[i] = b_conn - p_conn;
+                } else {
+                    best = NO_MOVE;
+                }
+            }
+            dest_cache[i] = best;
+            dest[i] = best;
+        }
+        std::vector<int32_t> cand;
+        for (int32_t i = 0; i < n; i++) {
+            if (dest[i] != NO_MOVE && lock_bit[i] == 0) {
+                cand.push_back(i);
+                pregain[i] = save_gp[i];
+            } else {
+                pregain[i] = std::numeric_limits<int64_t>::min();
+                lock_bit[i] = 0;
+            }
+        }
+        for (int32_t i : cand) {
+            int32_t best = dest[i];
+            int32_t p = part[i];
+            int64_t ig = pregain[i];
+            int64_t change = 0;
+            for (int64_t j = g->rowptr[i]; j < g->rowptr[i + 1]; j++) {
+                int32_t v = g->col[j];
+                int64_t vg = pregain[v];
+                if (vg > ig || (vg == ig && v < i)) {
+                    int64_t w = g->ew[j];
+                    int32_t vp = dest[v];
+                    if (vp == p) change -= w;
+                    else if (vp == best) change += w;
+                    vp = part[v];
+                    if (vp == p) change += w;
+                    else if (vp == best) change -= w;
+                }
+            }
+            if (ig + change >= 0) lock_bit[i] = 1;
+        }
+        std::vector<int32_t> moves;
+        for (int32_t i : cand)
+            if (lock_bit[i]) moves.push_back(i);
+        return moves;
+    }
+
+    int32_t sections_for(int32_t n) const {  // jet_refiner.hpp:423-428, :556-561
+        int32_t sections = MAX_SECTIONS;
+        int32_t section_size = (n + sections * k) / (sections * k);
+        if (section_size < 4096) {
+            section_size = 4096;
+            sections = (n + section_size * k) / (section_size * k);
+        }
+        return sections;
+    }
+
+    // jet_refiner.hpp:551-645 (rebalance_weak) and :359-412
+    // (get_evictions<false>). Position inside a minibucket: the reference takes
+    // atomic_fetch_add order; the order fixed here is increasing vertex id.
+    // A gain bucket of exactly 50 (gain_bucket does not cap it, :349-351)
+    // indexes past part p's minibuckets; for p = k-1 it lands past the scanned
+    // range, which the reference leaves unscanned. Here those `sections` extra
+    // minibuckets are zeroed and scanned with the rest.
+    std::vector<int32_t> rebalance_weak(const int32_t* part, const std::vector<int64_t>& sizes) {
+        int32_t n = g->n;
+        int32_t sections = sections_for(n);
+        int64_t t_mb = (int64_t)MAX_BUCKETS * k * sections;
+        int64_t width = (int64_t)MAX_BUCKETS * sections;
+        buckets.assign(t_mb + sections + 1, 0);
+        int64_t smax = size_max;
+        int64_t max_dest = (int64_t)(smax * 0.99);
+        if (max_dest < smax - 100) max_dest = smax - 100;
+        int32_t tu = 0;
+        for (int32_t p = 0; p < k; p++)
+            if (sizes[p] < max_dest) undersized[tu++] = p;
+        for (int32_t i = 0; i < n; i++) {
+            int32_t p = part[i];
+            if (sizes[p] > smax && (double)vw[i] < 1.5 * (double)(sizes[p] - opt)) {
+                int32_t best = p;
+                int64_t gain = 0, p_gain = 0;
+                int64_t start = conn_off[i], end = start + conn_size[i];
+                for (int64_t j = start; j < end; j++) {
+                    int32_t pj = conn_entry[j];
+                    if (pj > NULL_PART && sizes[pj] < max_dest) {
+                        int64_t jg = conn_val[j];
+                        if (jg > gain) {
+                            best = pj;
+                            gain = jg;
+                        } else if (jg == gain && gain > 0 && pj < best) {
+                            best = pj;
+                        }
+                    }
+                    if (pj == p) p_gain = conn_val[j];
+                }
+                if (gain > 0) {
+                    dest[i] = best;
+                    save[i] = gain - p_gain;
+                } else if (tu > 0) {
+                    dest[i] = undersized[i % tu];
+                    save[i] = -p_gain;
+                } else {
+                    dest[i] = p;  // no undersized part exists: the reference divides by zero here
+                }
+            } else {
+                dest[i] = p;
+            }
+        }
+        for (int32_t i = 0; i < n; i++) {
+            int32_t p = part[i];
+            bid[i] = -1;
+            if (p != dest[i]) {
+                int32_t gt = gain_bucket(save[i], vw[i]);
+                int64_t gid = ((int64_t)MAX_BUCKETS * p + gt) * sections + (i % sections);
+                bid[i] = (int32_t)gid;
+                pos[i] = buckets[gid];
+                buckets[gid] += vw[i];
+            }
+        }
+        int64_t run = 0;
+        for (size_t b = 0; b < buckets.size(); b++) {
+            int64_t x = buckets[b];
+            buckets[b] = run;
+            run += x;
+        }
+        std::vector<int32_t> moves;
+        for (int32_t i = 0; i < n; i++) {
+            int32_t b = bid[i];
+            if (b == -1) continue;
+            int32_t p = part[i];
+            int64_t begin = (int64_t)p * width;
+            int64_t score = pos[i] + buckets[b] - buckets[begin];
+            int64_t limit = sizes[p] - smax;
+            if (score < limit) moves.push_back(i);
+        }
+        return moves;
+    }
+
+    // jet_refiner.hpp:417-546 (rebalance_strong) and :359-412
+    // (get_evictions<true>). Minibucket positions: increasing vertex id. The
+    // destination re-draw (:538-541) races an atomic_fetch_add; the order fixed
+    // is the move list (increasing id), each vertex finishing before the next.
+    std::vector<int32_t> rebalance_strong(const int32_t* part, const std::vector<int64_t>& sizes) {
+        int32_t n = g->n;
+        int32_t sections = sections_for(n);
+        int64_t t_mb = (int64_t)MAX_BUCKETS * k * sections;
+        int64_t width = (int64_t)MAX_BUCKETS * sections;
+        buckets.assign(t_mb + 1, 0);
+        int64_t smax = size_max;
+        int64_t max_dest = std::max<int64_t>((int64_t)opt + 1, (int64_t)(smax * 0.99));
+        int64_t max_vwgt = std::numeric_limits<int64_t>::min();
+        for (int32_t p = 0; p < k; p++)
+            if (sizes[p] < max_dest) {
+                int64_t cap = max_dest - sizes[p];
+                if (cap > max_vwgt) max_vwgt = cap;
+            }
+        for (int32_t i = 0; i < n; i++) {
+            int32_t p = part[i];
+            bid[i] = -1;
+            bool vw_ok = (max_vwgt > 0) && ((int64_t)vw[i] <= 2 * max_vwgt);
+            if (sizes[p] > smax && vw_ok && (int64_t)vw[i] < 2 * (sizes[p] - opt)) {
+                uint64_t tk = 0, tg = 0;
+                int64_t p_gain = 0;
+                int64_t start = conn_off[i], end = start + conn_size[i];
+                for (int64_t j = start; j < end; j++) {
+                    int32_t pj = conn_entry[j];
+                    if (pj == p) {
+                        p_gain = conn_val[j];
+                    } else if (pj > NULL_PART) {
+                        if (sizes[pj] < max_dest) {
+                            tg += (uint64_t)conn_val[j];
+                            tk += 1;
+                        }
+                    }
+                }
+                if (tk == 0) tk = 1;
+                int64_t gain = (int64_t)(tg / tk) - p_gain;
+                int64_t vwgt = std::min<int64_t>(vw[i], sizes[p] - smax);
+                int32_t gt = gain_bucket(gain, vwgt);
+                if (gt < MAX_BUCKETS) {
+                    int64_t gid = ((int64_t)MAX_BUCKETS * p + gt) * sections + (i % sections) + 1;
+                    bid[i] = (int32_t)gid;
+                    pos[i] = buckets[gid];
+                    buckets[gid] += vw[i];
+                }
+            }
+        }
+        int64_t run = 0;
+        for (size_t b = 0; b < buckets.size(); b++) {
+            int64_t x = buckets[b];
+            buckets[b] = run;
+            run += x;
+        }
+        std::fill(evict_adjust.begin(), evict_adjust.end(), 0);
+        std::vector<int32_t> moves;
+        for (int32_t i = 0; i < n; i++) {
+            int32_t b = bid[i];
+            if (b == -1) continue;
+            int32_t p = part[i];
+            int64_t begin = (int64_t)p * width;
+            int64_t score = pos[i] + buckets[b] - buckets[begin];
+            int64_t limit = sizes[p] - smax;
+            if (score < limit) {
+                if (score + vw[i] >= limit) evict_adjust[p] = score + vw[i];
+                moves.push_back(i);
+            }
+        }
+        // cookie cutter (:491-518)
+        int64_t acc = 0;
+        for (int32_t p = 0; p < k; p++) {
+            int64_t add = evict_adjust[p];
+            int64_t begin = (int64_t)MAX_BUCKETS * p * sections;
+            if (add == 0) add = buckets[begin + width] - buckets[begin];
+            evict_adjust[p] = buckets[begin] - acc;
+            acc += add;
+        }
+        int64_t counter = acc;
+        evict_start[0] = 0;
+        int64_t acc2 = 0;
+        for (int32_t p = 0; p < k; p++) {
+            if (max_dest > sizes[p]) acc2 += max_dest - sizes[p];
+            evict_start[p + 1] = acc2;
+        }
+        for (int32_t v : moves) save[v] = pos[v] + buckets[bid[v]] - evict_adjust[part[v]];  // (:519-525)
+        for (int32_t v : moves) {                                                           // (:526-544)
+            int32_t p = 0;
+            bool done = false;
+            while (p < k) {
+                while (p <= k && evict_start[p] <= save[v]) p++;
+                p--;
+                if (p < k && vw[v] / 2 <= evict_start[p + 1] - save[v]) {
+                    dest[v] = p;
+                    done = true;
+                    break;
+                } else if (p < k) {
+                    save[v] = counter;
+                    counter += vw[v];
+                }
+            }
+            if (!done) dest[v] = part[v];
+        }
+        return moves;
+    }
+
+    // jet_refiner.hpp:891-901.
+    int64_t lookup(int32_t i, int32_t target) const {
+        int64_t start = conn_off[i];
+        int32_t size = conn_size[i];
+        for (int32_t q = 0; q < size; q++) {
+            int32_t pi = (target + q) % size;
+            if (conn_entry[start + pi] == target) return conn_val[start + pi];
+            if (conn_entry[start + pi] == NULL_PART) return 0;
+        }
+        return 0;
+    }
+
+    // jet_refiner.hpp:781-888 (update_small), serial order of moves and edges.
+    void update_small(const int32_t* part, const std::vector<int32_t>& moves) {
+        for (int32_t i : moves) {
+            int32_t p = dest[i];  // old part
+            for (int64_t j = g->rowptr[i]; j < g->rowptr[i + 1]; j++) {
+                int32_t v = g->col[j];
+                int64_t w = g->ew[j];
+                int64_t vs = conn_off[v];
+                int32_t vsz = conn_size[v];
+                int32_t po = p % vsz;
+                while (conn_entry[vs + po] != p) po = (po + 1) % vsz;
+                int64_t x = conn_val[vs + po];
+                conn_val[vs + po] -= w;
+                if (vsz < k && x == w) conn_entry[vs + po] = HASH_RECLAIM;
+            }
+        }
+        for (int32_t i : moves) {
+            int32_t best = part[i];  // new part
+            for (int64_t j = g->rowptr[i]; j < g->rowptr[i + 1]; j++) {
+                int32_t v = g->col[j];
+                int64_t w = g->ew[j];
+                dest_cache[v] = NULL_PART;
+                int64_t vs = conn_off[v];
+                int32_t vsz = conn_size[v];
+                int32_t po = best % vsz;
+                bool success = false;
+                for (int32_t q = 0; q < vsz; q++) {
+                    int32_t pi = (best + q) % vsz;
+                    if (conn_entry[vs + pi] == best) {
+                        success = true;
+                        po = pi;
+                        break;
+                    } else if (conn_entry[vs + pi] == NULL_PART) {
+                        break;
+                    }
+                }
+                int32_t count = 0;
+                while (!success && count < vsz) {
+                    int32_t px = conn_entry[vs + po];
+                    while (px != best && px > NULL_PART && count++ < vsz) {
+                        po = (po + 1) % vsz;
+                        px = conn_entry[vs + po];
+                    }
+                    if (px == best) {
+                        success = true;
+                    } else if (px <= NULL_PART) {
+                        conn_entry[vs + po] = best;
+                        success = true;
+                    }
+                }
+                if (!success) {
+                    po = vsz;
+                    while (!success) {
+                        int32_t px = conn_entry[vs + po];
+                        while (px != best && px > NULL_PART) {
+                            po++;
+                            px = conn_entry[vs + po];
+                        }
+                        if (px == best) {
+                            success = true;
+                        } else {
+                            conn_entry[vs + po] = best;
+                            conn_size[v]++;
+                            success = true;
+                        }
+                    }
+                }
+                conn_val[vs + po] += w;
+            }
+        }
+    }
+
+    // jet_refiner.hpp:743-777 (update_large).
+    void update_large(const int32_t* part, const std::vector<int32_t>& moves) {
+        for (int32_t i : moves)
+            for (int64_t j = g->rowptr[i]; j < g->rowptr[i + 1]; j++) swap_bit[g->col[j]] = 1;
+        for (int32_t v = 0; v < g->n; v++) {
+            if (swap_bit[v] != 1) continue;
+            for (int64_t j = conn_off[v]; j < conn_off[v + 1]; j++) {
+                conn_entry[j] = NULL_PART;
+                conn_val[j] = 0;
+            }
+            build_row(part, v);
+            swap_bit[v] = 0;
+            dest_cache[v] = NULL_PART;
+        }
+    }
+
+    // jet_refiner.hpp:905-955 (perform_moves).
+    void perform_moves(int32_t* part, const std::vector<int32_t>& moves, RefineData& st) {
+        int64_t cc1 = 0;
+        for (int32_t i : moves) cc1 += lookup(i, dest[i]) - lookup(i, part[i]);
+        for (int32_t i : moves) {
+            int32_t p = part[i];
+            int32_t b = dest[i];
+            dest_cache[i] = NULL_PART;
+            st.part_sizes[p] -= vw[i];
+            st.part_sizes[b] += vw[i];
+            part[i] = b;
+            dest[i] = p;
+        }
+        if ((int32_t)moves.size() > g->n / 10) update_large(part, moves);
+        else update_small(part, moves);
+        int64_t cc2 = 0;
+        for (int32_t i : moves) cc2 += lookup(i, part[i]) - lookup(i, dest[i]);
+        int64_t mx = std::numeric_limits<int64_t>::min();
+        for (int32_t p = 0; p < k; p++) mx = std::max(mx, st.part_sizes[p]);
+        st.total_imb = mx > opt ? mx - opt : 0;
+        st.cut -= cc1 + cc2;
+    }
+
+    // jet_refiner.hpp:1050-1142 (jet_refine).
+    LevelRecord refine(const Graph& lg, const int32_t* lvw, int32_t* best_part, bool uniform_ew, double imb, double tol,
+                       bool ultra, RefineData& best) {
+        g = &lg;
+        vw = lvw;
+        int32_t n = lg.n;
+        if (!best.init) {
+            best.cut = total_cut(lg, best_part);
+            best.part_sizes = part_sizes(n, lvw, best_part, k);
+            best.total_size = 0;
+            for (int32_t i = 0; i < n; i++) best.total_size += lvw[i];
+        }
+        opt = optimal_size((int32_t)best.total_size, k);
+        size_max = (int32_t)(opt * imb);
+        if (!best.init) {
+            best.init = true;
+            int64_t mx = 0;
+            for (int64_t s : best.part_sizes) mx = std::max(mx, s);
+            best.total_imb = mx > opt ? mx - opt : 0;
+        }
+        RefineData curr = best;
+        int64_t imb_max = (int64_t)size_max - opt;
+        std::vector<int32_t> part(best_part, best_part + n);
+        init_conn(part.data());
+        std::vector<double> temps;
+        if (ultra) {
+            for (double t = 0.85; t > 0; t -= 0.05) temps.push_back(t);
+        } else if (uniform_ew) {
+            temps.push_back(0.25);
+        } else {
+            temps.push_back(0.75);
+        }
+        LevelRecord rec;
+        int balance_counter = 0;
+        for (double fr : temps) {
+            int count = 0;
+            while (count++ <= 11) {
+                rec.iterations++;
+                std::vector<int32_t> moves;
+                if (curr.total_imb <= imb_max) {
+                    moves = jet_lp(part.data(), fr);
+                    balance_counter = 0;
+                    rec.lp_iterations++;
+                } else {
+                    if (balance_counter < 2) moves = rebalance_weak(part.data(), curr.part_sizes);
+                    else moves = rebalance_strong(part.data(), curr.part_sizes);
+                    balance_counter++;
+                }
+                perform_moves(part.data(), moves, curr);
+                if (best.total_imb > imb_max && curr.total_imb < best.total_imb) {
+                    best = curr;
+                    std::copy(part.begin(), part.end(), best_part);
+                    count = 0;
+                } else if (curr.cut < best.cut && (curr.total_imb <= imb_max || curr.total_imb <= best.total_imb)) {
+                    if ((double)curr.cut < tol * (double)best.cut) count = 0;
+                    best = curr;
+                    std::copy(part.begin(), part.end(), best_part);
+                }
+            }
+        }
+        rec.cut2 = best.cut;
+        rec.total_imb = best.total_imb;
+        rec.n = n;
+        rec.m = lg.m();
+        return rec;
+    }
+};
+
+static int64_t conn_capacity(const Graph& g, int32_t k) {
+    int64_t c = 0;
+    for (int32_t i = 0; i < g.n; i++) c += std::min<int64_t>(g.deg(i), k);
+    return c;
+}
+
+struct Level {
+    Graph g;
+    std::vector<int32_t> vw;
+    std::vector<int32_t> map;  // previous (finer) level -> this level
+    bool uniform = false;
+};
+
+// partitioner.hpp:61-125 with contract.hpp:494-533 and uncoarsen.hpp:88-158.
+static int partition(int32_t alg, int32_t k, double imb, double tol, bool ultra, const Graph& fine,
+                     const int32_t* fvw, bool uniform_ew, uint64_t seed, int32_t* part_out, int64_t* cut_out,
+                     std::vector<LevelRecord>* recs, std::vector<int32_t>* level_n) {
+    int32_t cutoff = k * 8;
+    if (cutoff > 1024) {
+        cutoff = k * 2;
+        cutoff = std::max(1024, cutoff);
+    }
+    int32_t min_allowed = cutoff / 4;
+    std::vector<Level> levels(1);
+    levels[0].g = fine;
+    levels[0].vw.assign(fvw, fvw + fine.n);
+    levels[0].uniform = uniform_ew;
+    while (levels.back().g.n > cutoff) {
+        Level& cur = levels.back();
+        std::vector<int32_t> map;
+        int32_t nc = coarsen_map(cur.g, cur.vw.data(), cur.uniform, alg, seed, (int32_t)levels.size() - 1, map);
+        if (nc < min_allowed) break;
+        Level next;
+        contract(cur.g, cur.vw.data(), map.data(), nc, next.g, next.vw);
+        next.map.swap(map);
+        next.uniform = false;
+        levels.push_back(std::move(next));
+        if (levels.size() > 200) break;
+    }
+    if (level_n) {
+        level_n->clear();
+        for (auto& L : levels) level_n->push_back(L.g.n);
+    }
+    const Level& cs = levels.back();
+    std::vector<int32_t> part(cs.g.n);
+    int rc = jet_host_initpart(cs.g.n, cs.g.rowptr.data(), cs.g.col.data(), cs.g.ew.data(), cs.vw.data(), k, imb,
+                               seed, part.data());
+    if (rc != 0) return rc;
+    Refiner ref(fine.n, conn_capacity(fine, k), k);
+    RefineData rfd;
+    for (size_t L = levels.size(); L-- > 0;) {
+        LevelRecord rec = ref.refine(levels[L].g, levels[L].vw.data(), part.data(), levels[L].uniform, imb, tol, ultra, rfd);
+        if (recs) recs->push_back(rec);
+        if (L > 0) {
+            const std::vector<int32_t>& map = levels[L].map;
+            std::vector<int32_t> fp(levels[L - 1].g.n);
+            for (size_t i = 0; i < fp.size(); i++) fp[i] = part[map[i]];  // uncoarsen.hpp:82-86
+            part.swap(fp);
+        }
+    }
+    std::copy(part.begin(), part.end(), part_out);
+    *cut_out = rfd.cut / 2;
+    return 0;
+}
+
+}  // namespace orc
+
+// ================================================================ C entry points
+using namespace orc;
+
+extern "C" {
+
+uint64_t orc_mix64(uint64_t x) { return mix64(x); }
+uint64_t orc_rng(uint64_t seed, uint32_t level, uint32_t site, uint32_t pass, uint32_t idx) {
+    return rng(seed, level, site, pass, idx);
+}
+uint32_t orc_murmur3_32(const void* data, int64_t len, uint32_t seed) { return murmur3_32(data, (size_t)len, seed); }
+uint32_t orc_pod_hash_i32(int32_t key) { return pod_hash_i32(key); }
+uint32_t orc_xorshift32(uint32_t x) { return xorshift32(x); }
+int32_t orc_gain_bucket(int64_t gain, int64_t vw) { return gain_bucket(gain, vw); }
+
+void orc_hec_hn(int32_t n, const int64_t* rowptr, const int32_t* col, const int32_t* ew, const int32_t* vw,
+                int32_t uniform, uint64_t seed, int32_t level, int32_t* hn_out) {
+    Graph g = make_graph(n, rowptr, col, ew);
+    std::vector<int32_t> hn = hec_heavy(g, vw, uniform != 0, seed, level);
+    std::copy(hn.begin(), hn.end(), hn_out);
+}
+
+int32_t orc_map_construct(int32_t n, const int32_t* hn, int32_t* map_out) {
+    std::vector<int32_t> h(hn, hn + n), map;
+    int32_t nc = map_construct(h, map);
+    std::copy(map.begin(), map.end(), map_out);
+    return nc;
+}
+
+int32_t orc_coarsen_map(int32_t alg, int32_t level, int32_t uniform, uint64_t seed, int32_t n, const int64_t* rowptr,
+                        const int32_t* col, const int32_t* ew, const int32_t* vw, int32_t* map_out) {
+    Graph g = make_graph(n, rowptr, col, ew);
+    std::vector<int32_t> map;
+    int32_t nc = coarsen_map(g, vw, uniform != 0, alg, seed, level, map);
+    std::copy(map.begin(), map.end(), map_out);
+    return nc;
+}
+
+int64_t orc_contract(int32_t n, const int64_t* rowptr, const int32_t* col, const int32_t* ew, const int32_t* vw,
+                     const int32_t* map, int32_t nc, int64_t* out_rowptr, int32_t* out_col, int32_t* out_ew,
+                     int32_t* out_vw) {
+    Graph g = make_graph(n, rowptr, col, ew);
+    Graph gc;
+    std::vector<int32_t> cvw;
+    contract(g, vw, map, nc, gc, cvw);
+    std::copy(gc.rowptr.begin(), gc.rowptr.end(), out_rowptr);
+    std::copy(gc.col.begin(), gc.col.end(), out_col);
+    std::copy(gc.ew.begin(), gc.ew.end(), out_ew);
+    std::copy(cvw.begin(), cvw.end(), out_vw);
+    return gc.m();
+}
+
+int64_t orc_total_cut(int32_t n, const int64_t* rowptr, const int32_t* col, const int32_t* ew, const int32_t* part) {
+    Graph g = make_graph(n, rowptr, col, ew);
+    return total_cut(g, part);
+}
+
+// Connectivity tables after init_conn_data, written canonically: row i's
+// entries (part, conn) sorted by part in [off[i], off[i+1]) with off from the
+// used counts, plus each row's table size (conn_table_sizes).
+int64_t orc_conn_build(int32_t n, const int64_t* rowptr, const int32_t* col, const int32_t* ew, const int32_t* part,
+                       int32_t k, int64_t* out_off, int32_t* out_part, int64_t* out_val, int32_t* out_tsize) {
+    Graph g = make_graph(n, rowptr, col, ew);
+    std::vector<int32_t> vw(n, 1);
+    Refiner r(n, conn_capacity(g, k), k);
+    r.g = &g;
+    r.vw = vw.data();
+    r.init_conn(part);
+    int64_t w = 0;
+    out_off[0] = 0;
+    for (int32_t i = 0; i < n; i++) {
+        std::vector<std::pair<int32_t, int64_t>> row;
+        for (int64_t j = r.conn_off[i]; j < r.conn_off[i] + r.conn_size[i]; j++)
+            if (r.conn_entry[j] > NULL_PART) row.push_back({r.conn_entry[j], r.conn_val[j]});
+        std::sort(row.begin(), row.end());
+        for (auto& e : row) {
+            out_part[w] = e.first;
+            out_val[w] = e.second;
+            w++;
+        }
+        out_off[i + 1] = w;
+        out_tsize[i] = r.conn_size[i];
+    }
+    return w;
+}
+
+// One refinement step on fresh state (empty destination cache, no locks) for
+// the given partition: kind 0 = jet_lp, 1 = rebalance_weak, 2 =
+// rebalance_strong. Writes the move list (increasing id) and each move's
+// destination; returns the number of moves.
+int32_t orc_refine_step(int32_t kind, int32_t n, const int64_t* rowptr, const int32_t* col, const int32_t* ew,
+                        const int32_t* vw, int32_t k, double imb, double filter_ratio, const int32_t* part,
+                        int32_t* moves_out, int32_t* dest_out) {
+    Graph g = make_graph(n, rowptr, col, ew);
+    Refiner r(n, conn_capacity(g, k), k);
+    r.g = &g;
+    r.vw = vw;
+    int64_t total = 0;
+    for (int32_t i = 0; i < n; i++) total += vw[i];
+    r.opt = optimal_size((int32_t)total, k);
+    r.size_max = (int32_t)(r.opt * imb);
+    r.init_conn(part);
+    std::vector<int64_t> sizes = part_sizes(n, vw, part, k);
+    std::vector<int32_t> moves;
+    if (kind == 0) moves = r.jet_lp(part, filter_ratio);
+    else if (kind == 1) moves = r.rebalance_weak(part, sizes);
+    else moves = r.rebalance_strong(part, sizes);
+    for (size_t x = 0; x < moves.size(); x++) {
+        moves_out[x] = moves[x];
+        dest_out[x] = r.dest[moves[x]];
+    }
+    return (int32_t)moves.size();
+}
+
+// One level of jet_refine from a given partition with fresh refine_data.
+// part_io: in = starting partition, out = best partition. rec_out (8 int64):
+// cut2, total_imb, n, m, iterations, lp_iterations, opt, size_max.
+int orc_refine_level(int32_t n, const int64_t* rowptr, const int32_t* col, const int32_t* ew, const int32_t* vw,
+                     int32_t k, double imb, double tol, int32_t ultra, int32_t uniform, int32_t* part_io,
+                     int64_t* rec_out) {
+    Graph g = make_graph(n, rowptr, col, ew);
+    Refiner r(n, conn_capacity(g, k), k);
+    RefineData rfd;
+    LevelRecord rec = r.refine(g, vw, part_io, uniform != 0, imb, tol, ultra != 0, rfd);
+    rec_out[0] = rec.cut2;
+    rec_out[1] = rec.total_imb;
+    rec_out[2] = rec.n;
+    rec_out[3] = rec.m;
+    rec_out[4] = rec.iterations;
+    rec_out[5] = rec.lp_iterations;
+    rec_out[6] = r.opt;
+    rec_out[7] = r.size_max;
+    return 0;
+}
+
+// Full multilevel partition. levels_out (optional, 8 int64 per level, coarsest
+// first): cut2, total_imb, n, m, iterations, lp_iterations, 0, 0.
+// Returns 0 or the initial partitioner's error code.
+int orc_partition(int32_t alg, int32_t k, double imb, double tol, int32_t ultra, int32_t n, const int64_t* rowptr,
+                  const int32_t* col, const int32_t* ew, const int32_t* vw, int32_t uniform_ew, uint64_t seed,
+                  int32_t* part_out, int64_t* cut_out, int64_t* levels_out, int32_t* nlevels_out) {
+    Graph g = make_graph(n, rowptr, col, ew);
+    std::vector<int32_t> unit;
+    if (!vw) {
+        unit.assign(n, 1);
+        vw = unit.data();
+    }
+    std::vector<LevelRecord> recs;
+    std::vector<int32_t> level_n;
+    int rc = partition(alg, k, imb, tol, ultra != 0, g, vw, uniform_ew != 0, seed, part_out, cut_out, &recs, &level_n);
+    if (rc != 0) return rc;
+    if (nlevels_out) *nlevels_out = (int32_t)recs.size();
+    if (levels_out) {
+        for (size_t L = 0; L < recs.size(); L++) {
+            int64_t* o = levels_out + 8 * L;
+            o[0] = recs[L].cut2;
+            o[1] = recs[L].total_imb;
+            o[2] = recs[L].n;
+            o[3] = recs[L].m;
+            o[4] = recs[L].iterations;
+            o[5] = recs[L].lp_iterations;
+            o[6] = 0;
+            o[7] = 0;
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"

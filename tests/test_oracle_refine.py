@@ -1,0 +1,249 @@
+"""Pins the oracle's refinement: connectivity tables, LP + afterburner,
+rebalancers, cut tracking through move application, and end-to-end sanity."""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import graphs
+import oracle
+
+NO_MOVE = -3
+
+
+def conn_dicts(g, part):
+    rp, col, ew = g["rowptr"], g["col"], g["ew"]
+    n = len(rp) - 1
+    out = []
+    for i in range(n):
+        d = {}
+        for j in range(rp[i], rp[i + 1]):
+            q = int(part[col[j]])
+            d[q] = d.get(q, 0) + int(ew[j])
+        out.append(d)
+    return out
+
+
+CASES = [
+    ("grid", lambda: graphs.grid2d(20, 17), 4),
+    ("wgrid", lambda: graphs.with_weights(graphs.grid2d(16, 16), seed=2, wmax=5, vwmax=3), 8),
+    ("stencil", lambda: graphs.stencil3d(8, 7, 6, seed=1), 16),
+    ("rmat", lambda: graphs.rmat(9, 8, seed=2), 8),
+    ("wrmat", lambda: graphs.with_weights(graphs.rmat(9, 6, seed=3), seed=4, wmax=3, vwmax=4), 32),
+    ("random", lambda: graphs.random_graph(300, 1500, seed=5, wmax=7), 5),
+]
+
+
+def rand_part(n, k, seed, skew=0.0):
+    rs = np.random.default_rng(seed)
+    p = np.arange(k)
+    probs = np.exp(-skew * p)
+    probs /= probs.sum()
+    return rs.choice(k, n, p=probs).astype(np.int32)
+
+
+@pytest.mark.parametrize("name,mk,k", CASES)
+def test_conn_tables_equal_bincount(name, mk, k):
+    g = mk()
+    n = len(g["rowptr"]) - 1
+    part = rand_part(n, k, 1)
+    off, prt, val, ts = oracle.conn_build(g, part, k)
+    cd = conn_dicts(g, part)
+    for i in range(n):
+        row = sorted(cd[i].items())
+        got = list(zip(prt[off[i]:off[i + 1]].tolist(), val[off[i]:off[i + 1]].tolist()))
+        assert got == row
+        cap = min(int(g["rowptr"][i + 1] - g["rowptr"][i]), k)
+        used = len(row)
+        shrunk = used + max(used // 4, 3)
+        assert ts[i] == (shrunk if shrunk < cap else cap)
+
+
+@pytest.mark.parametrize("name,mk,k", CASES)
+def test_total_cut_brute_force(name, mk, k):
+    g = mk()
+    n = len(g["rowptr"]) - 1
+    part = rand_part(n, k, 2)
+    A = sp.csr_matrix((g["ew"].astype(np.int64), g["col"], g["rowptr"]), shape=(n, n)).tocoo()
+    assert oracle.total_cut(g, part) == int(A.data[part[A.row] != part[A.col]].sum())
+
+
+def lp_closed_form(g, part, k, fr):
+    rp, col, ew = g["rowptr"], g["col"], g["ew"]
+    n = len(rp) - 1
+    cd = conn_dicts(g, part)
+    dest = np.full(n, NO_MOVE)
+    gain = np.zeros(n, dtype=np.int64)
+    for i in range(n):
+        p = int(part[i])
+        others = {q: w for q, w in cd[i].items() if q != p and w > 0}
+        if not others:
+            continue
+        b = max(others.values())
+        best = min(q for q, w in others.items() if w == b)
+        pc = cd[i].get(p, 0)
+        if b >= pc or (pc - b) < math.floor(fr * pc):
+            dest[i] = best
+            gain[i] = b - pc
+    cand = np.where(dest != NO_MOVE)[0]
+    pre = np.full(n, np.iinfo(np.int64).min, dtype=np.int64)
+    pre[cand] = gain[cand]
+    moves = []
+    for i in cand:
+        p, best, ig, ch = part[i], dest[i], pre[i], 0
+        for j in range(rp[i], rp[i + 1]):
+            v = col[j]
+            if pre[v] > ig or (pre[v] == ig and v < i):
+                w = int(ew[j])
+                ch += -w if dest[v] == p else (w if dest[v] == best else 0)
+                ch += w if part[v] == p else (-w if part[v] == best else 0)
+        if ig + ch >= 0:
+            moves.append(i)
+    return np.array(moves, dtype=np.int32), dest[moves]
+
+
+@pytest.mark.parametrize("name,mk,k", CASES)
+@pytest.mark.parametrize("fr", [0.75, 0.25])
+def test_lp_step_closed_form(name, mk, k, fr):
+    g = mk()
+    n = len(g["rowptr"]) - 1
+    part = rand_part(n, k, 3)
+    mv, de = oracle.refine_step(0, g, part, k, 1.03, fr)
+    emv, ede = lp_closed_form(g, part, k, fr)
+    assert np.array_equal(mv, emv) and np.array_equal(de, ede)
+
+
+def sections_for(n, k):
+    s = 128
+    size = (n + s * k) // (s * k)
+    if size < 4096:
+        size = 4096
+        s = (n + size * k) // (size * k)
+    return s
+
+
+def weak_closed_form(g, part, k, imb):
+    n = len(g["rowptr"]) - 1
+    vw = g["vw"].astype(np.int64)
+    sizes = np.bincount(part, weights=vw, minlength=k).astype(np.int64)
+    opt = (int(vw.sum()) + k - 1) // k
+    smax = int(opt * imb)
+    max_dest = int(smax * 0.99)
+    if max_dest < smax - 100:
+        max_dest = smax - 100
+    under = [p for p in range(k) if sizes[p] < max_dest]
+    cd = conn_dicts(g, part)
+    secs = sections_for(n, k)
+    dest = part.copy()
+    save = np.zeros(n, dtype=np.int64)
+    for i in range(n):
+        p = int(part[i])
+        if sizes[p] > smax and vw[i] < 1.5 * (sizes[p] - opt):
+            recv = {q: w for q, w in cd[i].items() if sizes[q] < max_dest and w > 0}
+            pg = cd[i].get(p, 0)
+            if recv:
+                b = max(recv.values())
+                dest[i] = min(q for q, w in recv.items() if w == b)
+                save[i] = b - pg
+            elif under:
+                dest[i] = under[i % len(under)]
+                save[i] = -pg
+    idx = np.where(dest != part)[0]
+    bid = np.array([(50 * part[i] + oracle.gain_bucket(save[i], vw[i])) * secs + i % secs for i in idx],
+                   dtype=np.int64)
+    # canonical position: order by (bucket, vertex id); prefix of weights
+    order = np.lexsort((idx, bid))
+    pref = np.concatenate([[0], np.cumsum(vw[idx[order]])])
+    score_of = {}
+    for r, o in enumerate(order):
+        i = idx[o]
+        p = part[i]
+        before_part = pref[np.searchsorted(bid[order], 50 * p * secs, side="left")]
+        score_of[i] = pref[r] - before_part
+    moves = [i for i in idx if score_of[i] < sizes[part[i]] - smax]
+    return np.array(moves, dtype=np.int32), dest[moves]
+
+
+@pytest.mark.parametrize("name,mk,k", CASES)
+@pytest.mark.parametrize("skew", [0.5, 2.0])
+def test_rebalance_weak_closed_form(name, mk, k, skew):
+    g = mk()
+    n = len(g["rowptr"]) - 1
+    part = rand_part(n, k, 4, skew)
+    mv, de = oracle.refine_step(1, g, part, k, 1.03)
+    emv, ede = weak_closed_form(g, part, k, 1.03)
+    assert np.array_equal(mv, emv) and np.array_equal(de, ede)
+
+
+@pytest.mark.parametrize("name,mk,k", CASES)
+@pytest.mark.parametrize("skew", [0.5, 2.0])
+def test_rebalance_strong_invariants(name, mk, k, skew):
+    g = mk()
+    n = len(g["rowptr"]) - 1
+    part = rand_part(n, k, 5, skew)
+    vw = g["vw"].astype(np.int64)
+    sizes = np.bincount(part, weights=vw, minlength=k).astype(np.int64)
+    opt = (int(vw.sum()) + k - 1) // k
+    smax = int(opt * 1.03)
+    max_dest = max(opt + 1, int(smax * 0.99))
+    mv, de = oracle.refine_step(2, g, part, k, 1.03)
+    assert np.all(np.diff(mv) > 0)
+    assert np.all(sizes[part[mv]] > smax)  # only oversized parts evict
+    moved = de != part[mv]
+    assert np.all(sizes[de[moved]] < max_dest)  # receivers are below max_dest
+    # receivers never get more than their chunk plus half of a straddling vertex
+    add = np.bincount(de[moved], weights=vw[mv][moved], minlength=k)
+    cap = np.maximum(max_dest - sizes, 0)
+    assert np.all(add <= cap + vw.max())
+    over = np.maximum(sizes - smax, 0)
+    if over.sum() > 0 and (sizes < max_dest).any():
+        assert len(mv) > 0
+
+
+@pytest.mark.parametrize("name,mk,k", CASES)
+@pytest.mark.parametrize("skew", [0.0, 1.5])
+def test_refine_level_tracks_cut_and_sizes(name, mk, k, skew):
+    g = mk()
+    n = len(g["rowptr"]) - 1
+    part0 = rand_part(n, k, 6, skew)
+    p, rec = oracle.refine_level(g, part0, k, 1.03, uniform=g["uniform"])
+    cut2, imb, _, _, iters, lp_iters, opt, smax = rec
+    assert cut2 == oracle.total_cut(g, p)  # perform_moves' incremental cut matches brute force
+    sizes = np.bincount(p, weights=g["vw"], minlength=k)
+    assert imb == max(0, int(sizes.max()) - opt)
+    assert iters >= 12
+    assert cut2 <= oracle.total_cut(g, part0) or imb < max(0, np.bincount(part0, weights=g["vw"]).max() - opt)
+
+
+def test_ultra_settings_run_more_iterations():
+    g = graphs.grid2d(24, 24)
+    part0 = rand_part(576, 4, 7)
+    _, rec_n = oracle.refine_level(g, part0, 4, 1.03)
+    _, rec_u = oracle.refine_level(g, part0, 4, 1.03, ultra=True)
+    assert rec_u[4] >= 17 * 12 > rec_n[4]
+
+
+@pytest.mark.parametrize("alg", [0, 1, 2])
+def test_partition_grid_balanced_and_good(alg):
+    g = graphs.grid2d(64, 64)
+    k = 8
+    part, cut, levels = oracle.partition(g, k, 1.03, alg=alg, seed=1)
+    sizes = np.bincount(part, minlength=k)
+    opt = (4096 + k - 1) // k
+    assert sizes.max() <= int(opt * 1.03)
+    assert cut == oracle.total_cut(g, part) // 2
+    assert cut < 64 * (k - 1)  # better than the k-way slab partition
+    part2, cut2, _ = oracle.partition(g, k, 1.03, alg=alg, seed=1)
+    assert cut2 == cut and np.array_equal(part, part2)
+
+
+def test_initpart_balance_and_quality():
+    g = graphs.grid2d(32, 32)
+    for k in (2, 3, 8, 13):
+        p = oracle.initpart(g, k, 1.03)
+        sizes = np.bincount(p, minlength=k)
+        assert sizes.min() > 0
+        assert sizes.max() <= math.ceil(1024 / k * 1.03) + 1
+    p = oracle.initpart(g, 2, 1.0)
+    assert oracle.total_cut(g, p) // 2 <= 40  # a straight cut is 32
