@@ -1,0 +1,209 @@
+/*
+ * C-ABI of the MI355X Jet partitioner (libjet_amd.so).
+ *
+ * The reference exposes C++ overloads over Kokkos views
+ * (header/jet.h:46-107, namespace jet_partitioner); an FFI cannot bind those.
+ * This header is the raw-CSR boundary that replaces them: plain pointers and
+ * sizes, caller-owned output, integer error codes, no exceptions.
+ *
+ * Graph input: symmetric CSR with both directions stored (the reference's
+ * unchecked precondition), row offsets monotone from 0, columns in [0, n),
+ * positive weights. Edge/vertex weight arrays may be NULL (unit weights).
+ * `ptr_space` says whether every array argument of the call lives in host
+ * memory (JET_PTR_HOST) or HIP device memory (JET_PTR_DEVICE).
+ */
+#ifndef JET_CAPI_H
+#define JET_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes. */
+#define JET_OK 0
+#define JET_EINVAL (-1)      /* malformed CSR, bad config, k out of range */
+#define JET_ENOMEM (-2)      /* device allocation failed */
+#define JET_EINIT (-3)       /* initial partitioner failed (the reference exits, initial_partition.hpp:106-109) */
+#define JET_EDEVICE (-4)     /* HIP error or no usable GPU */
+#define JET_EUNSUPPORTED (-5)
+
+#define JET_PTR_HOST 0
+#define JET_PTR_DEVICE 1
+
+#define JET_MAX_PARTS 8192
+#define JET_MAX_LEVELS 256
+
+/* config_t (header/jet_config.h:43-52): same fields, same meaning.
+ * coarsening_alg: 0 = 2-hop matching (MtMetis), 1 = HEC, 2 = pure matching,
+ * other = 2-hop matching (partitioner.hpp:75-87). num_iter: attempts, best
+ * edge cut kept by strict < (app/driver.cpp:110-130). dump_coarse is not
+ * supported and must be 0. */
+typedef struct jet_config_t {
+    int32_t coarsening_alg;
+    int32_t num_iter;
+    double max_imb_ratio;
+    int32_t num_parts;
+    double refine_tolerance;
+    int32_t dump_coarse;
+    int32_t verbose;
+    int32_t ultra_settings;
+} jet_config_t;
+
+/* experiment_data::CoarseLevel (header/experiment_data.hpp:102-123). */
+typedef struct jet_level_stats_t {
+    int64_t edge_cut;
+    double imb;
+    int64_t num_edges;
+    int64_t num_vertices;
+    double total_ref_time;
+    int32_t total_iterations;
+    int32_t lp_iterations;
+} jet_level_stats_t;
+
+/* experiment_data<T> (header/experiment_data.hpp:50-243): phase timers in
+ * seconds and per-level refinement records, coarsest level first. */
+typedef struct jet_stats_t {
+    double total_s;
+    double coarsen_s;
+    double map_s;
+    double build_s;
+    double init_partition_s;
+    double refine_s;
+    int32_t num_levels;
+    int32_t attempts;
+    double finest_imb_ratio;
+    int64_t finest_edge_cut;
+    int64_t largest_part;
+    int64_t smallest_part;
+    jet_level_stats_t levels[JET_MAX_LEVELS];
+} jet_stats_t;
+
+/* config_t defaults (header/jet_config.h:43-52). */
+void jet_config_default(jet_config_t* cfg);
+
+/* Message for the last non-zero return on this thread. */
+const char* jet_last_error(void);
+
+/* Number of visible HIP devices (0 when none). */
+int jet_device_count(void);
+
+/* Partition a graph into cfg->num_parts parts.
+ * Replaces partition_big / partition_host_big / partition_serial_big
+ * (header/jet.h:74-93; src/jet_big.cpp:47-75): int64 offsets, int32 columns,
+ * here int32 weights. part_out (n entries, caller-owned, in ptr_space)
+ * receives the best partition; *cut_out the undirected edge cut
+ * (uncoarsen.hpp:136). seed replaces the reference's std::time seed
+ * (contract.hpp:503); attempt i uses seed + i. stats may be NULL. */
+int jet_partition_csr(const jet_config_t* cfg, int64_t n, const int64_t* row_offsets, const int32_t* col,
+                      const int32_t* ew, const int32_t* vw, int32_t uniform_ew, int32_t ptr_space, uint64_t seed,
+                      int32_t* part_out, int64_t* cut_out, jet_stats_t* stats);
+
+/* Same with int32 row offsets: replaces partition / partition_host /
+ * partition_serial (header/jet.h:46-72; src/jet.cpp:47-75). */
+int jet_partition_csr32(const jet_config_t* cfg, int64_t n, const int32_t* row_offsets, const int32_t* col,
+                        const int32_t* ew, const int32_t* vw, int32_t uniform_ew, int32_t ptr_space, uint64_t seed,
+                        int32_t* part_out, int64_t* cut_out, jet_stats_t* stats);
+
+/* Same with int64 weights (big_val_t, header/jet_defs.h:48): values are
+ * narrowed to int32 after a range check (total weight < 2^31). */
+int jet_partition_csr_big(const jet_config_t* cfg, int64_t n, const int64_t* row_offsets, const int32_t* col,
+                          const int64_t* ew, const int64_t* vw, int32_t uniform_ew, int32_t ptr_space, uint64_t seed,
+                          int32_t* part_out, int64_t* cut_out, jet_stats_t* stats);
+
+/* ---- Sub-function entry points (one hot-path stage each; host or device
+ * pointers per ptr_space). They exist so integrators and parity tests can
+ * drive one stage of the path on the GPU. */
+
+/* contract.hpp:448-491 (generate_coarse_mapping) for one level: alg as
+ * config_t::coarsening_alg, level = index of the level being coarsened
+ * (0 = input graph), uniform = the level's uniform-edge-weight flag.
+ * Writes map_out[n] and *nc_out. */
+int jet_coarsen_map(int32_t alg, int32_t level, int32_t uniform, uint64_t seed, int64_t n, const int64_t* row_offsets,
+                    const int32_t* col, const int32_t* ew, const int32_t* vw, int32_t ptr_space, int32_t* map_out,
+                    int32_t* nc_out);
+
+/* heuristics.hpp:195-248: HEC heavy-neighbour choice (hn_out[n]). */
+int jet_hec_heavy_neighbours(int32_t level, int32_t uniform, uint64_t seed, int64_t n, const int64_t* row_offsets,
+                             const int32_t* col, const int32_t* ew, const int32_t* vw, int32_t ptr_space,
+                             int32_t* hn_out);
+
+/* heuristics.hpp:89-169: aggregate construction from hn[n]. */
+int jet_map_construct(int64_t n, const int32_t* hn, int32_t ptr_space, int32_t* map_out, int32_t* nc_out);
+
+/* contract.hpp:347-446: coarse graph for map (n -> nc). out_row_offsets has
+ * nc + 1 entries; out_col / out_ew must hold row_offsets[n] entries (an upper
+ * bound); rows come out sorted by column. */
+int jet_contract(int64_t n, const int64_t* row_offsets, const int32_t* col, const int32_t* ew, const int32_t* vw,
+                 const int32_t* map, int32_t nc, int32_t ptr_space, int64_t* out_row_offsets, int32_t* out_col,
+                 int32_t* out_ew, int32_t* out_vw, int64_t* out_m);
+
+/* jet_refiner.hpp:958-1048 (init_conn_data): per-vertex part-connectivity
+ * tables. out_off[n+1] gives each row's slot range (capacity min(deg, k));
+ * out_part/out_val the slots (NULL_PART = -1 when empty); out_size[n] the
+ * table sizes. Slot order inside a row is not part of the contract. */
+int jet_conn_tables(int64_t n, const int64_t* row_offsets, const int32_t* col, const int32_t* ew,
+                    const int32_t* part, int32_t k, int32_t ptr_space, int64_t* out_off, int32_t* out_part,
+                    int32_t* out_val, int32_t* out_size);
+
+/* One move-selection step on fresh refinement state for `part`: kind 0 =
+ * jet_lp (jet_refiner.hpp:200-323), 1 = rebalance_weak (:551-645), 2 =
+ * rebalance_strong (:417-546). Writes the moves in increasing vertex id and
+ * their destinations (n entries each at most). */
+int jet_refine_step(int32_t kind, int64_t n, const int64_t* row_offsets, const int32_t* col, const int32_t* ew,
+                    const int32_t* vw, const int32_t* part, int32_t k, double imb, double filter_ratio,
+                    int32_t ptr_space, int32_t* moves_out, int32_t* dest_out, int32_t* nmoves_out);
+
+/* jet_refiner.hpp:1050-1142 on one graph with fresh refine_data. part_io in:
+ * start partition, out: best partition. rec_out[8]: 2*cut, total_imb, n, m,
+ * iterations, lp_iterations, opt, size_max. */
+int jet_refine_level(int64_t n, const int64_t* row_offsets, const int32_t* col, const int32_t* ew, const int32_t* vw,
+                     int32_t k, double imb, double tol, int32_t ultra, int32_t uniform, int32_t ptr_space,
+                     int32_t* part_io, int64_t* rec_out);
+
+/* uncoarsen.hpp:82-86: fine_out[i] = coarse[map[i]]. */
+int jet_project(int64_t n_fine, const int32_t* map, const int32_t* coarse, int32_t ptr_space, int32_t* fine_out);
+
+/* part_stat.hpp:77-108: *cut2_out = sum over directed edges of w*[p_i != p_j]. */
+int jet_edge_cut(int64_t n, const int64_t* row_offsets, const int32_t* col, const int32_t* ew, const int32_t* part,
+                 int32_t ptr_space, int64_t* cut2_out);
+
+/* part_stat.hpp:146-153: sizes_out[k] = vertex weight per part. */
+int jet_part_sizes(int64_t n, const int32_t* vw, const int32_t* part, int32_t k, int32_t ptr_space,
+                   int64_t* sizes_out);
+
+/* jet_refiner.hpp:326-356 evaluated on the GPU for count (gain, vwgt) pairs
+ * (host arrays). */
+int jet_gain_bucket_batch(int64_t count, const int64_t* gains, const int64_t* vwgts, int32_t* out);
+
+/* ---- Synthetic graph generators (host memory; the measured workloads). */
+
+/* nx x ny 4-neighbour grid, id y*nx+x, unit weights. rowptr[n+1], col[m]
+ * with m = 2*(nx*(ny-1) + ny*(nx-1)). */
+int jet_generate_grid2d(int64_t nx, int64_t ny, int64_t* rowptr, int32_t* col);
+
+/* nx x ny x nz 7-point stencil, id (z*ny+y)*nx+x, edge weight
+ * 1 + splitmix64(seed ^ (min*N + max)) % wmax (wmax <= 0: unit weights). */
+int jet_generate_stencil3d(int64_t nx, int64_t ny, int64_t nz, uint64_t seed, int32_t wmax, int64_t* rowptr,
+                           int32_t* col, int32_t* ew);
+
+/* R-MAT (a, b, c, 1-a-b-c) with 2^scale vertices and ef*2^scale drawn edges,
+ * self-loops dropped, symmetrised, duplicates removed, rows sorted. col must
+ * hold 2*ef*2^scale entries; *m_out receives the stored edge count. */
+int jet_generate_rmat(int32_t scale, int32_t ef, double a, double b, double c, uint64_t seed, int64_t* rowptr,
+                      int32_t* col, int64_t* m_out);
+
+/* ---- Kernel timing for measurement tools: when enabled, the library records
+ * HIP events around every launch of the named kernel family on its stream
+ * and accumulates launch count, total milliseconds and algorithmic bytes.
+ * Families: "lp_select", "afterburner", "conn_build", "update", "contract",
+ * "hec", "project". */
+int jet_prof_enable(const char* family);
+int jet_prof_read(int64_t* launches, double* total_ms, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JET_CAPI_H */

@@ -1,0 +1,662 @@
+// Multilevel partition orchestration and the C-ABI (include/jet_capi.h).
+//
+// partition_once() follows partitioner::partition (partitioner.hpp:61-125):
+// heuristic choice and coarsening cutoff, the coarsening loop of
+// contracter::generate_coarse_graphs (contract.hpp:494-533), the initial
+// partition of the coarsest graph on the host (initial_partition.hpp:92-114),
+// then uncoarsener::multilevel_jet (uncoarsen.hpp:88-125): refine each level
+// from coarsest to finest and project onto the next finer one.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "arena.h"
+#include "coarsen.h"
+#include "contract.h"
+#include "jet_capi.h"
+#include "misc.h"
+#include "prims.h"
+#include "refine.h"
+
+extern "C" int jet_host_initpart(int32_t n, const int64_t* xadj, const int32_t* adj, const int32_t* adjw,
+                                 const int32_t* vw, int32_t k, double imb, uint64_t seed, int32_t* part);
+
+namespace jet {
+
+struct InvalidInput : std::runtime_error {
+    explicit InvalidInput(const std::string& s) : std::runtime_error(s) {}
+};
+struct InitFailure : std::runtime_error {
+    explicit InitFailure(const std::string& s) : std::runtime_error(s) {}
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Arena persist;
+    Arena scratch;
+    std::mutex mu;
+};
+
+static std::mutex g_ctx_mu;
+static Context* g_ctx = nullptr;
+
+static Context& context() {
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    if (!g_ctx) {
+        int cnt = 0;
+        hipError_t e = hipGetDeviceCount(&cnt);
+        if (e != hipSuccess || cnt < 1) {
+            (void)hipGetLastError();
+            throw DeviceError("no HIP device available");
+        }
+        Context* c = new Context();
+        JET_HIP(hipSetDevice(0));
+        JET_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        g_ctx = c;
+    }
+    return *g_ctx;
+}
+
+// Rewinds both arenas when a call ends (chunks stay cached).
+struct CallScope {
+    Context& c;
+    std::lock_guard<std::mutex> lk;
+    explicit CallScope(Context& ctx) : c(ctx), lk(ctx.mu) {
+        c.persist.reset();
+        c.scratch.reset();
+    }
+    ~CallScope() {
+        (void)hipStreamSynchronize(c.stream);
+        c.persist.reset();
+        c.scratch.reset();
+    }
+};
+
+static thread_local std::string g_err;
+
+using clk = std::chrono::steady_clock;
+static double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+template <class T>
+static T* to_device(Context& C, const T* src, int64_t count, int space) {
+    if (space == JET_PTR_DEVICE) return const_cast<T*>(src);
+    T* d = C.persist.alloc<T>(count);
+    if (count > 0) JET_HIP(hipMemcpyAsync(d, src, sizeof(T) * count, hipMemcpyHostToDevice, C.stream));
+    return d;
+}
+
+template <class T>
+static void from_device(Context& C, T* dst, const T* src, int64_t count, int space) {
+    if (count <= 0) return;
+    if (space == JET_PTR_DEVICE) {
+        if (dst != src) JET_HIP(hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyDeviceToDevice, C.stream));
+    } else {
+        JET_HIP(hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyDeviceToHost, C.stream));
+    }
+    JET_HIP(hipStreamSynchronize(C.stream));
+}
+
+template <class T>
+static T read_one(Context& C, const T* p, int space) {
+    if (space == JET_PTR_HOST) return *p;
+    return read_scalar(p, C.stream);
+}
+
+static void check_space(int space) {
+    if (space != JET_PTR_HOST && space != JET_PTR_DEVICE) throw InvalidInput("ptr_space must be JET_PTR_HOST or JET_PTR_DEVICE");
+}
+
+// Stages a CSR graph on the device (copies host arrays, widens int32 offsets,
+// narrows int64 weights, materialises unit weights) and validates it.
+static Level stage_graph(Context& C, int64_t n, const int64_t* rp64, const int32_t* rp32, const int32_t* col,
+                         const int32_t* ew32, const int64_t* ew64, const int32_t* vw32, const int64_t* vw64, int space,
+                         bool uniform, bool validate = true) {
+    check_space(space);
+    if (n < 1 || n > (int64_t)ORD_MAX - 2) throw InvalidInput("n must be in [1, 2^31 - 2)");
+    if ((!rp64 && !rp32) || !col) throw InvalidInput("row offsets and columns are required");
+    hipStream_t s = C.stream;
+    Level L;
+    L.n = (vid_t)n;
+    L.uniform = uniform;
+    if (rp64) {
+        L.m = read_one(C, rp64 + n, space);
+        int64_t first = read_one(C, rp64, space);
+        if (first != 0) throw InvalidInput("row_offsets[0] must be 0");
+        L.rowptr = to_device(C, rp64, n + 1, space);
+    } else {
+        L.m = read_one(C, rp32 + n, space);
+        int32_t first = read_one(C, rp32, space);
+        if (first != 0) throw InvalidInput("row_offsets[0] must be 0");
+        const int32_t* d32 = to_device(C, rp32, n + 1, space);
+        L.rowptr = C.persist.alloc<eoff_t>(n + 1);
+        widen_offsets(n + 1, d32, L.rowptr, s);
+    }
+    if (L.m < 0) throw InvalidInput("row_offsets[n] must be >= 0");
+    L.col = to_device(C, col, L.m, space);
+    if (ew64) {
+        const int64_t* d = to_device(C, ew64, L.m, space);
+        L.ew = C.persist.alloc<wgt_t>(L.m);
+        if (narrow_weights(L.m, d, L.ew, C.scratch, s)) throw InvalidInput("edge weights must be in [1, 2^31)");
+    } else if (ew32) {
+        L.ew = to_device(C, ew32, L.m, space);
+    } else {
+        L.ew = C.persist.alloc<wgt_t>(L.m);
+        fill_i32(L.ew, 1, L.m, s);
+    }
+    if (vw64) {
+        const int64_t* d = to_device(C, vw64, n, space);
+        L.vw = C.persist.alloc<wgt_t>(n);
+        if (narrow_weights(n, d, L.vw, C.scratch, s)) throw InvalidInput("vertex weights must be in [1, 2^31)");
+    } else if (vw32) {
+        L.vw = to_device(C, vw32, n, space);
+    } else {
+        L.vw = C.persist.alloc<wgt_t>(n);
+        fill_i32(L.vw, 1, n, s);
+    }
+    if (validate) {
+        int32_t e = validate_graph(L.n, L.rowptr, L.col, L.ew, L.vw, C.scratch, s);
+        if (e) {
+            std::string msg = "invalid graph:";
+            if (e & 1) msg += " row offsets not monotone;";
+            if (e & 2) msg += " column out of range;";
+            if (e & 4) msg += " non-positive edge weight;";
+            if (e & 8) msg += " non-positive vertex weight;";
+            if (e & 16) msg += " total weight does not fit in int32;";
+            throw InvalidInput(msg);
+        }
+    }
+    return L;
+}
+
+struct AttemptResult {
+    int64_t cut = 0;
+    double t_total = 0, t_coarsen = 0, t_map = 0, t_build = 0, t_init = 0, t_refine = 0;
+    std::vector<LevelRecord> recs;
+    int64_t largest = 0, smallest = 0, total = 0;
+    int32_t opt = 0;
+};
+
+static int32_t coarsen_level(const Level& cur, int32_t alg, uint64_t seed, int32_t level, vid_t* map, Arena& a,
+                             hipStream_t s) {
+    DGraph g = cur.view();
+    if (alg == 1) return coarsen_hec(g, cur.uniform, seed, level, map, a, s);
+    return coarsen_match(g, cur.uniform, seed, level, alg != 2, map, a, s);
+}
+
+// partitioner.hpp:61-125. part_out: device buffer of fine.n entries.
+static AttemptResult partition_once(Context& C, const jet_config_t& cfg, const Level& fine, uint64_t seed,
+                                    part_t* part_out) {
+    hipStream_t s = C.stream;
+    AttemptResult R;
+    auto t0 = clk::now();
+    const int32_t k = cfg.num_parts;
+    int32_t alg = cfg.coarsening_alg;
+    if (alg != 1 && alg != 2) alg = 0;  // default heuristic: MtMetis (2-hop matching)
+    int32_t cutoff = k * 8;
+    if (cutoff > 1024) {
+        cutoff = k * 2;
+        cutoff = std::max(1024, cutoff);
+    }
+    const int32_t min_allowed = cutoff / 4;
+    Arena::Mark level_mark = C.persist.mark();
+    std::vector<Level> levels{fine};
+    while (levels.back().n > cutoff) {
+        Level& cur = levels.back();
+        auto tm = clk::now();
+        vid_t* map = C.persist.alloc<vid_t>(cur.n);
+        int32_t nc = coarsen_level(cur, alg, seed, (int32_t)levels.size() - 1, map, C.scratch, s);
+        JET_HIP(hipStreamSynchronize(s));
+        auto tb = clk::now();
+        R.t_map += secs(tm, tb);
+        if (nc < min_allowed) break;
+        Level next;
+        contract(cur.view(), map, nc, C.persist, C.scratch, s, next);
+        next.map = map;
+        next.uniform = false;
+        JET_HIP(hipStreamSynchronize(s));
+        R.t_build += secs(tb, clk::now());
+        levels.push_back(next);
+        if (levels.size() > 200) break;
+    }
+    auto t1 = clk::now();
+    R.t_coarsen = secs(t0, t1);
+
+    // initial partition of the coarsest graph on the host
+    const Level& cs = levels.back();
+    std::vector<int64_t> h_rp(cs.n + 1);
+    std::vector<int32_t> h_col(cs.m), h_ew(cs.m), h_vw(cs.n), h_part(cs.n);
+    JET_HIP(hipMemcpyAsync(h_rp.data(), cs.rowptr, sizeof(int64_t) * (cs.n + 1), hipMemcpyDeviceToHost, s));
+    if (cs.m > 0) {
+        JET_HIP(hipMemcpyAsync(h_col.data(), cs.col, sizeof(int32_t) * cs.m, hipMemcpyDeviceToHost, s));
+        JET_HIP(hipMemcpyAsync(h_ew.data(), cs.ew, sizeof(int32_t) * cs.m, hipMemcpyDeviceToHost, s));
+    }
+    JET_HIP(hipMemcpyAsync(h_vw.data(), cs.vw, sizeof(int32_t) * cs.n, hipMemcpyDeviceToHost, s));
+    JET_HIP(hipStreamSynchronize(s));
+    if (jet_host_initpart(cs.n, h_rp.data(), h_col.data(), h_ew.data(), h_vw.data(), k, cfg.max_imb_ratio, seed,
+                          h_part.data()) != 0)
+        throw InitFailure("initial partitioner could not partition the coarsest graph");
+    part_t* cur = (levels.size() == 1) ? part_out : C.persist.alloc<part_t>(cs.n);
+    JET_HIP(hipMemcpyAsync(cur, h_part.data(), sizeof(int32_t) * cs.n, hipMemcpyHostToDevice, s));
+    JET_HIP(hipStreamSynchronize(s));
+    auto t2 = clk::now();
+    R.t_init = secs(t1, t2);
+
+    // uncoarsening (uncoarsen.hpp:88-125)
+    RefineParams rp;
+    rp.k = k;
+    rp.imb = cfg.max_imb_ratio;
+    rp.tol = cfg.refine_tolerance;
+    rp.ultra = cfg.ultra_settings != 0;
+    {
+        ArenaScope rscope(C.scratch);
+        int64_t cap = conn_capacity(fine.n, fine.rowptr, k, C.scratch, s);
+        Refiner ref(C.scratch, s, fine.n, cap, rp);
+        for (size_t L = levels.size(); L-- > 0;) {
+            LevelRecord rec = ref.refine(levels[L], cur);
+            if (cfg.verbose && L + 1 == levels.size())
+                std::printf("Initial %lld %.6f %d\n", (long long)(rec.cut2 / 2), 0.0, (int)levels[L].n);
+            R.recs.push_back(rec);
+            if (L > 0) {
+                part_t* fp = (L == 1) ? part_out : C.persist.alloc<part_t>(levels[L - 1].n);
+                project(levels[L - 1].n, levels[L].map, cur, fp, s);
+                cur = fp;
+            }
+        }
+        R.cut = ref.cut2() / 2;
+        R.total = ref.total_size();
+        R.opt = ref.opt();
+        std::vector<int64_t> hs(k);
+        JET_HIP(hipMemcpyAsync(hs.data(), ref.best_sizes_dev(), sizeof(int64_t) * k, hipMemcpyDeviceToHost, s));
+        JET_HIP(hipStreamSynchronize(s));
+        R.largest = 0;
+        R.smallest = R.total;
+        for (int64_t v : hs) {
+            R.largest = std::max(R.largest, v);
+            R.smallest = std::min(R.smallest, v);
+        }
+    }
+    auto t3 = clk::now();
+    R.t_refine = secs(t2, t3);
+    R.t_total = secs(t0, t3);
+    C.persist.release(level_mark);
+    return R;
+}
+
+static void fill_stats(const AttemptResult& R, int32_t attempts, int32_t k, jet_stats_t* st) {
+    if (!st) return;
+    std::memset(st, 0, sizeof(*st));
+    st->total_s = R.t_total;
+    st->coarsen_s = R.t_coarsen;
+    st->map_s = R.t_map;
+    st->build_s = R.t_build;
+    st->init_partition_s = R.t_init;
+    st->refine_s = R.t_refine;
+    st->attempts = attempts;
+    st->num_levels = (int32_t)std::min<size_t>(R.recs.size(), JET_MAX_LEVELS);
+    for (int32_t i = 0; i < st->num_levels; i++) {
+        const LevelRecord& r = R.recs[i];
+        jet_level_stats_t& o = st->levels[i];
+        o.edge_cut = r.cut2 / 2;
+        o.imb = R.opt > 0 ? (double)r.total_imb / (double)R.opt : 0.0;
+        o.num_edges = r.m;
+        o.num_vertices = r.n;
+        o.total_ref_time = r.seconds;
+        o.total_iterations = r.iterations;
+        o.lp_iterations = r.lp_iterations;
+    }
+    double opt = (double)((R.total + k - 1) / k);
+    st->finest_imb_ratio = opt > 0 ? (double)R.largest / opt : 0.0;
+    st->finest_edge_cut = R.cut;
+    st->largest_part = R.largest;
+    st->smallest_part = R.smallest;
+}
+
+static void check_config(const jet_config_t* cfg) {
+    if (!cfg) throw InvalidInput("config is NULL");
+    if (cfg->num_parts < 1 || cfg->num_parts > JET_MAX_PARTS) throw InvalidInput("num_parts must be in [1, 8192]");
+    if (!(cfg->max_imb_ratio >= 1.0)) throw InvalidInput("max_imb_ratio must be >= 1");
+    if (cfg->dump_coarse) throw InvalidInput("dump_coarse is not supported");
+}
+
+// Attempt loop of app/driver.cpp:110-130: best cut by strict <, seeds seed + i.
+static void partition_entry(const jet_config_t* cfg, int64_t n, const int64_t* rp64, const int32_t* rp32,
+                            const int32_t* col, const int32_t* ew32, const int64_t* ew64, const int32_t* vw32,
+                            const int64_t* vw64, int32_t uniform_ew, int32_t space, uint64_t seed, int32_t* part_out,
+                            int64_t* cut_out, jet_stats_t* stats) {
+    check_config(cfg);
+    if (!part_out || !cut_out) throw InvalidInput("part_out and cut_out are required");
+    Context& C = context();
+    CallScope call(C);
+    Level fine = stage_graph(C, n, rp64, rp32, col, ew32, ew64, vw32, vw64, space, uniform_ew != 0);
+    int32_t attempts = std::max(cfg->num_iter, 1);
+    part_t* best = (space == JET_PTR_DEVICE && attempts == 1) ? part_out : C.persist.alloc<part_t>(n);
+    part_t* trial = attempts > 1 ? C.persist.alloc<part_t>(n) : best;
+    AttemptResult best_r;
+    bool have = false;
+    for (int32_t i = 0; i < attempts; i++) {
+        AttemptResult r = partition_once(C, *cfg, fine, seed + (uint64_t)i, trial);
+        if (!have || r.cut < best_r.cut) {
+            if (trial != best) copy_i32(best, trial, n, C.stream);
+            best_r = r;
+            have = true;
+        }
+    }
+    from_device(C, part_out, best, n, space);
+    *cut_out = best_r.cut;
+    fill_stats(best_r, attempts, cfg->num_parts, stats);
+}
+
+}  // namespace jet
+
+using namespace jet;
+
+namespace jet {
+static int guard_impl(const std::function<void()>& f) {
+    try {
+        f();
+        g_err.clear();
+        return JET_OK;
+    } catch (const InvalidInput& e) {
+        g_err = e.what();
+        return JET_EINVAL;
+    } catch (const std::invalid_argument& e) {
+        g_err = e.what();
+        return JET_EINVAL;
+    } catch (const InitFailure& e) {
+        g_err = e.what();
+        return JET_EINIT;
+    } catch (const std::bad_alloc&) {
+        g_err = "device memory allocation failed";
+        return JET_ENOMEM;
+    } catch (const DeviceError& e) {
+        g_err = e.what();
+        return JET_EDEVICE;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return JET_EDEVICE;
+    }
+}
+}  // namespace jet
+
+extern "C" {
+
+void jet_config_default(jet_config_t* c) {
+    if (!c) return;
+    c->coarsening_alg = 0;
+    c->num_iter = 1;
+    c->max_imb_ratio = 1.03;
+    c->num_parts = 2;
+    c->refine_tolerance = 0.999;
+    c->dump_coarse = 0;
+    c->verbose = 0;
+    c->ultra_settings = 0;
+}
+
+const char* jet_last_error(void) { return g_err.c_str(); }
+
+int jet_device_count(void) {
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return cnt;
+}
+
+int jet_partition_csr(const jet_config_t* cfg, int64_t n, const int64_t* row_offsets, const int32_t* col,
+                      const int32_t* ew, const int32_t* vw, int32_t uniform_ew, int32_t ptr_space, uint64_t seed,
+                      int32_t* part_out, int64_t* cut_out, jet_stats_t* stats) {
+    return guard_impl([&] {
+        partition_entry(cfg, n, row_offsets, nullptr, col, ew, nullptr, vw, nullptr, uniform_ew, ptr_space, seed,
+                        part_out, cut_out, stats);
+    });
+}
+
+int jet_partition_csr32(const jet_config_t* cfg, int64_t n, const int32_t* row_offsets, const int32_t* col,
+                        const int32_t* ew, const int32_t* vw, int32_t uniform_ew, int32_t ptr_space, uint64_t seed,
+                        int32_t* part_out, int64_t* cut_out, jet_stats_t* stats) {
+    return guard_impl([&] {
+        partition_entry(cfg, n, nullptr, row_offsets, col, ew, nullptr, vw, nullptr, uniform_ew, ptr_space, seed,
+                        part_out, cut_out, stats);
+    });
+}
+
+int jet_partition_csr_big(const jet_config_t* cfg, int64_t n, const int64_t* row_offsets, const int32_t* col,
+                          const int64_t* ew, const int64_t* vw, int32_t uniform_ew, int32_t ptr_space, uint64_t seed,
+                          int32_t* part_out, int64_t* cut_out, jet_stats_t* stats) {
+    return guard_impl([&] {
+        partition_entry(cfg, n, row_offsets, nullptr, col, nullptr, ew, nullptr, vw, uniform_ew, ptr_space, seed,
+                        part_out, cut_out, stats);
+    });
+}
+
+int jet_coarsen_map(int32_t alg, int32_t level, int32_t uniform, uint64_t seed, int64_t n, const int64_t* row_offsets,
+                    const int32_t* col, const int32_t* ew, const int32_t* vw, int32_t ptr_space, int32_t* map_out,
+                    int32_t* nc_out) {
+    return guard_impl([&] {
+        Context& C = context();
+        CallScope call(C);
+        Level L = stage_graph(C, n, row_offsets, nullptr, col, ew, nullptr, vw, nullptr, ptr_space, uniform != 0);
+        vid_t* map = ptr_space == JET_PTR_DEVICE ? map_out : C.persist.alloc<vid_t>(n);
+        int32_t a = (alg == 1 || alg == 2) ? alg : 0;
+        *nc_out = coarsen_level(L, a, seed, level, map, C.scratch, C.stream);
+        from_device(C, map_out, map, n, ptr_space);
+    });
+}
+
+int jet_hec_heavy_neighbours(int32_t level, int32_t uniform, uint64_t seed, int64_t n, const int64_t* row_offsets,
+                             const int32_t* col, const int32_t* ew, const int32_t* vw, int32_t ptr_space,
+                             int32_t* hn_out) {
+    return guard_impl([&] {
+        Context& C = context();
+        CallScope call(C);
+        Level L = stage_graph(C, n, row_offsets, nullptr, col, ew, nullptr, vw, nullptr, ptr_space, uniform != 0);
+        vid_t* hn = ptr_space == JET_PTR_DEVICE ? hn_out : C.persist.alloc<vid_t>(n);
+        hec_heavy_neighbours(L.view(), uniform != 0, seed, level, hn, C.scratch, C.stream);
+        from_device(C, hn_out, hn, n, ptr_space);
+    });
+}
+
+int jet_map_construct(int64_t n, const int32_t* hn, int32_t ptr_space, int32_t* map_out, int32_t* nc_out) {
+    return guard_impl([&] {
+        check_space(ptr_space);
+        if (n < 1 || n > ORD_MAX - 2) throw InvalidInput("n out of range");
+        Context& C = context();
+        CallScope call(C);
+        const vid_t* dh = to_device(C, hn, n, ptr_space);
+        vid_t* map = ptr_space == JET_PTR_DEVICE ? map_out : C.persist.alloc<vid_t>(n);
+        *nc_out = map_construct((vid_t)n, dh, map, C.scratch, C.stream);
+        from_device(C, map_out, map, n, ptr_space);
+    });
+}
+
+int jet_contract(int64_t n, const int64_t* row_offsets, const int32_t* col, const int32_t* ew, const int32_t* vw,
+                 const int32_t* map, int32_t nc, int32_t ptr_space, int64_t* out_row_offsets, int32_t* out_col,
+                 int32_t* out_ew, int32_t* out_vw, int64_t* out_m) {
+    return guard_impl([&] {
+        Context& C = context();
+        CallScope call(C);
+        Level L = stage_graph(C, n, row_offsets, nullptr, col, ew, nullptr, vw, nullptr, ptr_space, false);
+        if (nc < 1 || nc > n) throw InvalidInput("nc must be in [1, n]");
+        const vid_t* dmap = to_device(C, map, n, ptr_space);
+        Level out;
+        contract(L.view(), dmap, nc, C.persist, C.scratch, C.stream, out);
+        from_device(C, out_row_offsets, out.rowptr, (int64_t)nc + 1, ptr_space);
+        from_device(C, out_col, out.col, out.m, ptr_space);
+        from_device(C, out_ew, out.ew, out.m, ptr_space);
+        from_device(C, out_vw, out.vw, nc, ptr_space);
+        *out_m = out.m;
+    });
+}
+
+int jet_conn_tables(int64_t n, const int64_t* row_offsets, const int32_t* col, const int32_t* ew,
+                    const int32_t* part, int32_t k, int32_t ptr_space, int64_t* out_off, int32_t* out_part,
+                    int32_t* out_val, int32_t* out_size) {
+    return guard_impl([&] {
+        if (ptr_space != JET_PTR_HOST) throw InvalidInput("jet_conn_tables takes host pointers");
+        Context& C = context();
+        CallScope call(C);
+        Level L = stage_graph(C, n, row_offsets, nullptr, col, ew, nullptr, nullptr, nullptr, ptr_space, false);
+        const part_t* dp = to_device(C, part, n, ptr_space);
+        RefineParams rp;
+        rp.k = k;
+        int64_t cap = conn_capacity(L.n, L.rowptr, k, C.scratch, C.stream);
+        Refiner ref(C.scratch, C.stream, L.n, cap, rp);
+        std::vector<int64_t> off;
+        std::vector<int32_t> sp, sv, sz;
+        ref.conn_tables(L, dp, off, sp, sv, sz);
+        std::copy(off.begin(), off.end(), out_off);
+        std::copy(sp.begin(), sp.end(), out_part);
+        std::copy(sv.begin(), sv.end(), out_val);
+        std::copy(sz.begin(), sz.end(), out_size);
+    });
+}
+
+int jet_refine_step(int32_t kind, int64_t n, const int64_t* row_offsets, const int32_t* col, const int32_t* ew,
+                    const int32_t* vw, const int32_t* part, int32_t k, double imb, double filter_ratio,
+                    int32_t ptr_space, int32_t* moves_out, int32_t* dest_out, int32_t* nmoves_out) {
+    return guard_impl([&] {
+        if (ptr_space != JET_PTR_HOST) throw InvalidInput("jet_refine_step takes host pointers");
+        if (kind < 0 || kind > 2) throw InvalidInput("kind must be 0, 1 or 2");
+        Context& C = context();
+        CallScope call(C);
+        Level L = stage_graph(C, n, row_offsets, nullptr, col, ew, nullptr, vw, nullptr, ptr_space, false);
+        const part_t* dp = to_device(C, part, n, ptr_space);
+        RefineParams rp;
+        rp.k = k;
+        rp.imb = imb;
+        int64_t cap = conn_capacity(L.n, L.rowptr, k, C.scratch, C.stream);
+        Refiner ref(C.scratch, C.stream, L.n, cap, rp);
+        std::vector<int32_t> mv, de;
+        ref.single_step(L, dp, kind, filter_ratio, mv, de);
+        std::copy(mv.begin(), mv.end(), moves_out);
+        std::copy(de.begin(), de.end(), dest_out);
+        *nmoves_out = (int32_t)mv.size();
+    });
+}
+
+int jet_refine_level(int64_t n, const int64_t* row_offsets, const int32_t* col, const int32_t* ew, const int32_t* vw,
+                     int32_t k, double imb, double tol, int32_t ultra, int32_t uniform, int32_t ptr_space,
+                     int32_t* part_io, int64_t* rec_out) {
+    return guard_impl([&] {
+        Context& C = context();
+        CallScope call(C);
+        Level L = stage_graph(C, n, row_offsets, nullptr, col, ew, nullptr, vw, nullptr, ptr_space, uniform != 0);
+        part_t* dp = C.persist.alloc<part_t>(n);
+        if (ptr_space == JET_PTR_DEVICE)
+            JET_HIP(hipMemcpyAsync(dp, part_io, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, C.stream));
+        else JET_HIP(hipMemcpyAsync(dp, part_io, sizeof(int32_t) * n, hipMemcpyHostToDevice, C.stream));
+        RefineParams rp;
+        rp.k = k;
+        rp.imb = imb;
+        rp.tol = tol;
+        rp.ultra = ultra != 0;
+        int64_t cap = conn_capacity(L.n, L.rowptr, k, C.scratch, C.stream);
+        Refiner ref(C.scratch, C.stream, L.n, cap, rp);
+        LevelRecord rec = ref.refine(L, dp);
+        from_device(C, part_io, dp, n, ptr_space);
+        rec_out[0] = rec.cut2;
+        rec_out[1] = rec.total_imb;
+        rec_out[2] = rec.n;
+        rec_out[3] = rec.m;
+        rec_out[4] = rec.iterations;
+        rec_out[5] = rec.lp_iterations;
+        rec_out[6] = ref.opt();
+        rec_out[7] = ref.size_max();
+    });
+}
+
+int jet_project(int64_t n_fine, const int32_t* map, const int32_t* coarse, int32_t ptr_space, int32_t* fine_out) {
+    return guard_impl([&] {
+        check_space(ptr_space);
+        if (n_fine < 0 || n_fine > ORD_MAX - 2) throw InvalidInput("n out of range");
+        Context& C = context();
+        CallScope call(C);
+        const vid_t* dmap = to_device(C, map, n_fine, ptr_space);
+        int32_t nc = 0;
+        if (ptr_space == JET_PTR_HOST) {
+            for (int64_t i = 0; i < n_fine; i++) {
+                if (map[i] < 0) throw InvalidInput("map entries must be >= 0");
+                nc = std::max(nc, map[i] + 1);
+            }
+        } else {
+            throw InvalidInput("jet_project with device pointers needs the coarse size; use host pointers");
+        }
+        const part_t* dc = to_device(C, coarse, nc, ptr_space);
+        part_t* df = C.persist.alloc<part_t>(n_fine);
+        project((vid_t)n_fine, dmap, dc, df, C.stream);
+        from_device(C, fine_out, df, n_fine, ptr_space);
+    });
+}
+
+int jet_edge_cut(int64_t n, const int64_t* row_offsets, const int32_t* col, const int32_t* ew, const int32_t* part,
+                 int32_t ptr_space, int64_t* cut2_out) {
+    return guard_impl([&] {
+        Context& C = context();
+        CallScope call(C);
+        Level L = stage_graph(C, n, row_offsets, nullptr, col, ew, nullptr, nullptr, nullptr, ptr_space, false);
+        const part_t* dp = to_device(C, part, n, ptr_space);
+        *cut2_out = total_cut2(L.view(), dp, C.scratch, C.stream);
+    });
+}
+
+int jet_part_sizes(int64_t n, const int32_t* vw, const int32_t* part, int32_t k, int32_t ptr_space,
+                   int64_t* sizes_out) {
+    return guard_impl([&] {
+        check_space(ptr_space);
+        if (n < 1 || k < 1 || k > JET_MAX_PARTS) throw InvalidInput("bad n or k");
+        Context& C = context();
+        CallScope call(C);
+        DGraph g;
+        g.n = (vid_t)n;
+        wgt_t* dvw;
+        if (vw) {
+            dvw = to_device(C, vw, n, ptr_space);
+        } else {
+            dvw = C.persist.alloc<wgt_t>(n);
+            fill_i32(dvw, 1, n, C.stream);
+        }
+        g.vw = dvw;
+        const part_t* dp = to_device(C, part, n, ptr_space);
+        int64_t* ds = ptr_space == JET_PTR_DEVICE ? sizes_out : C.persist.alloc<int64_t>(k);
+        part_sizes(g, dp, k, ds, C.stream);
+        from_device(C, sizes_out, ds, k, ptr_space);
+    });
+}
+
+int jet_gain_bucket_batch(int64_t count, const int64_t* gains, const int64_t* vwgts, int32_t* out) {
+    return guard_impl([&] {
+        if (count < 0) throw InvalidInput("count < 0");
+        Context& C = context();
+        CallScope call(C);
+        const int64_t* dg = to_device(C, gains, count, JET_PTR_HOST);
+        const int64_t* dw = to_device(C, vwgts, count, JET_PTR_HOST);
+        int32_t* dout = C.persist.alloc<int32_t>(count);
+        gain_bucket_batch(count, dg, dw, dout, C.stream);
+        from_device(C, out, dout, count, JET_PTR_HOST);
+    });
+}
+
+int jet_prof_enable(const char* family) {
+    (void)family;
+    g_err = "kernel timing is not built into this library";
+    return JET_EUNSUPPORTED;
+}
+
+int jet_prof_read(int64_t* launches, double* total_ms, double* bytes) {
+    (void)launches;
+    (void)total_ms;
+    (void)bytes;
+    g_err = "kernel timing is not built into this library";
+    return JET_EUNSUPPORTED;
+}
+
+}  // extern "C"

@@ -1,0 +1,222 @@
+// Device-wide primitives for the Jet hot path on gfx950.
+#include "prims.h"
+#include "wave.h"
+
+namespace jet {
+
+// ---------------------------------------------------------------- fills
+__global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_i64(int64_t* p, int64_t v, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_iota_i32(int32_t* p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = (int32_t)i;
+}
+__global__ void k_copy_i32(int32_t* d, const int32_t* s, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = s[i];
+}
+
+void fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    k_fill_i32<<<grid_for(n, 256, 8192), 256, 0, s>>>(p, v, n);
+    JET_LAUNCH_CHECK();
+}
+void fill_i64(int64_t* p, int64_t v, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    k_fill_i64<<<grid_for(n, 256, 8192), 256, 0, s>>>(p, v, n);
+    JET_LAUNCH_CHECK();
+}
+void iota_i32(int32_t* p, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    k_iota_i32<<<grid_for(n, 256, 8192), 256, 0, s>>>(p, n);
+    JET_LAUNCH_CHECK();
+}
+void copy_i32(int32_t* d, const int32_t* src, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    k_copy_i32<<<grid_for(n, 256, 8192), 256, 0, s>>>(d, src, n);
+    JET_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- scans
+constexpr int SB = 256;
+constexpr int SI = 16;
+constexpr int64_t STILE = (int64_t)SB * SI;
+
+template <class Tin>
+__global__ void __launch_bounds__(SB) k_scan_reduce(const Tin* in, int64_t n, int64_t* bsum) {
+    __shared__ int64_t red[SB / WAVE];
+    int64_t base = blockIdx.x * STILE;
+    int64_t s = 0;
+    for (int i = 0; i < SI; i++) {
+        int64_t idx = base + (int64_t)i * SB + threadIdx.x;
+        if (idx < n) s += (int64_t)in[idx];
+    }
+    s = block_reduce_sum<int64_t, SB>(s, red);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = s;
+}
+
+template <class Tin, class Tout>
+__global__ void __launch_bounds__(SB) k_scan_down(const Tin* in, Tout* out, int64_t n, const int64_t* boff) {
+    __shared__ int64_t ws[SB / WAVE + 1];
+    int64_t carry = boff ? boff[blockIdx.x] : 0;
+    int64_t base = blockIdx.x * STILE;
+    for (int i = 0; i < SI; i++) {
+        int64_t idx = base + (int64_t)i * SB + threadIdx.x;
+        int64_t x = idx < n ? (int64_t)in[idx] : 0;
+        int64_t total;
+        int64_t ex = block_excl_scan<int64_t, SB>(x, total, ws);
+        if (idx < n) out[idx] = (Tout)(carry + ex);
+        carry += total;
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = (Tout)carry;
+}
+
+template <class Tin, class Tout>
+static void excl_scan_impl(const Tin* in, Tout* out, int64_t n, Arena& a, hipStream_t s) {
+    if (n <= 0) {
+        if (sizeof(Tout) == 8) fill_i64(reinterpret_cast<int64_t*>(out), 0, 1, s);
+        else fill_i32(reinterpret_cast<int32_t*>(out), 0, 1, s);
+        return;
+    }
+    int64_t nb = ceil_div(n, STILE);
+    if (nb == 1) {
+        k_scan_down<Tin, Tout><<<1, SB, 0, s>>>(in, out, n, nullptr);
+        JET_LAUNCH_CHECK();
+        return;
+    }
+    ArenaScope scope(a);
+    int64_t* bsum = a.alloc<int64_t>(nb);
+    int64_t* boff = a.alloc<int64_t>(nb + 1);
+    k_scan_reduce<Tin><<<(unsigned)nb, SB, 0, s>>>(in, n, bsum);
+    JET_LAUNCH_CHECK();
+    excl_scan_impl<int64_t, int64_t>(bsum, boff, nb, a, s);
+    k_scan_down<Tin, Tout><<<(unsigned)nb, SB, 0, s>>>(in, out, n, boff);
+    JET_LAUNCH_CHECK();
+}
+
+void excl_scan_i32_i32(const int32_t* in, int32_t* out, int64_t n, Arena& a, hipStream_t s) {
+    excl_scan_impl<int32_t, int32_t>(in, out, n, a, s);
+}
+void excl_scan_i32_i64(const int32_t* in, int64_t* out, int64_t n, Arena& a, hipStream_t s) {
+    excl_scan_impl<int32_t, int64_t>(in, out, n, a, s);
+}
+void excl_scan_i64_i64(const int64_t* in, int64_t* out, int64_t n, Arena& a, hipStream_t s) {
+    excl_scan_impl<int64_t, int64_t>(in, out, n, a, s);
+}
+
+// ---------------------------------------------------------------- radix sort
+constexpr int RB = 256;
+constexpr int RI = 16;
+constexpr int64_t RTILE = (int64_t)RB * RI;
+constexpr int RWAVES = RB / WAVE;
+
+template <class K>
+__global__ void __launch_bounds__(RB) k_rs_hist(const K* keys, int64_t n, int shift, int32_t* hist, int64_t nblocks) {
+    __shared__ int32_t h[256];
+    for (int t = threadIdx.x; t < 256; t += RB) h[t] = 0;
+    __syncthreads();
+    int64_t base = blockIdx.x * RTILE;
+    for (int i = 0; i < RI; i++) {
+        int64_t idx = base + (int64_t)i * RB + threadIdx.x;
+        if (idx < n) atomicAdd(&h[(uint32_t)(keys[idx] >> shift) & 255u], 1);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 256; t += RB) hist[(int64_t)t * nblocks + blockIdx.x] = h[t];
+}
+
+// Stable scatter: inside a tile, items are ranked chunk by chunk (RB items),
+// wave by wave and lane by lane, so equal digits keep their input order.
+template <class K>
+__global__ void __launch_bounds__(RB) k_rs_scatter(const K* keys, const uint32_t* vals, int64_t n, int shift,
+                                                   const int64_t* offs, int64_t nblocks, K* kout, uint32_t* vout) {
+    __shared__ int64_t base_off[256];
+    __shared__ int32_t wcnt[RWAVES][256];
+    __shared__ int32_t ctot[256];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wid = threadIdx.x / WAVE;
+    for (int t = threadIdx.x; t < 256; t += RB) base_off[t] = offs[(int64_t)t * nblocks + blockIdx.x];
+    int64_t base = blockIdx.x * RTILE;
+    for (int i = 0; i < RI; i++) {
+        for (int t = threadIdx.x; t < RWAVES * 256; t += RB) (&wcnt[0][0])[t] = 0;
+        __syncthreads();
+        int64_t idx = base + (int64_t)i * RB + threadIdx.x;
+        bool valid = idx < n;
+        K key = valid ? keys[idx] : K(0);
+        uint32_t val = valid ? vals[idx] : 0u;
+        uint32_t d = (uint32_t)(key >> shift) & 255u;
+        uint64_t same = __ballot(valid);
+        for (int b = 0; b < 8; b++) {
+            bool bit = (d >> b) & 1u;
+            uint64_t bb = __ballot(valid && bit);
+            same &= bit ? bb : ~bb;
+        }
+        uint32_t rank = (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        uint32_t cnt = (uint32_t)__popcll(same);
+        if (valid && rank == cnt - 1) wcnt[wid][d] = (int32_t)cnt;
+        __syncthreads();
+        for (int t = threadIdx.x; t < 256; t += RB) {
+            int32_t run = 0;
+            for (int w = 0; w < RWAVES; w++) {
+                int32_t c = wcnt[w][t];
+                wcnt[w][t] = run;
+                run += c;
+            }
+            ctot[t] = run;
+        }
+        __syncthreads();
+        if (valid) {
+            int64_t pos = base_off[d] + wcnt[wid][d] + rank;
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < 256; t += RB) base_off[t] += ctot[t];
+        __syncthreads();
+    }
+}
+
+template <class K>
+static void radix_sort_impl(K* keys, uint32_t* vals, int64_t n, int key_bits, K* ktmp, uint32_t* vtmp, Arena& a,
+                            hipStream_t s) {
+    if (n <= 1 || key_bits <= 0) return;
+    ArenaScope scope(a);
+    int64_t nb = ceil_div(n, RTILE);
+    int32_t* hist = a.alloc<int32_t>(256 * nb);
+    int64_t* offs = a.alloc<int64_t>(256 * nb + 1);
+    K* src_k = keys;
+    uint32_t* src_v = vals;
+    K* dst_k = ktmp;
+    uint32_t* dst_v = vtmp;
+    int passes = (key_bits + 7) / 8;
+    for (int p = 0; p < passes; p++) {
+        int shift = 8 * p;
+        k_rs_hist<K><<<(unsigned)nb, RB, 0, s>>>(src_k, n, shift, hist, nb);
+        JET_LAUNCH_CHECK();
+        excl_scan_i32_i64(hist, offs, 256 * nb, a, s);
+        k_rs_scatter<K><<<(unsigned)nb, RB, 0, s>>>(src_k, src_v, n, shift, offs, nb, dst_k, dst_v);
+        JET_LAUNCH_CHECK();
+        std::swap(src_k, dst_k);
+        std::swap(src_v, dst_v);
+    }
+    if (src_k != keys) {
+        JET_HIP(hipMemcpyAsync(keys, src_k, sizeof(K) * n, hipMemcpyDeviceToDevice, s));
+        JET_HIP(hipMemcpyAsync(vals, src_v, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+    }
+}
+
+void radix_sort_u32(uint32_t* keys, uint32_t* vals, int64_t n, int key_bits, uint32_t* ktmp, uint32_t* vtmp,
+                    Arena& a, hipStream_t s) {
+    radix_sort_impl<uint32_t>(keys, vals, n, key_bits, ktmp, vtmp, a, s);
+}
+void radix_sort_u64(uint64_t* keys, uint32_t* vals, int64_t n, int key_bits, uint64_t* ktmp, uint32_t* vtmp,
+                    Arena& a, hipStream_t s) {
+    radix_sort_impl<uint64_t>(keys, vals, n, key_bits, ktmp, vtmp, a, s);
+}
+
+}  // namespace jet

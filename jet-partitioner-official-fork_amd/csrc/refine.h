@@ -1,0 +1,123 @@
+// Jet refinement on the GPU (jet_refiner.hpp and uncoarsen.hpp of the reference).
+#pragma once
+#include <vector>
+
+#include "arena.h"
+#include "contract.h"
+#include "jet_types.h"
+
+namespace jet {
+
+// Per-level refinement record (experiment_data::CoarseLevel, experiment_data.hpp:102-123).
+struct LevelRecord {
+    int64_t cut2 = 0;       // best_state.cut after the level (2 x cut)
+    int64_t total_imb = 0;  // best_state.total_imb after the level
+    int32_t n = 0;
+    int64_t m = 0;
+    int32_t iterations = 0;
+    int32_t lp_iterations = 0;
+    double seconds = 0;
+};
+
+struct RefineParams {
+    int32_t k = 2;
+    double imb = 1.03;
+    double tol = 0.999;
+    bool ultra = false;
+};
+
+// Holds the refiner's scratch (sized by the finest level, jet_refiner.hpp:134-181)
+// and the refine_data persisting across levels (jet_refiner.hpp:100-106).
+class Refiner {
+  public:
+    Refiner(Arena& a, hipStream_t s, vid_t finest_n, int64_t finest_conn_cap, const RefineParams& p);
+
+    // jet_refiner.hpp:1050-1142 on one level; best_part (device, L.n) in/out.
+    LevelRecord refine(const Level& L, part_t* best_part);
+
+    // One move-selection step on fresh state (empty cache, no locks) for
+    // parity tests: kind 0 = jet_lp, 1 = rebalance_weak, 2 = rebalance_strong.
+    // Returns moves (sorted by id) and their destinations on the host.
+    void single_step(const Level& L, const part_t* part, int kind, double filter_ratio, std::vector<int32_t>& moves,
+                     std::vector<int32_t>& dest);
+
+    // Connectivity tables for `part` (init_conn_data), copied to the host.
+    void conn_tables(const Level& L, const part_t* part, std::vector<int64_t>& off, std::vector<int32_t>& slot_part,
+                     std::vector<int32_t>& slot_val, std::vector<int32_t>& size);
+
+    int64_t cut2() const { return cut_; }
+    int64_t total_size() const { return total_size_; }
+    int64_t total_imb() const { return total_imb_; }
+    int32_t opt() const { return opt_; }
+    int32_t size_max() const { return size_max_; }
+    const int64_t* best_sizes_dev() const { return best_sizes_; }
+
+  private:
+    struct Moves {
+        int32_t nsmall = 0, nbig = 0;
+        int32_t total() const { return nsmall + nbig; }
+    };
+    void init_conn(const Level& L, const part_t* part, const int32_t* mark);
+    void build_tiers(const Level& L);
+    Moves jet_lp(const Level& L, const part_t* part, double filter_ratio);
+    Moves rebalance_weak(const Level& L, const part_t* part, const int64_t* sizes);
+    Moves rebalance_strong(const Level& L, const part_t* part, const int64_t* sizes);
+    void perform_moves(const Level& L, part_t* part, Moves mv, int64_t* sizes, int64_t& cut_change,
+                       int64_t& max_size);
+    void sort_candidates(const Level& L, int32_t ncand, int key_bits);
+
+    Arena& a_;
+    hipStream_t s_;
+    RefineParams p_;
+    vid_t nmax_;
+    // connectivity tables
+    eoff_t* conn_off_;
+    int2* conn_;
+    int32_t* conn_size_;
+    part_t* dest_cache_;
+    int32_t* lock_;
+    int32_t* gain_p_;
+    // iteration scratch
+    int32_t* pregain_;
+    part_t* dest_;
+    int32_t* save_;
+    int32_t* bid_;
+    int32_t* flags_;
+    vid_t* cand_;
+    vid_t* moves_;
+    int32_t* swap_bit_;
+    uint32_t* skey_;
+    uint32_t* sval_;
+    uint32_t* ktmp_;
+    uint32_t* vtmp_;
+    int64_t* prefix_;
+    int64_t* score_;
+    part_t* cur_part_;
+    vid_t* tier_mid_;
+    vid_t* tier_big_;
+    int32_t ntier_mid_ = 0, ntier_big_ = 0;
+    // small device state
+    int32_t* counters_;  // [0] ncand small, [1] ncand big, [2] moves small, [3] moves big, [4] redraws
+    int64_t* acc_;       // [0] cut change 1, [1] cut change 2, [2] max part, [3] scratch
+    int64_t* curr_sizes_;
+    int64_t* best_sizes_;
+    int64_t* evict_adjust_;
+    int64_t* evict_start_;
+    int32_t* part_begin_;
+    part_t* undersized_;
+    int32_t* misc_;  // [0] undersized count
+    vid_t* redraw_;
+    // refine_data mirror on the host
+    bool init_ = false;
+    int64_t cut_ = 0, total_imb_ = 0, total_size_ = 0;
+    int32_t opt_ = 0, size_max_ = 0;
+};
+
+// uncoarsen.hpp:82-86.
+void project(vid_t n_fine, const vid_t* map, const part_t* coarse, part_t* fine, hipStream_t s);
+
+// part_stat.hpp:77-108 (2 x cut), :146-153.
+int64_t total_cut2(const DGraph& g, const part_t* part, Arena& a, hipStream_t s);
+void part_sizes(const DGraph& g, const part_t* part, int32_t k, int64_t* sizes, hipStream_t s);
+
+}  // namespace jet

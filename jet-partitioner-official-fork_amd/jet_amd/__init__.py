@@ -1,0 +1,329 @@
+"""Python binding of the MI355X Jet partitioner (ctypes over libjet_amd.so).
+
+Mirrors the reference's entry points (header/jet.h:46-107): ``partition``,
+``partition_host``, ``partition_serial`` and their ``_big`` variants, taking a
+CSR graph and a ``Config`` with the fields of ``config_t``
+(header/jet_config.h:43-52). Every call runs the HIP path; there is no CPU
+fallback, and loading fails loudly when the native library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libjet_amd.so")
+
+JET_OK, JET_EINVAL, JET_ENOMEM, JET_EINIT, JET_EDEVICE, JET_EUNSUPPORTED = 0, -1, -2, -3, -4, -5
+PTR_HOST, PTR_DEVICE = 0, 1
+MAX_LEVELS = 256
+
+
+class JetError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"jet error {code}: {msg}")
+        self.code = code
+
+
+class Config(ctypes.Structure):
+    """config_t (header/jet_config.h:43-52)."""
+    _fields_ = [("coarsening_alg", ctypes.c_int32), ("num_iter", ctypes.c_int32),
+                ("max_imb_ratio", ctypes.c_double), ("num_parts", ctypes.c_int32),
+                ("refine_tolerance", ctypes.c_double), ("dump_coarse", ctypes.c_int32),
+                ("verbose", ctypes.c_int32), ("ultra_settings", ctypes.c_int32)]
+
+    def __init__(self, num_parts=2, max_imb_ratio=1.03, coarsening_alg=0, num_iter=1, refine_tolerance=0.999,
+                 ultra_settings=False, verbose=False):
+        super().__init__(coarsening_alg, num_iter, max_imb_ratio, num_parts, refine_tolerance, 0, int(verbose),
+                         int(ultra_settings))
+
+
+class LevelStats(ctypes.Structure):
+    _fields_ = [("edge_cut", ctypes.c_int64), ("imb", ctypes.c_double), ("num_edges", ctypes.c_int64),
+                ("num_vertices", ctypes.c_int64), ("total_ref_time", ctypes.c_double),
+                ("total_iterations", ctypes.c_int32), ("lp_iterations", ctypes.c_int32)]
+
+
+class Stats(ctypes.Structure):
+    """experiment_data (header/experiment_data.hpp:50-243)."""
+    _fields_ = [("total_s", ctypes.c_double), ("coarsen_s", ctypes.c_double), ("map_s", ctypes.c_double),
+                ("build_s", ctypes.c_double), ("init_partition_s", ctypes.c_double), ("refine_s", ctypes.c_double),
+                ("num_levels", ctypes.c_int32), ("attempts", ctypes.c_int32), ("finest_imb_ratio", ctypes.c_double),
+                ("finest_edge_cut", ctypes.c_int64), ("largest_part", ctypes.c_int64),
+                ("smallest_part", ctypes.c_int64), ("levels", LevelStats * MAX_LEVELS)]
+
+
+_lib = None
+VP = ctypes.c_void_p
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"native library missing: {LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        c = ctypes
+        i32, i64, u64, dbl = c.c_int32, c.c_int64, c.c_uint64, c.c_double
+        L.jet_last_error.restype = c.c_char_p
+        L.jet_device_count.restype = c.c_int
+        sigs = {
+            "jet_config_default": [c.POINTER(Config)],
+            "jet_partition_csr": [c.POINTER(Config), i64, VP, VP, VP, VP, i32, i32, u64, VP, c.POINTER(i64),
+                                  c.POINTER(Stats)],
+            "jet_partition_csr32": [c.POINTER(Config), i64, VP, VP, VP, VP, i32, i32, u64, VP, c.POINTER(i64),
+                                    c.POINTER(Stats)],
+            "jet_partition_csr_big": [c.POINTER(Config), i64, VP, VP, VP, VP, i32, i32, u64, VP, c.POINTER(i64),
+                                      c.POINTER(Stats)],
+            "jet_coarsen_map": [i32, i32, i32, u64, i64, VP, VP, VP, VP, i32, VP, c.POINTER(i32)],
+            "jet_hec_heavy_neighbours": [i32, i32, u64, i64, VP, VP, VP, VP, i32, VP],
+            "jet_map_construct": [i64, VP, i32, VP, c.POINTER(i32)],
+            "jet_contract": [i64, VP, VP, VP, VP, VP, i32, i32, VP, VP, VP, VP, c.POINTER(i64)],
+            "jet_conn_tables": [i64, VP, VP, VP, VP, i32, i32, VP, VP, VP, VP],
+            "jet_refine_step": [i32, i64, VP, VP, VP, VP, VP, i32, dbl, dbl, i32, VP, VP, c.POINTER(i32)],
+            "jet_refine_level": [i64, VP, VP, VP, VP, i32, dbl, dbl, i32, i32, i32, VP, VP],
+            "jet_project": [i64, VP, VP, i32, VP],
+            "jet_edge_cut": [i64, VP, VP, VP, VP, i32, c.POINTER(i64)],
+            "jet_part_sizes": [i64, VP, VP, i32, i32, VP],
+            "jet_gain_bucket_batch": [i64, VP, VP, VP],
+            "jet_generate_grid2d": [i64, i64, VP, VP],
+            "jet_generate_stencil3d": [i64, i64, i64, u64, i32, VP, VP, VP],
+            "jet_generate_rmat": [i32, i32, dbl, dbl, dbl, u64, VP, VP, c.POINTER(i64)],
+            "jet_prof_enable": [c.c_char_p],
+            "jet_prof_read": [c.POINTER(i64), c.POINTER(dbl), c.POINTER(dbl)],
+        }
+        for name, args in sigs.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = None if name == "jet_config_default" else c.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != JET_OK:
+        raise JetError(rc, lib().jet_last_error().decode())
+
+
+def _p(a):
+    """Pointer of a numpy array, or an int device address, or None."""
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return a
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    return a.ctypes.data
+
+
+def _c(a, dt):
+    return None if a is None else np.ascontiguousarray(a, dtype=dt)
+
+
+def device_count():
+    return lib().jet_device_count()
+
+
+def _graph(g):
+    return (_c(g["rowptr"], np.int64), _c(g["col"], np.int32), _c(g.get("ew"), np.int32),
+            _c(g.get("vw"), np.int32))
+
+
+def partition_csr(g, cfg, seed=0, uniform=None, stats=False):
+    """jet_partition_csr on host arrays; returns (part, cut[, Stats])."""
+    rp, col, ew, vw = _graph(g)
+    n = len(rp) - 1
+    if uniform is None:
+        uniform = bool(g.get("uniform", False))
+    part = np.empty(n, dtype=np.int32)
+    cut = ctypes.c_int64(0)
+    st = Stats()
+    _check(lib().jet_partition_csr(ctypes.byref(cfg), n, _p(rp), _p(col), _p(ew), _p(vw), int(uniform), PTR_HOST,
+                                   seed, _p(part), ctypes.byref(cut), ctypes.byref(st)))
+    return (part, cut.value, st) if stats else (part, cut.value)
+
+
+def partition_device(n, rowptr_ptr, col_ptr, ew_ptr, vw_ptr, part_ptr, cfg, seed=0, uniform=False):
+    """jet_partition_csr on device pointers (ints); returns (cut, Stats)."""
+    cut = ctypes.c_int64(0)
+    st = Stats()
+    _check(lib().jet_partition_csr(ctypes.byref(cfg), n, rowptr_ptr, col_ptr, ew_ptr, vw_ptr, int(uniform),
+                                   PTR_DEVICE, seed, part_ptr, ctypes.byref(cut), ctypes.byref(st)))
+    return cut.value, st
+
+
+# Reference entry-point names (header/jet.h:46-107). They return
+# (part, edge_cut, stats) for a CSR dict {rowptr, col, ew, vw}.
+def _partition_named(g, config, uniform_ew, seed, big):
+    rp = np.ascontiguousarray(g["rowptr"])
+    n = len(rp) - 1
+    part = np.empty(n, dtype=np.int32)
+    cut = ctypes.c_int64(0)
+    st = Stats()
+    if big:
+        ew = _c(g.get("ew"), np.int64)
+        vw = _c(g.get("vw"), np.int64)
+        rc = lib().jet_partition_csr_big(ctypes.byref(config), n, _p(_c(rp, np.int64)), _p(_c(g["col"], np.int32)),
+                                         _p(ew), _p(vw), int(uniform_ew), PTR_HOST, seed, _p(part),
+                                         ctypes.byref(cut), ctypes.byref(st))
+    else:
+        rp32 = _c(rp, np.int32)
+        rc = lib().jet_partition_csr32(ctypes.byref(config), n, _p(rp32), _p(_c(g["col"], np.int32)),
+                                       _p(_c(g.get("ew"), np.int32)), _p(_c(g.get("vw"), np.int32)), int(uniform_ew),
+                                       PTR_HOST, seed, _p(part), ctypes.byref(cut), ctypes.byref(st))
+    _check(rc)
+    return part, cut.value, st
+
+
+def partition(g, config, uniform_ew=False, seed=0):
+    return _partition_named(g, config, uniform_ew, seed, False)
+
+
+partition_host = partition
+partition_serial = partition
+
+
+def partition_big(g, config, uniform_ew=False, seed=0):
+    return _partition_named(g, config, uniform_ew, seed, True)
+
+
+partition_host_big = partition_big
+partition_serial_big = partition_big
+
+
+# ---- stage entry points
+def coarsen_map(g, alg, uniform, seed, level=0):
+    rp, col, ew, vw = _graph(g)
+    n = len(rp) - 1
+    out = np.empty(n, dtype=np.int32)
+    nc = ctypes.c_int32(0)
+    _check(lib().jet_coarsen_map(alg, level, int(uniform), seed, n, _p(rp), _p(col), _p(ew), _p(vw), PTR_HOST,
+                                 _p(out), ctypes.byref(nc)))
+    return out, nc.value
+
+
+def hec_hn(g, uniform, seed, level):
+    rp, col, ew, vw = _graph(g)
+    n = len(rp) - 1
+    out = np.empty(n, dtype=np.int32)
+    _check(lib().jet_hec_heavy_neighbours(level, int(uniform), seed, n, _p(rp), _p(col), _p(ew), _p(vw), PTR_HOST,
+                                          _p(out)))
+    return out
+
+
+def map_construct(hn):
+    hn = _c(hn, np.int32)
+    out = np.empty(len(hn), dtype=np.int32)
+    nc = ctypes.c_int32(0)
+    _check(lib().jet_map_construct(len(hn), _p(hn), PTR_HOST, _p(out), ctypes.byref(nc)))
+    return out, nc.value
+
+
+def contract(g, mp, nc):
+    rp, col, ew, vw = _graph(g)
+    n = len(rp) - 1
+    m = len(col)
+    orp = np.empty(nc + 1, dtype=np.int64)
+    ocol = np.empty(max(m, 1), dtype=np.int32)
+    oew = np.empty(max(m, 1), dtype=np.int32)
+    ovw = np.empty(max(nc, 1), dtype=np.int32)
+    mc = ctypes.c_int64(0)
+    _check(lib().jet_contract(n, _p(rp), _p(col), _p(ew), _p(vw), _p(_c(mp, np.int32)), nc, PTR_HOST, _p(orp),
+                              _p(ocol), _p(oew), _p(ovw), ctypes.byref(mc)))
+    return {"rowptr": orp, "col": ocol[:mc.value].copy(), "ew": oew[:mc.value].copy(), "vw": ovw[:nc].copy()}
+
+
+def conn_tables(g, part, k):
+    """Raw tables: (off[n+1], slot_part, slot_val, size[n])."""
+    rp, col, ew, _ = _graph(g)
+    n = len(rp) - 1
+    cap = int(np.minimum(np.diff(rp), k).sum())
+    off = np.empty(n + 1, dtype=np.int64)
+    sp = np.empty(max(cap, 1), dtype=np.int32)
+    sv = np.empty(max(cap, 1), dtype=np.int32)
+    sz = np.empty(n, dtype=np.int32)
+    _check(lib().jet_conn_tables(n, _p(rp), _p(col), _p(ew), _p(_c(part, np.int32)), k, PTR_HOST, _p(off), _p(sp),
+                                 _p(sv), _p(sz)))
+    return off, sp[:cap], sv[:cap], sz
+
+
+def refine_step(kind, g, part, k, imb, filter_ratio=0.75):
+    rp, col, ew, vw = _graph(g)
+    n = len(rp) - 1
+    mv = np.empty(max(n, 1), dtype=np.int32)
+    de = np.empty(max(n, 1), dtype=np.int32)
+    cnt = ctypes.c_int32(0)
+    _check(lib().jet_refine_step(kind, n, _p(rp), _p(col), _p(ew), _p(vw), _p(_c(part, np.int32)), k, imb,
+                                 filter_ratio, PTR_HOST, _p(mv), _p(de), ctypes.byref(cnt)))
+    return mv[:cnt.value].copy(), de[:cnt.value].copy()
+
+
+def refine_level(g, part, k, imb, tol=0.999, ultra=False, uniform=False):
+    rp, col, ew, vw = _graph(g)
+    n = len(rp) - 1
+    p = np.array(part, dtype=np.int32)
+    rec = np.zeros(8, dtype=np.int64)
+    _check(lib().jet_refine_level(n, _p(rp), _p(col), _p(ew), _p(vw), k, imb, tol, int(ultra), int(uniform),
+                                  PTR_HOST, _p(p), _p(rec)))
+    return p, rec
+
+
+def project(mp, coarse):
+    mp = _c(mp, np.int32)
+    out = np.empty(len(mp), dtype=np.int32)
+    _check(lib().jet_project(len(mp), _p(mp), _p(_c(coarse, np.int32)), PTR_HOST, _p(out)))
+    return out
+
+
+def edge_cut2(g, part):
+    rp, col, ew, _ = _graph(g)
+    out = ctypes.c_int64(0)
+    _check(lib().jet_edge_cut(len(rp) - 1, _p(rp), _p(col), _p(ew), _p(_c(part, np.int32)), PTR_HOST,
+                              ctypes.byref(out)))
+    return out.value
+
+
+def part_sizes(vw, part, k):
+    out = np.empty(k, dtype=np.int64)
+    _check(lib().jet_part_sizes(len(part), _p(_c(vw, np.int32)), _p(_c(part, np.int32)), k, PTR_HOST, _p(out)))
+    return out
+
+
+def gain_bucket_batch(gains, vws):
+    g = _c(gains, np.int64)
+    w = _c(vws, np.int64)
+    out = np.empty(len(g), dtype=np.int32)
+    _check(lib().jet_gain_bucket_batch(len(g), _p(g), _p(w), _p(out)))
+    return out
+
+
+# ---- generators (host)
+def grid2d(nx, ny):
+    n = nx * ny
+    m = 2 * (nx * (ny - 1) + ny * (nx - 1))
+    rp = np.empty(n + 1, dtype=np.int64)
+    col = np.empty(max(m, 1), dtype=np.int32)
+    _check(lib().jet_generate_grid2d(nx, ny, _p(rp), _p(col)))
+    return {"rowptr": rp, "col": col[:m], "ew": np.ones(m, dtype=np.int32), "vw": np.ones(n, dtype=np.int32),
+            "uniform": True}
+
+
+def stencil3d(nx, ny, nz, seed=0, wmax=10):
+    n = nx * ny * nz
+    m = 2 * (nx * ny * (nz - 1) + nx * nz * (ny - 1) + ny * nz * (nx - 1))
+    rp = np.empty(n + 1, dtype=np.int64)
+    col = np.empty(max(m, 1), dtype=np.int32)
+    ew = np.empty(max(m, 1), dtype=np.int32)
+    _check(lib().jet_generate_stencil3d(nx, ny, nz, seed, wmax, _p(rp), _p(col), _p(ew)))
+    return {"rowptr": rp, "col": col[:m], "ew": ew[:m], "vw": np.ones(n, dtype=np.int32), "uniform": wmax <= 1}
+
+
+def rmat(scale, ef, seed=0, a=0.57, b=0.19, c=0.19):
+    n = 1 << scale
+    rp = np.empty(n + 1, dtype=np.int64)
+    col = np.empty(2 * ef * n, dtype=np.int32)
+    m = ctypes.c_int64(0)
+    _check(lib().jet_generate_rmat(scale, ef, a, b, c, seed, _p(rp), _p(col), ctypes.byref(m)))
+    col = col[:m.value].copy()
+    return {"rowptr": rp, "col": col, "ew": np.ones(m.value, dtype=np.int32), "vw": np.ones(n, dtype=np.int32),
+            "uniform": True}

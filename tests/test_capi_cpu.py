@@ -1,0 +1,80 @@
+"""CPU checks of the C-ABI library: it loads, exports every declared symbol,
+its host generators produce the measured workloads, and it reports a clean
+error (no crash) when no GPU is present."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import graphs
+import jet_amd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "jet_capi.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(jet_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = jet_amd.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(L, s), s
+
+
+def test_config_defaults_match_reference():
+    c = jet_amd.Config()
+    jet_amd.lib().jet_config_default(c)
+    assert (c.coarsening_alg, c.num_iter, c.max_imb_ratio, c.num_parts, c.refine_tolerance, c.dump_coarse,
+            c.verbose, c.ultra_settings) == (0, 1, 1.03, 2, 0.999, 0, 0, 0)
+
+
+def test_grid_generator_matches_numpy_builder():
+    g = jet_amd.grid2d(37, 23)
+    h = graphs.grid2d(37, 23)
+    assert np.array_equal(g["rowptr"], h["rowptr"]) and np.array_equal(g["col"], h["col"])
+
+
+def test_stencil_generator_matches_numpy_builder():
+    g = jet_amd.stencil3d(9, 7, 5, seed=3, wmax=10)
+    h = graphs.stencil3d(9, 7, 5, seed=3)
+    for key in ("rowptr", "col", "ew"):
+        assert np.array_equal(g[key], h[key])
+    assert g["ew"].min() >= 1 and g["ew"].max() <= 10
+
+
+def test_stencil_c2_shape():
+    g = jet_amd.stencil3d(128, 128, 128, seed=0)
+    assert len(g["rowptr"]) - 1 == 2097152
+    assert len(g["col"]) == 12484608
+
+
+def test_rmat_generator_is_simple_symmetric():
+    g = jet_amd.rmat(12, 8, seed=1)
+    rp, col = g["rowptr"], g["col"]
+    n = len(rp) - 1
+    src = np.repeat(np.arange(n), np.diff(rp))
+    assert np.all(src != col)
+    a = src.astype(np.int64) * n + col
+    b = col.astype(np.int64) * n + src
+    assert np.array_equal(np.sort(a), np.sort(b))
+    assert len(np.unique(a)) == len(a)
+    for i in range(0, n, 97):
+        assert np.all(np.diff(col[rp[i]:rp[i + 1]]) > 0)
+    deg = np.diff(rp)
+    assert deg.max() > 20 * deg.mean()  # power-law hubs
+    assert (deg == 0).mean() > 0.2       # many isolated vertices at edge factor 8
+
+
+def test_missing_gpu_is_a_clean_error():
+    if jet_amd.device_count() > 0:
+        pytest.skip("a GPU is present")
+    g = graphs.grid2d(8, 8)
+    with pytest.raises(jet_amd.JetError) as e:
+        jet_amd.partition_csr(g, jet_amd.Config(num_parts=2))
+    assert e.value.code == jet_amd.JET_EDEVICE
