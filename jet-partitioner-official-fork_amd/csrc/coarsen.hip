@@ -34,7 +34,8 @@ __device__ inline T group_sum(T x) {
 // ---------------------------------------------------------------- heavy list
 __global__ void k_heavy_list(DGraph g, vid_t* list, int32_t* cnt) {
     FOR_THREAD(i, g.n) {
-        if (g.rowptr[i + 1] - g.rowptr[i] > HEAVY_DEG) list[atomicAdd(cnt, 1)] = (vid_t)i;
+        int32_t p = wave_append(cnt, g.rowptr[i + 1] - g.rowptr[i] > HEAVY_DEG);
+        if (p >= 0) list[p] = (vid_t)i;
     }
 }
 
@@ -208,7 +209,8 @@ __global__ void k_mc_resolve(const vid_t* wl, int32_t len, int32_t round, int sw
 __global__ void k_mc_next(const vid_t* wl, int32_t len, vid_t n, const vid_t* vcmap, vid_t* next, int32_t* cnt) {
     FOR_THREAD(x, len) {
         vid_t u = wl ? wl[x] : (vid_t)x;
-        if (vcmap[u] >= n) next[atomicAdd(cnt, 1)] = u;
+        int32_t p = wave_append(cnt, vcmap[u] >= n);
+        if (p >= 0) next[p] = u;
     }
 }
 
@@ -443,7 +445,8 @@ __global__ void k_match_next(const vid_t* wl, int32_t len, const vid_t* vcmap, c
                              int32_t* cnt) {
     FOR_THREAD(x, len) {
         vid_t u = wl ? wl[x] : (vid_t)x;
-        if (vcmap[u] == ORD_MAX && hn[u] != ORD_MAX) next[atomicAdd(cnt, 1)] = u;
+        int32_t p = wave_append(cnt, vcmap[u] == ORD_MAX && hn[u] != ORD_MAX);
+        if (p >= 0) next[p] = u;
     }
 }
 

@@ -43,9 +43,12 @@ __global__ void k_members(vid_t n, const vid_t* map, const int32_t* moff, int32_
 __global__ void k_tiers(vid_t nc, const eoff_t* hoff, vid_t* l1, vid_t* l2, vid_t* l3, int32_t* cnt) {
     FOR_THREAD(c, nc) {
         eoff_t hs = hoff[c + 1] - hoff[c];
-        if (hs <= T1_MAX) l1[atomicAdd(&cnt[0], 1)] = (vid_t)c;
-        else if (hs <= T2_MAX) l2[atomicAdd(&cnt[1], 1)] = (vid_t)c;
-        else l3[atomicAdd(&cnt[2], 1)] = (vid_t)c;
+        int32_t a = wave_append(&cnt[0], hs <= T1_MAX);
+        int32_t b = wave_append(&cnt[1], hs > T1_MAX && hs <= T2_MAX);
+        int32_t d = wave_append(&cnt[2], hs > T2_MAX);
+        if (a >= 0) l1[a] = (vid_t)c;
+        if (b >= 0) l2[b] = (vid_t)c;
+        if (d >= 0) l3[d] = (vid_t)c;
     }
 }
 

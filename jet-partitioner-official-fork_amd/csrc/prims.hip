@@ -61,20 +61,43 @@ __global__ void __launch_bounds__(SB) k_scan_reduce(const Tin* in, int64_t n, in
     if (threadIdx.x == 0) bsum[blockIdx.x] = s;
 }
 
+// Tile of STILE items: coalesced load into LDS (padded against bank
+// conflicts), each thread scans SI consecutive items, one workgroup scan of
+// the thread totals, coalesced store.
+__device__ inline int pad_idx(int li) { return li + (li >> 5); }
+
 template <class Tin, class Tout>
 __global__ void __launch_bounds__(SB) k_scan_down(const Tin* in, Tout* out, int64_t n, const int64_t* boff) {
+    __shared__ int64_t tile[STILE + STILE / 32];
     __shared__ int64_t ws[SB / WAVE + 1];
-    int64_t carry = boff ? boff[blockIdx.x] : 0;
-    int64_t base = blockIdx.x * STILE;
+    const int64_t base = blockIdx.x * STILE;
     for (int i = 0; i < SI; i++) {
-        int64_t idx = base + (int64_t)i * SB + threadIdx.x;
-        int64_t x = idx < n ? (int64_t)in[idx] : 0;
-        int64_t total;
-        int64_t ex = block_excl_scan<int64_t, SB>(x, total, ws);
-        if (idx < n) out[idx] = (Tout)(carry + ex);
-        carry += total;
+        int li = i * SB + threadIdx.x;
+        int64_t idx = base + li;
+        tile[pad_idx(li)] = idx < n ? (int64_t)in[idx] : 0;
     }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = (Tout)carry;
+    __syncthreads();
+    int64_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < SI; j++) sum += tile[pad_idx(threadIdx.x * SI + j)];
+    int64_t total;
+    int64_t ex = block_excl_scan<int64_t, SB>(sum, total, ws);
+    const int64_t b0 = boff ? boff[blockIdx.x] : 0;
+    int64_t carry = b0 + ex;
+#pragma unroll
+    for (int j = 0; j < SI; j++) {
+        int li = pad_idx(threadIdx.x * SI + j);
+        int64_t v = tile[li];
+        tile[li] = carry;
+        carry += v;
+    }
+    __syncthreads();
+    for (int i = 0; i < SI; i++) {
+        int li = i * SB + threadIdx.x;
+        int64_t idx = base + li;
+        if (idx < n) out[idx] = (Tout)tile[pad_idx(li)];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = (Tout)(b0 + total);
 }
 
 template <class Tin, class Tout>

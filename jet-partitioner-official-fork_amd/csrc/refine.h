@@ -31,6 +31,9 @@ struct RefineParams {
 class Refiner {
   public:
     Refiner(Arena& a, hipStream_t s, vid_t finest_n, int64_t finest_conn_cap, const RefineParams& p);
+    ~Refiner();
+    Refiner(const Refiner&) = delete;
+    Refiner& operator=(const Refiner&) = delete;
 
     // jet_refiner.hpp:1050-1142 on one level; best_part (device, L.n) in/out.
     LevelRecord refine(const Level& L, part_t* best_part);
@@ -53,18 +56,17 @@ class Refiner {
     const int64_t* best_sizes_dev() const { return best_sizes_; }
 
   private:
-    struct Moves {
-        int32_t nsmall = 0, nbig = 0;
-        int32_t total() const { return nsmall + nbig; }
-    };
     void init_conn(const Level& L, const part_t* part, const int32_t* mark);
     void build_tiers(const Level& L);
-    Moves jet_lp(const Level& L, const part_t* part, double filter_ratio);
-    Moves rebalance_weak(const Level& L, const part_t* part, const int64_t* sizes);
-    Moves rebalance_strong(const Level& L, const part_t* part, const int64_t* sizes);
-    void perform_moves(const Level& L, part_t* part, Moves mv, int64_t* sizes, int64_t& cut_change,
-                       int64_t& max_size);
-    void sort_candidates(const Level& L, int32_t ncand, int key_bits);
+    void jet_lp(const Level& L, const part_t* part, double filter_ratio);
+    void rebalance(const Level& L, const part_t* part, const int64_t* sizes, bool strong);
+    void rank_flagged(const Level& L, const part_t* part, const int64_t* sizes, int64_t width, int key_bits,
+                      bool strong);
+    void perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_t& cut_change, int64_t& max_size);
+    // Two counter sets alternate between iterations: the first kernel of an
+    // iteration clears the other set, so no fill launch is needed.
+    int32_t* cnt() const { return counters_ + 8 * par_; }
+    int32_t* cnt_next() const { return counters_ + 8 * (1 - par_); }
 
     Arena& a_;
     hipStream_t s_;
@@ -80,15 +82,11 @@ class Refiner {
     // iteration scratch
     int32_t* pregain_;
     part_t* dest_;
-    int32_t* save_;
     int32_t* bid_;
-    int32_t* flags_;
     vid_t* cand_;
     vid_t* moves_;
     int32_t* swap_bit_;
-    uint32_t* skey_;
     uint32_t* sval_;
-    uint32_t* ktmp_;
     uint32_t* vtmp_;
     int64_t* prefix_;
     int64_t* score_;
@@ -96,17 +94,25 @@ class Refiner {
     vid_t* tier_mid_;
     vid_t* tier_big_;
     int32_t ntier_mid_ = 0, ntier_big_ = 0;
-    // small device state
-    int32_t* counters_;  // [0] ncand small, [1] ncand big, [2] moves small, [3] moves big, [4] redraws
-    int64_t* acc_;       // [0] cut change 1, [1] cut change 2, [2] max part, [3] scratch
+    bool has_big_ = false;
+    // small device state; counter set layout: [0] small candidates, [1] big
+    // candidates, [2] small moves, [3] big moves, [4] re-draws, [5] ranked
+    // bucket members, [6] update_large chosen
+    int32_t* counters_;
+    int par_ = 0;
+    int64_t* acc_;  // [0] cut change 1, [1] cut change 2, [2] max part, [3] strong capacity / re-draw counter, [4] ticket
     int64_t* curr_sizes_;
     int64_t* best_sizes_;
     int64_t* evict_adjust_;
     int64_t* evict_start_;
-    int32_t* part_begin_;
     part_t* undersized_;
-    int32_t* misc_;  // [0] undersized count
+    int32_t* misc_;  // [0] undersized count, [1] error bits
     vid_t* redraw_;
+    int64_t* buckets_;
+    int32_t* bflag_;
+    int64_t nbuckets_ = 0;
+    int64_t* res_host_ = nullptr;  // pinned: (cut change, largest part) of the last iteration
+    int64_t* res_dev_ = nullptr;
     // refine_data mirror on the host
     bool init_ = false;
     int64_t cut_ = 0, total_imb_ = 0, total_size_ = 0;

@@ -42,9 +42,14 @@ __device__ inline int32_t lookup(const int2* tab, int32_t size, part_t target) {
     return 0;
 }
 
-__device__ inline void push_list(vid_t* list, int32_t* counters, int idx_small, vid_t nmax, vid_t v, eoff_t deg) {
-    if (deg > BIG_DEG) list[nmax - 1 - atomicAdd(&counters[idx_small + 1], 1)] = v;
-    else list[atomicAdd(&counters[idx_small], 1)] = v;
+// Appends v (when pred) to the small-degree list growing from the front or
+// the large-degree list growing from the back; wave-aggregated atomics.
+__device__ inline void push_list(vid_t* list, int32_t* counters, int idx_small, vid_t nmax, bool pred, vid_t v,
+                                 eoff_t deg) {
+    int32_t a = wave_append(&counters[idx_small], pred && deg <= BIG_DEG);
+    int32_t b = wave_append(&counters[idx_small + 1], pred && deg > BIG_DEG);
+    if (a >= 0) list[a] = v;
+    if (b >= 0) list[nmax - 1 - b] = v;
 }
 
 __device__ inline vid_t list_at(const vid_t* list, int32_t nsmall, vid_t nmax, int64_t x) {
@@ -129,10 +134,13 @@ __global__ void k_conn_caps(DGraph g, int32_t k, int32_t* caps) {
 __global__ void k_conn_tiers(DGraph g, int32_t k, vid_t* mid, vid_t* big, int32_t* cnt) {
     FOR_THREAD(i, g.n) {
         int64_t d = g.rowptr[i + 1] - g.rowptr[i];
-        if (d <= SMALL_DEG) continue;
         int64_t cap = d > k ? k : d;
-        if (d <= HEAVY_DEG_REFINE && cap <= WAVE_TABLE_MAX) mid[atomicAdd(&cnt[0], 1)] = (vid_t)i;
-        else big[atomicAdd(&cnt[1], 1)] = (vid_t)i;
+        bool is_mid = d > SMALL_DEG && d <= HEAVY_DEG_REFINE && cap <= WAVE_TABLE_MAX;
+        bool is_big = d > SMALL_DEG && !is_mid;
+        int32_t a = wave_append(&cnt[0], is_mid);
+        int32_t b = wave_append(&cnt[1], is_big);
+        if (a >= 0) mid[a] = (vid_t)i;
+        if (b >= 0) big[b] = (vid_t)i;
     }
 }
 
@@ -147,7 +155,8 @@ constexpr int SB_BLOCK = 256;
 
 __global__ void __launch_bounds__(SB_BLOCK)
     k_conn_build_small(DGraph g, const part_t* part, int32_t k, const eoff_t* conn_off, int2* conn,
-                       int32_t* conn_size, int32_t* mark, part_t* dest_cache) {
+                       int32_t* conn_size, int32_t* mark, part_t* dest_cache, const int32_t* gate) {
+    if (gate && gate[0] == 0) return;
     __shared__ int2 tab[SMALL_DEG][SB_BLOCK];
     const int t = threadIdx.x;
     FOR_THREAD(i, g.n) {
@@ -236,7 +245,9 @@ constexpr int WB_WAVES = 4;
 
 __global__ void __launch_bounds__(WB_WAVES* WAVE)
     k_conn_build_wave(DGraph g, const part_t* part, int32_t k, const eoff_t* conn_off, int2* conn,
-                      int32_t* conn_size, int32_t* mark, part_t* dest_cache, const vid_t* list, int32_t nlist) {
+                      int32_t* conn_size, int32_t* mark, part_t* dest_cache, const vid_t* list, int32_t nlist,
+                      const int32_t* gate) {
+    if (gate && gate[0] == 0) return;
     __shared__ int2 T[WB_WAVES][WAVE_TABLE_MAX];
     __shared__ int2 T2[WB_WAVES][WAVE_TABLE_MAX];
     __shared__ int32_t used[WB_WAVES];
@@ -282,7 +293,9 @@ constexpr int BB_BLOCK = 512;
 
 __global__ void __launch_bounds__(BB_BLOCK)
     k_conn_build_block(DGraph g, const part_t* part, int32_t k, const eoff_t* conn_off, int2* conn,
-                       int32_t* conn_size, int32_t* mark, part_t* dest_cache, const vid_t* list, int32_t nlist) {
+                       int32_t* conn_size, int32_t* mark, part_t* dest_cache, const vid_t* list, int32_t nlist,
+                       const int32_t* gate) {
+    if (gate && gate[0] == 0) return;
     __shared__ int2 T[MAX_K];
     __shared__ int2 T2[MAX_K];
     __shared__ int32_t used;
@@ -322,45 +335,65 @@ __global__ void __launch_bounds__(BB_BLOCK)
 
 // ---------------------------------------------------------------- jet_lp
 // jet_refiner.hpp:211-268: destination selection, viable-move filter.
-__global__ void k_lp_select(DGraph g, const part_t* part, const eoff_t* conn_off, const int2* conn,
-                            const int32_t* conn_size, part_t* dest_cache, int32_t* gain_p, int32_t* lock, double fr,
-                            part_t* dest, int32_t* pregain, vid_t* cand, int32_t* counters, vid_t nmax) {
-    FOR_THREAD(i, g.n) {
-        part_t best = dest_cache[i];
-        if (best == NULL_PART) {
-            part_t p = part[i];
-            best = NO_MOVE;
-            int32_t b_conn = 0, p_conn = 0;
-            const int2* tab = conn + conn_off[i];
-            int32_t sz = conn_size[i];
-            for (int32_t s = 0; s < sz; s++) {
-                int2 e = tab[s];
-                if (e.y > b_conn && e.x != p) {
-                    best = e.x;
-                    b_conn = e.y;
-                } else if (e.y == b_conn && b_conn > 0 && e.x != p && e.x < best) {
-                    best = e.x;
-                } else if (e.y > 0 && e.x == p) {
-                    p_conn = e.y;
+// Each wave owns a contiguous chunk of LS_CHW vertices; candidates are staged
+// in LDS and published with one atomic per workgroup.
+constexpr int LS_WAVES = 4;
+constexpr int LS_CHW = 512;
+
+__global__ void __launch_bounds__(LS_WAVES* WAVE)
+    k_lp_select(DGraph g, const part_t* part, const eoff_t* conn_off, const int2* conn, const int32_t* conn_size,
+                part_t* dest_cache, int32_t* gain_p, int32_t* lock, double fr, part_t* dest, int32_t* pregain,
+                vid_t* cand, int32_t* counters, vid_t nmax, int32_t* next_counters) {
+    __shared__ vid_t sbuf[LS_WAVES * LS_CHW];
+    __shared__ int32_t sh[2 * LS_WAVES + 2];
+    StagedLists<LS_WAVES, LS_CHW> st(sbuf);
+    if (blockIdx.x == 0 && threadIdx.x < 8) next_counters[threadIdx.x] = 0;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int64_t wbase = ((int64_t)blockIdx.x * LS_WAVES + threadIdx.x / WAVE) * LS_CHW;
+    for (int32_t off = 0; off < LS_CHW; off += WAVE) {
+        int64_t i = wbase + off + lane;
+        bool is_cand = false;
+        eoff_t deg = 0;
+        if (i < g.n) {
+            part_t best = dest_cache[i];
+            if (best == NULL_PART) {
+                part_t p = part[i];
+                best = NO_MOVE;
+                int32_t b_conn = 0, p_conn = 0;
+                const int2* tab = conn + conn_off[i];
+                int32_t sz = conn_size[i];
+                for (int32_t s = 0; s < sz; s++) {
+                    int2 e = tab[s];
+                    if (e.y > b_conn && e.x != p) {
+                        best = e.x;
+                        b_conn = e.y;
+                    } else if (e.y == b_conn && b_conn > 0 && e.x != p && e.x < best) {
+                        best = e.x;
+                    } else if (e.y > 0 && e.x == p) {
+                        p_conn = e.y;
+                    }
                 }
+                int32_t gsv = 0;
+                if (best != NO_MOVE) {
+                    if (b_conn >= p_conn || ((double)(p_conn - b_conn) < floor(fr * (double)p_conn))) gsv = b_conn - p_conn;
+                    else best = NO_MOVE;
+                }
+                gain_p[i] = gsv;
+                dest_cache[i] = best;
             }
-            int32_t gsv = 0;
-            if (best != NO_MOVE) {
-                if (b_conn >= p_conn || ((double)(p_conn - b_conn) < floor(fr * (double)p_conn))) gsv = b_conn - p_conn;
-                else best = NO_MOVE;
+            dest[i] = best;
+            is_cand = best != NO_MOVE && lock[i] == 0;
+            if (is_cand) {
+                pregain[i] = gain_p[i];
+                deg = g.rowptr[i + 1] - g.rowptr[i];
+            } else {
+                pregain[i] = GAIN_MIN;
+                lock[i] = 0;
             }
-            gain_p[i] = gsv;
-            dest_cache[i] = best;
         }
-        dest[i] = best;
-        if (best != NO_MOVE && lock[i] == 0) {
-            pregain[i] = gain_p[i];
-            push_list(cand, counters, 0, nmax, (vid_t)i, g.rowptr[i + 1] - g.rowptr[i]);
-        } else {
-            pregain[i] = GAIN_MIN;
-            lock[i] = 0;
-        }
+        st.push(is_cand && deg <= BIG_DEG, is_cand && deg > BIG_DEG, (vid_t)i);
     }
+    st.flush(cand, &counters[0], &counters[1], nmax, sh);
 }
 
 // jet_refiner.hpp:275-307 (afterburner): gain correction assuming every
@@ -380,22 +413,45 @@ __device__ inline int64_t afterburn_edge(vid_t v, vid_t i, int32_t ig, int32_t w
     return ch;
 }
 
-__global__ void k_afterburner_small(DGraph g, const vid_t* cand, const int32_t* counters, const part_t* part,
-                                    const part_t* dest, const int32_t* pregain, int32_t* lock, vid_t* moves,
-                                    int32_t* mcounters, vid_t nmax) {
+constexpr int AB_WAVES = 4;
+constexpr int AB_CHW = 256;
+
+template <int G>
+__global__ void __launch_bounds__(AB_WAVES* WAVE)
+    k_afterburner_small(DGraph g, const vid_t* cand, const int32_t* counters, const part_t* part, const part_t* dest,
+                        const int32_t* pregain, int32_t* lock, vid_t* moves, int32_t* mcounters, vid_t nmax) {
+    __shared__ vid_t sbuf[AB_WAVES * AB_CHW];
+    __shared__ int32_t sh[2 * AB_WAVES + 2];
+    StagedLists<AB_WAVES, AB_CHW> st(sbuf);
     const int32_t nc = counters[0];
-    FOR_THREAD(x, nc) {
-        vid_t i = cand[x];
-        part_t p = part[i], best = dest[i];
-        int32_t ig = pregain[i];
-        int64_t change = 0;
-        eoff_t rs = g.rowptr[i], re = g.rowptr[i + 1];
-        for (eoff_t j = rs; j < re; j++) change += afterburn_edge(g.col[j], i, ig, g.ew[j], p, best, part, dest, pregain);
-        if ((int64_t)ig + change >= 0) {
-            lock[i] = 1;
-            push_list(moves, mcounters, 2, nmax, i, re - rs);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int sub = lane & (G - 1);
+    // candidates per wave: spread the actual count over every wave of the grid
+    const int64_t waves = (int64_t)gridDim.x * AB_WAVES;
+    int64_t chunk = (nc + waves - 1) / waves;
+    chunk = ((chunk + WAVE / G - 1) / (WAVE / G)) * (WAVE / G);
+    if (chunk > AB_CHW) chunk = AB_CHW;
+    if (chunk < WAVE / G) chunk = WAVE / G;
+    const int64_t wbase = ((int64_t)blockIdx.x * AB_WAVES + threadIdx.x / WAVE) * chunk;
+    for (int32_t off = 0; off < chunk; off += WAVE / G) {
+        int64_t x = wbase + off + lane / G;
+        bool keep = false;
+        vid_t i = 0;
+        if (x < nc) {
+            i = cand[x];
+            part_t p = part[i], best = dest[i];
+            int32_t ig = pregain[i];
+            int64_t change = 0;
+            for (eoff_t j = g.rowptr[i] + sub; j < g.rowptr[i + 1]; j += G)
+                change += afterburn_edge(g.col[j], i, ig, g.ew[j], p, best, part, dest, pregain);
+#pragma unroll
+            for (int d = G / 2; d >= 1; d >>= 1) change += __shfl_xor(change, d, G);
+            keep = (int64_t)ig + change >= 0;
+            if (keep && sub == 0) lock[i] = 1;
         }
+        st.push(keep && sub == 0, false, i);
     }
+    st.flush(moves, &mcounters[2], &mcounters[3], nmax, sh);
 }
 
 __global__ void k_afterburner_big(DGraph g, const vid_t* cand, const int32_t* counters, const part_t* part,
@@ -415,16 +471,18 @@ __global__ void k_afterburner_big(DGraph g, const vid_t* cand, const int32_t* co
         change = wave_reduce_sum(change);
         if (lane == 0 && (int64_t)ig + change >= 0) {
             lock[i] = 1;
-            push_list(moves, mcounters, 2, nmax, i, re - rs);
+            if (re - rs > BIG_DEG) moves[nmax - 1 - atomicAdd(&mcounters[3], 1)] = i;
+            else moves[atomicAdd(&mcounters[2], 1)] = i;
         }
     }
 }
 
 // ---------------------------------------------------------------- perform_moves
 // jet_refiner.hpp:905-955.
-__global__ void k_moves_apply(const vid_t* moves, const int32_t* mc, vid_t nmax, const wgt_t* vw, part_t* part,
+__global__ void k_moves_apply(DGraph g, const vid_t* moves, int32_t* mc, vid_t nmax, const wgt_t* vw, part_t* part,
                               part_t* dest, part_t* dest_cache, const eoff_t* conn_off, const int2* conn,
-                              const int32_t* conn_size, unsigned long long* sizes, int32_t k, int64_t* acc) {
+                              const int32_t* conn_size, unsigned long long* sizes, int32_t k, int64_t* acc,
+                              int32_t* mark) {
     __shared__ int64_t red[256 / WAVE];
     __shared__ unsigned long long ls[LDS_K];
     const bool lds = k <= LDS_K;
@@ -433,6 +491,10 @@ __global__ void k_moves_apply(const vid_t* moves, const int32_t* mc, vid_t nmax,
         __syncthreads();
     }
     const int32_t ns = mc[2], total = mc[2] + mc[3];
+    // perform_moves' choice (jet_refiner.hpp:933): update_large when more
+    // than n/10 vertices move; update_large rebuilds the neighbours' tables
+    const bool large = total > g.n / 10;
+    if (blockIdx.x == 0 && threadIdx.x == 0) mc[6] = large ? 1 : 0;
     int64_t cc = 0;
     FOR_THREAD(x, total) {
         vid_t v = list_at(moves, ns, nmax, x);
@@ -451,6 +513,8 @@ __global__ void k_moves_apply(const vid_t* moves, const int32_t* mc, vid_t nmax,
         }
         part[v] = b;
         dest[v] = p;
+        if (large)
+            for (eoff_t j = g.rowptr[v]; j < g.rowptr[v + 1]; j++) mark[g.col[j]] = 1;
     }
     cc = block_reduce_sum<int64_t, 256>(cc, red);
     if (threadIdx.x == 0 && cc) atomicAdd((unsigned long long*)&acc[0], (unsigned long long)cc);
@@ -556,16 +620,17 @@ __device__ inline void update_edge(vid_t v, int32_t w, part_t q, const eoff_t* c
 }
 
 // ADD == false: subtract from the old part (held in dest); ADD == true: add to
-// the new part (held in part).
-template <bool ADD>
+// the new part (held in part). G lanes stride the edges of one move.
+template <bool ADD, int G>
 __global__ void k_update_small(DGraph g, const vid_t* moves, const int32_t* mc, vid_t nmax, const part_t* part,
                                const part_t* dest, const eoff_t* conn_off, int2* conn, int32_t* conn_size,
                                part_t* dest_cache, int32_t k, int32_t* err) {
+    if (mc[6] != 0) return;  // update_large chosen
     const int32_t ns = mc[2];
-    FOR_THREAD(x, ns) {
+    FOR_GROUP(x, ns) {
         vid_t i = moves[x];
         part_t q = ADD ? part[i] : dest[i];
-        for (eoff_t j = g.rowptr[i]; j < g.rowptr[i + 1]; j++)
+        for (eoff_t j = g.rowptr[i] + sub_; j < g.rowptr[i + 1]; j += G)
             update_edge<ADD>(g.col[j], g.ew[j], q, conn_off, conn, conn_size, dest_cache, k, err);
     }
 }
@@ -574,6 +639,7 @@ template <bool ADD>
 __global__ void k_update_big(DGraph g, const vid_t* moves, const int32_t* mc, vid_t nmax, const part_t* part,
                              const part_t* dest, const eoff_t* conn_off, int2* conn, int32_t* conn_size,
                              part_t* dest_cache, int32_t k, int32_t* err) {
+    if (mc[6] != 0) return;  // update_large chosen
     const int32_t nb = mc[3];
     const int lane = threadIdx.x & (WAVE - 1);
     const int64_t wstride = ((int64_t)gridDim.x * blockDim.x) / WAVE;
@@ -585,19 +651,10 @@ __global__ void k_update_big(DGraph g, const vid_t* moves, const int32_t* mc, vi
     }
 }
 
-// update_large mark step (jet_refiner.hpp:748-755).
-__global__ void k_mark_adjacent(DGraph g, const vid_t* moves, const int32_t* mc, vid_t nmax, int32_t* mark) {
-    const int32_t total = mc[2] + mc[3];
-    FOR_THREAD(x, total) {
-        vid_t i = list_at(moves, mc[2], nmax, x);
-        for (eoff_t j = g.rowptr[i]; j < g.rowptr[i + 1]; j++) mark[g.col[j]] = 1;
-    }
-}
-
 // cut change part 2 and the largest part (jet_refiner.hpp:937-948).
 __global__ void k_moves_finish(const vid_t* moves, const int32_t* mc, vid_t nmax, const part_t* part,
                                const part_t* dest, const eoff_t* conn_off, const int2* conn, const int32_t* conn_size,
-                               const int64_t* sizes, int32_t k, int64_t* acc) {
+                               const int64_t* sizes, int32_t k, int64_t* acc, int64_t* host_out) {
     __shared__ int64_t red[256 / WAVE];
     const int32_t ns = mc[2], total = mc[2] + mc[3];
     int64_t cc = 0;
@@ -615,34 +672,127 @@ __global__ void k_moves_finish(const vid_t* moves, const int32_t* mc, vid_t nmax
         mx = block_reduce_max<int64_t, 256>(mx, red);
         if (threadIdx.x == 0) acc[2] = mx;
     }
+    // the last workgroup to finish publishes (cut change, largest part)
+    if (threadIdx.x == 0) {
+        __threadfence();
+        unsigned int ticket = atomicAdd(reinterpret_cast<unsigned int*>(&acc[4]), 1u);
+        if (ticket == gridDim.x - 1) {
+            __threadfence();
+            int64_t c1 = __hip_atomic_load(&acc[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t c2 = __hip_atomic_load(&acc[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t m2 = __hip_atomic_load(&acc[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            host_out[0] = c1 + c2;
+            host_out[1] = m2;
+            __threadfence_system();
+            acc[0] = 0;
+            acc[1] = 0;
+            acc[4] = 0;
+        }
+    }
 }
 
 // ---------------------------------------------------------------- rebalancing
-__global__ void k_undersized(const int64_t* sizes, int32_t k, int64_t max_dest, part_t* undersized, int32_t* misc) {
-    __shared__ int32_t ws[256 / WAVE + 1];
-    int32_t base = 0;
-    for (int32_t p0 = 0; p0 < k; p0 += 256) {
-        int32_t p = p0 + threadIdx.x;
-        int32_t f = (p < k && sizes[p] < max_dest) ? 1 : 0;
-        int32_t tot;
-        int32_t ex = block_excl_scan<int32_t, 256>(f, tot, ws);
-        if (f) undersized[base + ex] = p;
-        base += tot;
+// Rebalancer minibuckets (jet_refiner.hpp:359-412, :449-481, :632-643).
+// Bucket sums are order-free, so they are accumulated with atomics and
+// scanned. A vertex is evicted when pos + off[b] - off[begin_p] < limit_p,
+// where pos is the weight of same-bucket candidates with smaller id. Whole
+// buckets below a part's limit are decided without pos; only the members of
+// buckets that straddle a limit (or, for the strong rebalancer, every bucket
+// below it, whose scores feed the destination chunks) are ranked by id.
+
+// Zeroes the bucket arrays and the next iteration's counters; workgroup 0
+// also builds the undersized-part list (weak, jet_refiner.hpp:578-591) or
+// the largest receiver capacity (strong, :440-448) and clears the eviction
+// adjustments.
+__global__ void __launch_bounds__(256)
+    k_reset_buckets(int64_t* buckets, int32_t* bflag, int64_t nb, int32_t* next_counters, const int64_t* sizes,
+                    int32_t k, int64_t max_dest, int strong, part_t* undersized, int32_t* misc, int64_t* acc,
+                    int64_t* evict_adjust) {
+    FOR_THREAD(i, nb) {
+        buckets[i] = 0;
+        bflag[i] = 0;
     }
-    if (threadIdx.x == 0) misc[0] = base;
+    if (blockIdx.x != 0) return;
+    if (threadIdx.x < 8) next_counters[threadIdx.x] = 0;
+    __shared__ int32_t ws[256 / WAVE + 1];
+    __shared__ int64_t red[256 / WAVE];
+    if (strong) {
+        int64_t mx = INT64_MIN;
+        for (int t = threadIdx.x; t < k; t += blockDim.x) {
+            int64_t sz = sizes[t];
+            if (sz < max_dest && max_dest - sz > mx) mx = max_dest - sz;
+            evict_adjust[t] = 0;
+        }
+        mx = block_reduce_max<int64_t, 256>(mx, red);
+        if (threadIdx.x == 0) acc[3] = mx;
+    } else {
+        int32_t base = 0;
+        for (int32_t p0 = 0; p0 < k; p0 += 256) {
+            int32_t p = p0 + threadIdx.x;
+            int32_t f = (p < k && sizes[p] < max_dest) ? 1 : 0;
+            int32_t tot;
+            int32_t ex = block_excl_scan<int32_t, 256>(f, tot, ws);
+            if (f) undersized[base + ex] = p;
+            base += tot;
+        }
+        if (threadIdx.x == 0) misc[0] = base;
+    }
 }
+
+// Per-workgroup LDS map of bucket sums (consecutive vertices share few
+// buckets); overflow goes straight to global atomics.
+constexpr int BH_SLOTS = 1024;
+
+struct BucketMap {
+    int32_t* key;
+    unsigned long long* val;
+    __device__ void init() {
+        for (int t = threadIdx.x; t < BH_SLOTS; t += blockDim.x) {
+            key[t] = -1;
+            val[t] = 0;
+        }
+        __syncthreads();
+    }
+    __device__ void add(int32_t b, unsigned long long w, unsigned long long* global) {
+        uint32_t h = ((uint32_t)b * 0x9E3779B1u) >> 22;
+        for (int probe = 0; probe < 16; probe++) {
+            int32_t k = key[h];
+            if (k == -1) {
+                int32_t old = atomicCAS(&key[h], -1, b);
+                if (old == -1 || old == b) {
+                    atomicAdd(&val[h], w);
+                    return;
+                }
+            } else if (k == b) {
+                atomicAdd(&val[h], w);
+                return;
+            }
+            h = (h + 1) & (BH_SLOTS - 1);
+        }
+        atomicAdd(&global[b], w);
+    }
+    __device__ void flush(unsigned long long* global) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < BH_SLOTS; t += blockDim.x)
+            if (key[t] != -1) atomicAdd(&global[key[t]], val[t]);
+    }
+};
 
 // jet_refiner.hpp:592-643: destination and bucket of each vertex of an
 // oversized part.
 __global__ void k_weak_select(DGraph g, const part_t* part, const int64_t* sizes, int64_t size_max, int32_t opt,
                               int64_t max_dest, const part_t* undersized, const int32_t* misc,
                               const eoff_t* conn_off, const int2* conn, const int32_t* conn_size, int32_t sections,
-                              part_t* dest, int32_t* bid, int32_t* flags) {
+                              part_t* dest, int32_t* bid, unsigned long long* buckets) {
+    __shared__ int32_t bk[BH_SLOTS];
+    __shared__ unsigned long long bv[BH_SLOTS];
+    BucketMap bm{bk, bv};
+    bm.init();
     const int32_t tu = misc[0];
     FOR_THREAD(i, g.n) {
         part_t p = part[i];
         part_t d = p;
-        int32_t f = 0;
+        int32_t b = -1;
         int64_t sp = sizes[p];
         if (sp > size_max && (double)g.vw[i] < 1.5 * (double)(sp - opt)) {
             part_t best = p;
@@ -671,34 +821,29 @@ __global__ void k_weak_select(DGraph g, const part_t* part, const int64_t* sizes
             }
             if (d != p) {
                 int32_t gt = gain_bucket(sv, g.vw[i]);
-                bid[i] = (MAX_BUCKETS * p + gt) * sections + (int32_t)(i % sections);
-                f = 1;
+                b = (MAX_BUCKETS * p + gt) * sections + (int32_t)(i % sections);
+                bm.add(b, (unsigned long long)g.vw[i], buckets);
             }
         }
         dest[i] = d;
-        flags[i] = f;
+        bid[i] = b;
     }
-}
-
-__global__ void k_strong_prep(const int64_t* sizes, int32_t k, int64_t max_dest, int64_t* acc) {
-    __shared__ int64_t red[256 / WAVE];
-    int64_t mx = INT64_MIN;
-    for (int t = threadIdx.x; t < k; t += blockDim.x) {
-        int64_t sz = sizes[t];
-        if (sz < max_dest && max_dest - sz > mx) mx = max_dest - sz;
-    }
-    mx = block_reduce_max<int64_t, 256>(mx, red);
-    if (threadIdx.x == 0) acc[3] = mx;
+    bm.flush(buckets);
 }
 
 // jet_refiner.hpp:449-481.
 __global__ void k_strong_select(DGraph g, const part_t* part, const int64_t* sizes, int64_t size_max, int32_t opt,
                                 int64_t max_dest, const int64_t* acc, const eoff_t* conn_off, const int2* conn,
-                                const int32_t* conn_size, int32_t sections, int32_t* bid, int32_t* flags) {
+                                const int32_t* conn_size, int32_t sections, int32_t* bid,
+                                unsigned long long* buckets) {
+    __shared__ int32_t bk[BH_SLOTS];
+    __shared__ unsigned long long bv[BH_SLOTS];
+    BucketMap bm{bk, bv};
+    bm.init();
     const int64_t max_vwgt = acc[3];
     FOR_THREAD(i, g.n) {
         part_t p = part[i];
-        int32_t f = 0;
+        int32_t b = -1;
         int64_t sp = sizes[p];
         int64_t w = g.vw[i];
         bool vw_ok = max_vwgt > 0 && w <= 2 * max_vwgt;
@@ -721,81 +866,237 @@ __global__ void k_strong_select(DGraph g, const part_t* part, const int64_t* siz
             int64_t vwgt = w < sp - size_max ? w : sp - size_max;
             int32_t gt = gain_bucket(gain, vwgt);
             if (gt < MAX_BUCKETS) {
-                bid[i] = (MAX_BUCKETS * p + gt) * sections + (int32_t)(i % sections) + 1;
-                f = 1;
+                b = (MAX_BUCKETS * p + gt) * sections + (int32_t)(i % sections) + 1;
+                bm.add(b, (unsigned long long)w, buckets);
             }
         }
-        flags[i] = f;
+        bid[i] = b;
     }
+    bm.flush(buckets);
 }
 
-__global__ void k_gather_cands(vid_t n, const int32_t* pos, const int32_t* bid, uint32_t* skey, uint32_t* sval) {
-    FOR_THREAD(i, n) {
-        int32_t p0 = pos[i];
-        if (pos[i + 1] != p0) {
-            skey[p0] = (uint32_t)bid[i];
-            sval[p0] = (uint32_t)i;
-        }
-    }
-}
-
-__global__ void k_sorted_weights(int32_t nc, const uint32_t* sval, const wgt_t* vw, int64_t* w) {
-    FOR_THREAD(t, nc) w[t] = vw[sval[t]];
-}
-
-__device__ inline int32_t lower_bound_u32(const uint32_t* a, int32_t n, uint32_t key) {
-    int32_t lo = 0, hi = n;
-    while (lo < hi) {
-        int32_t mid = (lo + hi) >> 1;
-        if (a[mid] < key) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-__global__ void k_part_begin(const uint32_t* skey, int32_t nc, int32_t k, int64_t width, int32_t* pb) {
-    FOR_THREAD(p, (int64_t)k + 1) pb[p] = lower_bound_u32(skey, nc, (uint32_t)((int64_t)p * width));
-}
-
-// get_evictions (jet_refiner.hpp:359-412): score = weight of this part's
-// candidates ranked before the vertex (bucket order, then vertex id).
-template <bool ADJUST>
-__global__ void k_evict(DGraph g, const uint32_t* sval, int32_t nc, const int64_t* prefix, const int32_t* pb,
-                        const part_t* part, const int64_t* sizes, int64_t size_max, int64_t* evict_adjust,
-                        int64_t* score, vid_t* moves, int32_t* mcounters, vid_t nmax) {
-    FOR_THREAD(t, nc) {
-        vid_t v = (vid_t)sval[t];
-        part_t p = part[v];
-        int64_t sc = prefix[t] - prefix[pb[p]];
-        int64_t limit = sizes[p] - size_max;
-        if (sc < limit) {
-            if (ADJUST) {
-                if (sc + g.vw[v] >= limit) evict_adjust[p] = sc + g.vw[v];
-                score[v] = prefix[t];
+// off = exclusive scan of the bucket sums. WEAK: evicts whole buckets below the
+// limit and flags straddling ones; STRONG: flags every bucket below the limit.
+template <bool STRONG>
+__global__ void __launch_bounds__(LS_WAVES* WAVE)
+    k_classify(DGraph g, const part_t* part, const int32_t* bid, const int64_t* off, const int64_t* sizes,
+               int64_t size_max, int64_t width, int32_t* bflag, vid_t* moves, int32_t* mcounters, vid_t nmax) {
+    __shared__ vid_t sbuf[LS_WAVES * LS_CHW];
+    __shared__ int32_t sh[2 * LS_WAVES + 2];
+    StagedLists<LS_WAVES, LS_CHW> st(sbuf);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int64_t wbase = ((int64_t)blockIdx.x * LS_WAVES + threadIdx.x / WAVE) * LS_CHW;
+    for (int32_t o = 0; o < LS_CHW; o += WAVE) {
+        int64_t i = wbase + o + lane;
+        bool ev = false;
+        eoff_t deg = 0;
+        if (i < g.n) {
+            int32_t b = bid[i];
+            if (b >= 0) {
+                part_t p = part[i];
+                int64_t base = off[b] - off[(int64_t)p * width];
+                int64_t limit = sizes[p] - size_max;
+                int64_t bs = off[b + 1] - off[b];
+                if (STRONG) {
+                    if (base < limit) bflag[b] = 1;
+                } else if (base + bs <= limit) {
+                    ev = true;
+                    deg = g.rowptr[i + 1] - g.rowptr[i];
+                } else if (base < limit) {
+                    bflag[b] = 1;
+                }
             }
-            push_list(moves, mcounters, 2, nmax, v, g.rowptr[v + 1] - g.rowptr[v]);
         }
+        st.push(ev && deg <= BIG_DEG, ev && deg > BIG_DEG, (vid_t)i);
+    }
+    st.flush(moves, &mcounters[2], &mcounters[3], nmax, sh);
+}
+
+// Members of flagged buckets -> rlist (counter in counters[5]).
+__global__ void __launch_bounds__(LS_WAVES* WAVE)
+    k_collect(vid_t n, const int32_t* bid, const int32_t* bflag, vid_t* rlist, int32_t* counters) {
+    __shared__ vid_t sbuf[LS_WAVES * LS_CHW];
+    __shared__ int32_t sh[2 * LS_WAVES + 2];
+    StagedLists<LS_WAVES, LS_CHW> st(sbuf);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int64_t wbase = ((int64_t)blockIdx.x * LS_WAVES + threadIdx.x / WAVE) * LS_CHW;
+    for (int32_t o = 0; o < LS_CHW; o += WAVE) {
+        int64_t i = wbase + o + lane;
+        bool take = false;
+        if (i < n) {
+            int32_t b = bid[i];
+            take = b >= 0 && bflag[b] != 0;
+        }
+        st.push(take, false, (vid_t)i);
+    }
+    st.flush(rlist, &counters[5], &counters[7], n, sh);
+}
+
+// Decision for one ranked vertex: pos = weight of same-bucket members with a
+// smaller id.
+template <bool STRONG>
+__device__ inline void rank_decide(vid_t v, int32_t b, int64_t pos, const part_t* part, const int64_t* off,
+                                   const int64_t* sizes, int64_t size_max, int64_t width, const wgt_t* vw,
+                                   const eoff_t* rowptr, int64_t* evict_adjust, int64_t* score, vid_t* moves,
+                                   int32_t* mcounters, vid_t nmax) {
+    part_t p = part[v];
+    int64_t base = off[b] - off[(int64_t)p * width];
+    int64_t limit = sizes[p] - size_max;
+    int64_t bs = off[b + 1] - off[b];
+    bool ev = false;
+    if (STRONG) {
+        if (base < limit && base + pos < limit) {
+            ev = true;
+            if (base + pos + vw[v] >= limit) evict_adjust[p] = base + pos + vw[v];
+            score[v] = off[b] + pos;
+        }
+    } else if (base < limit && base + bs > limit) {
+        ev = base + pos < limit;
+    }
+    if (ev) {
+        if (rowptr[v + 1] - rowptr[v] > BIG_DEG) moves[nmax - 1 - atomicAdd(&mcounters[3], 1)] = v;
+        else moves[atomicAdd(&mcounters[2], 1)] = v;
+    }
+}
+
+constexpr int RS_CAP = 8192;
+constexpr int RS_BLOCK = 1024;
+
+// Ranks up to RS_CAP members in one workgroup: bitonic sort of
+// (bucket << 32 | id) in LDS, prefix of weights, decisions.
+template <bool STRONG>
+__global__ void __launch_bounds__(RS_BLOCK)
+    k_rank_small(const vid_t* rlist, const int32_t* counters, const int32_t* bid, const part_t* part,
+                 const int64_t* off, const int64_t* sizes, int64_t size_max, int64_t width, const wgt_t* vw,
+                 const eoff_t* rowptr, int64_t* evict_adjust, int64_t* score, vid_t* moves, int32_t* mcounters,
+                 vid_t nmax, int32_t* err) {
+    __shared__ uint64_t keys[RS_CAP];
+    __shared__ int64_t pre[RS_CAP];
+    __shared__ int64_t ws[RS_BLOCK / WAVE + 1];
+    const int32_t R = counters[5];
+    if (R > RS_CAP) {
+        if (threadIdx.x == 0) atomicOr(err, 8);
+        return;
+    }
+    uint32_t cap = 1;
+    while ((int32_t)cap < R) cap <<= 1;
+    for (uint32_t t = threadIdx.x; t < cap; t += RS_BLOCK) {
+        if ((int32_t)t < R) {
+            vid_t v = rlist[t];
+            keys[t] = ((uint64_t)(uint32_t)bid[v] << 32) | (uint32_t)v;
+        } else {
+            keys[t] = ~0ull;
+        }
+    }
+    for (uint32_t size = 2; size <= cap; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            __syncthreads();
+            for (uint32_t t = threadIdx.x; t < cap / 2; t += RS_BLOCK) {
+                uint32_t lo = 2 * t - (t & (stride - 1));
+                uint32_t hi = lo + stride;
+                bool up = (lo & size) == 0;
+                uint64_t x = keys[lo], y = keys[hi];
+                if ((x > y) == up) {
+                    keys[lo] = y;
+                    keys[hi] = x;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // exclusive prefix of weights in sorted order (RS_CAP / RS_BLOCK items per thread)
+    constexpr int PER = RS_CAP / RS_BLOCK;
+    int64_t loc[PER];
+    int64_t sum = 0;
+    for (int j = 0; j < PER; j++) {
+        int32_t t = threadIdx.x * PER + j;
+        int64_t w = t < R ? (int64_t)vw[(vid_t)(keys[t] & 0xffffffffull)] : 0;
+        loc[j] = sum;
+        sum += w;
+    }
+    int64_t tot;
+    int64_t ex = block_excl_scan<int64_t, RS_BLOCK>(sum, tot, ws);
+    for (int j = 0; j < PER; j++) pre[threadIdx.x * PER + j] = ex + loc[j];
+    __syncthreads();
+    for (int32_t t = threadIdx.x; t < R; t += RS_BLOCK) {
+        uint64_t key = keys[t];
+        int32_t b = (int32_t)(key >> 32);
+        uint64_t lo_key = (uint64_t)(uint32_t)b << 32;
+        int32_t lo = 0, hi = t;
+        while (lo < hi) {
+            int32_t mid = (lo + hi) >> 1;
+            if (keys[mid] < lo_key) lo = mid + 1;
+            else hi = mid;
+        }
+        rank_decide<STRONG>((vid_t)(key & 0xffffffffull), b, pre[t] - pre[lo], part, off, sizes, size_max, width, vw,
+                            rowptr, evict_adjust, score, moves, mcounters, nmax);
+    }
+}
+
+__global__ void k_rank_keys(const vid_t* rlist, int32_t R, const int32_t* bid, uint64_t* keys, uint32_t* vals) {
+    FOR_THREAD(t, R) {
+        vid_t v = rlist[t];
+        keys[t] = ((uint64_t)(uint32_t)bid[v] << 32) | (uint32_t)v;
+        vals[t] = (uint32_t)v;
+    }
+}
+
+__global__ void k_rank_weights(int32_t R, const uint32_t* vals, const wgt_t* vw, int64_t* w) {
+    FOR_THREAD(t, R) w[t] = vw[vals[t]];
+}
+
+template <bool STRONG>
+__global__ void k_rank_apply(int32_t R, const uint64_t* keys, const int64_t* pre, const part_t* part,
+                             const int64_t* off, const int64_t* sizes, int64_t size_max, int64_t width,
+                             const wgt_t* vw, const eoff_t* rowptr, int64_t* evict_adjust, int64_t* score,
+                             vid_t* moves, int32_t* mcounters, vid_t nmax) {
+    FOR_THREAD(t, R) {
+        uint64_t key = keys[t];
+        int32_t b = (int32_t)(key >> 32);
+        uint64_t lo_key = (uint64_t)(uint32_t)b << 32;
+        int64_t lo = 0, hi = t;
+        while (lo < hi) {
+            int64_t mid = (lo + hi) >> 1;
+            if (keys[mid] < lo_key) lo = mid + 1;
+            else hi = mid;
+        }
+        rank_decide<STRONG>((vid_t)(key & 0xffffffffull), b, pre[t] - pre[lo], part, off, sizes, size_max, width, vw,
+                            rowptr, evict_adjust, score, moves, mcounters, nmax);
     }
 }
 
 // Cookie cutter (jet_refiner.hpp:491-518): per-part eviction totals become a
-// global eviction order; receivers get consecutive chunks of capacity.
-__global__ void k_cookie(int32_t k, const int64_t* prefix, const int32_t* pb, const int64_t* sizes, int64_t max_dest,
-                         int64_t* evict_adjust, int64_t* evict_start, int64_t* acc) {
-    if (threadIdx.x != 0) return;
-    int64_t run = 0;
-    for (int32_t p = 0; p < k; p++) {
-        int64_t add = evict_adjust[p];
-        if (add == 0) add = prefix[pb[p + 1]] - prefix[pb[p]];
-        evict_adjust[p] = prefix[pb[p]] - run;
-        run += add;
+// global eviction order; receivers get consecutive chunks of capacity. Two
+// workgroup scans over the parts.
+constexpr int CK_BLOCK = 1024;
+__global__ void __launch_bounds__(CK_BLOCK)
+    k_cookie(int32_t k, int64_t width, const int64_t* off, const int64_t* sizes, int64_t max_dest,
+             int64_t* evict_adjust, int64_t* evict_start, int64_t* acc) {
+    __shared__ int64_t ws[CK_BLOCK / WAVE + 1];
+    int64_t run = 0, run2 = 0;
+    for (int32_t p0 = 0; p0 < k; p0 += CK_BLOCK) {
+        int32_t p = p0 + threadIdx.x;
+        int64_t add = 0, cap = 0, beg = 0;
+        if (p < k) {
+            int64_t begin = (int64_t)p * width;
+            beg = off[begin];
+            add = evict_adjust[p];
+            if (add == 0) add = off[begin + width] - beg;
+            if (max_dest > sizes[p]) cap = max_dest - sizes[p];
+        }
+        int64_t t1, t2;
+        int64_t e1 = block_excl_scan<int64_t, CK_BLOCK>(add, t1, ws);
+        int64_t e2 = block_excl_scan<int64_t, CK_BLOCK>(cap, t2, ws);
+        if (p < k) {
+            evict_adjust[p] = beg - (run + e1);
+            evict_start[p + 1] = run2 + e2 + cap;
+        }
+        run += t1;
+        run2 += t2;
     }
-    acc[3] = run;  // re-draw counter starts after every eviction
-    int64_t run2 = 0;
-    evict_start[0] = 0;
-    for (int32_t p = 0; p < k; p++) {
-        if (max_dest > sizes[p]) run2 += max_dest - sizes[p];
-        evict_start[p + 1] = run2;
+    if (threadIdx.x == 0) {
+        acc[3] = run;  // re-draw counter starts after every eviction
+        evict_start[0] = 0;
     }
 }
 
@@ -908,6 +1209,21 @@ static inline int32_t sections_for(vid_t n, int32_t k) {
     return sections;
 }
 
+static int group_for_avg(const DGraph& g) {
+    double avg = g.n > 0 ? (double)g.m / g.n : 0.0;
+    return avg <= 4 ? 4 : avg <= 8 ? 8 : avg <= 16 ? 16 : avg <= 32 ? 32 : 64;
+}
+
+__global__ void k_max_degree(DGraph g, int32_t* out) {
+    int32_t mx = 0;
+    FOR_THREAD(i, g.n) {
+        int64_t d = g.rowptr[i + 1] - g.rowptr[i];
+        if (d > mx) mx = (int32_t)(d > ORD_MAX ? ORD_MAX : d);
+    }
+    mx = wave_reduce_max(mx);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && mx) atomicMax(out, mx);
+}
+
 Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefineParams& p) : a_(a), s_(s), p_(p), nmax_(n) {
     if (p.k < 1 || p.k > MAX_K) throw std::invalid_argument("k out of range");
     int64_t N = std::max<int64_t>(n, 1);
@@ -919,50 +1235,58 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
     gain_p_ = a.alloc<int32_t>(N);
     pregain_ = a.alloc<int32_t>(N);
     dest_ = a.alloc<part_t>(N);
-    save_ = a.alloc<int32_t>(N);
     bid_ = a.alloc<int32_t>(N);
-    flags_ = a.alloc<int32_t>(N + 1);
     cand_ = a.alloc<vid_t>(N);
     moves_ = a.alloc<vid_t>(N);
     swap_bit_ = a.alloc<int32_t>(N);
-    skey_ = a.alloc<uint32_t>(N);
     sval_ = a.alloc<uint32_t>(N);
-    ktmp_ = a.alloc<uint32_t>(N);
     vtmp_ = a.alloc<uint32_t>(N);
     prefix_ = a.alloc<int64_t>(N + 1);
     score_ = a.alloc<int64_t>(N);
     cur_part_ = a.alloc<part_t>(N);
     tier_mid_ = a.alloc<vid_t>(N);
     tier_big_ = a.alloc<vid_t>(N);
-    counters_ = a.alloc<int32_t>(16);
+    counters_ = a.alloc<int32_t>(32);
     acc_ = a.alloc<int64_t>(8);
     curr_sizes_ = a.alloc<int64_t>(p.k);
     best_sizes_ = a.alloc<int64_t>(p.k);
     evict_adjust_ = a.alloc<int64_t>(p.k);
     evict_start_ = a.alloc<int64_t>(p.k + 1);
-    part_begin_ = a.alloc<int32_t>(p.k + 1);
     undersized_ = a.alloc<part_t>(p.k);
     misc_ = a.alloc<int32_t>(8);
     redraw_ = a.alloc<vid_t>(RD_MAX);
+    nbuckets_ = (int64_t)MAX_BUCKETS * p.k * sections_for(n, p.k) + MAX_SECTIONS + 4;
+    buckets_ = a.alloc<int64_t>(nbuckets_);
+    bflag_ = a.alloc<int32_t>(nbuckets_);
     fill_i32(swap_bit_, 0, N, s);
-    fill_i32(counters_, 0, 16, s);
+    fill_i32(counters_, 0, 32, s);
+    fill_i64(acc_, 0, 8, s);
+    JET_HIP(hipHostMalloc(reinterpret_cast<void**>(&res_host_), 4 * sizeof(int64_t), hipHostMallocMapped | hipHostMallocCoherent));
+    JET_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&res_dev_), res_host_, 0));
+}
+
+Refiner::~Refiner() {
+    if (res_host_) (void)hipHostFree(res_host_);
 }
 
 void Refiner::build_tiers(const Level& L) {
     DGraph g = L.view();
-    fill_i32(counters_ + 8, 0, 2, s_);
-    k_conn_tiers<<<grid_for(g.n, 256, 8192), 256, 0, s_>>>(g, p_.k, tier_mid_, tier_big_, counters_ + 8);
+    fill_i32(counters_ + 16, 0, 3, s_);
+    k_conn_tiers<<<grid_for(g.n, 256, 8192), 256, 0, s_>>>(g, p_.k, tier_mid_, tier_big_, counters_ + 16);
+    k_max_degree<<<grid_for(g.n, 256, 2048), 256, 0, s_>>>(g, counters_ + 18);
     JET_LAUNCH_CHECK();
-    int32_t c[2];
-    JET_HIP(hipMemcpyAsync(c, counters_ + 8, sizeof(c), hipMemcpyDeviceToHost, s_));
+    int32_t c[3];
+    JET_HIP(hipMemcpyAsync(c, counters_ + 16, sizeof(c), hipMemcpyDeviceToHost, s_));
     JET_HIP(hipStreamSynchronize(s_));
     ntier_mid_ = c[0];
     ntier_big_ = c[1];
+    has_big_ = c[2] > BIG_DEG;
 }
 
 void Refiner::init_conn(const Level& L, const part_t* part, const int32_t* mark) {
     DGraph g = L.view();
     int32_t* mk = const_cast<int32_t*>(mark);
+    const int32_t* gate = mark ? cnt() + 6 : nullptr;
     if (!mark) {
         ArenaScope sc(a_);
         int32_t* caps = a_.alloc<int32_t>((int64_t)g.n + 1);
@@ -973,156 +1297,176 @@ void Refiner::init_conn(const Level& L, const part_t* part, const int32_t* mark)
         fill_i32(lock_, 0, g.n, s_);
     }
     k_conn_build_small<<<grid_for(g.n, SB_BLOCK, 8192), SB_BLOCK, 0, s_>>>(g, part, p_.k, conn_off_, conn_, conn_size_,
-                                                                           mk, dest_cache_);
+                                                                           mk, dest_cache_, gate);
     JET_LAUNCH_CHECK();
     if (ntier_mid_ > 0) {
         k_conn_build_wave<<<grid_for(ntier_mid_, WB_WAVES, 8192), WB_WAVES * WAVE, 0, s_>>>(
-            g, part, p_.k, conn_off_, conn_, conn_size_, mk, dest_cache_, tier_mid_, ntier_mid_);
+            g, part, p_.k, conn_off_, conn_, conn_size_, mk, dest_cache_, tier_mid_, ntier_mid_, gate);
         JET_LAUNCH_CHECK();
     }
     if (ntier_big_ > 0) {
         k_conn_build_block<<<std::min(ntier_big_, 2048), BB_BLOCK, 0, s_>>>(g, part, p_.k, conn_off_, conn_, conn_size_,
-                                                                            mk, dest_cache_, tier_big_, ntier_big_);
+                                                                            mk, dest_cache_, tier_big_, ntier_big_, gate);
         JET_LAUNCH_CHECK();
     }
 }
 
-Refiner::Moves Refiner::jet_lp(const Level& L, const part_t* part, double fr) {
+void Refiner::jet_lp(const Level& L, const part_t* part, double fr) {
     DGraph g = L.view();
-    fill_i32(counters_, 0, 4, s_);
-    int nb = grid_for(g.n, 256, 8192);
-    k_lp_select<<<nb, 256, 0, s_>>>(g, part, conn_off_, conn_, conn_size_, dest_cache_, gain_p_, lock_, fr, dest_,
-                                    pregain_, cand_, counters_, nmax_);
+    int32_t* c = cnt();
+    int nb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_WAVES * LS_CHW));
+    k_lp_select<<<nb, LS_WAVES * WAVE, 0, s_>>>(g, part, conn_off_, conn_, conn_size_, dest_cache_, gain_p_, lock_, fr,
+                                                dest_, pregain_, cand_, c, nmax_, cnt_next());
     JET_LAUNCH_CHECK();
-    k_afterburner_small<<<nb, 256, 0, s_>>>(g, cand_, counters_, part, dest_, pregain_, lock_, moves_, counters_, nmax_);
-    k_afterburner_big<<<grid_for((int64_t)g.n * WAVE, 256, 4096), 256, 0, s_>>>(g, cand_, counters_, part, dest_,
-                                                                               pregain_, lock_, moves_, counters_, nmax_);
-    JET_LAUNCH_CHECK();
-    int32_t c[4];
-    JET_HIP(hipMemcpyAsync(c, counters_, sizeof(c), hipMemcpyDeviceToHost, s_));
-    JET_HIP(hipStreamSynchronize(s_));
-    return Moves{c[2], c[3]};
-}
-
-// Candidates flagged in flags_ with keys in bid_: ordered compaction, stable
-// sort by key, prefix of weights in sorted order, part boundaries.
-void Refiner::sort_candidates(const Level& L, int32_t ncand, int key_bits) {
-    DGraph g = L.view();
-    k_gather_cands<<<grid_for(g.n, 256, 8192), 256, 0, s_>>>(g.n, flags_, bid_, skey_, sval_);
-    JET_LAUNCH_CHECK();
-    radix_sort_u32(skey_, sval_, ncand, key_bits, ktmp_, vtmp_, a_, s_);
-    if (ncand > 0) {
-        k_sorted_weights<<<grid_for(ncand, 256, 8192), 256, 0, s_>>>(ncand, sval_, g.vw, prefix_);
-        JET_LAUNCH_CHECK();
+    unsigned ab = (unsigned)std::max<int64_t>(256, ceil_div((int64_t)g.n, (int64_t)AB_WAVES * AB_CHW));
+    switch (group_for_avg(g)) {
+        case 4: k_afterburner_small<4><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_); break;
+        case 8: k_afterburner_small<8><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_); break;
+        case 16: k_afterburner_small<16><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_); break;
+        case 32: k_afterburner_small<32><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_); break;
+        default: k_afterburner_small<64><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_); break;
     }
-    excl_scan_i64_i64(prefix_, prefix_, ncand, a_, s_);
+    if (has_big_)
+        k_afterburner_big<<<grid_for((int64_t)g.n * WAVE, 256, 2048), 256, 0, s_>>>(g, cand_, c, part, dest_, pregain_,
+                                                                                   lock_, moves_, c, nmax_);
+    JET_LAUNCH_CHECK();
 }
 
-Refiner::Moves Refiner::rebalance_weak(const Level& L, const part_t* part, const int64_t* sizes) {
+// Ranks the members of flagged buckets (counter 5 of them) and applies the
+// eviction decisions.
+void Refiner::rank_flagged(const Level& L, const part_t* part, const int64_t* sizes, int64_t width, int key_bits,
+                           bool strong) {
+    DGraph g = L.view();
+    int32_t* c = cnt();
+    int64_t smax = size_max_;
+    int32_t R = read_scalar(c + 5, s_);
+    if (R == 0) return;
+    if (R <= RS_CAP) {
+        if (strong)
+            k_rank_small<true><<<1, RS_BLOCK, 0, s_>>>(cand_, c, bid_, part, buckets_, sizes, smax, width, g.vw,
+                                                      g.rowptr, evict_adjust_, score_, moves_, c, nmax_, misc_ + 1);
+        else
+            k_rank_small<false><<<1, RS_BLOCK, 0, s_>>>(cand_, c, bid_, part, buckets_, sizes, smax, width, g.vw,
+                                                       g.rowptr, evict_adjust_, score_, moves_, c, nmax_, misc_ + 1);
+        JET_LAUNCH_CHECK();
+        return;
+    }
+    ArenaScope sc(a_);
+    uint64_t* keys = a_.alloc<uint64_t>(R);
+    uint64_t* ktmp = a_.alloc<uint64_t>(R);
+    int rb = grid_for(R, 256, 8192);
+    k_rank_keys<<<rb, 256, 0, s_>>>(cand_, R, bid_, keys, sval_);
+    JET_LAUNCH_CHECK();
+    radix_sort_u64(keys, sval_, R, 32 + key_bits, ktmp, vtmp_, a_, s_);
+    k_rank_weights<<<rb, 256, 0, s_>>>(R, sval_, g.vw, prefix_);
+    JET_LAUNCH_CHECK();
+    excl_scan_i64_i64(prefix_, prefix_, R, a_, s_);
+    if (strong)
+        k_rank_apply<true><<<rb, 256, 0, s_>>>(R, keys, prefix_, part, buckets_, sizes, smax, width, g.vw, g.rowptr,
+                                              evict_adjust_, score_, moves_, c, nmax_);
+    else
+        k_rank_apply<false><<<rb, 256, 0, s_>>>(R, keys, prefix_, part, buckets_, sizes, smax, width, g.vw, g.rowptr,
+                                               evict_adjust_, score_, moves_, c, nmax_);
+    JET_LAUNCH_CHECK();
+}
+
+void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes, bool strong) {
     DGraph g = L.view();
     const int32_t k = p_.k;
+    int32_t* c = cnt();
     int32_t sections = sections_for(g.n, k);
     int64_t width = (int64_t)MAX_BUCKETS * sections;
+    // weak: gain bucket 50 is not capped (jet_refiner.hpp:349-351) and spills
+    // past the last part's minibuckets, so `sections` more are kept and scanned
+    int64_t nbk = strong ? (int64_t)k * width + 1 : (int64_t)k * width + sections + 1;
     int64_t smax = size_max_;
-    int64_t max_dest = (int64_t)(smax * 0.99);
-    if (max_dest < smax - 100) max_dest = smax - 100;
-    fill_i32(counters_, 0, 4, s_);
-    k_undersized<<<1, 256, 0, s_>>>(sizes, k, max_dest, undersized_, misc_);
-    int nb = grid_for(g.n, 256, 8192);
-    k_weak_select<<<nb, 256, 0, s_>>>(g, part, sizes, smax, opt_, max_dest, undersized_, misc_, conn_off_, conn_,
-                                      conn_size_, sections, dest_, bid_, flags_);
-    JET_LAUNCH_CHECK();
-    excl_scan_i32_i32(flags_, flags_, g.n, a_, s_);
-    int32_t ncand = read_scalar(flags_ + g.n, s_);
-    int64_t maxkey = (int64_t)k * width + sections;
-    int bits = 1;
-    while ((int64_t(1) << bits) <= maxkey) bits++;
-    sort_candidates(L, ncand, bits);
-    k_part_begin<<<grid_for((int64_t)k + 1, 256, 64), 256, 0, s_>>>(skey_, ncand, k, width, part_begin_);
-    if (ncand > 0) {
-        k_evict<false><<<grid_for(ncand, 256, 8192), 256, 0, s_>>>(g, sval_, ncand, prefix_, part_begin_, part, sizes,
-                                                                   smax, evict_adjust_, score_, moves_, counters_,
-                                                                   nmax_);
+    int64_t max_dest;
+    if (strong) {
+        max_dest = std::max<int64_t>((int64_t)opt_ + 1, (int64_t)(smax * 0.99));
+    } else {
+        max_dest = (int64_t)(smax * 0.99);
+        if (max_dest < smax - 100) max_dest = smax - 100;
     }
-    JET_LAUNCH_CHECK();
-    int32_t c[4];
-    JET_HIP(hipMemcpyAsync(c, counters_, sizeof(c), hipMemcpyDeviceToHost, s_));
-    JET_HIP(hipStreamSynchronize(s_));
-    return Moves{c[2], c[3]};
-}
-
-Refiner::Moves Refiner::rebalance_strong(const Level& L, const part_t* part, const int64_t* sizes) {
-    DGraph g = L.view();
-    const int32_t k = p_.k;
-    int32_t sections = sections_for(g.n, k);
-    int64_t width = (int64_t)MAX_BUCKETS * sections;
-    int64_t smax = size_max_;
-    int64_t max_dest = std::max<int64_t>((int64_t)opt_ + 1, (int64_t)(smax * 0.99));
-    fill_i32(counters_, 0, 5, s_);
-    k_strong_prep<<<1, 256, 0, s_>>>(sizes, k, max_dest, acc_);
+    k_reset_buckets<<<grid_for(nbk + 1, 256, 2048), 256, 0, s_>>>(buckets_, bflag_, nbk + 1, cnt_next(), sizes, k,
+                                                                    max_dest, strong ? 1 : 0, undersized_, misc_, acc_,
+                                                                    evict_adjust_);
     int nb = grid_for(g.n, 256, 8192);
-    k_strong_select<<<nb, 256, 0, s_>>>(g, part, sizes, smax, opt_, max_dest, acc_, conn_off_, conn_, conn_size_,
-                                        sections, bid_, flags_);
+    int nsel = grid_for(g.n, 256, 1024);  // each workgroup amortises its LDS bucket map
+    if (strong)
+        k_strong_select<<<nsel, 256, 0, s_>>>(g, part, sizes, smax, opt_, max_dest, acc_, conn_off_, conn_, conn_size_,
+                                            sections, bid_, reinterpret_cast<unsigned long long*>(buckets_));
+    else
+        k_weak_select<<<nsel, 256, 0, s_>>>(g, part, sizes, smax, opt_, max_dest, undersized_, misc_, conn_off_, conn_,
+                                          conn_size_, sections, dest_, bid_,
+                                          reinterpret_cast<unsigned long long*>(buckets_));
     JET_LAUNCH_CHECK();
-    excl_scan_i32_i32(flags_, flags_, g.n, a_, s_);
-    int32_t ncand = read_scalar(flags_ + g.n, s_);
-    int64_t maxkey = (int64_t)k * width + 1;
+    excl_scan_i64_i64(buckets_, buckets_, nbk, a_, s_);
+    int cb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_WAVES * LS_CHW));
+    if (strong)
+        k_classify<true><<<cb, LS_WAVES * WAVE, 0, s_>>>(g, part, bid_, buckets_, sizes, smax, width, bflag_, moves_, c,
+                                                         nmax_);
+    else
+        k_classify<false><<<cb, LS_WAVES * WAVE, 0, s_>>>(g, part, bid_, buckets_, sizes, smax, width, bflag_, moves_,
+                                                          c, nmax_);
+    k_collect<<<cb, LS_WAVES * WAVE, 0, s_>>>(g.n, bid_, bflag_, cand_, c);
+    JET_LAUNCH_CHECK();
     int bits = 1;
-    while ((int64_t(1) << bits) <= maxkey) bits++;
-    sort_candidates(L, ncand, bits);
-    k_part_begin<<<grid_for((int64_t)k + 1, 256, 64), 256, 0, s_>>>(skey_, ncand, k, width, part_begin_);
-    fill_i64(evict_adjust_, 0, k, s_);
-    if (ncand > 0) {
-        k_evict<true><<<grid_for(ncand, 256, 8192), 256, 0, s_>>>(g, sval_, ncand, prefix_, part_begin_, part, sizes,
-                                                                  smax, evict_adjust_, score_, moves_, counters_, nmax_);
-    }
-    k_cookie<<<1, 64, 0, s_>>>(k, prefix_, part_begin_, sizes, max_dest, evict_adjust_, evict_start_, acc_);
-    k_strong_dest<<<nb, 256, 0, s_>>>(moves_, counters_, nmax_, part, g.vw, evict_adjust_, evict_start_, k, score_,
-                                      dest_, redraw_, counters_ + 4, RD_MAX);
-    k_strong_redraw<<<1, 256, 0, s_>>>(redraw_, counters_ + 4, part, g.vw, evict_start_, k, score_, dest_, acc_,
-                                       misc_ + 1);
-    JET_LAUNCH_CHECK();
-    int32_t c[4];
-    JET_HIP(hipMemcpyAsync(c, counters_, sizeof(c), hipMemcpyDeviceToHost, s_));
-    JET_HIP(hipStreamSynchronize(s_));
-    return Moves{c[2], c[3]};
-}
-
-void Refiner::perform_moves(const Level& L, part_t* part, Moves mv, int64_t* sizes, int64_t& cut_change,
-                            int64_t& max_size) {
-    DGraph g = L.view();
-    const int32_t k = p_.k;
-    fill_i64(acc_, 0, 3, s_);
-    int32_t total = mv.total();
-    int nb = grid_for(std::max(total, 1), 256, 8192);
-    k_moves_apply<<<nb, 256, 0, s_>>>(moves_, counters_, nmax_, g.vw, part, dest_, dest_cache_, conn_off_, conn_,
-                                      conn_size_, reinterpret_cast<unsigned long long*>(sizes), k, acc_);
-    JET_LAUNCH_CHECK();
-    if (total > g.n / 10) {
-        k_mark_adjacent<<<nb, 256, 0, s_>>>(g, moves_, counters_, nmax_, swap_bit_);
-        JET_LAUNCH_CHECK();
-        init_conn(L, part, swap_bit_);
-    } else if (total > 0) {
-        int nbs = grid_for(std::max(mv.nsmall, 1), 256, 8192);
-        int nbb = grid_for((int64_t)std::max(mv.nbig, 1) * WAVE, 256, 4096);
-        k_update_small<false><<<nbs, 256, 0, s_>>>(g, moves_, counters_, nmax_, part, dest_, conn_off_, conn_, conn_size_,
-                                                   dest_cache_, k, misc_ + 1);
-        k_update_big<false><<<nbb, 256, 0, s_>>>(g, moves_, counters_, nmax_, part, dest_, conn_off_, conn_, conn_size_,
-                                                 dest_cache_, k, misc_ + 1);
-        k_update_small<true><<<nbs, 256, 0, s_>>>(g, moves_, counters_, nmax_, part, dest_, conn_off_, conn_, conn_size_,
-                                                  dest_cache_, k, misc_ + 1);
-        k_update_big<true><<<nbb, 256, 0, s_>>>(g, moves_, counters_, nmax_, part, dest_, conn_off_, conn_, conn_size_,
-                                                dest_cache_, k, misc_ + 1);
+    while ((int64_t(1) << bits) <= nbk) bits++;
+    rank_flagged(L, part, sizes, width, bits, strong);
+    if (strong) {
+        k_cookie<<<1, CK_BLOCK, 0, s_>>>(k, width, buckets_, sizes, max_dest, evict_adjust_, evict_start_, acc_);
+        k_strong_dest<<<nb, 256, 0, s_>>>(moves_, c, nmax_, part, g.vw, evict_adjust_, evict_start_, k, score_, dest_,
+                                          redraw_, c + 4, RD_MAX);
+        k_strong_redraw<<<1, 256, 0, s_>>>(redraw_, c + 4, part, g.vw, evict_start_, k, score_, dest_, acc_,
+                                           misc_ + 1);
         JET_LAUNCH_CHECK();
     }
-    k_moves_finish<<<nb, 256, 0, s_>>>(moves_, counters_, nmax_, part, dest_, conn_off_, conn_, conn_size_, sizes, k,
-                                       acc_);
+}
+
+void Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_t& cut_change, int64_t& max_size) {
+    DGraph g = L.view();
+    const int32_t k = p_.k;
+    int32_t* c = cnt();
+    // moves are at most n/10 on the update_small path; a modest grid keeps the
+    // completion ticket of k_moves_finish cheap
+    int nb = grid_for(g.n, 256, 512);
+    k_moves_apply<<<nb, 256, 0, s_>>>(g, moves_, c, nmax_, g.vw, part, dest_, dest_cache_, conn_off_, conn_,
+                                      conn_size_, reinterpret_cast<unsigned long long*>(sizes), k, acc_, swap_bit_);
     JET_LAUNCH_CHECK();
-    int64_t h[3];
-    JET_HIP(hipMemcpyAsync(h, acc_, sizeof(h), hipMemcpyDeviceToHost, s_));
+    init_conn(L, part, swap_bit_);  // update_large: rebuild marked rows (gated on the device)
+    int G = group_for_avg(g);
+    int nbs = grid_for((int64_t)g.n * G, 256, 4096);
+    int nbb = grid_for((int64_t)g.n * WAVE, 256, 2048);
+    for (int add = 0; add < 2; add++) {
+#define JET_UPD(GG)                                                                                                   \
+    if (add) k_update_small<true, GG><<<nbs, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,       \
+                                                          conn_size_, dest_cache_, k, misc_ + 1);                    \
+    else k_update_small<false, GG><<<nbs, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,          \
+                                                        conn_size_, dest_cache_, k, misc_ + 1);
+        switch (G) {
+            case 4: JET_UPD(4) break;
+            case 8: JET_UPD(8) break;
+            case 16: JET_UPD(16) break;
+            case 32: JET_UPD(32) break;
+            default: JET_UPD(64) break;
+        }
+#undef JET_UPD
+        if (has_big_) {
+            if (add) k_update_big<true><<<nbb, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,
+                                                             conn_size_, dest_cache_, k, misc_ + 1);
+            else k_update_big<false><<<nbb, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,
+                                                           conn_size_, dest_cache_, k, misc_ + 1);
+        }
+        JET_LAUNCH_CHECK();
+    }
+    k_moves_finish<<<nb, 256, 0, s_>>>(moves_, c, nmax_, part, dest_, conn_off_, conn_, conn_size_, sizes, k, acc_,
+                                       res_dev_);
+    JET_LAUNCH_CHECK();
     JET_HIP(hipStreamSynchronize(s_));
-    cut_change = h[0] + h[1];
-    max_size = h[2];
+    volatile int64_t* res = res_host_;
+    cut_change = res[0];
+    max_size = res[1];
+    par_ ^= 1;
 }
 
 LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
@@ -1130,6 +1474,8 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
     DGraph g = L.view();
     const int32_t k = p_.k;
     fill_i32(misc_, 0, 8, s_);
+    fill_i32(counters_, 0, 16, s_);
+    par_ = 0;
     if (!init_) {
         cut_ = total_cut2(g, best_part, a_, s_);
         part_sizes(g, best_part, k, best_sizes_, s_);
@@ -1170,18 +1516,16 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
         int count = 0;
         while (count++ <= 11) {
             rec.iterations++;
-            Moves mv;
             if (curr_imb <= imb_max) {
-                mv = jet_lp(L, cur_part_, fr);
+                jet_lp(L, cur_part_, fr);
                 balance_counter = 0;
                 rec.lp_iterations++;
             } else {
-                if (balance_counter < 2) mv = rebalance_weak(L, cur_part_, curr_sizes_);
-                else mv = rebalance_strong(L, cur_part_, curr_sizes_);
+                rebalance(L, cur_part_, curr_sizes_, balance_counter >= 2);
                 balance_counter++;
             }
             int64_t cc = 0, mx = 0;
-            perform_moves(L, cur_part_, mv, curr_sizes_, cc, mx);
+            perform_moves(L, cur_part_, curr_sizes_, cc, mx);
             curr_imb = mx > opt_ ? mx - opt_ : 0;
             curr_cut -= cc;
             bool take = false;
@@ -1215,6 +1559,8 @@ void Refiner::single_step(const Level& L, const part_t* part, int kind, double f
     DGraph g = L.view();
     const int32_t k = p_.k;
     fill_i32(misc_, 0, 8, s_);
+    fill_i32(counters_, 0, 16, s_);
+    par_ = 0;
     copy_i32(cur_part_, part, g.n, s_);
     part_sizes(g, cur_part_, k, curr_sizes_, s_);
     std::vector<int64_t> hs(k);
@@ -1226,16 +1572,17 @@ void Refiner::single_step(const Level& L, const part_t* part, int kind, double f
     size_max_ = (int32_t)(opt_ * p_.imb);
     build_tiers(L);
     init_conn(L, cur_part_, nullptr);
-    Moves mv;
-    if (kind == 0) mv = jet_lp(L, cur_part_, fr);
-    else if (kind == 1) mv = rebalance_weak(L, cur_part_, curr_sizes_);
-    else mv = rebalance_strong(L, cur_part_, curr_sizes_);
-    int32_t total = mv.total();
+    if (kind == 0) jet_lp(L, cur_part_, fr);
+    else rebalance(L, cur_part_, curr_sizes_, kind == 2);
+    int32_t c4[4];
+    JET_HIP(hipMemcpyAsync(c4, cnt(), sizeof(c4), hipMemcpyDeviceToHost, s_));
+    JET_HIP(hipStreamSynchronize(s_));
+    int32_t total = c4[2] + c4[3];
     ArenaScope sc(a_);
     vid_t* ov = a_.alloc<vid_t>(std::max(total, 1));
     part_t* od = a_.alloc<part_t>(std::max(total, 1));
     if (total > 0) {
-        k_collect_moves<<<grid_for(total, 256, 8192), 256, 0, s_>>>(moves_, counters_, nmax_, dest_, ov, od);
+        k_collect_moves<<<grid_for(total, 256, 8192), 256, 0, s_>>>(moves_, cnt(), nmax_, dest_, ov, od);
         JET_LAUNCH_CHECK();
     }
     std::vector<int32_t> v(total), d(total);

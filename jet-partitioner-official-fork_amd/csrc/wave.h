@@ -24,6 +24,65 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Appends the lanes with pred set to a list guarded by *counter: one atomic
+// per wave; returns each such lane's slot (or -1). Call from every active lane.
+__device__ inline int32_t wave_append(int32_t* counter, bool pred) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    uint64_t mask = __ballot(pred);
+    if (mask == 0) return -1;
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    int32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (int32_t)__popcll(mask));
+    base = __shfl(base, leader, WAVE);
+    return pred ? base + (int32_t)__popcll(mask & ((1ull << lane) - 1ull)) : -1;
+}
+
+// Unordered two-list append with wave-private LDS staging and one global
+// atomic per workgroup and list. Each wave stages up to CHW items (front list
+// grows up, back list grows down inside its region); flush() publishes them:
+// front items to list[base..], back items to list[nmax-1-base..].
+template <int WAVES, int CHW>
+struct StagedLists {
+    vid_t* buf;  // LDS, WAVES * CHW
+    int32_t nf = 0, nb = 0;
+    __device__ explicit StagedLists(vid_t* lds) : buf(lds + (threadIdx.x / WAVE) * CHW) {}
+    __device__ inline void push(bool pf, bool pb, vid_t v) {
+        const int lane = threadIdx.x & (WAVE - 1);
+        const uint64_t lt = (1ull << lane) - 1ull;
+        uint64_t mf = __ballot(pf), mb = __ballot(pb);
+        if (pf) buf[nf + (int32_t)__popcll(mf & lt)] = v;
+        if (pb) buf[CHW - 1 - (nb + (int32_t)__popcll(mb & lt))] = v;
+        nf += (int32_t)__popcll(mf);
+        nb += (int32_t)__popcll(mb);
+    }
+    // All threads of the workgroup must call flush().
+    __device__ inline void flush(vid_t* list, int32_t* cnt_front, int32_t* cnt_back, vid_t nmax, int32_t* sh) {
+        const int lane = threadIdx.x & (WAVE - 1);
+        const int w = threadIdx.x / WAVE;
+        if (lane == 0) {
+            sh[2 * w] = nf;
+            sh[2 * w + 1] = nb;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int32_t tf = 0, tb = 0;
+            for (int q = 0; q < WAVES; q++) {
+                int32_t a = sh[2 * q], b = sh[2 * q + 1];
+                sh[2 * q] = tf;
+                sh[2 * q + 1] = tb;
+                tf += a;
+                tb += b;
+            }
+            sh[2 * WAVES] = tf ? atomicAdd(cnt_front, tf) : 0;
+            sh[2 * WAVES + 1] = tb ? atomicAdd(cnt_back, tb) : 0;
+        }
+        __syncthreads();
+        int32_t bf = sh[2 * WAVES] + sh[2 * w], bb = sh[2 * WAVES + 1] + sh[2 * w + 1];
+        for (int32_t t = lane; t < nf; t += WAVE) list[bf + t] = buf[t];
+        for (int32_t t = lane; t < nb; t += WAVE) list[nmax - 1 - (bb + t)] = buf[CHW - 1 - t];
+    }
+};
+
 template <class T>
 __device__ inline T wave_incl_scan(T x) {
     const int lane = threadIdx.x & (WAVE - 1);
