@@ -23,6 +23,7 @@
 #include "jet_capi.h"
 #include "misc.h"
 #include "prims.h"
+#include "prof.h"
 #include "refine.h"
 
 extern "C" int jet_host_initpart(int32_t n, const int64_t* xadj, const int32_t* adj, const int32_t* adjw,
@@ -646,17 +647,22 @@ int jet_gain_bucket_batch(int64_t count, const int64_t* gains, const int64_t* vw
 }
 
 int jet_prof_enable(const char* family) {
-    (void)family;
-    g_err = "kernel timing is not built into this library";
-    return JET_EUNSUPPORTED;
+    return guard_impl([&] {
+        if (!family) throw InvalidInput("family is NULL");
+        context();
+        if (!prof_enable(family)) throw InvalidInput(std::string("unknown kernel family: ") + family);
+    });
 }
 
 int jet_prof_read(int64_t* launches, double* total_ms, double* bytes) {
-    (void)launches;
-    (void)total_ms;
-    (void)bytes;
-    g_err = "kernel timing is not built into this library";
-    return JET_EUNSUPPORTED;
+    return guard_impl([&] {
+        int64_t l = 0;
+        double ms = 0, b = 0;
+        if (!prof_read(l, ms, b)) throw InvalidInput("no kernel family enabled");
+        *launches = l;
+        *total_ms = ms;
+        *bytes = b;
+    });
 }
 
 }  // extern "C"

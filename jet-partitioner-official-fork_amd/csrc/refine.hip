@@ -17,6 +17,7 @@
 #include <cmath>
 
 #include "prims.h"
+#include "prof.h"
 #include "refine.h"
 #include "wave.h"
 
@@ -343,9 +344,11 @@ constexpr int LS_CHW = 512;
 __global__ void __launch_bounds__(LS_WAVES* WAVE)
     k_lp_select(DGraph g, const part_t* part, const eoff_t* conn_off, const int2* conn, const int32_t* conn_size,
                 part_t* dest_cache, int32_t* gain_p, int32_t* lock, double fr, part_t* dest, int32_t* pregain,
-                vid_t* cand, int32_t* counters, vid_t nmax, int32_t* next_counters) {
+                vid_t* cand, int32_t* counters, vid_t nmax, int32_t* next_counters, unsigned long long* pc) {
     __shared__ vid_t sbuf[LS_WAVES * LS_CHW];
     __shared__ int32_t sh[2 * LS_WAVES + 2];
+    __shared__ int64_t pred[LS_WAVES];
+    int64_t n_miss = 0, n_slot = 0, n_cand = 0;
     StagedLists<LS_WAVES, LS_CHW> st(sbuf);
     if (blockIdx.x == 0 && threadIdx.x < 8) next_counters[threadIdx.x] = 0;
     const int lane = threadIdx.x & (WAVE - 1);
@@ -362,6 +365,8 @@ __global__ void __launch_bounds__(LS_WAVES* WAVE)
                 int32_t b_conn = 0, p_conn = 0;
                 const int2* tab = conn + conn_off[i];
                 int32_t sz = conn_size[i];
+                n_miss++;
+                n_slot += sz;
                 for (int32_t s = 0; s < sz; s++) {
                     int2 e = tab[s];
                     if (e.y > b_conn && e.x != p) {
@@ -384,6 +389,7 @@ __global__ void __launch_bounds__(LS_WAVES* WAVE)
             dest[i] = best;
             is_cand = best != NO_MOVE && lock[i] == 0;
             if (is_cand) {
+                n_cand++;
                 pregain[i] = gain_p[i];
                 deg = g.rowptr[i + 1] - g.rowptr[i];
             } else {
@@ -394,6 +400,13 @@ __global__ void __launch_bounds__(LS_WAVES* WAVE)
         st.push(is_cand && deg <= BIG_DEG, is_cand && deg > BIG_DEG, (vid_t)i);
     }
     st.flush(cand, &counters[0], &counters[1], nmax, sh);
+    if (pc) {
+        int64_t v3[3] = {n_miss, n_slot, n_cand};
+        for (int q = 0; q < 3; q++) {
+            int64_t v = block_reduce_sum<int64_t, LS_WAVES * WAVE>(v3[q], pred);
+            if (threadIdx.x == 0 && v) atomicAdd(&pc[q], (unsigned long long)v);
+        }
+    }
 }
 
 // jet_refiner.hpp:275-307 (afterburner): gain correction assuming every
@@ -1315,9 +1328,13 @@ void Refiner::jet_lp(const Level& L, const part_t* part, double fr) {
     DGraph g = L.view();
     int32_t* c = cnt();
     int nb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_WAVES * LS_CHW));
+    prof_begin(PF_LP_SELECT, s_);
     k_lp_select<<<nb, LS_WAVES * WAVE, 0, s_>>>(g, part, conn_off_, conn_, conn_size_, dest_cache_, gain_p_, lock_, fr,
-                                                dest_, pregain_, cand_, c, nmax_, cnt_next());
+                                                dest_, pregain_, cand_, c, nmax_, cnt_next(),
+                                                prof_counters(PF_LP_SELECT));
     JET_LAUNCH_CHECK();
+    // per vertex: cache read 4, destination write 4, lock read 4, pre-gain write 4
+    prof_end(PF_LP_SELECT, s_, 16.0 * g.n);
     unsigned ab = (unsigned)std::max<int64_t>(256, ceil_div((int64_t)g.n, (int64_t)AB_WAVES * AB_CHW));
     switch (group_for_avg(g)) {
         case 4: k_afterburner_small<4><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_); break;
