@@ -112,6 +112,15 @@ int jet_partition_csr_big(const jet_config_t* cfg, int64_t n, const int64_t* row
                           const int64_t* ew, const int64_t* vw, int32_t uniform_ew, int32_t ptr_space, uint64_t seed,
                           int32_t* part_out, int64_t* cut_out, jet_stats_t* stats);
 
+/* Same with int64 columns: replaces partition_biggest / partition_host_biggest
+ * / partition_serial_biggest (header/jet.h:88-107; src/jet_biggest.cpp:47-75,
+ * big_ordinal_t = int64, header/jet_defs.h:89-92). The device path keeps
+ * 32-bit vertex ids, so n must be below 2^31 - 2 (else JET_EINVAL); columns
+ * are narrowed on the device after a range check. */
+int jet_partition_csr_biggest(const jet_config_t* cfg, int64_t n, const int64_t* row_offsets, const int64_t* col,
+                              const int64_t* ew, const int64_t* vw, int32_t uniform_ew, int32_t ptr_space,
+                              uint64_t seed, int32_t* part_out, int64_t* cut_out, jet_stats_t* stats);
+
 /* ---- Sub-function entry points (one hot-path stage each; host or device
  * pointers per ptr_space). They exist so integrators and parity tests can
  * drive one stage of the path on the GPU. */

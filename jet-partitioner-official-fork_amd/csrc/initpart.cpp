@@ -10,11 +10,17 @@
 // tolerances along any root-to-leaf path stays within `imb` (METIS's ubvec).
 //
 // It runs on the host in both the GPU path and the CPU oracle so that the two
-// start refinement from the same coarsest partition.
+// start refinement from the same coarsest partition. Independent work (the two
+// halves of a bisection, the trials of one bisection) runs on up to
+// JET_HOST_THREADS_MAX host threads; every thread writes its own output and the
+// best trial is picked in trial order, so the result does not depend on the
+// thread count.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <queue>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -132,8 +138,9 @@ struct Bisector {
         }
     }
 
-    // Balance-constrained 2-way FM. Keeps the best prefix of each pass.
-    void fm(const double maxw[2], int passes) {
+    // Balance-constrained 2-way FM. Keeps the best prefix of each pass; a pass
+    // ends after `limit` moves without improvement (0: max(50, n/10)).
+    void fm(const double maxw[2], int passes, int32_t limit_in = 0) {
         int32_t n = s.n;
         std::vector<uint8_t> locked(n);
         std::vector<int32_t> moved;
@@ -153,7 +160,7 @@ struct Bisector {
             int64_t best_cut = cut;
             size_t best_len = 0;
             moved.clear();
-            int32_t limit = std::max<int32_t>(50, n / 10);
+            int32_t limit = limit_in > 0 ? limit_in : std::max<int32_t>(50, n / 10);
             int32_t since_best = 0;
             while (since_best < limit) {
                 int from = -1;
@@ -213,53 +220,264 @@ struct Bisector {
     }
 };
 
-// Bisects `s` so side 0 carries about frac of its weight. Returns side per
-// local vertex.
-static std::vector<int8_t> bisect(const Sub& s, double frac, double ub, uint64_t seed) {
+constexpr int JET_HOST_THREADS_MAX = 16;
+// Below this many adjacency entries a subproblem is not worth a thread.
+constexpr int64_t PAR_MIN_EDGES = 200000;
+
+struct Trial {
+    std::vector<int8_t> side;
+    double over = 0;
+    int64_t cut = 0;
+};
+
+static int32_t pseudo_peripheral(const Sub& s) {
+    int32_t n = s.n, seedv = 0;
+    for (int rep = 0; rep < 2; rep++) {  // BFS twice from vertex 0
+        std::vector<int32_t> dist(n, -1);
+        std::vector<int32_t> qv;
+        qv.push_back(seedv);
+        dist[seedv] = 0;
+        for (size_t h = 0; h < qv.size(); h++) {
+            int32_t v = qv[h];
+            for (int64_t j = s.xadj[v]; j < s.xadj[v + 1]; j++)
+                if (dist[s.adj[j]] < 0) {
+                    dist[s.adj[j]] = dist[v] + 1;
+                    qv.push_back(s.adj[j]);
+                }
+        }
+        seedv = qv.back();
+    }
+    return seedv;
+}
+
+static void run_trial(const Sub& s, int t, uint64_t seed, double target0, const double maxw[2], int passes,
+                      Trial& out) {
+    int32_t n = s.n;
+    int32_t seedv = t == 0 ? pseudo_peripheral(s)
+                           : (int32_t)(mix(seed + (uint64_t)t * 0x9E37ULL + (uint64_t)n) % (uint64_t)n);
+    Bisector b(s);
+    b.grow(seedv, target0, maxw[0]);
+    b.fm(maxw, passes);
+    out.over = 0;
+    for (int side = 0; side < 2; side++)
+        if ((double)b.w[side] > maxw[side]) out.over += (double)b.w[side] - maxw[side];
+    out.cut = b.cut;
+    out.side = std::move(b.side);
+}
+
+// Bisects `s` so side 0 carries about frac of its weight: several trials of
+// greedy growing + FM on the whole graph, on up to `par` threads. Returns
+// side per local vertex.
+static std::vector<int8_t> bisect_flat(const Sub& s, double frac, double ub, uint64_t seed, int par) {
     double W = (double)s.total_w;
     double target0 = W * frac;
     double maxw[2] = {target0 * ub, (W - target0) * ub};
     int32_t n = s.n;
     int trials = n > 20000 ? 2 : (n > 2000 ? 4 : 8);
     int passes = n > 20000 ? 4 : 8;
-    std::vector<int8_t> best;
-    double best_over = 0;
-    int64_t best_cut = 0;
-    Bisector b(s);
-    for (int t = 0; t < trials; t++) {
-        int32_t seedv;
-        if (t == 0) {  // pseudo-peripheral vertex: BFS twice from vertex 0
-            seedv = 0;
-            for (int rep = 0; rep < 2; rep++) {
-                std::vector<int32_t> dist(n, -1);
-                std::vector<int32_t> qv;
-                qv.push_back(seedv);
-                dist[seedv] = 0;
-                for (size_t h = 0; h < qv.size(); h++) {
-                    int32_t v = qv[h];
-                    for (int64_t j = s.xadj[v]; j < s.xadj[v + 1]; j++)
-                        if (dist[s.adj[j]] < 0) {
-                            dist[s.adj[j]] = dist[v] + 1;
-                            qv.push_back(s.adj[j]);
-                        }
-                }
-                seedv = qv.back();
-            }
-        } else {
-            seedv = (int32_t)(mix(seed + (uint64_t)t * 0x9E37ULL + (uint64_t)n) % (uint64_t)n);
+    std::vector<Trial> res(trials);
+    if (par > 1 && (int64_t)s.adj.size() >= PAR_MIN_EDGES) {
+        for (int t0 = 0; t0 < trials; t0 += par) {
+            std::vector<std::thread> th;
+            int t1 = std::min(trials, t0 + par);
+            for (int t = t0 + 1; t < t1; t++)
+                th.emplace_back([&, t]() { run_trial(s, t, seed, target0, maxw, passes, res[t]); });
+            run_trial(s, t0, seed, target0, maxw, passes, res[t0]);
+            for (auto& x : th) x.join();
         }
-        b.grow(seedv, target0, maxw[0]);
-        b.fm(maxw, passes);
-        double over = 0;
-        for (int side = 0; side < 2; side++)
-            if ((double)b.w[side] > maxw[side]) over += (double)b.w[side] - maxw[side];
-        if (best.empty() || over < best_over || (over == best_over && b.cut < best_cut)) {
-            best = b.side;
-            best_over = over;
-            best_cut = b.cut;
+    } else {
+        for (int t = 0; t < trials; t++) run_trial(s, t, seed, target0, maxw, passes, res[t]);
+    }
+    int best = 0;
+    for (int t = 1; t < trials; t++)
+        if (res[t].over < res[best].over || (res[t].over == res[best].over && res[t].cut < res[best].cut)) best = t;
+    return std::move(res[best].side);
+}
+
+// ---- multilevel bisection for large subgraphs (coarsen, bisect, refine up)
+
+// Graphs above this size are coarsened before bisecting; coarsening stops at
+// about ML_COARSEST vertices or when a level shrinks by less than 10%.
+constexpr int32_t ML_MIN_N = 2000;
+constexpr int32_t ML_COARSEST = 1000;
+
+static void shuffle_ids(std::vector<int32_t>& p, uint64_t seed) {
+    for (int32_t i = (int32_t)p.size() - 1; i > 0; i--) {
+        int32_t j = (int32_t)(mix(seed + (uint64_t)i) % (uint64_t)(i + 1));
+        std::swap(p[i], p[j]);
+    }
+}
+
+// One coarsening level: heavy-edge matching in a seeded order under a vertex
+// weight cap, then pairs of still-unmatched vertices that share their first
+// neighbour (leaves of one hub), then pairs of unmatched isolated vertices.
+// cmap receives the coarse id of every vertex (ids in order of first member).
+static Sub coarsen_once(const Sub& s, uint64_t seed, int par, std::vector<int32_t>& cmap) {
+    const int32_t n = s.n;
+    const int64_t cap = std::max<int64_t>(1, (int64_t)(1.5 * (double)s.total_w / ML_COARSEST));
+    std::vector<int32_t> mate(n, -1), order(n);
+    for (int32_t i = 0; i < n; i++) order[i] = i;
+    shuffle_ids(order, seed);
+    for (int32_t v : order) {
+        if (mate[v] >= 0) continue;
+        int32_t best = -1;
+        int64_t bw = -1;
+        for (int64_t j = s.xadj[v]; j < s.xadj[v + 1]; j++) {
+            int32_t u = s.adj[j];
+            if (u == v || mate[u] >= 0 || (int64_t)s.vw[v] + s.vw[u] > cap) continue;
+            if ((int64_t)s.adjw[j] > bw) {
+                bw = s.adjw[j];
+                best = u;
+            }
+        }
+        if (best >= 0) {
+            mate[v] = best;
+            mate[best] = v;
         }
     }
-    return best;
+    // leaves sharing a neighbour, and isolated vertices, in increasing id
+    std::vector<std::pair<int32_t, int32_t>> keyed;
+    int32_t last_isolated = -1;
+    for (int32_t v = 0; v < n; v++) {
+        if (mate[v] >= 0) continue;
+        if (s.xadj[v + 1] == s.xadj[v]) {
+            if (last_isolated >= 0 && (int64_t)s.vw[v] + s.vw[last_isolated] <= cap) {
+                mate[v] = last_isolated;
+                mate[last_isolated] = v;
+                last_isolated = -1;
+            } else {
+                last_isolated = v;
+            }
+            continue;
+        }
+        keyed.push_back({s.adj[s.xadj[v]], v});
+    }
+    std::sort(keyed.begin(), keyed.end());
+    for (size_t i = 0; i + 1 < keyed.size();) {
+        int32_t a = keyed[i].second, b = keyed[i + 1].second;
+        if (keyed[i].first == keyed[i + 1].first && (int64_t)s.vw[a] + s.vw[b] <= cap) {
+            mate[a] = b;
+            mate[b] = a;
+            i += 2;
+        } else {
+            i += 1;
+        }
+    }
+    cmap.assign(n, -1);
+    int32_t nc = 0;
+    for (int32_t v = 0; v < n; v++) {
+        if (cmap[v] >= 0) continue;
+        cmap[v] = nc;
+        if (mate[v] >= 0) cmap[mate[v]] = nc;
+        nc++;
+    }
+    Sub c;
+    c.n = nc;
+    c.vw.assign(nc, 0);
+    c.gid.resize(nc);
+    std::vector<int32_t> first(nc, -1), second(nc, -1);
+    for (int32_t v = 0; v < n; v++) {
+        int32_t cv = cmap[v];
+        c.vw[cv] += s.vw[v];
+        if (first[cv] < 0) first[cv] = v;
+        else second[cv] = v;
+    }
+    c.total_w = s.total_w;
+    for (int32_t cv = 0; cv < nc; cv++) c.gid[cv] = cv;
+    // Rows are built in contiguous chunks of coarse ids, one per thread, each
+    // with its own position table; the chunks are then concatenated in order,
+    // so the result is the same for any thread count.
+    int T = std::max(1, std::min<int>(par, (int)(s.adj.size() / PAR_MIN_EDGES) + 1));
+    std::vector<int32_t> bounds(T + 1, nc);
+    {
+        const int64_t per = ((int64_t)s.adj.size() + T - 1) / T;
+        int64_t acc = 0;
+        int t = 1;
+        bounds[0] = 0;
+        for (int32_t cv = 0; cv < nc && t < T; cv++) {
+            for (int32_t v : {first[cv], second[cv]})
+                if (v >= 0) acc += s.xadj[v + 1] - s.xadj[v];
+            if (acc >= per * t) bounds[t++] = cv + 1;
+        }
+        for (; t < T; t++) bounds[t] = nc;
+    }
+    // pass 1 counts each coarse row's distinct neighbours, pass 2 writes the
+    // rows at their final offsets; a row's entries keep first-seen order
+    std::vector<int64_t> rowlen(nc, 0);
+    auto pass = [&](int t, bool write) {
+        std::vector<int64_t> slot(nc, -1);  // position of cu in the current row, if added there
+        for (int32_t cv = bounds[t]; cv < bounds[t + 1]; cv++) {
+            const int64_t row0 = write ? c.xadj[cv] : 0;
+            int64_t len = 0;
+            int64_t tag = write ? row0 : -2 - (int64_t)cv;  // count pass: slot holds -2 - row id
+            for (int32_t v : {first[cv], second[cv]}) {
+                if (v < 0) continue;
+                for (int64_t j = s.xadj[v]; j < s.xadj[v + 1]; j++) {
+                    int32_t cu = cmap[s.adj[j]];
+                    if (cu == cv) continue;
+                    if (write) {
+                        if (slot[cu] < row0) {
+                            slot[cu] = row0 + len++;
+                            c.adj[slot[cu]] = cu;
+                            c.adjw[slot[cu]] = s.adjw[j];
+                        } else {
+                            c.adjw[slot[cu]] += s.adjw[j];
+                        }
+                    } else if (slot[cu] != tag) {
+                        slot[cu] = tag;
+                        len++;
+                    }
+                }
+            }
+            if (!write) rowlen[cv] = len;
+        }
+    };
+    auto run_all = [&](bool write) {
+        std::vector<std::thread> th;
+        for (int t = 1; t < T; t++) th.emplace_back(pass, t, write);
+        pass(0, write);
+        for (auto& x : th) x.join();
+    };
+    run_all(false);
+    c.xadj.assign(nc + 1, 0);
+    for (int32_t cv = 0; cv < nc; cv++) c.xadj[cv + 1] = c.xadj[cv] + rowlen[cv];
+    c.adj.resize(c.xadj[nc]);
+    c.adjw.resize(c.xadj[nc]);
+    run_all(true);
+    return c;
+}
+
+static std::vector<int8_t> bisect_flat(const Sub& s, double frac, double ub, uint64_t seed, int par);
+
+// Coarsens s, bisects the coarsest graph with the flat method, then projects
+// level by level with boundary FM passes.
+static std::vector<int8_t> bisect_multilevel(const Sub& s, double frac, double ub, uint64_t seed, int par) {
+    std::vector<Sub> levels;
+    std::vector<std::vector<int32_t>> maps;
+    const Sub* cur = &s;
+    for (int lvl = 0; cur->n > ML_COARSEST && lvl < 64; lvl++) {
+        std::vector<int32_t> cmap;
+        Sub c = coarsen_once(*cur, seed + 0x51ED27ULL * (uint64_t)(lvl + 1), par, cmap);
+        if ((double)c.n > 0.9 * (double)cur->n) break;
+        maps.push_back(std::move(cmap));
+        levels.push_back(std::move(c));
+        cur = &levels.back();
+    }
+    std::vector<int8_t> side = bisect_flat(*cur, frac, ub, seed, par);
+    const double W = (double)s.total_w;
+    const double target0 = W * frac;
+    const double maxw[2] = {target0 * ub, (W - target0) * ub};
+    for (int lvl = (int)levels.size() - 1; lvl >= 0; lvl--) {
+        const Sub& fine = lvl == 0 ? s : levels[lvl - 1];
+        std::vector<int8_t> fs(fine.n);
+        for (int32_t v = 0; v < fine.n; v++) fs[v] = side[maps[lvl][v]];
+        Bisector b(fine);
+        b.side = std::move(fs);
+        b.compute();
+        b.fm(maxw, 4, std::max<int32_t>(25, std::min<int32_t>(200, fine.n / 100)));
+        side = std::move(b.side);
+    }
+    return side;
 }
 
 static Sub induced(const Sub& s, const std::vector<int8_t>& side, int8_t which) {
@@ -287,17 +505,46 @@ static Sub induced(const Sub& s, const std::vector<int8_t>& side, int8_t which) 
     return r;
 }
 
-static void recurse(const Sub& s, int32_t k, int32_t first, double ub, uint64_t seed, int32_t* part) {
+static void recurse(const Sub& s, int32_t k, int32_t first, double ub, uint64_t seed, int32_t* part, int par) {
     if (k <= 1 || s.n == 0) {
         for (int32_t v = 0; v < s.n; v++) part[s.gid[v]] = first;
         return;
     }
     int32_t k0 = k / 2, k1 = k - k0;
-    std::vector<int8_t> side = bisect(s, (double)k0 / (double)k, ub, seed + (uint64_t)first * 1315423911ULL);
-    Sub a = induced(s, side, 0);
-    Sub b = induced(s, side, 1);
-    recurse(a, k0, first, ub, seed, part);
-    recurse(b, k1, first + k0, ub, seed, part);
+    const double frac = (double)k0 / (double)k;
+    const uint64_t bseed = seed + (uint64_t)first * 1315423911ULL;
+    std::vector<int8_t> side = s.n > ML_MIN_N ? bisect_multilevel(s, frac, ub, bseed, par)
+                                              : bisect_flat(s, frac, ub, bseed, par);
+    Sub a, b;
+    if (par > 1 && (int64_t)s.adj.size() >= PAR_MIN_EDGES) {
+        std::thread th([&]() { b = induced(s, side, 1); });
+        a = induced(s, side, 0);
+        th.join();
+    } else {
+        a = induced(s, side, 0);
+        b = induced(s, side, 1);
+    }
+    side.clear();
+    side.shrink_to_fit();
+    if (par > 1 && (int64_t)(a.adj.size() + b.adj.size()) >= PAR_MIN_EDGES) {
+        int pa = par / 2;
+        std::thread th([&]() { recurse(a, k0, first, ub, seed, part, pa); });
+        recurse(b, k1, first + k0, ub, seed, part, par - pa);
+        th.join();
+    } else {
+        recurse(a, k0, first, ub, seed, part, par);
+        recurse(b, k1, first + k0, ub, seed, part, par);
+    }
+}
+
+static int host_threads() {
+    int hw = (int)std::thread::hardware_concurrency();
+    if (hw < 1) hw = 1;
+    if (const char* e = std::getenv("JET_HOST_THREADS")) {
+        int v = std::atoi(e);
+        if (v >= 1) hw = v;
+    }
+    return std::min(hw, JET_HOST_THREADS_MAX);
 }
 
 }  // namespace
@@ -324,6 +571,6 @@ extern "C" int jet_host_initpart(int32_t n, const int64_t* xadj, const int32_t* 
     int depth = 0;
     while ((1 << depth) < k) depth++;
     double ub = depth > 0 ? std::pow(imb, 1.0 / depth) : imb;
-    recurse(s, k, 0, ub, seed, part);
+    recurse(s, k, 0, ub, seed, part, host_threads());
     return 0;
 }

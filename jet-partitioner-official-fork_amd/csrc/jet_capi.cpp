@@ -119,10 +119,10 @@ static void check_space(int space) {
 // narrows int64 weights, materialises unit weights) and validates it.
 static Level stage_graph(Context& C, int64_t n, const int64_t* rp64, const int32_t* rp32, const int32_t* col,
                          const int32_t* ew32, const int64_t* ew64, const int32_t* vw32, const int64_t* vw64, int space,
-                         bool uniform, bool validate = true) {
+                         bool uniform, bool validate = true, const int64_t* col64 = nullptr) {
     check_space(space);
     if (n < 1 || n > (int64_t)ORD_MAX - 2) throw InvalidInput("n must be in [1, 2^31 - 2)");
-    if ((!rp64 && !rp32) || !col) throw InvalidInput("row offsets and columns are required");
+    if ((!rp64 && !rp32) || (!col && !col64)) throw InvalidInput("row offsets and columns are required");
     hipStream_t s = C.stream;
     Level L;
     L.n = (vid_t)n;
@@ -141,7 +141,13 @@ static Level stage_graph(Context& C, int64_t n, const int64_t* rp64, const int32
         widen_offsets(n + 1, d32, L.rowptr, s);
     }
     if (L.m < 0) throw InvalidInput("row_offsets[n] must be >= 0");
-    L.col = to_device(C, col, L.m, space);
+    if (col64) {
+        const int64_t* d = to_device(C, col64, L.m, space);
+        L.col = C.persist.alloc<vid_t>(L.m);
+        if (narrow_columns(L.m, d, n, L.col, C.scratch, s)) throw InvalidInput("invalid graph: column out of range;");
+    } else {
+        L.col = to_device(C, col, L.m, space);
+    }
     if (ew64) {
         const int64_t* d = to_device(C, ew64, L.m, space);
         L.ew = C.persist.alloc<wgt_t>(L.m);
@@ -209,8 +215,17 @@ static AttemptResult partition_once(Context& C, const jet_config_t& cfg, const L
     const int32_t min_allowed = cutoff / 4;
     Arena::Mark level_mark = C.persist.mark();
     std::vector<Level> levels{fine};
+    bool fixed_point = false;
     while (levels.back().n > cutoff) {
         Level& cur = levels.back();
+        if (fixed_point) {
+            // identity level: same arrays, map == nullptr means map[i] = i
+            Level same = cur;
+            same.map = nullptr;
+            levels.push_back(same);
+            if (levels.size() > 200) break;
+            continue;
+        }
         auto tm = clk::now();
         vid_t* map = C.persist.alloc<vid_t>(cur.n);
         int32_t nc = coarsen_level(cur, alg, seed, (int32_t)levels.size() - 1, map, C.scratch, s);
@@ -218,6 +233,21 @@ static AttemptResult partition_once(Context& C, const jet_config_t& cfg, const L
         auto tb = clk::now();
         R.t_map += secs(tm, tb);
         if (nc < min_allowed) break;
+        // Weighted HEC draws no random numbers and ignores the level index, and
+        // map ids follow root order, so nc == n means map[i] = i. Contracting a
+        // contracted level (rows sorted, no self-loops or duplicates) through
+        // the identity returns the same graph, so every later level up to the
+        // level limit repeats it: they share its arrays instead of being
+        // rebuilt (contract.hpp:508-527 would rebuild the same graph each time).
+        if (alg == 1 && levels.size() >= 2 && nc == cur.n) {
+            fixed_point = true;
+            Level same = cur;
+            same.map = nullptr;
+            R.t_build += secs(tb, clk::now());
+            levels.push_back(same);
+            if (levels.size() > 200) break;
+            continue;
+        }
         Level next;
         contract(cur.view(), map, nc, C.persist, C.scratch, s, next);
         next.map = map;
@@ -266,8 +296,12 @@ static AttemptResult partition_once(Context& C, const jet_config_t& cfg, const L
                 std::printf("Initial %lld %.6f %d\n", (long long)(rec.cut2 / 2), 0.0, (int)levels[L].n);
             R.recs.push_back(rec);
             if (L > 0) {
+                if (levels[L].map == nullptr && L > 1) continue;  // identity level: same buffer
                 part_t* fp = (L == 1) ? part_out : C.persist.alloc<part_t>(levels[L - 1].n);
-                project(levels[L - 1].n, levels[L].map, cur, fp, s);
+                if (levels[L].map == nullptr)
+                    JET_HIP(hipMemcpyAsync(fp, cur, sizeof(part_t) * levels[L].n, hipMemcpyDeviceToDevice, s));
+                else
+                    project(levels[L - 1].n, levels[L].map, cur, fp, s);
                 cur = fp;
             }
         }
@@ -331,12 +365,12 @@ static void check_config(const jet_config_t* cfg) {
 static void partition_entry(const jet_config_t* cfg, int64_t n, const int64_t* rp64, const int32_t* rp32,
                             const int32_t* col, const int32_t* ew32, const int64_t* ew64, const int32_t* vw32,
                             const int64_t* vw64, int32_t uniform_ew, int32_t space, uint64_t seed, int32_t* part_out,
-                            int64_t* cut_out, jet_stats_t* stats) {
+                            int64_t* cut_out, jet_stats_t* stats, const int64_t* col64 = nullptr) {
     check_config(cfg);
     if (!part_out || !cut_out) throw InvalidInput("part_out and cut_out are required");
     Context& C = context();
     CallScope call(C);
-    Level fine = stage_graph(C, n, rp64, rp32, col, ew32, ew64, vw32, vw64, space, uniform_ew != 0);
+    Level fine = stage_graph(C, n, rp64, rp32, col, ew32, ew64, vw32, vw64, space, uniform_ew != 0, true, col64);
     int32_t attempts = std::max(cfg->num_iter, 1);
     part_t* best = (space == JET_PTR_DEVICE && attempts == 1) ? part_out : C.persist.alloc<part_t>(n);
     part_t* trial = attempts > 1 ? C.persist.alloc<part_t>(n) : best;
@@ -436,6 +470,15 @@ int jet_partition_csr_big(const jet_config_t* cfg, int64_t n, const int64_t* row
     return guard_impl([&] {
         partition_entry(cfg, n, row_offsets, nullptr, col, nullptr, ew, nullptr, vw, uniform_ew, ptr_space, seed,
                         part_out, cut_out, stats);
+    });
+}
+
+int jet_partition_csr_biggest(const jet_config_t* cfg, int64_t n, const int64_t* row_offsets, const int64_t* col,
+                              const int64_t* ew, const int64_t* vw, int32_t uniform_ew, int32_t ptr_space,
+                              uint64_t seed, int32_t* part_out, int64_t* cut_out, jet_stats_t* stats) {
+    return guard_impl([&] {
+        partition_entry(cfg, n, row_offsets, nullptr, nullptr, nullptr, ew, nullptr, vw, uniform_ew, ptr_space, seed,
+                        part_out, cut_out, stats, col);
     });
 }
 

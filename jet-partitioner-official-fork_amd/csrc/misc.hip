@@ -16,32 +16,30 @@ __global__ void k_validate_offsets(vid_t n, const eoff_t* rowptr, eoff_t m, int3
     if (e) atomicOr(err, e);
 }
 
-// err bits: 1 offsets not monotone, 2 column out of range, 4 non-positive
-// edge weight, 8 non-positive vertex weight, 16 weight total overflows int32.
-__global__ void k_validate(vid_t n, const eoff_t* rowptr, const vid_t* col, const wgt_t* ew, const wgt_t* vw,
-                           int32_t* err, unsigned long long* sums) {
+// err bits: 2 column out of range, 4 non-positive edge weight, 8 non-positive
+// vertex weight, 16 weight total overflows int32 (set on the host). Flat
+// passes over vertices and over edges, so hubs do not serialise a thread.
+__global__ void k_validate(vid_t n, eoff_t m, const vid_t* col, const wgt_t* ew, const wgt_t* vw, int32_t* err,
+                           unsigned long long* sums) {
     __shared__ int64_t red[256 / WAVE];
     int64_t se = 0, sv = 0;
     int32_t e = 0;
     FOR_THREAD(i, n) {
-        eoff_t a = rowptr[i], b = rowptr[i + 1];
-        if (b < a) e |= 1;
         if (vw) {
             if (vw[i] <= 0) e |= 8;
             sv += vw[i];
         } else {
             sv += 1;
         }
-        if (b < a) continue;
-        for (eoff_t j = a; j < b; j++) {
-            vid_t c = col[j];
-            if (c < 0 || c >= n) e |= 2;
-            if (ew) {
-                if (ew[j] <= 0) e |= 4;
-                se += ew[j];
-            } else {
-                se += 1;
-            }
+    }
+    FOR_THREAD(j, m) {
+        vid_t c = col[j];
+        if (c < 0 || c >= n) e |= 2;
+        if (ew) {
+            if (ew[j] <= 0) e |= 4;
+            se += ew[j];
+        } else {
+            se += 1;
         }
     }
     if (e) atomicOr(err, e);
@@ -64,7 +62,7 @@ int32_t validate_graph(vid_t n, const eoff_t* rowptr, const vid_t* col, const wg
     k_validate_offsets<<<grid_for(n, 256, 4096), 256, 0, s>>>(n, rowptr, m, err);
     JET_LAUNCH_CHECK();
     if (read_scalar(err, s)) return 1;
-    k_validate<<<grid_for(n, 256, 4096), 256, 0, s>>>(n, rowptr, col, ew, vw, err, sums);
+    k_validate<<<grid_for(std::max<int64_t>(n, m), 256, 4096), 256, 0, s>>>(n, m, col, ew, vw, err, sums);
     JET_LAUNCH_CHECK();
     int32_t e = 0;
     unsigned long long h[2];
@@ -97,6 +95,25 @@ int32_t narrow_weights(int64_t cnt, const int64_t* in, int32_t* out, Arena& a, h
     int32_t* err = a.alloc<int32_t>(1);
     fill_i32(err, 0, 1, s);
     k_narrow<<<grid_for(cnt, 256, 8192), 256, 0, s>>>(cnt, in, out, err);
+    JET_LAUNCH_CHECK();
+    return read_scalar(err, s);
+}
+
+// Column narrowing with a range check: err |= 2 when a column is outside [0, n).
+__global__ void k_narrow_cols(int64_t cnt, const int64_t* in, int64_t n, int32_t* out, int32_t* err) {
+    FOR_THREAD(i, cnt) {
+        int64_t v = in[i];
+        if (v < 0 || v >= n) atomicOr(err, 2);
+        out[i] = (int32_t)v;
+    }
+}
+
+int32_t narrow_columns(int64_t cnt, const int64_t* in, int64_t n, int32_t* out, Arena& a, hipStream_t s) {
+    if (cnt <= 0) return 0;
+    ArenaScope sc(a);
+    int32_t* err = a.alloc<int32_t>(1);
+    fill_i32(err, 0, 1, s);
+    k_narrow_cols<<<grid_for(cnt, 256, 8192), 256, 0, s>>>(cnt, in, n, out, err);
     JET_LAUNCH_CHECK();
     return read_scalar(err, s);
 }

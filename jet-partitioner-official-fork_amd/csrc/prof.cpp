@@ -24,19 +24,17 @@ static hipEvent_t take_event() {
 
 unsigned long long* prof_counters(int fam) { return prof_on(fam) ? prof().dev : nullptr; }
 
-void prof_begin(int fam, hipStream_t s) {
-    if (!prof_on(fam)) return;
+std::pair<hipEvent_t, hipEvent_t> prof_events(int fam) {
+    if (!prof_on(fam)) return {nullptr, nullptr};
     hipEvent_t a = take_event();
-    JET_HIP(hipEventRecord(a, s));
-    prof().pending.push_back({a, nullptr});
+    hipEvent_t b = take_event();
+    prof().pending.push_back({a, b});
+    return {a, b};
 }
 
-void prof_end(int fam, hipStream_t s, double host_bytes) {
+void prof_account(int fam, double host_bytes) {
     if (!prof_on(fam)) return;
     Prof& P = prof();
-    hipEvent_t b = take_event();
-    JET_HIP(hipEventRecord(b, s));
-    P.pending.back().second = b;
     P.host_bytes += host_bytes;
     P.launches++;
 }

@@ -1,5 +1,5 @@
-// Per-kernel timing for measurement tools: HIP events on the library's stream
-// around every launch of one kernel family, plus device-side item counters
+// Per-kernel timing for measurement tools: HIP events bound to every launch of
+// one kernel family on the library's stream, plus device-side item counters
 // from which the family's algorithmic bytes are computed.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -25,8 +25,13 @@ Prof& prof();
 inline bool prof_on(int fam) { return prof().family == fam; }
 // Device counters when `fam` is being timed, else nullptr.
 unsigned long long* prof_counters(int fam);
-void prof_begin(int fam, hipStream_t s);
-void prof_end(int fam, hipStream_t s, double host_bytes);
+// Event pair for the next launch of `fam` ({nullptr, nullptr} when the family
+// is not being timed). The launch passes them to hipExtLaunchKernelGGL, which
+// stamps them at the kernel's own start and end, so host submission gaps on
+// an idle stream are not counted.
+std::pair<hipEvent_t, hipEvent_t> prof_events(int fam);
+// Accounts one launch of `fam` whose events came from prof_events.
+void prof_account(int fam, double host_bytes);
 // Enables a family by name; returns false for an unknown name.
 bool prof_enable(const std::string& name);
 // Synchronizes, then returns launches, total milliseconds and algorithmic bytes.

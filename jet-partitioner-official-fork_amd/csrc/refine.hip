@@ -18,6 +18,8 @@
 
 #include "prims.h"
 #include "prof.h"
+
+#include <hip/hip_ext.h>
 #include "refine.h"
 #include "wave.h"
 
@@ -60,12 +62,11 @@ __device__ inline vid_t list_at(const vid_t* list, int32_t nsmall, vid_t nmax, i
 // ---------------------------------------------------------------- statistics
 __global__ void k_cut2(DGraph g, const part_t* part, unsigned long long* out) {
     __shared__ int64_t red[256 / WAVE];
+    __shared__ EdgeWalkLds walk;
     int64_t c = 0;
-    FOR_THREAD(i, g.n) {
-        part_t p = part[i];
-        for (eoff_t j = g.rowptr[i]; j < g.rowptr[i + 1]; j++)
-            if (part[g.col[j]] != p) c += g.ew[j];
-    }
+    block_edges<256>(g.rowptr, g.n, g.m, walk, [&](vid_t r, eoff_t j) {
+        if (part[g.col[j]] != part[r]) c += g.ew[j];
+    });
     c = block_reduce_sum<int64_t, 256>(c, red);
     if (threadIdx.x == 0 && c) atomicAdd(out, (unsigned long long)c);
 }
@@ -74,8 +75,8 @@ int64_t total_cut2(const DGraph& g, const part_t* part, Arena& a, hipStream_t s)
     ArenaScope sc(a);
     unsigned long long* out = a.alloc<unsigned long long>(1);
     fill_i64(reinterpret_cast<int64_t*>(out), 0, 1, s);
-    if (g.n > 0) {
-        k_cut2<<<grid_for(g.n, 256, 8192), 256, 0, s>>>(g, part, out);
+    if (g.n > 0 && g.m > 0) {
+        k_cut2<<<grid_for(g.m, EB_TILE, 8192), 256, 0, s>>>(g, part, out);
         JET_LAUNCH_CHECK();
     }
     return (int64_t)read_scalar(out, s);
@@ -494,8 +495,7 @@ __global__ void k_afterburner_big(DGraph g, const vid_t* cand, const int32_t* co
 // jet_refiner.hpp:905-955.
 __global__ void k_moves_apply(DGraph g, const vid_t* moves, int32_t* mc, vid_t nmax, const wgt_t* vw, part_t* part,
                               part_t* dest, part_t* dest_cache, const eoff_t* conn_off, const int2* conn,
-                              const int32_t* conn_size, unsigned long long* sizes, int32_t k, int64_t* acc,
-                              int32_t* mark) {
+                              const int32_t* conn_size, unsigned long long* sizes, int32_t k, int64_t* acc) {
     __shared__ int64_t red[256 / WAVE];
     __shared__ unsigned long long ls[LDS_K];
     const bool lds = k <= LDS_K;
@@ -504,13 +504,10 @@ __global__ void k_moves_apply(DGraph g, const vid_t* moves, int32_t* mc, vid_t n
         __syncthreads();
     }
     const int32_t ns = mc[2], total = mc[2] + mc[3];
-    // perform_moves' choice (jet_refiner.hpp:933): update_large when more
-    // than n/10 vertices move; update_large rebuilds the neighbours' tables
-    const bool large = total > g.n / 10;
-    if (blockIdx.x == 0 && threadIdx.x == 0) mc[6] = large ? 1 : 0;
-    int64_t cc = 0;
+    int64_t cc = 0, sdeg = 0;
     FOR_THREAD(x, total) {
         vid_t v = list_at(moves, ns, nmax, x);
+        sdeg += g.rowptr[v + 1] - g.rowptr[v];
         part_t p = part[v], b = dest[v];
         const int2* tab = conn + conn_off[v];
         int32_t sz = conn_size[v];
@@ -526,11 +523,11 @@ __global__ void k_moves_apply(DGraph g, const vid_t* moves, int32_t* mc, vid_t n
         }
         part[v] = b;
         dest[v] = p;
-        if (large)
-            for (eoff_t j = g.rowptr[v]; j < g.rowptr[v + 1]; j++) mark[g.col[j]] = 1;
     }
     cc = block_reduce_sum<int64_t, 256>(cc, red);
+    sdeg = block_reduce_sum<int64_t, 256>(sdeg, red);
     if (threadIdx.x == 0 && cc) atomicAdd((unsigned long long*)&acc[0], (unsigned long long)cc);
+    if (threadIdx.x == 0 && sdeg) atomicAdd((unsigned long long*)&acc[5], (unsigned long long)sdeg);
     if (lds) {
         __syncthreads();
         for (int t = threadIdx.x; t < k; t += blockDim.x)
@@ -632,16 +629,37 @@ __device__ inline void update_edge(vid_t v, int32_t w, part_t q, const eoff_t* c
     }
 }
 
-// ADD == false: subtract from the old part (held in dest); ADD == true: add to
-// the new part (held in part). G lanes stride the edges of one move.
+// perform_moves' choice between update_small (push each moved vertex's
+// contribution into its neighbours' tables) and update_large (rebuild the
+// tables of every neighbour of a moved vertex). The reference rebuilds when
+// more than n/10 vertices move (jet_refiner.hpp:933). Both leave the same
+// part -> connectivity content and every decision reads tables by content, so
+// the rebuild is also taken when the moved vertices' adjacency exceeds m/8:
+// pushing that many atomics into hub tables serialises on a few hot slots,
+// while the rebuild streams each touched row once. acc[5] holds the degree
+// sum of the moves (k_moves_apply).
+__device__ inline bool update_by_rebuild(const int32_t* mc, const int64_t* acc, vid_t n, eoff_t m) {
+    const int32_t total = mc[2] + mc[3];
+    return total > n / 10 || 8 * __hip_atomic_load(&acc[5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > m;
+}
+
+// ADD == false: subtract from the old part (held in dest), or, when the tables
+// are rebuilt instead, mark the neighbours; ADD == true: add to the new part
+// (held in part). G lanes stride the edges of one move.
 template <bool ADD, int G>
-__global__ void k_update_small(DGraph g, const vid_t* moves, const int32_t* mc, vid_t nmax, const part_t* part,
+__global__ void k_update_small(DGraph g, const vid_t* moves, int32_t* mc, vid_t nmax, const part_t* part,
                                const part_t* dest, const eoff_t* conn_off, int2* conn, int32_t* conn_size,
-                               part_t* dest_cache, int32_t k, int32_t* err) {
-    if (mc[6] != 0) return;  // update_large chosen
+                               part_t* dest_cache, int32_t k, int32_t* err, const int64_t* acc, int32_t* mark) {
+    const bool rebuild = update_by_rebuild(mc, acc, g.n, g.m);
+    if (ADD && rebuild) return;
+    if (!ADD && rebuild && blockIdx.x == 0 && threadIdx.x == 0) mc[6] = 1;  // gates the rebuild
     const int32_t ns = mc[2];
     FOR_GROUP(x, ns) {
         vid_t i = moves[x];
+        if (!ADD && rebuild) {
+            for (eoff_t j = g.rowptr[i] + sub_; j < g.rowptr[i + 1]; j += G) mark[g.col[j]] = 1;
+            continue;
+        }
         part_t q = ADD ? part[i] : dest[i];
         for (eoff_t j = g.rowptr[i] + sub_; j < g.rowptr[i + 1]; j += G)
             update_edge<ADD>(g.col[j], g.ew[j], q, conn_off, conn, conn_size, dest_cache, k, err);
@@ -651,13 +669,18 @@ __global__ void k_update_small(DGraph g, const vid_t* moves, const int32_t* mc, 
 template <bool ADD>
 __global__ void k_update_big(DGraph g, const vid_t* moves, const int32_t* mc, vid_t nmax, const part_t* part,
                              const part_t* dest, const eoff_t* conn_off, int2* conn, int32_t* conn_size,
-                             part_t* dest_cache, int32_t k, int32_t* err) {
-    if (mc[6] != 0) return;  // update_large chosen
+                             part_t* dest_cache, int32_t k, int32_t* err, const int64_t* acc, int32_t* mark) {
+    const bool rebuild = update_by_rebuild(mc, acc, g.n, g.m);
+    if (ADD && rebuild) return;
     const int32_t nb = mc[3];
     const int lane = threadIdx.x & (WAVE - 1);
     const int64_t wstride = ((int64_t)gridDim.x * blockDim.x) / WAVE;
     for (int64_t x = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE; x < nb; x += wstride) {
         vid_t i = moves[nmax - 1 - x];
+        if (!ADD && rebuild) {
+            for (eoff_t j = g.rowptr[i] + lane; j < g.rowptr[i + 1]; j += WAVE) mark[g.col[j]] = 1;
+            continue;
+        }
         part_t q = ADD ? part[i] : dest[i];
         for (eoff_t j = g.rowptr[i] + lane; j < g.rowptr[i + 1]; j += WAVE)
             update_edge<ADD>(g.col[j], g.ew[j], q, conn_off, conn, conn_size, dest_cache, k, err);
@@ -700,6 +723,7 @@ __global__ void k_moves_finish(const vid_t* moves, const int32_t* mc, vid_t nmax
             acc[0] = 0;
             acc[1] = 0;
             acc[4] = 0;
+            acc[5] = 0;
         }
     }
 }
@@ -1328,13 +1352,14 @@ void Refiner::jet_lp(const Level& L, const part_t* part, double fr) {
     DGraph g = L.view();
     int32_t* c = cnt();
     int nb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_WAVES * LS_CHW));
-    prof_begin(PF_LP_SELECT, s_);
-    k_lp_select<<<nb, LS_WAVES * WAVE, 0, s_>>>(g, part, conn_off_, conn_, conn_size_, dest_cache_, gain_p_, lock_, fr,
-                                                dest_, pregain_, cand_, c, nmax_, cnt_next(),
-                                                prof_counters(PF_LP_SELECT));
+    auto ev = prof_events(PF_LP_SELECT);
+    hipExtLaunchKernelGGL(k_lp_select, dim3(nb), dim3(LS_WAVES * WAVE), 0, s_, ev.first, ev.second, 0, g, part,
+                          (const int64_t*)conn_off_, (const int2*)conn_, (const int32_t*)conn_size_,
+                          dest_cache_, gain_p_, lock_, fr, dest_, pregain_, cand_, c, nmax_, cnt_next(),
+                          prof_counters(PF_LP_SELECT));
     JET_LAUNCH_CHECK();
     // per vertex: cache read 4, destination write 4, lock read 4, pre-gain write 4
-    prof_end(PF_LP_SELECT, s_, 16.0 * g.n);
+    prof_account(PF_LP_SELECT, 16.0 * g.n);
     unsigned ab = (unsigned)std::max<int64_t>(256, ceil_div((int64_t)g.n, (int64_t)AB_WAVES * AB_CHW));
     switch (group_for_avg(g)) {
         case 4: k_afterburner_small<4><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_); break;
@@ -1444,22 +1469,21 @@ void Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
     DGraph g = L.view();
     const int32_t k = p_.k;
     int32_t* c = cnt();
-    // moves are at most n/10 on the update_small path; a modest grid keeps the
+    // moves are at most n/10 on the push path; a modest grid keeps the
     // completion ticket of k_moves_finish cheap
     int nb = grid_for(g.n, 256, 512);
     k_moves_apply<<<nb, 256, 0, s_>>>(g, moves_, c, nmax_, g.vw, part, dest_, dest_cache_, conn_off_, conn_,
-                                      conn_size_, reinterpret_cast<unsigned long long*>(sizes), k, acc_, swap_bit_);
+                                      conn_size_, reinterpret_cast<unsigned long long*>(sizes), k, acc_);
     JET_LAUNCH_CHECK();
-    init_conn(L, part, swap_bit_);  // update_large: rebuild marked rows (gated on the device)
     int G = group_for_avg(g);
     int nbs = grid_for((int64_t)g.n * G, 256, 4096);
     int nbb = grid_for((int64_t)g.n * WAVE, 256, 2048);
     for (int add = 0; add < 2; add++) {
 #define JET_UPD(GG)                                                                                                   \
     if (add) k_update_small<true, GG><<<nbs, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,       \
-                                                          conn_size_, dest_cache_, k, misc_ + 1);                    \
+                                                          conn_size_, dest_cache_, k, misc_ + 1, acc_, swap_bit_);   \
     else k_update_small<false, GG><<<nbs, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,          \
-                                                        conn_size_, dest_cache_, k, misc_ + 1);
+                                                        conn_size_, dest_cache_, k, misc_ + 1, acc_, swap_bit_);
         switch (G) {
             case 4: JET_UPD(4) break;
             case 8: JET_UPD(8) break;
@@ -1470,12 +1494,13 @@ void Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
 #undef JET_UPD
         if (has_big_) {
             if (add) k_update_big<true><<<nbb, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,
-                                                             conn_size_, dest_cache_, k, misc_ + 1);
+                                                             conn_size_, dest_cache_, k, misc_ + 1, acc_, swap_bit_);
             else k_update_big<false><<<nbb, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,
-                                                           conn_size_, dest_cache_, k, misc_ + 1);
+                                                           conn_size_, dest_cache_, k, misc_ + 1, acc_, swap_bit_);
         }
         JET_LAUNCH_CHECK();
     }
+    init_conn(L, part, swap_bit_);  // rebuild marked rows (gated on the device)
     k_moves_finish<<<nb, 256, 0, s_>>>(moves_, c, nmax_, part, dest_, conn_off_, conn_, conn_size_, sizes, k, acc_,
                                        res_dev_);
     JET_LAUNCH_CHECK();

@@ -164,4 +164,60 @@ __device__ inline T block_excl_scan(T x, T& total, T* lds) {
     return res;
 }
 
+// Largest r in [lo, hi] with rowptr[r] <= e: the row holding edge e when
+// rowptr[lo] <= e < rowptr[hi + 1].
+__device__ inline vid_t row_of(const eoff_t* rowptr, vid_t lo, vid_t hi, eoff_t e) {
+    while (lo < hi) {
+        vid_t mid = lo + (hi - lo + 1) / 2;
+        if (rowptr[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Edge-balanced CSR walk, independent of the degree distribution: workgroups
+// take tiles of EB_TILE consecutive edges (grid-stride) and find each edge's
+// row by binary search over the rows the tile spans, held in LDS when at most
+// EB_SEG of them (a tile over many empty rows searches global memory). Calls
+// f(row, edge) for every edge. All threads of the workgroup must call it.
+constexpr int EB_TILE = 2048;
+constexpr int EB_SEG = 1024;
+struct EdgeWalkLds {
+    eoff_t seg[EB_SEG];
+    vid_t r[2];
+};
+template <int BLOCK, class F>
+__device__ inline void block_edges(const eoff_t* rowptr, vid_t n, eoff_t m, EdgeWalkLds& L, F&& f) {
+    for (int64_t t0 = blockIdx.x * (int64_t)EB_TILE; t0 < m; t0 += (int64_t)gridDim.x * EB_TILE) {
+        const int64_t t1 = t0 + EB_TILE < m ? t0 + EB_TILE : m;
+        if (threadIdx.x == 0) {
+            L.r[0] = row_of(rowptr, 0, n - 1, t0);
+            L.r[1] = row_of(rowptr, L.r[0], n - 1, t1 - 1);
+        }
+        __syncthreads();
+        const vid_t r0 = L.r[0], r1 = L.r[1];
+        const int32_t span = r1 - r0 + 1;
+        const bool in_lds = span <= EB_SEG;
+        if (in_lds)
+            for (int q = threadIdx.x; q < span; q += BLOCK) L.seg[q] = rowptr[r0 + q];
+        __syncthreads();
+        for (int64_t e = t0 + threadIdx.x; e < t1; e += BLOCK) {
+            vid_t r;
+            if (in_lds) {
+                int32_t lo = 0, hi = span - 1;
+                while (lo < hi) {
+                    int32_t mid = (lo + hi + 1) >> 1;
+                    if (L.seg[mid] <= e) lo = mid;
+                    else hi = mid - 1;
+                }
+                r = r0 + lo;
+            } else {
+                r = row_of(rowptr, r0, r1, e);
+            }
+            f(r, e);
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace jet

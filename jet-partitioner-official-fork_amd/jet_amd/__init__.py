@@ -75,6 +75,8 @@ def lib():
                                     c.POINTER(Stats)],
             "jet_partition_csr_big": [c.POINTER(Config), i64, VP, VP, VP, VP, i32, i32, u64, VP, c.POINTER(i64),
                                       c.POINTER(Stats)],
+            "jet_partition_csr_biggest": [c.POINTER(Config), i64, VP, VP, VP, VP, i32, i32, u64, VP,
+                                          c.POINTER(i64), c.POINTER(Stats)],
             "jet_coarsen_map": [i32, i32, i32, u64, i64, VP, VP, VP, VP, i32, VP, c.POINTER(i32)],
             "jet_hec_heavy_neighbours": [i32, i32, u64, i64, VP, VP, VP, VP, i32, VP],
             "jet_map_construct": [i64, VP, i32, VP, c.POINTER(i32)],
@@ -160,7 +162,12 @@ def _partition_named(g, config, uniform_ew, seed, big):
     part = np.empty(n, dtype=np.int32)
     cut = ctypes.c_int64(0)
     st = Stats()
-    if big:
+    if big == "biggest":
+        rc = lib().jet_partition_csr_biggest(ctypes.byref(config), n, _p(_c(rp, np.int64)), _p(_c(g["col"], np.int64)),
+                                             _p(_c(g.get("ew"), np.int64)), _p(_c(g.get("vw"), np.int64)),
+                                             int(uniform_ew), PTR_HOST, seed, _p(part), ctypes.byref(cut),
+                                             ctypes.byref(st))
+    elif big:
         ew = _c(g.get("ew"), np.int64)
         vw = _c(g.get("vw"), np.int64)
         rc = lib().jet_partition_csr_big(ctypes.byref(config), n, _p(_c(rp, np.int64)), _p(_c(g["col"], np.int32)),
@@ -189,6 +196,15 @@ def partition_big(g, config, uniform_ew=False, seed=0):
 
 partition_host_big = partition_big
 partition_serial_big = partition_big
+
+
+def partition_biggest(g, config, uniform_ew=False, seed=0):
+    """64-bit ordinals (header/jet.h:88-107); n must be below 2^31 - 2."""
+    return _partition_named(g, config, uniform_ew, seed, "biggest")
+
+
+partition_host_biggest = partition_biggest
+partition_serial_biggest = partition_biggest
 
 
 # ---- stage entry points

@@ -107,3 +107,24 @@ def path(n):
 def clique(n):
     u, v = np.triu_indices(n, 1)
     return from_edges(n, u, v)
+
+
+def heavy_bipartite(hubs, leaves, seed=0, density=0.5, hub_w=1000, leaf_pairs=False):
+    """Hubs of vertex weight hub_w joined to unit-weight leaves at random, edge
+    weights in [1, 5]. Weighted HEC (heuristics.hpp:211-248) maps every vertex
+    to itself here once leaves only see hubs, so coarsening reaches a fixed
+    point and runs to the 200-level limit (contract.hpp:508-527). With
+    leaf_pairs, leaves 2i and 2i+1 are also joined, so the first level merges
+    them and the fixed point starts one level later."""
+    rng = np.random.default_rng(seed)
+    hu, lv = np.nonzero(rng.random((hubs, leaves)) < density)
+    u = list(hu)
+    v = list(hubs + lv)
+    if leaf_pairs:
+        for i in range(0, leaves - 1, 2):
+            u.append(hubs + i)
+            v.append(hubs + i + 1)
+    g = from_edges(hubs + leaves, np.array(u), np.array(v), w=rng.integers(1, 6, len(u)))
+    g["vw"] = np.ones(hubs + leaves, dtype=np.int32)
+    g["vw"][:hubs] = hub_w
+    return g

@@ -3,6 +3,7 @@ its host generators produce the measured workloads, and it reports a clean
 error (no crash) when no GPU is present."""
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -78,3 +79,24 @@ def test_missing_gpu_is_a_clean_error():
     with pytest.raises(jet_amd.JetError) as e:
         jet_amd.partition_csr(g, jet_amd.Config(num_parts=2))
     assert e.value.code == jet_amd.JET_EDEVICE
+
+
+def build_jet_h_example(tmp_path):
+    """Compiles tests/cpp/jet_h_partitions.cpp against include/jet.h."""
+    src = os.path.join(ROOT, "tests", "cpp", "jet_h_partitions.cpp")
+    exe = str(tmp_path / "jet_h_partitions")
+    pkg = os.path.join(ROOT, "jet-partitioner-official-fork_amd")
+    cmd = ["g++", "-O1", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           "-I" + os.path.join(ROOT, "include"), src, "-o", exe, "-L" + pkg, "-ljet_amd", "-Wl,-rpath," + pkg,
+           "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"]
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    return exe
+
+
+def test_jet_h_mirror_compiles_and_reports_no_device(tmp_path):
+    if jet_amd.device_count() > 0:
+        pytest.skip("a GPU is present")
+    exe = build_jet_h_example(tmp_path)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 3
+    assert r.stdout.startswith("error -4 ")

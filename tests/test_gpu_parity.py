@@ -1,5 +1,6 @@
 """GPU parity: every hot-path stage through the C-ABI against the CPU oracle
 on the same inputs, bit-exact (integer work), then end-to-end partitions."""
+import subprocess
 import sys
 
 import numpy as np
@@ -184,6 +185,8 @@ E2E = [
     ("rmat-hec", lambda: graphs.rmat(14, 16, seed=0), 256, 1.10, 1),
     ("rmat-2hop", lambda: graphs.rmat(13, 8, seed=3), 32, 1.03, 0),
     ("wrmat-hec", lambda: graphs.with_weights(graphs.rmat(12, 8, seed=4), seed=2, wmax=4, vwmax=4), 16, 1.03, 1),
+    ("hec-fixed-point", lambda: graphs.heavy_bipartite(6, 600, seed=3), 4, 1.10, 1),
+    ("hec-fixed-point-late", lambda: graphs.heavy_bipartite(8, 900, seed=4, leaf_pairs=True), 4, 1.10, 1),
 ]
 
 
@@ -202,6 +205,7 @@ def test_partition_end_to_end_bit_exact(name, mk, k, imb, alg):
         for L in range(st.num_levels):
             assert st.levels[L].num_vertices == elv[L][2]
             assert st.levels[L].total_iterations == elv[L][4]
+            assert 2 * st.levels[L].edge_cut == elv[L][0]
         assert cut == oracle.total_cut(g, part) // 2
         sizes = np.bincount(part, weights=g["vw"], minlength=k)
         opt = (int(g["vw"].sum()) + k - 1) // k
@@ -234,3 +238,30 @@ def test_invalid_inputs_rejected():
     neg["ew"][0] = -1
     with pytest.raises(jet_amd.JetError):
         jet_amd.partition_csr(neg, jet_amd.Config(num_parts=2))
+
+
+def _fnv(part):
+    h = 1469598103934665603
+    for v in part.tolist():
+        h = ((h ^ (v & 0xFFFFFFFF)) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def test_jet_h_nine_entry_points(tmp_path):
+    """include/jet.h: the nine reference names (header/jet.h:46-107), host and
+    device variants, all widths, give the partition the C-ABI gives."""
+    from test_capi_cpu import build_jet_h_example
+    exe = build_jet_h_example(tmp_path)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = [ln.split() for ln in r.stdout.strip().splitlines()]
+    assert [ln[0] for ln in lines] == ["partition_host", "partition_serial", "partition_host_big",
+                                       "partition_serial_big", "partition_host_biggest", "partition_serial_biggest",
+                                       "partition", "partition_big", "partition_biggest"]
+    g = graphs.grid2d(96, 64)
+    rp, col = g["rowptr"], g["col"]
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    g["ew"] = (1 + (7 * np.minimum(rows, col) + np.maximum(rows, col)) % 4).astype(np.int32)
+    part, cut = jet_amd.partition_csr(g, jet_amd.Config(num_parts=8, coarsening_alg=1), seed=5, uniform=False)
+    for ln in lines:
+        assert int(ln[1]) == cut and int(ln[2]) == _fnv(part)

@@ -147,3 +147,20 @@ def test_hec_map_invariants(name, mk):
         # an aggregate that others join through non-heavy vertices
         hn = oracle.hec_hn(g, False, 3, 0)
         assert np.all(~heavy[hn] | (hn == np.arange(n)))
+
+
+@pytest.mark.parametrize("pairs", [False, True])
+def test_hec_fixed_point_runs_to_level_limit(pairs):
+    """Weighted HEC leaves every vertex alone once leaves only see hubs heavier
+    than 6x the mean vertex weight (heuristics.hpp:217-243), so the level loop
+    stops only at its 200-level limit (contract.hpp:508-527): 201 levels, all
+    of one size after the first merge."""
+    g = graphs.heavy_bipartite(8, 900, seed=4, leaf_pairs=pairs)
+    part, cut, lv = oracle.partition(g, 4, 1.10, alg=1, seed=0)
+    assert len(lv) == 201
+    sizes = {int(r[2]) for r in lv[:-1]}
+    assert len(sizes) == 1
+    n = len(g["rowptr"]) - 1
+    assert int(lv[-1][2]) == n
+    assert (int(lv[0][2]) == n) == (not pairs)
+    assert cut == oracle.total_cut(g, part) // 2
