@@ -23,6 +23,7 @@ extern "C" {
 
 /* Return codes. */
 #define JET_OK 0
+#define JET_EAGAIN 1         /* jet_poll: the job has not finished yet */
 #define JET_EINVAL (-1)      /* malformed CSR, bad config, k out of range */
 #define JET_ENOMEM (-2)      /* device allocation failed */
 #define JET_EINIT (-3)       /* initial partitioner failed (the reference exits, initial_partition.hpp:106-109) */
@@ -202,6 +203,58 @@ int jet_generate_stencil3d(int64_t nx, int64_t ny, int64_t nz, uint64_t seed, in
  * hold 2*ef*2^scale entries; *m_out receives the stored edge count. */
 int jet_generate_rmat(int32_t scale, int32_t ef, double a, double b, double c, uint64_t seed, int64_t* rowptr,
                       int32_t* col, int64_t* m_out);
+
+/* ---- Batched and served partitioning (one GPU, several jobs in flight).
+ *
+ * A context owns `workers` GPU workers, each with its own HIP stream and
+ * device memory, and one host thread per worker. Jobs queue in submission
+ * order and start when their estimated device footprint fits the context's
+ * budget beside the running jobs, so jobs overlap on the device within a
+ * memory bound. Each job runs exactly jet_partition_csr with its own seed on
+ * private state, so its result equals the same call made alone. The
+ * reference has no batched or served mode: these entry points have no
+ * counterpart in it and serve callers that partition many graphs. */
+#define JET_MAX_WORKERS 16
+
+typedef struct jet_ctx jet_ctx_t;
+
+/* Seconds since the context was created. */
+typedef struct jet_job_times_t {
+    double submit_s;
+    double start_s;
+    double end_s;
+} jet_job_times_t;
+
+/* workers in [1, 16]; budget_bytes <= 0 means 80% of the device's free memory. */
+int jet_ctx_create(int32_t device, int32_t workers, int64_t budget_bytes, jet_ctx_t** out);
+/* Waits for running jobs, then frees the context (uncollected results are dropped). */
+int jet_ctx_destroy(jet_ctx_t* ctx);
+
+/* Queues one job with jet_partition_csr's arguments. The arrays (host or
+ * device per ptr_space) and part_out must stay valid until jet_poll returns
+ * the job's result. JET_ENOMEM when the job alone exceeds the budget. */
+int jet_submit(jet_ctx_t* ctx, const jet_config_t* cfg, int64_t n, const int64_t* row_offsets, const int32_t* col,
+               const int32_t* ew, const int32_t* vw, int32_t uniform_ew, int32_t ptr_space, uint64_t seed,
+               int32_t* part_out, int64_t* ticket_out);
+
+/* Result of a job: JET_OK (writes *cut_out, stats, times; the ticket is then
+ * consumed), the job's error code, or JET_EAGAIN while it is queued or
+ * running. wait_ms: 0 polls, > 0 waits up to that long, < 0 waits until done. */
+int jet_poll(jet_ctx_t* ctx, int64_t ticket, int32_t wait_ms, int64_t* cut_out, jet_stats_t* stats,
+             jet_job_times_t* times);
+
+/* Partitions `count` graphs given as graph-tagged concatenated arrays and
+ * waits for all of them. Graph g owns vertices [vtx_offsets[g],
+ * vtx_offsets[g+1]) of vw / part_out and edges [edge_offsets[g],
+ * edge_offsets[g+1]) of col / ew; its n_g + 1 local row offsets (starting at
+ * 0) are at row_offsets + vtx_offsets[g] + g; its columns are local ids.
+ * cfgs[g], seeds[g] (NULL: 0) and uniform_ew[g] (NULL: 0) are per graph;
+ * part_out receives local part ids, cuts_out[g] the cut, stats[g] (NULL: not
+ * written) the record. Returns the first failing graph's code. */
+int jet_partition_batch(jet_ctx_t* ctx, int32_t count, const jet_config_t* cfgs, const uint64_t* seeds,
+                        const int64_t* vtx_offsets, const int64_t* edge_offsets, const int64_t* row_offsets,
+                        const int32_t* col, const int32_t* ew, const int32_t* vw, const int32_t* uniform_ew,
+                        int32_t ptr_space, int32_t* part_out, int64_t* cuts_out, jet_stats_t* stats);
 
 /* ---- Kernel timing for measurement tools: when enabled, the library records
  * HIP events around every launch of the named kernel family on its stream

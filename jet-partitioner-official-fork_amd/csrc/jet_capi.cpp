@@ -19,6 +19,7 @@
 
 #include "arena.h"
 #include "coarsen.h"
+#include "engine.h"
 #include "contract.h"
 #include "jet_capi.h"
 #include "misc.h"
@@ -38,31 +39,39 @@ struct InitFailure : std::runtime_error {
     explicit InitFailure(const std::string& s) : std::runtime_error(s) {}
 };
 
-struct Context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    Arena persist;
-    Arena scratch;
-    std::mutex mu;
-};
+Context* context_create(int device) {
+    int cnt = 0;
+    hipError_t e = hipGetDeviceCount(&cnt);
+    if (e != hipSuccess || cnt < 1) {
+        (void)hipGetLastError();
+        throw DeviceError("no HIP device available");
+    }
+    if (device < 0 || device >= cnt) throw InvalidInput("device index out of range");
+    JET_HIP(hipSetDevice(device));
+    Context* c = new Context();
+    c->device = device;
+    JET_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    return c;
+}
+
+void context_destroy(Context* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->persist.free_all();
+    c->scratch.free_all();
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
 
 static std::mutex g_ctx_mu;
 static Context* g_ctx = nullptr;
 
+// The process-wide context of the plain entry points (device 0).
 static Context& context() {
     std::lock_guard<std::mutex> lk(g_ctx_mu);
-    if (!g_ctx) {
-        int cnt = 0;
-        hipError_t e = hipGetDeviceCount(&cnt);
-        if (e != hipSuccess || cnt < 1) {
-            (void)hipGetLastError();
-            throw DeviceError("no HIP device available");
-        }
-        Context* c = new Context();
-        JET_HIP(hipSetDevice(0));
-        JET_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        g_ctx = c;
-    }
+    if (!g_ctx) g_ctx = context_create(0);
+    JET_HIP(hipSetDevice(g_ctx->device));
     return *g_ctx;
 }
 
@@ -290,8 +299,24 @@ static AttemptResult partition_once(Context& C, const jet_config_t& cfg, const L
         ArenaScope rscope(C.scratch);
         int64_t cap = conn_capacity(fine.n, fine.rowptr, k, C.scratch, s);
         Refiner ref(C.scratch, s, fine.n, cap, rp);
+        // A level that shares its graph with the level just refined (the
+        // coarsening fixed point) starts from that level's output partition
+        // and refine_data. When that level changed neither, this level
+        // repeats the same deterministic computation from the same state, so
+        // its record is the same and the partition stays as it is.
+        bool unchanged = false;
+        LevelRecord last;
         for (size_t L = levels.size(); L-- > 0;) {
-            LevelRecord rec = ref.refine(levels[L], cur);
+            const bool same_graph = L + 1 < levels.size() && levels[L + 1].map == nullptr;
+            LevelRecord rec;
+            if (same_graph && unchanged) {
+                rec = last;
+                rec.seconds = 0;
+            } else {
+                rec = ref.refine(levels[L], cur);
+                unchanged = !ref.changed_last();
+                last = rec;
+            }
             if (cfg.verbose && L + 1 == levels.size())
                 std::printf("Initial %lld %.6f %d\n", (long long)(rec.cut2 / 2), 0.0, (int)levels[L].n);
             R.recs.push_back(rec);
@@ -365,10 +390,11 @@ static void check_config(const jet_config_t* cfg) {
 static void partition_entry(const jet_config_t* cfg, int64_t n, const int64_t* rp64, const int32_t* rp32,
                             const int32_t* col, const int32_t* ew32, const int64_t* ew64, const int32_t* vw32,
                             const int64_t* vw64, int32_t uniform_ew, int32_t space, uint64_t seed, int32_t* part_out,
-                            int64_t* cut_out, jet_stats_t* stats, const int64_t* col64 = nullptr) {
+                            int64_t* cut_out, jet_stats_t* stats, const int64_t* col64 = nullptr,
+                            Context* ctx = nullptr) {
     check_config(cfg);
     if (!part_out || !cut_out) throw InvalidInput("part_out and cut_out are required");
-    Context& C = context();
+    Context& C = ctx ? *ctx : context();
     CallScope call(C);
     Level fine = stage_graph(C, n, rp64, rp32, col, ew32, ew64, vw32, vw64, space, uniform_ew != 0, true, col64);
     int32_t attempts = std::max(cfg->num_iter, 1);
@@ -419,6 +445,17 @@ static int guard_impl(const std::function<void()>& f) {
         return JET_EDEVICE;
     }
 }
+int partition_in_context(Context& C, const jet_config_t* cfg, int64_t n, const int64_t* row_offsets,
+                         const int32_t* col, const int32_t* ew, const int32_t* vw, int32_t uniform_ew,
+                         int32_t ptr_space, uint64_t seed, int32_t* part_out, int64_t* cut_out, jet_stats_t* stats) {
+    return guard_impl([&] {
+        partition_entry(cfg, n, row_offsets, nullptr, col, ew, nullptr, vw, nullptr, uniform_ew, ptr_space, seed,
+                        part_out, cut_out, stats, nullptr, &C);
+    });
+}
+
+void set_last_error(const std::string& msg) { g_err = msg; }
+
 }  // namespace jet
 
 extern "C" {

@@ -1,6 +1,8 @@
 // Kernel-family timing (see prof.h).
 #include "prof.h"
 
+#include <vector>
+
 #include "jet_types.h"
 
 namespace jet {
@@ -10,26 +12,13 @@ Prof& prof() {
     return *p;
 }
 
-static hipEvent_t take_event() {
-    Prof& P = prof();
-    if (!P.pool.empty()) {
-        hipEvent_t e = P.pool.back();
-        P.pool.pop_back();
-        return e;
-    }
-    hipEvent_t e;
-    JET_HIP(hipEventCreate(&e));
-    return e;
-}
-
 unsigned long long* prof_counters(int fam) { return prof_on(fam) ? prof().dev : nullptr; }
 
-std::pair<hipEvent_t, hipEvent_t> prof_events(int fam) {
-    if (!prof_on(fam)) return {nullptr, nullptr};
-    hipEvent_t a = take_event();
-    hipEvent_t b = take_event();
-    prof().pending.push_back({a, b});
-    return {a, b};
+unsigned long long* prof_span(int fam) {
+    if (!prof_on(fam)) return nullptr;
+    Prof& P = prof();
+    if (P.spans >= PROF_MAX_SPANS) return nullptr;
+    return P.span + 2 * (P.spans++);
 }
 
 void prof_account(int fam, double host_bytes) {
@@ -47,15 +36,19 @@ bool prof_enable(const std::string& name) {
     else return false;
     Prof& P = prof();
     if (!P.dev) JET_HIP(hipMalloc(&P.dev, 8 * sizeof(unsigned long long)));
+    if (!P.span) JET_HIP(hipMalloc(&P.span, 2 * PROF_MAX_SPANS * sizeof(unsigned long long)));
     JET_HIP(hipMemset(P.dev, 0, 8 * sizeof(unsigned long long)));
-    JET_HIP(hipDeviceSynchronize());
-    for (auto& pr : P.pending) {
-        P.pool.push_back(pr.first);
-        if (pr.second) P.pool.push_back(pr.second);
+    // starts at all-ones (atomic min), ends at zero (atomic max)
+    std::vector<unsigned long long> init(2 * PROF_MAX_SPANS);
+    for (int64_t i = 0; i < PROF_MAX_SPANS; i++) {
+        init[2 * i] = ~0ull;
+        init[2 * i + 1] = 0;
     }
-    P.pending.clear();
+    JET_HIP(hipMemcpy(P.span, init.data(), init.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
+    JET_HIP(hipDeviceSynchronize());
     P.host_bytes = 0;
     P.launches = 0;
+    P.spans = 0;
     P.family = fam;
     return true;
 }
@@ -64,12 +57,19 @@ bool prof_read(int64_t& launches, double& ms, double& bytes) {
     Prof& P = prof();
     if (P.family == PF_NONE) return false;
     JET_HIP(hipDeviceSynchronize());
-    ms = 0;
-    for (auto& pr : P.pending) {
-        float t = 0;
-        JET_HIP(hipEventElapsedTime(&t, pr.first, pr.second));
-        ms += t;
-    }
+    int dev = 0, khz = 0;
+    JET_HIP(hipGetDevice(&dev));
+    JET_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    std::vector<unsigned long long> sp(2 * P.spans);
+    if (P.spans > 0)
+        JET_HIP(hipMemcpy(sp.data(), P.span, sp.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    double ticks = 0;
+    int64_t timed = 0;
+    for (int64_t i = 0; i < P.spans; i++)
+        if (sp[2 * i + 1] >= sp[2 * i] && sp[2 * i] != ~0ull) {
+            ticks += (double)(sp[2 * i + 1] - sp[2 * i]);
+            timed++;
+        }
     unsigned long long c[8];
     JET_HIP(hipMemcpy(c, P.dev, sizeof(c), hipMemcpyDeviceToHost));
     bytes = P.host_bytes;
@@ -78,7 +78,11 @@ bool prof_read(int64_t& launches, double& ms, double& bytes) {
         // cache write 4; per table slot 8; per candidate 4 (list write)
         bytes += 24.0 * (double)c[0] + 8.0 * (double)c[1] + 4.0 * (double)c[2];
     }
-    launches = P.launches;
+    // the span count bounds the launches whose time is known; scale the
+    // byte total to the timed launches when spans ran out
+    if (timed < P.launches && P.launches > 0) bytes *= (double)timed / (double)P.launches;
+    launches = timed;
+    ms = khz > 0 ? ticks / (double)khz : 0.0;
     return true;
 }
 

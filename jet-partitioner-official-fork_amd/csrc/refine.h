@@ -48,6 +48,9 @@ class Refiner {
     void conn_tables(const Level& L, const part_t* part, std::vector<int64_t>& off, std::vector<int32_t>& slot_part,
                      std::vector<int32_t>& slot_val, std::vector<int32_t>& size);
 
+    // Whether the last refine() replaced the best partition (and so the
+    // refine_data carried to the next level).
+    bool changed_last() const { return changed_; }
     int64_t cut2() const { return cut_; }
     int64_t total_size() const { return total_size_; }
     int64_t total_imb() const { return total_imb_; }
@@ -115,6 +118,7 @@ class Refiner {
     int64_t* res_dev_ = nullptr;
     // refine_data mirror on the host
     bool init_ = false;
+    bool changed_ = false;
     int64_t cut_ = 0, total_imb_ = 0, total_size_ = 0;
     int32_t opt_ = 0, size_max_ = 0;
 };

@@ -18,8 +18,6 @@
 
 #include "prims.h"
 #include "prof.h"
-
-#include <hip/hip_ext.h>
 #include "refine.h"
 #include "wave.h"
 
@@ -250,11 +248,14 @@ __global__ void __launch_bounds__(WB_WAVES* WAVE)
                       int32_t* conn_size, int32_t* mark, part_t* dest_cache, const vid_t* list, int32_t nlist,
                       const int32_t* gate) {
     if (gate && gate[0] == 0) return;
-    __shared__ int2 T[WB_WAVES][WAVE_TABLE_MAX];
-    __shared__ int2 T2[WB_WAVES][WAVE_TABLE_MAX];
+    // per wave two tables of tcap = min(k, WAVE_TABLE_MAX) slots, sized at launch
+    extern __shared__ int2 tabs[];
     __shared__ int32_t used[WB_WAVES];
     const int lane = threadIdx.x & (WAVE - 1);
     const int w = threadIdx.x / WAVE;
+    const int32_t tcap = k < WAVE_TABLE_MAX ? k : WAVE_TABLE_MAX;
+    int2* Tw = tabs + (size_t)(2 * w) * tcap;
+    int2* T2w = Tw + tcap;
     for (int32_t r = blockIdx.x * WB_WAVES + w; r < nlist; r += gridDim.x * WB_WAVES) {
         vid_t i = list[r];
         if (mark) {
@@ -264,21 +265,21 @@ __global__ void __launch_bounds__(WB_WAVES* WAVE)
         int32_t deg = (int32_t)(re - rs);
         int32_t cap = deg < k ? deg : k;
         for (int32_t s = lane; s < cap; s += WAVE) {
-            T[w][s] = make_int2(NULL_PART, 0);
-            T2[w][s] = make_int2(NULL_PART, 0);
+            Tw[s] = make_int2(NULL_PART, 0);
+            T2w[s] = make_int2(NULL_PART, 0);
         }
         if (lane == 0) used[w] = 0;
         wave_sync();
-        for (eoff_t j = rs + lane; j < re; j += WAVE) lds_table_add(T[w], cap, k, part[g.col[j]], g.ew[j], &used[w]);
+        for (eoff_t j = rs + lane; j < re; j += WAVE) lds_table_add(Tw, cap, k, part[g.col[j]], g.ew[j], &used[w]);
         wave_sync();
         int32_t size = shrink_size(used[w], cap);
         int2* out = conn + conn_off[i];
         if (size < cap) {
-            lds_table_shrink(T[w], cap, T2[w], size, lane, WAVE);
+            lds_table_shrink(Tw, cap, T2w, size, lane, WAVE);
             wave_sync();
-            for (int32_t s = lane; s < cap; s += WAVE) out[s] = T2[w][s];
+            for (int32_t s = lane; s < cap; s += WAVE) out[s] = T2w[s];
         } else {
-            for (int32_t s = lane; s < cap; s += WAVE) out[s] = T[w][s];
+            for (int32_t s = lane; s < cap; s += WAVE) out[s] = Tw[s];
         }
         if (lane == 0) {
             conn_size[i] = size;
@@ -298,8 +299,11 @@ __global__ void __launch_bounds__(BB_BLOCK)
                        int32_t* conn_size, int32_t* mark, part_t* dest_cache, const vid_t* list, int32_t nlist,
                        const int32_t* gate) {
     if (gate && gate[0] == 0) return;
-    __shared__ int2 T[MAX_K];
-    __shared__ int2 T2[MAX_K];
+    // two tables of min(k, max degree) slots, sized at launch
+    extern __shared__ int2 tabs[];
+    const int32_t tcap = k;
+    int2* T = tabs;
+    int2* T2 = tabs + tcap;
     __shared__ int32_t used;
     for (int32_t r = blockIdx.x; r < nlist; r += gridDim.x) {
         vid_t i = list[r];
@@ -345,7 +349,9 @@ constexpr int LS_CHW = 512;
 __global__ void __launch_bounds__(LS_WAVES* WAVE)
     k_lp_select(DGraph g, const part_t* part, const eoff_t* conn_off, const int2* conn, const int32_t* conn_size,
                 part_t* dest_cache, int32_t* gain_p, int32_t* lock, double fr, part_t* dest, int32_t* pregain,
-                vid_t* cand, int32_t* counters, vid_t nmax, int32_t* next_counters, unsigned long long* pc) {
+                vid_t* cand, int32_t* counters, vid_t nmax, int32_t* next_counters, unsigned long long* pc,
+                unsigned long long* span) {
+    prof_span_begin(span);
     __shared__ vid_t sbuf[LS_WAVES * LS_CHW];
     __shared__ int32_t sh[2 * LS_WAVES + 2];
     __shared__ int64_t pred[LS_WAVES];
@@ -407,6 +413,10 @@ __global__ void __launch_bounds__(LS_WAVES* WAVE)
             int64_t v = block_reduce_sum<int64_t, LS_WAVES * WAVE>(v3[q], pred);
             if (threadIdx.x == 0 && v) atomicAdd(&pc[q], (unsigned long long)v);
         }
+    }
+    if (span) {
+        __syncthreads();
+        prof_span_end(span);
     }
 }
 
@@ -1263,6 +1273,11 @@ __global__ void k_max_degree(DGraph g, int32_t* out) {
 
 Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefineParams& p) : a_(a), s_(s), p_(p), nmax_(n) {
     if (p.k < 1 || p.k > MAX_K) throw std::invalid_argument("k out of range");
+    if (2 * sizeof(int2) * (size_t)p.k > 65536) {
+        // dynamic LDS above 64 KB must be allowed per kernel (gfx950 has 160 KB)
+        JET_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conn_build_block),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)(2 * sizeof(int2) * MAX_K)));
+    }
     int64_t N = std::max<int64_t>(n, 1);
     conn_off_ = a.alloc<eoff_t>(N + 1);
     conn_ = a.alloc<int2>(std::max<int64_t>(cap, 1));
@@ -1337,13 +1352,16 @@ void Refiner::init_conn(const Level& L, const part_t* part, const int32_t* mark)
                                                                            mk, dest_cache_, gate);
     JET_LAUNCH_CHECK();
     if (ntier_mid_ > 0) {
-        k_conn_build_wave<<<grid_for(ntier_mid_, WB_WAVES, 8192), WB_WAVES * WAVE, 0, s_>>>(
+        const size_t lds = 2 * sizeof(int2) * (size_t)WB_WAVES * std::min(p_.k, WAVE_TABLE_MAX);
+        k_conn_build_wave<<<grid_for(ntier_mid_, WB_WAVES, 8192), WB_WAVES * WAVE, lds, s_>>>(
             g, part, p_.k, conn_off_, conn_, conn_size_, mk, dest_cache_, tier_mid_, ntier_mid_, gate);
         JET_LAUNCH_CHECK();
     }
     if (ntier_big_ > 0) {
-        k_conn_build_block<<<std::min(ntier_big_, 2048), BB_BLOCK, 0, s_>>>(g, part, p_.k, conn_off_, conn_, conn_size_,
-                                                                            mk, dest_cache_, tier_big_, ntier_big_, gate);
+        // a row's capacity is min(deg, k) <= k: two tables of k slots suffice
+        const size_t lds = 2 * sizeof(int2) * (size_t)p_.k;
+        k_conn_build_block<<<std::min(ntier_big_, 2048), BB_BLOCK, lds, s_>>>(
+            g, part, p_.k, conn_off_, conn_, conn_size_, mk, dest_cache_, tier_big_, ntier_big_, gate);
         JET_LAUNCH_CHECK();
     }
 }
@@ -1352,11 +1370,9 @@ void Refiner::jet_lp(const Level& L, const part_t* part, double fr) {
     DGraph g = L.view();
     int32_t* c = cnt();
     int nb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_WAVES * LS_CHW));
-    auto ev = prof_events(PF_LP_SELECT);
-    hipExtLaunchKernelGGL(k_lp_select, dim3(nb), dim3(LS_WAVES * WAVE), 0, s_, ev.first, ev.second, 0, g, part,
-                          (const int64_t*)conn_off_, (const int2*)conn_, (const int32_t*)conn_size_,
-                          dest_cache_, gain_p_, lock_, fr, dest_, pregain_, cand_, c, nmax_, cnt_next(),
-                          prof_counters(PF_LP_SELECT));
+    k_lp_select<<<nb, LS_WAVES * WAVE, 0, s_>>>(g, part, conn_off_, conn_, conn_size_, dest_cache_, gain_p_, lock_, fr,
+                                                dest_, pregain_, cand_, c, nmax_, cnt_next(),
+                                                prof_counters(PF_LP_SELECT), prof_span(PF_LP_SELECT));
     JET_LAUNCH_CHECK();
     // per vertex: cache read 4, destination write 4, lock read 4, pre-gain write 4
     prof_account(PF_LP_SELECT, 16.0 * g.n);
@@ -1513,6 +1529,7 @@ void Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
 
 LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
     auto t0 = std::chrono::steady_clock::now();
+    changed_ = false;
     DGraph g = L.view();
     const int32_t k = p_.k;
     fill_i32(misc_, 0, 8, s_);
@@ -1579,6 +1596,7 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
                 take = true;
             }
             if (take) {
+                changed_ = true;
                 cut_ = curr_cut;
                 total_imb_ = curr_imb;
                 JET_HIP(hipMemcpyAsync(best_sizes_, curr_sizes_, sizeof(int64_t) * k, hipMemcpyDeviceToDevice, s_));

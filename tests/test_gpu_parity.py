@@ -132,6 +132,25 @@ def test_conn_tables_bit_exact(name, mk, k):
         assert got[i] == list(zip(ep[eoff[i]:eoff[i + 1]].tolist(), ev[eoff[i]:eoff[i + 1]].tolist())), i
 
 
+@pytest.mark.parametrize("k", [300, 5000, 8192])
+def test_conn_tables_large_k(k):
+    """Workgroup-built rows with table capacity up to k = 8192 slots (two
+    tables of 8192 int2 in LDS: 128 KB of the 160 KB per workgroup)."""
+    rs = np.random.default_rng(k)
+    hubs, n = 3, 30000
+    u = np.concatenate([np.full(n - hubs, h) for h in range(hubs)] + [rs.integers(hubs, n, 20000)])
+    v = np.concatenate([np.arange(hubs, n) for _ in range(hubs)] + [rs.integers(hubs, n, 20000)])
+    g = graphs.from_edges(n, u, v, w=rs.integers(1, 9, len(u)))
+    part = rs.integers(0, k, n).astype(np.int32)
+    say("conn-large-k", k)
+    off, sp, sv, sz = jet_amd.conn_tables(g, part, k)
+    eoff, ep, ev, esz = oracle.conn_build(g, part, k)
+    assert np.array_equal(sz, esz)
+    got = canonical_tables(off, sp, sv, sz)
+    for i in range(n):
+        assert got[i] == list(zip(ep[eoff[i]:eoff[i + 1]].tolist(), ev[eoff[i]:eoff[i + 1]].tolist())), i
+
+
 def rand_part(n, k, seed, skew=0.0):
     rs = np.random.default_rng(seed)
     pr = np.exp(-skew * np.arange(k))
