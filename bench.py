@@ -42,12 +42,245 @@ def make_graph(name):
     raise SystemExit(f"unknown workload {name}")
 
 
+def batch_graphs(count, seed):
+    """The batched mix: half 2-D/3-D grids, half R-MAT (a=.57, b=.19, c=.19,
+    edge factor 8); vertex counts log-uniform in [1K, 200K] (R-MAT: the power
+    of two nearest the draw); edge weights uniform in [1, 4]; per graph k from
+    {2..64}, imbalance from {1.01, 1.03, 1.10}, 2-hop or HEC coarsening."""
+    rs = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = float(np.exp(rs.uniform(np.log(1e3), np.log(2e5))))
+        if i % 2 == 0:
+            if rs.random() < 0.5:
+                side = max(2, int(round(n ** 0.5)))
+                g = jet_amd.grid2d(side, side)
+            else:
+                side = max(2, int(round(n ** (1 / 3))))
+                g = jet_amd.stencil3d(side, side, side, seed=i, wmax=0)
+        else:
+            g = jet_amd.rmat(max(10, int(round(np.log2(n)))), 8, seed=i)
+        g["ew"] = rs.integers(1, 5, len(g["col"])).astype(np.int32)
+        rp, col = g["rowptr"], g["col"]
+        rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+        # symmetric weights: w(u, v) from the unordered pair
+        lo, hi = np.minimum(rows, col).astype(np.int64), np.maximum(rows, col).astype(np.int64)
+        g["ew"] = (1 + ((lo * 0x9E3779B1 + hi * 0x85EBCA77 + i) % 4)).astype(np.int32)
+        g["uniform"] = False
+        cfg = jet_amd.Config(num_parts=int(rs.choice([2, 4, 8, 16, 32, 64])),
+                             max_imb_ratio=float(rs.choice([1.01, 1.03, 1.10])),
+                             coarsening_alg=int(rs.integers(0, 2)))
+        out.append((g, cfg, 1000 + i))
+    return out
+
+
+def run_batch(args):
+    """Batched mode: the same graphs partitioned one call at a time (plain
+    loop) and through jet_partition_batch; graphs/s for both."""
+    import torch
+
+    count = args.graphs
+    t_gen = time.time()
+    jobs = batch_graphs(count, args.seed)
+    print(f"# {count} graphs generated in {time.time() - t_gen:.1f}s", file=sys.stderr, flush=True)
+    dev = torch.device("cuda:0")
+    ns = [len(g["rowptr"]) - 1 for g, _, _ in jobs]
+    ms = [len(g["col"]) for g, _, _ in jobs]
+    voff = np.zeros(count + 1, dtype=np.int64)
+    voff[1:] = np.cumsum(ns)
+    eoff = np.zeros(count + 1, dtype=np.int64)
+    eoff[1:] = np.cumsum(ms)
+    d_rp = torch.from_numpy(np.concatenate([g["rowptr"].astype(np.int64) for g, _, _ in jobs])).to(dev)
+    d_col = torch.from_numpy(np.concatenate([g["col"] for g, _, _ in jobs])).to(dev)
+    d_ew = torch.from_numpy(np.concatenate([g["ew"] for g, _, _ in jobs])).to(dev)
+    d_vw = torch.ones(int(voff[-1]), dtype=torch.int32, device=dev)
+    d_part = torch.empty(int(voff[-1]), dtype=torch.int32, device=dev)
+    d_part2 = torch.empty(int(voff[-1]), dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    cfgs = (jet_amd.Config * count)(*[c for _, c, _ in jobs])
+    seeds = np.array([s for _, _, s in jobs], dtype=np.uint64)
+    uni = np.zeros(count, dtype=np.int32)
+    ctx = jet_amd.Context(workers=args.workers)
+    cuts_b = np.empty(count, dtype=np.int64)
+    L = jet_amd.lib()
+    i4 = torch.tensor([], dtype=torch.int32).element_size()
+
+    def plain():
+        cuts = []
+        for i in range(count):
+            c, _ = jet_amd.partition_device(ns[i], d_rp.data_ptr() + 8 * int(voff[i] + i),
+                                            d_col.data_ptr() + i4 * int(eoff[i]), d_ew.data_ptr() + i4 * int(eoff[i]),
+                                            d_vw.data_ptr() + i4 * int(voff[i]), d_part.data_ptr() + i4 * int(voff[i]),
+                                            cfgs[i], seed=int(seeds[i]))
+            cuts.append(c)
+        return cuts
+
+    def batched():
+        jet_amd._check(L.jet_partition_batch(ctx._h, count, cfgs, seeds.ctypes.data, voff.ctypes.data,
+                                             eoff.ctypes.data, d_rp.data_ptr(), d_col.data_ptr(), d_ew.data_ptr(),
+                                             d_vw.data_ptr(), uni.ctypes.data, jet_amd.PTR_DEVICE,
+                                             d_part2.data_ptr(), cuts_b.ctypes.data, None))
+
+    for w in range(args.warmup):
+        plain()
+        batched()
+        print(f"# warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        cuts_p = plain()
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t0) / args.steps
+    print(f"# plain loop {t_plain:.3f}s per pass", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batched()
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / args.steps
+    print(f"# batched {t_batch:.3f}s per pass", file=sys.stderr, flush=True)
+    same = bool(np.array_equal(np.array(cuts_p), cuts_b) and torch.equal(d_part, d_part2))
+    ctx.close()
+    out = {
+        "metric": "batched graphs/s (1 GPU)",
+        "value": round(count / t_batch, 2),
+        "unit": "graphs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t_batch, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"batch-{count}", "graphs": count, "vertices": int(voff[-1]),
+                   "undirected_edges": int(eoff[-1]) // 2, "workers": args.workers, "seed": args.seed},
+        "plain_loop_graphs_per_s": round(count / t_plain, 2),
+        "batched_vs_plain": round(t_plain / t_batch, 3),
+        "identical_to_plain": same,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def serve_pool(size, seed):
+    """Graphs requests draw from: grids and R-MAT (edge factor 8), vertex
+    counts log-uniform in [10K, 4M]."""
+    rs = np.random.default_rng(seed)
+    pool = []
+    for i in range(size):
+        n = float(np.exp(rs.uniform(np.log(1e4), np.log(4e6))))
+        if i % 2 == 0:
+            side = int(round(n ** 0.5))
+            g = jet_amd.grid2d(side, side)
+        else:
+            g = jet_amd.rmat(int(round(np.log2(n))), 8, seed=i)
+        g["uniform"] = True
+        pool.append(g)
+    return pool
+
+
+def run_serve(args):
+    """Serving: Poisson arrivals of partition jobs (each a pooled graph with a
+    per-request seed and k), admitted under an HBM budget; latency from
+    submission to completion."""
+    import torch
+
+    rs = np.random.default_rng(args.seed)
+    t_gen = time.time()
+    pool = serve_pool(args.pool, args.seed)
+    print(f"# pool of {len(pool)} graphs generated in {time.time() - t_gen:.1f}s", file=sys.stderr, flush=True)
+    dev = torch.device("cuda:0")
+    dpool = []
+    for g in pool:
+        dpool.append((len(g["rowptr"]) - 1, torch.from_numpy(g["rowptr"]).to(dev), torch.from_numpy(g["col"]).to(dev)))
+    torch.cuda.synchronize()
+    nreq = args.requests
+    gaps = rs.exponential(1.0 / args.rate, nreq)
+    which = rs.integers(0, len(pool), nreq)
+    ks = rs.choice([8, 16, 32, 64, 128, 256], nreq)
+    cfgs = [jet_amd.Config(num_parts=int(k), max_imb_ratio=1.03, coarsening_alg=1) for k in ks]
+    parts = [torch.empty(dpool[w][0], dtype=torch.int32, device=dev) for w in which]
+    ctx = jet_amd.Context(workers=args.workers, budget=int(args.budget_gb * (1 << 30)))
+    tickets = [None] * nreq
+    done = {}
+    t0 = time.perf_counter()
+    next_t = 0.0
+    collected = 0
+    for r in range(nreq):
+        next_t += gaps[r]
+        while True:  # collect finished jobs while waiting for the next arrival
+            now = time.perf_counter() - t0
+            if now >= next_t:
+                break
+            for q in range(collected, r):
+                if q in done or tickets[q] is None:
+                    continue
+                res = ctx.poll(tickets[q], wait_ms=0)
+                if res is not None:
+                    done[q] = res
+            time.sleep(min(0.001, max(0.0, next_t - now)))
+        n, d_rp, d_col = dpool[which[r]]
+        tickets[r] = ctx.submit_device(n, d_rp.data_ptr(), d_col.data_ptr(), None, None, parts[r].data_ptr(),
+                                       cfgs[r], seed=r, uniform=True)
+        if (r + 1) % 100 == 0:
+            print(f"# {r + 1}/{nreq} submitted, {len(done)} done", file=sys.stderr, flush=True)
+    for q in range(nreq):
+        if q not in done:
+            done[q] = ctx.poll(tickets[q], wait_ms=-1)
+        if (q + 1) % 100 == 0:
+            print(f"# {q + 1}/{nreq} collected", file=sys.stderr, flush=True)
+    t_all = time.perf_counter() - t0
+    lat = np.array([1e3 * (done[q][3].end_s - done[q][3].submit_s) for q in range(nreq)])
+    first = min(done[q][3].submit_s for q in range(nreq))
+    last = max(done[q][3].end_s for q in range(nreq))
+    # invariance: a sample of requests re-run alone on the plain path
+    same = True
+    for q in range(0, nreq, max(1, nreq // 8)):
+        n, d_rp, d_col = dpool[which[q]]
+        alone = torch.empty(n, dtype=torch.int32, device=dev)
+        c, _ = jet_amd.partition_device(n, d_rp.data_ptr(), d_col.data_ptr(), None, None, alone.data_ptr(), cfgs[q],
+                                        seed=q, uniform=True)
+        same = same and c == done[q][1] and bool(torch.equal(alone, parts[q]))
+    ctx.close()
+    out = {
+        "metric": "serving p99 latency ms (1 GPU)",
+        "value": round(float(np.percentile(lat, 99)), 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": nreq,
+        "warmup": 0,
+        "ms_per_step": round(1e3 * t_all / nreq, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "serve", "requests": nreq, "rate_per_s": args.rate, "pool": len(pool),
+                   "workers": args.workers, "budget_gb": args.budget_gb, "seed": args.seed},
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "graphs_per_s": round(nreq / (last - first), 3),
+        "same_result_alone": same,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="stencil3d-128", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="stencil3d-128", choices=sorted(WORKLOADS) + ["batch", "serve"])
+    ap.add_argument("--graphs", type=int, default=512, help="batch: number of graphs")
+    ap.add_argument("--workers", type=int, default=4, help="batch/serve: concurrent GPU workers")
+    ap.add_argument("--requests", type=int, default=400, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrival rate per second")
+    ap.add_argument("--pool", type=int, default=16, help="serve: distinct graphs requests draw from")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM budget for admitted jobs")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--prof", default="lp_select", help="kernel family timed with HIP events")
@@ -62,6 +295,10 @@ def main():
 
     if not torch.cuda.is_available():
         raise SystemExit("no GPU visible")
+    if args.workload == "batch":
+        return run_batch(args)
+    if args.workload == "serve":
+        return run_serve(args)
     wl = WORKLOADS[args.workload]
     t_gen = time.time()
     g = make_graph(args.workload)

@@ -353,8 +353,7 @@ void contract(const DGraph& g, const vid_t* map, vid_t nc, Arena& persist, Arena
     k_tiers<<<grid_for(nc, 256, 8192), 256, 0, s>>>(nc, hoff, l1, l2, l3, cnt);
     JET_LAUNCH_CHECK();
     int32_t hcnt[4];
-    JET_HIP(hipMemcpyAsync(hcnt, cnt, sizeof(hcnt), hipMemcpyDeviceToHost, s));
-    JET_HIP(hipStreamSynchronize(s));
+    read_i32_words(cnt, 4, hcnt, s);
     eoff_t H = g.m;
     vid_t* tkey = a.alloc<vid_t>(H);
     wgt_t* tval = a.alloc<wgt_t>(H);

@@ -1,5 +1,7 @@
 // Device-wide primitives: fills, exclusive scans, stable LSD radix sort.
 #pragma once
+#include <cstring>
+
 #include "arena.h"
 #include "jet_types.h"
 
@@ -9,6 +11,23 @@ void fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
 void fill_i64(int64_t* p, int64_t v, int64_t n, hipStream_t s);
 void iota_i32(int32_t* p, int64_t n, hipStream_t s);
 void copy_i32(int32_t* dst, const int32_t* src, int64_t n, hipStream_t s);
+void copy_i64(int64_t* dst, const int64_t* src, int64_t n, hipStream_t s);
+
+// Pinned, device-mapped host words owned by the calling thread (allocated on
+// first use, MAILBOX_SLOTS of them). A kernel stores a result there with a
+// system-scope fence and the host reads it after synchronising the stream:
+// no copy command and no staging buffer on the way.
+constexpr int MAILBOX_SLOTS = 64;
+struct Mailbox {
+    int64_t* host = nullptr;
+    int64_t* dev = nullptr;
+    ~Mailbox();
+};
+Mailbox& thread_mailbox();
+// Copies `bytes` (<= 8) from device memory into mailbox slot `slot` (stream-ordered).
+void peek_to_mailbox(const void* src, int bytes, int slot, hipStream_t s);
+// Reads `count` (<= MAILBOX_SLOTS) int32 words from device memory, synchronously.
+void read_i32_words(const int32_t* src, int count, int32_t* out, hipStream_t s);
 
 // out[i] = in[0] + ... + in[i-1] for i in [0, n]; out has n + 1 entries
 // (out[n] is the total). `in` and `out` may alias when the types match.
@@ -30,12 +49,15 @@ inline int grid_for(int64_t n, int block, int max_blocks = 65536) {
     return (int)b;
 }
 
-// Reads one value from device memory into host memory, synchronously.
+// Reads one value from device memory into host memory, synchronously,
+// through this thread's mailbox.
 template <class T>
 T read_scalar(const T* dev, hipStream_t s) {
-    T h;
-    JET_HIP(hipMemcpyAsync(&h, dev, sizeof(T), hipMemcpyDeviceToHost, s));
+    static_assert(sizeof(T) <= 8, "read_scalar reads at most 8 bytes");
+    peek_to_mailbox(dev, (int)sizeof(T), 0, s);
     JET_HIP(hipStreamSynchronize(s));
+    T h;
+    std::memcpy(&h, (const void*)thread_mailbox().host, sizeof(T));
     return h;
 }
 

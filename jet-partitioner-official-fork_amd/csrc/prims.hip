@@ -1,5 +1,7 @@
 // Device-wide primitives for the Jet hot path on gfx950.
 #include "prims.h"
+
+#include <stdexcept>
 #include "wave.h"
 
 namespace jet {
@@ -20,6 +22,58 @@ __global__ void k_iota_i32(int32_t* p, int64_t n) {
 __global__ void k_copy_i32(int32_t* d, const int32_t* s, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         d[i] = s[i];
+}
+
+__global__ void k_copy_i64(int64_t* d, const int64_t* s, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = s[i];
+}
+__global__ void k_peek(const unsigned char* src, int bytes, int64_t* dst) {
+    int64_t v = 0;
+    for (int b = 0; b < bytes; b++) v |= (int64_t)src[b] << (8 * b);
+    *dst = v;
+    __threadfence_system();
+}
+
+Mailbox::~Mailbox() {
+    if (host) (void)hipHostFree(host);
+}
+
+Mailbox& thread_mailbox() {
+    static thread_local Mailbox mb;
+    if (!mb.host) {
+        JET_HIP(hipHostMalloc(reinterpret_cast<void**>(&mb.host), MAILBOX_SLOTS * sizeof(int64_t),
+                              hipHostMallocMapped | hipHostMallocCoherent));
+        JET_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&mb.dev), mb.host, 0));
+    }
+    return mb;
+}
+
+void peek_to_mailbox(const void* src, int bytes, int slot, hipStream_t s) {
+    Mailbox& mb = thread_mailbox();
+    k_peek<<<1, 1, 0, s>>>(static_cast<const unsigned char*>(src), bytes, mb.dev + slot);
+    JET_LAUNCH_CHECK();
+}
+
+__global__ void k_peek_words(const int32_t* src, int count, int64_t* dst) {
+    if (threadIdx.x < count) dst[threadIdx.x] = src[threadIdx.x];
+    __threadfence_system();
+}
+
+void read_i32_words(const int32_t* src, int count, int32_t* out, hipStream_t s) {
+    if (count <= 0) return;
+    if (count > MAILBOX_SLOTS) throw std::invalid_argument("read_i32_words: too many words");
+    Mailbox& mb = thread_mailbox();
+    k_peek_words<<<1, 64, 0, s>>>(src, count, mb.dev);
+    JET_LAUNCH_CHECK();
+    JET_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < count; i++) out[i] = (int32_t)((volatile int64_t*)mb.host)[i];
+}
+
+void copy_i64(int64_t* d, const int64_t* src, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    k_copy_i64<<<grid_for(n, 256, 8192), 256, 0, s>>>(d, src, n);
+    JET_LAUNCH_CHECK();
 }
 
 void fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s) {

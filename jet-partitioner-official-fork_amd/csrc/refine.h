@@ -31,7 +31,7 @@ struct RefineParams {
 class Refiner {
   public:
     Refiner(Arena& a, hipStream_t s, vid_t finest_n, int64_t finest_conn_cap, const RefineParams& p);
-    ~Refiner();
+    ~Refiner() = default;
     Refiner(const Refiner&) = delete;
     Refiner& operator=(const Refiner&) = delete;
 
@@ -96,6 +96,12 @@ class Refiner {
     part_t* cur_part_;
     vid_t* tier_mid_;
     vid_t* tier_big_;
+    // rows of tier_big_ ("hubs") take their update_small changes through
+    // k_update_hubs: flags per vertex for hub, hub with a moved neighbour,
+    // and moved in this iteration
+    uint8_t* hub_;
+    uint8_t* hub_dirty_;
+    uint8_t* moved_;
     int32_t ntier_mid_ = 0, ntier_big_ = 0;
     bool has_big_ = false;
     // small device state; counter set layout: [0] small candidates, [1] big
@@ -114,7 +120,7 @@ class Refiner {
     int64_t* buckets_;
     int32_t* bflag_;
     int64_t nbuckets_ = 0;
-    int64_t* res_host_ = nullptr;  // pinned: (cut change, largest part) of the last iteration
+    int64_t* res_host_ = nullptr;  // thread mailbox: (cut change, largest part) of the last iteration
     int64_t* res_dev_ = nullptr;
     // refine_data mirror on the host
     bool init_ = false;

@@ -131,12 +131,13 @@ __global__ void k_conn_caps(DGraph g, int32_t k, int32_t* caps) {
     }
 }
 
-__global__ void k_conn_tiers(DGraph g, int32_t k, vid_t* mid, vid_t* big, int32_t* cnt) {
+__global__ void k_conn_tiers(DGraph g, int32_t k, vid_t* mid, vid_t* big, int32_t* cnt, uint8_t* hub) {
     FOR_THREAD(i, g.n) {
         int64_t d = g.rowptr[i + 1] - g.rowptr[i];
         int64_t cap = d > k ? k : d;
         bool is_mid = d > SMALL_DEG && d <= HEAVY_DEG_REFINE && cap <= WAVE_TABLE_MAX;
         bool is_big = d > SMALL_DEG && !is_mid;
+        hub[i] = is_big ? 1 : 0;
         int32_t a = wave_append(&cnt[0], is_mid);
         int32_t b = wave_append(&cnt[1], is_big);
         if (a >= 0) mid[a] = (vid_t)i;
@@ -505,7 +506,8 @@ __global__ void k_afterburner_big(DGraph g, const vid_t* cand, const int32_t* co
 // jet_refiner.hpp:905-955.
 __global__ void k_moves_apply(DGraph g, const vid_t* moves, int32_t* mc, vid_t nmax, const wgt_t* vw, part_t* part,
                               part_t* dest, part_t* dest_cache, const eoff_t* conn_off, const int2* conn,
-                              const int32_t* conn_size, unsigned long long* sizes, int32_t k, int64_t* acc) {
+                              const int32_t* conn_size, unsigned long long* sizes, int32_t k, int64_t* acc,
+                              uint8_t* moved) {
     __shared__ int64_t red[256 / WAVE];
     __shared__ unsigned long long ls[LDS_K];
     const bool lds = k <= LDS_K;
@@ -514,10 +516,9 @@ __global__ void k_moves_apply(DGraph g, const vid_t* moves, int32_t* mc, vid_t n
         __syncthreads();
     }
     const int32_t ns = mc[2], total = mc[2] + mc[3];
-    int64_t cc = 0, sdeg = 0;
+    int64_t cc = 0;
     FOR_THREAD(x, total) {
         vid_t v = list_at(moves, ns, nmax, x);
-        sdeg += g.rowptr[v + 1] - g.rowptr[v];
         part_t p = part[v], b = dest[v];
         const int2* tab = conn + conn_off[v];
         int32_t sz = conn_size[v];
@@ -533,11 +534,10 @@ __global__ void k_moves_apply(DGraph g, const vid_t* moves, int32_t* mc, vid_t n
         }
         part[v] = b;
         dest[v] = p;
+        moved[v] = 1;
     }
     cc = block_reduce_sum<int64_t, 256>(cc, red);
-    sdeg = block_reduce_sum<int64_t, 256>(sdeg, red);
     if (threadIdx.x == 0 && cc) atomicAdd((unsigned long long*)&acc[0], (unsigned long long)cc);
-    if (threadIdx.x == 0 && sdeg) atomicAdd((unsigned long long*)&acc[5], (unsigned long long)sdeg);
     if (lds) {
         __syncthreads();
         for (int t = threadIdx.x; t < k; t += blockDim.x)
@@ -641,17 +641,12 @@ __device__ inline void update_edge(vid_t v, int32_t w, part_t q, const eoff_t* c
 
 // perform_moves' choice between update_small (push each moved vertex's
 // contribution into its neighbours' tables) and update_large (rebuild the
-// tables of every neighbour of a moved vertex). The reference rebuilds when
-// more than n/10 vertices move (jet_refiner.hpp:933). Both leave the same
-// part -> connectivity content and every decision reads tables by content, so
-// the rebuild is also taken when the moved vertices' adjacency exceeds m/8:
-// pushing that many atomics into hub tables serialises on a few hot slots,
-// while the rebuild streams each touched row once. acc[5] holds the degree
-// sum of the moves (k_moves_apply).
-__device__ inline bool update_by_rebuild(const int32_t* mc, const int64_t* acc, vid_t n, eoff_t m) {
-    const int32_t total = mc[2] + mc[3];
-    return total > n / 10 || 8 * __hip_atomic_load(&acc[5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > m;
-}
+// tables of every neighbour of a moved vertex): rebuild when more than n/10
+// vertices move (jet_refiner.hpp:933). The two do not leave the same tables:
+// pushing keeps a part whose connectivity falls to 0 in a table of k slots
+// (jet_refiner.hpp:806-810) and the rebalancers read such entries, so the
+// choice follows the reference exactly.
+__device__ inline bool update_by_rebuild(const int32_t* mc, vid_t n) { return mc[2] + mc[3] > n / 10; }
 
 // ADD == false: subtract from the old part (held in dest), or, when the tables
 // are rebuilt instead, mark the neighbours; ADD == true: add to the new part
@@ -659,8 +654,9 @@ __device__ inline bool update_by_rebuild(const int32_t* mc, const int64_t* acc, 
 template <bool ADD, int G>
 __global__ void k_update_small(DGraph g, const vid_t* moves, int32_t* mc, vid_t nmax, const part_t* part,
                                const part_t* dest, const eoff_t* conn_off, int2* conn, int32_t* conn_size,
-                               part_t* dest_cache, int32_t k, int32_t* err, const int64_t* acc, int32_t* mark) {
-    const bool rebuild = update_by_rebuild(mc, acc, g.n, g.m);
+                               part_t* dest_cache, int32_t k, int32_t* err, int32_t* mark, const uint8_t* hub,
+                               uint8_t* hub_dirty) {
+    const bool rebuild = update_by_rebuild(mc, g.n);
     if (ADD && rebuild) return;
     if (!ADD && rebuild && blockIdx.x == 0 && threadIdx.x == 0) mc[6] = 1;  // gates the rebuild
     const int32_t ns = mc[2];
@@ -671,16 +667,23 @@ __global__ void k_update_small(DGraph g, const vid_t* moves, int32_t* mc, vid_t 
             continue;
         }
         part_t q = ADD ? part[i] : dest[i];
-        for (eoff_t j = g.rowptr[i] + sub_; j < g.rowptr[i + 1]; j += G)
-            update_edge<ADD>(g.col[j], g.ew[j], q, conn_off, conn, conn_size, dest_cache, k, err);
+        for (eoff_t j = g.rowptr[i] + sub_; j < g.rowptr[i + 1]; j += G) {
+            vid_t v = g.col[j];
+            if (hub[v]) {  // updated by k_update_hubs
+                if (!ADD) hub_dirty[v] = 1;
+                continue;
+            }
+            update_edge<ADD>(v, g.ew[j], q, conn_off, conn, conn_size, dest_cache, k, err);
+        }
     }
 }
 
 template <bool ADD>
 __global__ void k_update_big(DGraph g, const vid_t* moves, const int32_t* mc, vid_t nmax, const part_t* part,
                              const part_t* dest, const eoff_t* conn_off, int2* conn, int32_t* conn_size,
-                             part_t* dest_cache, int32_t k, int32_t* err, const int64_t* acc, int32_t* mark) {
-    const bool rebuild = update_by_rebuild(mc, acc, g.n, g.m);
+                             part_t* dest_cache, int32_t k, int32_t* err, int32_t* mark, const uint8_t* hub,
+                             uint8_t* hub_dirty) {
+    const bool rebuild = update_by_rebuild(mc, g.n);
     if (ADD && rebuild) return;
     const int32_t nb = mc[3];
     const int lane = threadIdx.x & (WAVE - 1);
@@ -692,20 +695,72 @@ __global__ void k_update_big(DGraph g, const vid_t* moves, const int32_t* mc, vi
             continue;
         }
         part_t q = ADD ? part[i] : dest[i];
-        for (eoff_t j = g.rowptr[i] + lane; j < g.rowptr[i + 1]; j += WAVE)
-            update_edge<ADD>(g.col[j], g.ew[j], q, conn_off, conn, conn_size, dest_cache, k, err);
+        for (eoff_t j = g.rowptr[i] + lane; j < g.rowptr[i + 1]; j += WAVE) {
+            vid_t v = g.col[j];
+            if (hub[v]) {  // updated by k_update_hubs
+                if (!ADD) hub_dirty[v] = 1;
+                continue;
+            }
+            update_edge<ADD>(v, g.ew[j], q, conn_off, conn, conn_size, dest_cache, k, err);
+        }
+    }
+}
+
+// update_small for hub rows (the workgroup-built tier), one workgroup per hub
+// with a moved neighbour: the row's moved neighbours are found by a scan of
+// its adjacency and their weights summed per part in LDS, then each part's
+// total is applied to the table once, with the subtraction phase before the
+// addition phase as for the other rows. The resulting table equals the one
+// per-edge pushes leave: a part is reclaimed when its connectivity reaches 0
+// in a table of fewer than k slots (jet_refiner.hpp:806-810), new parts take
+// free or reclaimed slots and then overflow slots (:828-884). Pushing instead
+// would land every moved neighbour's atomics on the same few hub slots.
+constexpr int HB_BLOCK = 256;
+
+template <bool ADD>
+__global__ void __launch_bounds__(HB_BLOCK)
+    k_update_hubs(DGraph g, const vid_t* list, int32_t nlist, const int32_t* mc, const uint8_t* moved,
+                  uint8_t* hub_dirty, const part_t* part, const part_t* dest, const eoff_t* conn_off, int2* conn,
+                  int32_t* conn_size, part_t* dest_cache, int32_t k, int32_t* err) {
+    if (update_by_rebuild(mc, g.n)) return;
+    extern __shared__ int32_t delta[];  // k
+    for (int32_t r = blockIdx.x; r < nlist; r += gridDim.x) {
+        const vid_t u = list[r];
+        if (!hub_dirty[u]) continue;
+        for (int32_t p = threadIdx.x; p < k; p += HB_BLOCK) delta[p] = 0;
+        __syncthreads();
+        for (eoff_t j = g.rowptr[u] + threadIdx.x; j < g.rowptr[u + 1]; j += HB_BLOCK) {
+            vid_t v = g.col[j];
+            if (moved[v]) atomicAdd(&delta[ADD ? part[v] : dest[v]], g.ew[j]);
+        }
+        __syncthreads();
+        int2* tab = conn + conn_off[u];
+        const int32_t cap = (int32_t)(conn_off[u + 1] - conn_off[u]);
+        const int32_t vsz = conn_size[u];
+        for (int32_t p = threadIdx.x; p < k; p += HB_BLOCK) {
+            const int32_t w = delta[p];
+            if (w == 0) continue;
+            if (ADD) conn_add(tab, &conn_size[u], cap, p, w, err);
+            else conn_sub(tab, vsz, p, w, k, err);
+        }
+        if (ADD && threadIdx.x == 0) {
+            dest_cache[u] = NULL_PART;
+            hub_dirty[u] = 0;
+        }
+        __syncthreads();
     }
 }
 
 // cut change part 2 and the largest part (jet_refiner.hpp:937-948).
 __global__ void k_moves_finish(const vid_t* moves, const int32_t* mc, vid_t nmax, const part_t* part,
                                const part_t* dest, const eoff_t* conn_off, const int2* conn, const int32_t* conn_size,
-                               const int64_t* sizes, int32_t k, int64_t* acc, int64_t* host_out) {
+                               const int64_t* sizes, int32_t k, int64_t* acc, int64_t* host_out, uint8_t* moved) {
     __shared__ int64_t red[256 / WAVE];
     const int32_t ns = mc[2], total = mc[2] + mc[3];
     int64_t cc = 0;
     FOR_THREAD(x, total) {
         vid_t v = list_at(moves, ns, nmax, x);
+        moved[v] = 0;
         const int2* tab = conn + conn_off[v];
         int32_t sz = conn_size[v];
         cc += (int64_t)lookup(tab, sz, part[v]) - lookup(tab, sz, dest[v]);
@@ -733,7 +788,6 @@ __global__ void k_moves_finish(const vid_t* moves, const int32_t* mc, vid_t nmax
             acc[0] = 0;
             acc[1] = 0;
             acc[4] = 0;
-            acc[5] = 0;
         }
     }
 }
@@ -1298,6 +1352,9 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
     cur_part_ = a.alloc<part_t>(N);
     tier_mid_ = a.alloc<vid_t>(N);
     tier_big_ = a.alloc<vid_t>(N);
+    hub_ = a.alloc<uint8_t>(N);
+    hub_dirty_ = a.alloc<uint8_t>(N);
+    moved_ = a.alloc<uint8_t>(N);
     counters_ = a.alloc<int32_t>(32);
     acc_ = a.alloc<int64_t>(8);
     curr_sizes_ = a.alloc<int64_t>(p.k);
@@ -1311,25 +1368,24 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
     buckets_ = a.alloc<int64_t>(nbuckets_);
     bflag_ = a.alloc<int32_t>(nbuckets_);
     fill_i32(swap_bit_, 0, N, s);
+    JET_HIP(hipMemsetAsync(hub_dirty_, 0, (size_t)N, s));
+    JET_HIP(hipMemsetAsync(moved_, 0, (size_t)N, s));
     fill_i32(counters_, 0, 32, s);
     fill_i64(acc_, 0, 8, s);
-    JET_HIP(hipHostMalloc(reinterpret_cast<void**>(&res_host_), 4 * sizeof(int64_t), hipHostMallocMapped | hipHostMallocCoherent));
-    JET_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&res_dev_), res_host_, 0));
-}
-
-Refiner::~Refiner() {
-    if (res_host_) (void)hipHostFree(res_host_);
+    // (cut change, largest part) of each iteration land in mailbox slots 8, 9
+    Mailbox& mb = thread_mailbox();
+    res_host_ = mb.host + 8;
+    res_dev_ = mb.dev + 8;
 }
 
 void Refiner::build_tiers(const Level& L) {
     DGraph g = L.view();
     fill_i32(counters_ + 16, 0, 3, s_);
-    k_conn_tiers<<<grid_for(g.n, 256, 8192), 256, 0, s_>>>(g, p_.k, tier_mid_, tier_big_, counters_ + 16);
+    k_conn_tiers<<<grid_for(g.n, 256, 8192), 256, 0, s_>>>(g, p_.k, tier_mid_, tier_big_, counters_ + 16, hub_);
     k_max_degree<<<grid_for(g.n, 256, 2048), 256, 0, s_>>>(g, counters_ + 18);
     JET_LAUNCH_CHECK();
     int32_t c[3];
-    JET_HIP(hipMemcpyAsync(c, counters_ + 16, sizeof(c), hipMemcpyDeviceToHost, s_));
-    JET_HIP(hipStreamSynchronize(s_));
+    read_i32_words(counters_ + 16, 3, c, s_);
     ntier_mid_ = c[0];
     ntier_big_ = c[1];
     has_big_ = c[2] > BIG_DEG;
@@ -1489,7 +1545,7 @@ void Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
     // completion ticket of k_moves_finish cheap
     int nb = grid_for(g.n, 256, 512);
     k_moves_apply<<<nb, 256, 0, s_>>>(g, moves_, c, nmax_, g.vw, part, dest_, dest_cache_, conn_off_, conn_,
-                                      conn_size_, reinterpret_cast<unsigned long long*>(sizes), k, acc_);
+                                      conn_size_, reinterpret_cast<unsigned long long*>(sizes), k, acc_, moved_);
     JET_LAUNCH_CHECK();
     int G = group_for_avg(g);
     int nbs = grid_for((int64_t)g.n * G, 256, 4096);
@@ -1497,9 +1553,11 @@ void Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
     for (int add = 0; add < 2; add++) {
 #define JET_UPD(GG)                                                                                                   \
     if (add) k_update_small<true, GG><<<nbs, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,       \
-                                                          conn_size_, dest_cache_, k, misc_ + 1, acc_, swap_bit_);   \
+                                                          conn_size_, dest_cache_, k, misc_ + 1, swap_bit_,      \
+                                                          hub_, hub_dirty_);                                     \
     else k_update_small<false, GG><<<nbs, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,          \
-                                                        conn_size_, dest_cache_, k, misc_ + 1, acc_, swap_bit_);
+                                                        conn_size_, dest_cache_, k, misc_ + 1, swap_bit_,        \
+                                                        hub_, hub_dirty_);
         switch (G) {
             case 4: JET_UPD(4) break;
             case 8: JET_UPD(8) break;
@@ -1510,15 +1568,27 @@ void Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
 #undef JET_UPD
         if (has_big_) {
             if (add) k_update_big<true><<<nbb, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,
-                                                             conn_size_, dest_cache_, k, misc_ + 1, acc_, swap_bit_);
+                                                             conn_size_, dest_cache_, k, misc_ + 1, swap_bit_, hub_,
+                                                             hub_dirty_);
             else k_update_big<false><<<nbb, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,
-                                                           conn_size_, dest_cache_, k, misc_ + 1, acc_, swap_bit_);
+                                                           conn_size_, dest_cache_, k, misc_ + 1, swap_bit_, hub_,
+                                                           hub_dirty_);
+        }
+        if (ntier_big_ > 0) {
+            const size_t lds = sizeof(int32_t) * (size_t)k;
+            const int hb = std::min(ntier_big_, 2048);
+            if (add) k_update_hubs<true><<<hb, HB_BLOCK, lds, s_>>>(g, tier_big_, ntier_big_, c, moved_, hub_dirty_,
+                                                                    part, dest_, conn_off_, conn_, conn_size_,
+                                                                    dest_cache_, k, misc_ + 1);
+            else k_update_hubs<false><<<hb, HB_BLOCK, lds, s_>>>(g, tier_big_, ntier_big_, c, moved_, hub_dirty_,
+                                                                 part, dest_, conn_off_, conn_, conn_size_,
+                                                                 dest_cache_, k, misc_ + 1);
         }
         JET_LAUNCH_CHECK();
     }
     init_conn(L, part, swap_bit_);  // rebuild marked rows (gated on the device)
     k_moves_finish<<<nb, 256, 0, s_>>>(moves_, c, nmax_, part, dest_, conn_off_, conn_, conn_size_, sizes, k, acc_,
-                                       res_dev_);
+                                       res_dev_, moved_);
     JET_LAUNCH_CHECK();
     JET_HIP(hipStreamSynchronize(s_));
     volatile int64_t* res = res_host_;
@@ -1556,7 +1626,7 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
         total_imb_ = mx > opt_ ? mx - opt_ : 0;
     }
     int64_t curr_cut = cut_, curr_imb = total_imb_;
-    JET_HIP(hipMemcpyAsync(curr_sizes_, best_sizes_, sizeof(int64_t) * k, hipMemcpyDeviceToDevice, s_));
+    copy_i64(curr_sizes_, best_sizes_, k, s_);
     const int64_t imb_max = (int64_t)size_max_ - opt_;
     copy_i32(cur_part_, best_part, g.n, s_);
     build_tiers(L);
@@ -1599,7 +1669,7 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
                 changed_ = true;
                 cut_ = curr_cut;
                 total_imb_ = curr_imb;
-                JET_HIP(hipMemcpyAsync(best_sizes_, curr_sizes_, sizeof(int64_t) * k, hipMemcpyDeviceToDevice, s_));
+                copy_i64(best_sizes_, curr_sizes_, k, s_);
                 copy_i32(best_part, cur_part_, g.n, s_);
             }
         }

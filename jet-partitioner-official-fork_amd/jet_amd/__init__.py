@@ -15,6 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "libjet_amd.so")
 
 JET_OK, JET_EINVAL, JET_ENOMEM, JET_EINIT, JET_EDEVICE, JET_EUNSUPPORTED = 0, -1, -2, -3, -4, -5
+JET_EAGAIN = 1
 PTR_HOST, PTR_DEVICE = 0, 1
 MAX_LEVELS = 256
 
@@ -51,6 +52,11 @@ class Stats(ctypes.Structure):
                 ("num_levels", ctypes.c_int32), ("attempts", ctypes.c_int32), ("finest_imb_ratio", ctypes.c_double),
                 ("finest_edge_cut", ctypes.c_int64), ("largest_part", ctypes.c_int64),
                 ("smallest_part", ctypes.c_int64), ("levels", LevelStats * MAX_LEVELS)]
+
+
+class JobTimes(ctypes.Structure):
+    """jet_job_times_t: seconds since the context was created."""
+    _fields_ = [("submit_s", ctypes.c_double), ("start_s", ctypes.c_double), ("end_s", ctypes.c_double)]
 
 
 _lib = None
@@ -91,6 +97,11 @@ def lib():
             "jet_generate_grid2d": [i64, i64, VP, VP],
             "jet_generate_stencil3d": [i64, i64, i64, u64, i32, VP, VP, VP],
             "jet_generate_rmat": [i32, i32, dbl, dbl, dbl, u64, VP, VP, c.POINTER(i64)],
+            "jet_ctx_create": [i32, i32, i64, c.POINTER(VP)],
+            "jet_ctx_destroy": [VP],
+            "jet_submit": [VP, c.POINTER(Config), i64, VP, VP, VP, VP, i32, i32, u64, VP, c.POINTER(i64)],
+            "jet_poll": [VP, i64, i32, c.POINTER(i64), c.POINTER(Stats), c.POINTER(JobTimes)],
+            "jet_partition_batch": [VP, i32, c.POINTER(Config), VP, VP, VP, VP, VP, VP, VP, VP, i32, VP, VP, VP],
             "jet_prof_enable": [c.c_char_p],
             "jet_prof_read": [c.POINTER(i64), c.POINTER(dbl), c.POINTER(dbl)],
         }
@@ -343,3 +354,99 @@ def rmat(scale, ef, seed=0, a=0.57, b=0.19, c=0.19):
     col = col[:m.value].copy()
     return {"rowptr": rp, "col": col, "ew": np.ones(m.value, dtype=np.int32), "vw": np.ones(n, dtype=np.int32),
             "uniform": True}
+
+
+# ---------------------------------------------------------------- batch / serve
+class Context:
+    """jet_ctx_t: `workers` GPU workers (own HIP stream and device memory each)
+    admitting queued jobs under a device-memory budget (bytes; 0 = 80% of
+    free memory). Use as a context manager or call close()."""
+
+    def __init__(self, workers=4, budget=0, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().jet_ctx_create(device, workers, int(budget), ctypes.byref(h)))
+        self._h = h
+        self._keep = {}
+
+    def close(self):
+        if self._h:
+            lib().jet_ctx_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, g, cfg, seed=0, uniform=None, part_out=None):
+        """Queues a job on host arrays; returns its ticket. The result lands in
+        part_out (allocated when None) and is returned by poll()."""
+        rp, col, ew, vw = _graph(g)
+        n = len(rp) - 1
+        if uniform is None:
+            uniform = bool(g.get("uniform", False))
+        part = np.empty(n, dtype=np.int32) if part_out is None else part_out
+        t = ctypes.c_int64(0)
+        c = Config()
+        ctypes.pointer(c)[0] = cfg
+        _check(lib().jet_submit(self._h, ctypes.byref(c), n, _p(rp), _p(col), _p(ew), _p(vw), int(uniform), PTR_HOST,
+                                seed, _p(part), ctypes.byref(t)))
+        self._keep[t.value] = (rp, col, ew, vw, part, c)
+        return t.value
+
+    def submit_device(self, n, rowptr_ptr, col_ptr, ew_ptr, vw_ptr, part_ptr, cfg, seed=0, uniform=False):
+        """Queues a job on device pointers (ints); returns its ticket."""
+        t = ctypes.c_int64(0)
+        _check(lib().jet_submit(self._h, ctypes.byref(cfg), n, rowptr_ptr, col_ptr, ew_ptr, vw_ptr, int(uniform),
+                                PTR_DEVICE, seed, part_ptr, ctypes.byref(t)))
+        self._keep[t.value] = (cfg,)
+        return t.value
+
+    def poll(self, ticket, wait_ms=-1):
+        """None while the job runs (wait_ms >= 0 and not finished), else
+        (part or None, cut, Stats, JobTimes); raises JetError on failure."""
+        cut = ctypes.c_int64(0)
+        st = Stats()
+        tm = JobTimes()
+        rc = lib().jet_poll(self._h, ticket, int(wait_ms), ctypes.byref(cut), ctypes.byref(st), ctypes.byref(tm))
+        if rc == JET_EAGAIN:
+            return None
+        kept = self._keep.pop(ticket, None)
+        _check(rc)
+        part = kept[4] if kept is not None and len(kept) > 4 else None
+        return part, cut.value, st, tm
+
+    def batch(self, graphs, cfgs, seeds=None, stats=False):
+        """jet_partition_batch over a list of host CSR dicts; returns
+        (list of parts, cuts[, list of Stats])."""
+        count = len(graphs)
+        ns = [len(g["rowptr"]) - 1 for g in graphs]
+        ms = [len(g["col"]) for g in graphs]
+        voff = np.zeros(count + 1, dtype=np.int64)
+        voff[1:] = np.cumsum(ns)
+        eoff = np.zeros(count + 1, dtype=np.int64)
+        eoff[1:] = np.cumsum(ms)
+        rp = np.concatenate([np.asarray(g["rowptr"], dtype=np.int64) for g in graphs])
+        col = np.concatenate([np.asarray(g["col"], dtype=np.int32) for g in graphs])
+        ew = np.concatenate([np.asarray(g["ew"], dtype=np.int32) if g.get("ew") is not None
+                             else np.ones(len(g["col"]), dtype=np.int32) for g in graphs])
+        vw = np.concatenate([np.asarray(g["vw"], dtype=np.int32) if g.get("vw") is not None
+                             else np.ones(len(g["rowptr"]) - 1, dtype=np.int32) for g in graphs])
+        uni = np.array([int(bool(g.get("uniform", False))) for g in graphs], dtype=np.int32)
+        carr = (Config * count)(*cfgs)
+        sd = np.array(seeds if seeds is not None else [0] * count, dtype=np.uint64)
+        part = np.empty(int(voff[-1]), dtype=np.int32)
+        cuts = np.empty(count, dtype=np.int64)
+        st = (Stats * count)()
+        _check(lib().jet_partition_batch(self._h, count, carr, _p(sd), _p(voff), _p(eoff), _p(rp), _p(col), _p(ew),
+                                         _p(vw), _p(uni), PTR_HOST, _p(part), _p(cuts), st))
+        parts = [part[voff[i]:voff[i + 1]] for i in range(count)]
+        return (parts, cuts, list(st)) if stats else (parts, cuts)

@@ -100,3 +100,11 @@ def test_jet_h_mirror_compiles_and_reports_no_device(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert r.returncode == 3
     assert r.stdout.startswith("error -4 ")
+
+
+def test_batch_context_without_gpu_is_a_clean_error():
+    if jet_amd.device_count() > 0:
+        pytest.skip("a GPU is present")
+    with pytest.raises(jet_amd.JetError) as e:
+        jet_amd.Context(workers=2)
+    assert e.value.code == jet_amd.JET_EDEVICE

@@ -204,6 +204,8 @@ E2E = [
     ("rmat-hec", lambda: graphs.rmat(14, 16, seed=0), 256, 1.10, 1),
     ("rmat-2hop", lambda: graphs.rmat(13, 8, seed=3), 32, 1.03, 0),
     ("wrmat-hec", lambda: graphs.with_weights(graphs.rmat(12, 8, seed=4), seed=2, wmax=4, vwmax=4), 16, 1.03, 1),
+    ("wrmat-tight", lambda: graphs.with_weights(graphs.rmat(13, 8, seed=1), seed=1, wmax=4), 8, 1.01, 1),
+    ("rmat16-tight", lambda: graphs.rmat(16, 16, seed=5), 64, 1.01, 1),
     ("hec-fixed-point", lambda: graphs.heavy_bipartite(6, 600, seed=3), 4, 1.10, 1),
     ("hec-fixed-point-late", lambda: graphs.heavy_bipartite(8, 900, seed=4, leaf_pairs=True), 4, 1.10, 1),
 ]
