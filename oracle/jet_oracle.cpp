@@ -926,8 +926,11 @@ class Refiner {
     }
 
     // jet_refiner.hpp:1050-1142 (jet_refine).
+    bool changed = false;  // whether the last refine() replaced the best partition
+
     LevelRecord refine(const Graph& lg, const int32_t* lvw, int32_t* best_part, bool uniform_ew, double imb, double tol,
                        bool ultra, RefineData& best) {
+        changed = false;
         g = &lg;
         vw = lvw;
         int32_t n = lg.n;
@@ -978,10 +981,12 @@ class Refiner {
                     best = curr;
                     std::copy(part.begin(), part.end(), best_part);
                     count = 0;
+                    changed = true;
                 } else if (curr.cut < best.cut && (curr.total_imb <= imb_max || curr.total_imb <= best.total_imb)) {
                     if ((double)curr.cut < tol * (double)best.cut) count = 0;
                     best = curr;
                     std::copy(part.begin(), part.end(), best_part);
+                    changed = true;
                 }
             }
         }
@@ -1007,9 +1012,17 @@ struct Level {
 };
 
 // partitioner.hpp:61-125 with contract.hpp:494-533 and uncoarsen.hpp:88-158.
+//
+// With `repeat_shortcut`, two exact shortcuts are taken, for graphs whose
+// weighted-HEC coarsening reaches a fixed point (nc == n on a contracted
+// level, so every later level up to the 200-level limit is the same graph):
+// those levels are not rebuilt, and a repeated level is not refined again once
+// a refinement of the same graph left the partition and refine_data unchanged
+// (the next one would start from the same state and repeat it). Without it the
+// loop is the literal one; tests check that both give the same results.
 static int partition(int32_t alg, int32_t k, double imb, double tol, bool ultra, const Graph& fine,
                      const int32_t* fvw, bool uniform_ew, uint64_t seed, int32_t* part_out, int64_t* cut_out,
-                     std::vector<LevelRecord>* recs, std::vector<int32_t>* level_n) {
+                     std::vector<LevelRecord>* recs, std::vector<int32_t>* level_n, bool repeat_shortcut = false) {
     int32_t cutoff = k * 8;
     if (cutoff > 1024) {
         cutoff = k * 2;
@@ -1020,11 +1033,16 @@ static int partition(int32_t alg, int32_t k, double imb, double tol, bool ultra,
     levels[0].g = fine;
     levels[0].vw.assign(fvw, fvw + fine.n);
     levels[0].uniform = uniform_ew;
+    size_t repeats = 0;  // shortcut: identity levels above levels.back()
     while (levels.back().g.n > cutoff) {
         Level& cur = levels.back();
         std::vector<int32_t> map;
         int32_t nc = coarsen_map(cur.g, cur.vw.data(), cur.uniform, alg, seed, (int32_t)levels.size() - 1, map);
         if (nc < min_allowed) break;
+        if (repeat_shortcut && alg == 1 && levels.size() >= 2 && nc == cur.g.n) {
+            repeats = 201 - levels.size();  // the literal loop pushes copies until it holds 201 levels
+            break;
+        }
         Level next;
         contract(cur.g, cur.vw.data(), map.data(), nc, next.g, next.vw);
         next.map.swap(map);
@@ -1035,6 +1053,7 @@ static int partition(int32_t alg, int32_t k, double imb, double tol, bool ultra,
     if (level_n) {
         level_n->clear();
         for (auto& L : levels) level_n->push_back(L.g.n);
+        for (size_t r = 0; r < repeats; r++) level_n->push_back(levels.back().g.n);
     }
     const Level& cs = levels.back();
     std::vector<int32_t> part(cs.g.n);
@@ -1044,8 +1063,21 @@ static int partition(int32_t alg, int32_t k, double imb, double tol, bool ultra,
     Refiner ref(fine.n, conn_capacity(fine, k), k);
     RefineData rfd;
     for (size_t L = levels.size(); L-- > 0;) {
-        LevelRecord rec = ref.refine(levels[L].g, levels[L].vw.data(), part.data(), levels[L].uniform, imb, tol, ultra, rfd);
-        if (recs) recs->push_back(rec);
+        const size_t rounds = (L + 1 == levels.size()) ? repeats + 1 : 1;
+        bool unchanged = false;
+        LevelRecord last;
+        for (size_t r = 0; r < rounds; r++) {
+            LevelRecord rec;
+            if (unchanged) {
+                rec = last;
+            } else {
+                rec = ref.refine(levels[L].g, levels[L].vw.data(), part.data(), levels[L].uniform, imb, tol, ultra,
+                                 rfd);
+                unchanged = !ref.changed;
+                last = rec;
+            }
+            if (recs) recs->push_back(rec);
+        }
         if (L > 0) {
             const std::vector<int32_t>& map = levels[L].map;
             std::vector<int32_t> fp(levels[L - 1].g.n);
@@ -1199,7 +1231,8 @@ int orc_refine_level(int32_t n, const int64_t* rowptr, const int32_t* col, const
 // Returns 0 or the initial partitioner's error code.
 int orc_partition(int32_t alg, int32_t k, double imb, double tol, int32_t ultra, int32_t n, const int64_t* rowptr,
                   const int32_t* col, const int32_t* ew, const int32_t* vw, int32_t uniform_ew, uint64_t seed,
-                  int32_t* part_out, int64_t* cut_out, int64_t* levels_out, int32_t* nlevels_out) {
+                  int32_t* part_out, int64_t* cut_out, int64_t* levels_out, int32_t* nlevels_out,
+                  int32_t repeat_shortcut) {
     Graph g = make_graph(n, rowptr, col, ew);
     std::vector<int32_t> unit;
     if (!vw) {
@@ -1208,7 +1241,8 @@ int orc_partition(int32_t alg, int32_t k, double imb, double tol, int32_t ultra,
     }
     std::vector<LevelRecord> recs;
     std::vector<int32_t> level_n;
-    int rc = partition(alg, k, imb, tol, ultra != 0, g, vw, uniform_ew != 0, seed, part_out, cut_out, &recs, &level_n);
+    int rc = partition(alg, k, imb, tol, ultra != 0, g, vw, uniform_ew != 0, seed, part_out, cut_out, &recs, &level_n,
+                       repeat_shortcut != 0);
     if (rc != 0) return rc;
     if (nlevels_out) *nlevels_out = (int32_t)recs.size();
     if (levels_out) {

@@ -63,7 +63,7 @@ def lib():
         L.orc_partition.restype = c.c_int
         L.orc_partition.argtypes = [c.c_int32, c.c_int32, c.c_double, c.c_double, c.c_int32, c.c_int32, i64p,
                                     i32p, i32p, i32p, c.c_int32, c.c_uint64, i32p, c.POINTER(c.c_int64), i64p,
-                                    c.POINTER(c.c_int32)]
+                                    c.POINTER(c.c_int32), c.c_int32]
         L.jet_host_initpart.restype = c.c_int
         L.jet_host_initpart.argtypes = [c.c_int32, i64p, i32p, i32p, i32p, c.c_int32, c.c_double, c.c_uint64, i32p]
         _lib = L
@@ -175,7 +175,11 @@ def refine_level(g, part, k, imb, tol=0.999, ultra=False, uniform=False):
     return p, rec
 
 
-def partition(g, k, imb=1.03, alg=0, seed=0, tol=0.999, ultra=False, uniform=None):
+def partition(g, k, imb=1.03, alg=0, seed=0, tol=0.999, ultra=False, uniform=None, repeat_shortcut=False):
+    """Full multilevel partition; returns (part, cut, level records). With
+    repeat_shortcut, the exact shortcuts for a weighted-HEC coarsening fixed
+    point are taken (shared repeated levels, no re-refinement of an unchanged
+    repeated level); the default is the literal level loop."""
     n, rp, col, ew = _g(g)
     if uniform is None:
         uniform = bool(g.get("uniform", False))
@@ -184,7 +188,7 @@ def partition(g, k, imb=1.03, alg=0, seed=0, tol=0.999, ultra=False, uniform=Non
     levels = np.zeros(8 * 256, dtype=np.int64)
     nl = ctypes.c_int32(0)
     rc = lib().orc_partition(alg, k, imb, tol, int(ultra), n, rp, col, ew, _vw(g), int(uniform), seed, part,
-                             ctypes.byref(cut), levels, ctypes.byref(nl))
+                             ctypes.byref(cut), levels, ctypes.byref(nl), int(repeat_shortcut))
     if rc != 0:
         raise RuntimeError(f"oracle partition failed: {rc}")
     return part, cut.value, levels[: 8 * nl.value].reshape(-1, 8)

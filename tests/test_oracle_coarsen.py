@@ -164,3 +164,22 @@ def test_hec_fixed_point_runs_to_level_limit(pairs):
     assert int(lv[-1][2]) == n
     assert (int(lv[0][2]) == n) == (not pairs)
     assert cut == oracle.total_cut(g, part) // 2
+
+
+@pytest.mark.parametrize("name,mk,k,imb", [
+    ("bipartite", lambda: graphs.heavy_bipartite(8, 900, seed=4), 4, 1.10),
+    ("bipartite-late", lambda: graphs.heavy_bipartite(8, 900, seed=4, leaf_pairs=True), 4, 1.10),
+    ("rmat16", lambda: graphs.rmat(16, 16, seed=5), 64, 1.01),
+    ("wrmat13", lambda: graphs.with_weights(graphs.rmat(13, 8, seed=1), seed=1, wmax=4), 8, 1.01),
+    ("grid", lambda: graphs.grid2d(60, 50), 8, 1.03),
+])
+def test_repeat_shortcut_equals_literal_loop(name, mk, k, imb):
+    """The oracle's fixed-point shortcuts (levels shared, an unchanged repeated
+    level not refined again) give exactly the literal loop's partition, cut
+    and per-level records."""
+    g = mk()
+    for seed in (0, 3):
+        p1, c1, l1 = oracle.partition(g, k, imb, alg=1, seed=seed)
+        p2, c2, l2 = oracle.partition(g, k, imb, alg=1, seed=seed, repeat_shortcut=True)
+        assert c1 == c2 and np.array_equal(p1, p2)
+        assert np.array_equal(l1, l2)
