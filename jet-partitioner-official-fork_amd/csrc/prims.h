@@ -26,7 +26,9 @@ struct Mailbox {
 Mailbox& thread_mailbox();
 // Copies `bytes` (<= 8) from device memory into mailbox slot `slot` (stream-ordered).
 void peek_to_mailbox(const void* src, int bytes, int slot, hipStream_t s);
-// Reads `count` (<= MAILBOX_SLOTS) int32 words from device memory, synchronously.
+// Reads `count` (<= 8) int32 words from device memory, synchronously. Slots
+// 0-7 serve these reads and read_scalar; slots 8-15 carry the refiner's
+// per-iteration results.
 void read_i32_words(const int32_t* src, int count, int32_t* out, hipStream_t s);
 
 // out[i] = in[0] + ... + in[i-1] for i in [0, n]; out has n + 1 entries

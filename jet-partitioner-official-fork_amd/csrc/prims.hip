@@ -62,7 +62,7 @@ __global__ void k_peek_words(const int32_t* src, int count, int64_t* dst) {
 
 void read_i32_words(const int32_t* src, int count, int32_t* out, hipStream_t s) {
     if (count <= 0) return;
-    if (count > MAILBOX_SLOTS) throw std::invalid_argument("read_i32_words: too many words");
+    if (count > 8) throw std::invalid_argument("read_i32_words: at most 8 words (mailbox slots 0-7)");
     Mailbox& mb = thread_mailbox();
     k_peek_words<<<1, 64, 0, s>>>(src, count, mb.dev);
     JET_LAUNCH_CHECK();

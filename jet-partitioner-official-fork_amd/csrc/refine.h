@@ -63,9 +63,20 @@ class Refiner {
     void build_tiers(const Level& L);
     void jet_lp(const Level& L, const part_t* part, double filter_ratio);
     void rebalance(const Level& L, const part_t* part, const int64_t* sizes, bool strong);
-    void rank_flagged(const Level& L, const part_t* part, const int64_t* sizes, int64_t width, int key_bits,
-                      bool strong);
-    void perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_t& cut_change, int64_t& max_size);
+    // Bucket layout of one rebalancing step (jet_refiner.hpp:417-428, 551-560).
+    struct RebalanceShape {
+        int32_t sections = 0;
+        int64_t width = 0, nbk = 0, max_dest = 0;
+        int key_bits = 1;
+    };
+    RebalanceShape rebalance_shape(vid_t n, bool strong) const;
+    void rank_flagged(const Level& L, const part_t* part, const int64_t* sizes, int64_t width, bool strong);
+    void rank_radix(const Level& L, const part_t* part, const int64_t* sizes, int64_t width, int key_bits,
+                    bool strong);
+    void strong_tail(const Level& L, const part_t* part, const int64_t* sizes, const RebalanceShape& r);
+    void rank_deferred(const Level& L, const part_t* part, const int64_t* sizes, bool strong);
+    // Returns true when the step's ranking was deferred (nothing applied).
+    bool perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_t& cut_change, int64_t& max_size);
     // Two counter sets alternate between iterations: the first kernel of an
     // iteration clears the other set, so no fill launch is needed.
     int32_t* cnt() const { return counters_ + 8 * par_; }
@@ -96,13 +107,14 @@ class Refiner {
     part_t* cur_part_;
     vid_t* tier_mid_;
     vid_t* tier_big_;
-    // rows of tier_big_ ("hubs") take their update_small changes through
-    // k_update_hubs: flags per vertex for hub, hub with a moved neighbour,
-    // and moved in this iteration
+    // rows of degree > BIG_DEG ("hubs": tier_big_ and pull_mid_) take their
+    // update_small changes through k_update_hubs: flags per vertex for hub,
+    // hub with a moved neighbour, and moved in this iteration
     uint8_t* hub_;
     uint8_t* hub_dirty_;
     uint8_t* moved_;
-    int32_t ntier_mid_ = 0, ntier_big_ = 0;
+    vid_t* pull_mid_;  // mid-tier rows of degree > BIG_DEG
+    int32_t ntier_mid_ = 0, ntier_big_ = 0, npull_mid_ = 0;
     bool has_big_ = false;
     // small device state; counter set layout: [0] small candidates, [1] big
     // candidates, [2] small moves, [3] big moves, [4] re-draws, [5] ranked
@@ -115,7 +127,7 @@ class Refiner {
     int64_t* evict_adjust_;
     int64_t* evict_start_;
     part_t* undersized_;
-    int32_t* misc_;  // [0] undersized count, [1] error bits
+    int32_t* misc_;  // [0] undersized count, [1] error bits, [2] ranking deferred
     vid_t* redraw_;
     int64_t* buckets_;
     int32_t* bflag_;

@@ -131,17 +131,24 @@ __global__ void k_conn_caps(DGraph g, int32_t k, int32_t* caps) {
     }
 }
 
-__global__ void k_conn_tiers(DGraph g, int32_t k, vid_t* mid, vid_t* big, int32_t* cnt, uint8_t* hub) {
+// Table-build tiers (mid: one wave per row, big: one workgroup per row) and
+// the rows whose update_small changes are pulled (degree > BIG_DEG): big-tier
+// rows and pull_mid rows; hub[] flags both kinds.
+__global__ void k_conn_tiers(DGraph g, int32_t k, vid_t* mid, vid_t* big, vid_t* pull_mid, int32_t* cnt,
+                             uint8_t* hub) {
     FOR_THREAD(i, g.n) {
         int64_t d = g.rowptr[i + 1] - g.rowptr[i];
         int64_t cap = d > k ? k : d;
         bool is_mid = d > SMALL_DEG && d <= HEAVY_DEG_REFINE && cap <= WAVE_TABLE_MAX;
         bool is_big = d > SMALL_DEG && !is_mid;
-        hub[i] = is_big ? 1 : 0;
+        bool is_pmid = is_mid && d > BIG_DEG;
+        hub[i] = (is_big || is_pmid) ? 1 : 0;
         int32_t a = wave_append(&cnt[0], is_mid);
         int32_t b = wave_append(&cnt[1], is_big);
+        int32_t c = wave_append(&cnt[3], is_pmid);
         if (a >= 0) mid[a] = (vid_t)i;
         if (b >= 0) big[b] = (vid_t)i;
+        if (c >= 0) pull_mid[c] = (vid_t)i;
     }
 }
 
@@ -507,7 +514,8 @@ __global__ void k_afterburner_big(DGraph g, const vid_t* cand, const int32_t* co
 __global__ void k_moves_apply(DGraph g, const vid_t* moves, int32_t* mc, vid_t nmax, const wgt_t* vw, part_t* part,
                               part_t* dest, part_t* dest_cache, const eoff_t* conn_off, const int2* conn,
                               const int32_t* conn_size, unsigned long long* sizes, int32_t k, int64_t* acc,
-                              uint8_t* moved) {
+                              uint8_t* moved, const int32_t* deferred) {
+    if (*deferred) return;
     __shared__ int64_t red[256 / WAVE];
     __shared__ unsigned long long ls[LDS_K];
     const bool lds = k <= LDS_K;
@@ -655,7 +663,8 @@ template <bool ADD, int G>
 __global__ void k_update_small(DGraph g, const vid_t* moves, int32_t* mc, vid_t nmax, const part_t* part,
                                const part_t* dest, const eoff_t* conn_off, int2* conn, int32_t* conn_size,
                                part_t* dest_cache, int32_t k, int32_t* err, int32_t* mark, const uint8_t* hub,
-                               uint8_t* hub_dirty) {
+                               uint8_t* hub_dirty, const int32_t* deferred) {
+    if (*deferred) return;
     const bool rebuild = update_by_rebuild(mc, g.n);
     if (ADD && rebuild) return;
     if (!ADD && rebuild && blockIdx.x == 0 && threadIdx.x == 0) mc[6] = 1;  // gates the rebuild
@@ -682,7 +691,8 @@ template <bool ADD>
 __global__ void k_update_big(DGraph g, const vid_t* moves, const int32_t* mc, vid_t nmax, const part_t* part,
                              const part_t* dest, const eoff_t* conn_off, int2* conn, int32_t* conn_size,
                              part_t* dest_cache, int32_t k, int32_t* err, int32_t* mark, const uint8_t* hub,
-                             uint8_t* hub_dirty) {
+                             uint8_t* hub_dirty, const int32_t* deferred) {
+    if (*deferred) return;
     const bool rebuild = update_by_rebuild(mc, g.n);
     if (ADD && rebuild) return;
     const int32_t nb = mc[3];
@@ -706,8 +716,8 @@ __global__ void k_update_big(DGraph g, const vid_t* moves, const int32_t* mc, vi
     }
 }
 
-// update_small for hub rows (the workgroup-built tier), one workgroup per hub
-// with a moved neighbour: the row's moved neighbours are found by a scan of
+// update_small for rows of degree > BIG_DEG ("hubs": the workgroup-built tier
+// and the pull_mid rows), one workgroup per row with a moved neighbour: the row's moved neighbours are found by a scan of
 // its adjacency and their weights summed per part in LDS, then each part's
 // total is applied to the table once, with the subtraction phase before the
 // addition phase as for the other rows. The resulting table equals the one
@@ -715,21 +725,19 @@ __global__ void k_update_big(DGraph g, const vid_t* moves, const int32_t* mc, vi
 // in a table of fewer than k slots (jet_refiner.hpp:806-810), new parts take
 // free or reclaimed slots and then overflow slots (:828-884). Pushing instead
 // would land every moved neighbour's atomics on the same few hub slots.
-constexpr int HB_BLOCK = 256;
-
-template <bool ADD>
-__global__ void __launch_bounds__(HB_BLOCK)
+template <bool ADD, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
     k_update_hubs(DGraph g, const vid_t* list, int32_t nlist, const int32_t* mc, const uint8_t* moved,
                   uint8_t* hub_dirty, const part_t* part, const part_t* dest, const eoff_t* conn_off, int2* conn,
-                  int32_t* conn_size, part_t* dest_cache, int32_t k, int32_t* err) {
-    if (update_by_rebuild(mc, g.n)) return;
+                  int32_t* conn_size, part_t* dest_cache, int32_t k, int32_t* err, const int32_t* deferred) {
+    if (*deferred || update_by_rebuild(mc, g.n)) return;
     extern __shared__ int32_t delta[];  // k
     for (int32_t r = blockIdx.x; r < nlist; r += gridDim.x) {
         const vid_t u = list[r];
         if (!hub_dirty[u]) continue;
-        for (int32_t p = threadIdx.x; p < k; p += HB_BLOCK) delta[p] = 0;
+        for (int32_t p = threadIdx.x; p < k; p += BLOCK) delta[p] = 0;
         __syncthreads();
-        for (eoff_t j = g.rowptr[u] + threadIdx.x; j < g.rowptr[u + 1]; j += HB_BLOCK) {
+        for (eoff_t j = g.rowptr[u] + threadIdx.x; j < g.rowptr[u + 1]; j += BLOCK) {
             vid_t v = g.col[j];
             if (moved[v]) atomicAdd(&delta[ADD ? part[v] : dest[v]], g.ew[j]);
         }
@@ -737,7 +745,7 @@ __global__ void __launch_bounds__(HB_BLOCK)
         int2* tab = conn + conn_off[u];
         const int32_t cap = (int32_t)(conn_off[u + 1] - conn_off[u]);
         const int32_t vsz = conn_size[u];
-        for (int32_t p = threadIdx.x; p < k; p += HB_BLOCK) {
+        for (int32_t p = threadIdx.x; p < k; p += BLOCK) {
             const int32_t w = delta[p];
             if (w == 0) continue;
             if (ADD) conn_add(tab, &conn_size[u], cap, p, w, err);
@@ -754,8 +762,16 @@ __global__ void __launch_bounds__(HB_BLOCK)
 // cut change part 2 and the largest part (jet_refiner.hpp:937-948).
 __global__ void k_moves_finish(const vid_t* moves, const int32_t* mc, vid_t nmax, const part_t* part,
                                const part_t* dest, const eoff_t* conn_off, const int2* conn, const int32_t* conn_size,
-                               const int64_t* sizes, int32_t k, int64_t* acc, int64_t* host_out, uint8_t* moved) {
+                               const int64_t* sizes, int32_t k, int64_t* acc, int64_t* host_out, uint8_t* moved,
+                               const int32_t* deferred) {
     __shared__ int64_t red[256 / WAVE];
+    if (*deferred) {  // nothing was applied: report the deferral
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            host_out[2] = 1;
+            __threadfence_system();
+        }
+        return;
+    }
     const int32_t ns = mc[2], total = mc[2] + mc[3];
     int64_t cc = 0;
     FOR_THREAD(x, total) {
@@ -784,6 +800,7 @@ __global__ void k_moves_finish(const vid_t* moves, const int32_t* mc, vid_t nmax
             int64_t m2 = __hip_atomic_load(&acc[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             host_out[0] = c1 + c2;
             host_out[1] = m2;
+            host_out[2] = 0;
             __threadfence_system();
             acc[0] = 0;
             acc[1] = 0;
@@ -1070,13 +1087,16 @@ __global__ void __launch_bounds__(RS_BLOCK)
     k_rank_small(const vid_t* rlist, const int32_t* counters, const int32_t* bid, const part_t* part,
                  const int64_t* off, const int64_t* sizes, int64_t size_max, int64_t width, const wgt_t* vw,
                  const eoff_t* rowptr, int64_t* evict_adjust, int64_t* score, vid_t* moves, int32_t* mcounters,
-                 vid_t nmax, int32_t* err) {
+                 vid_t nmax, int32_t* deferred) {
     __shared__ uint64_t keys[RS_CAP];
     __shared__ int64_t pre[RS_CAP];
     __shared__ int64_t ws[RS_BLOCK / WAVE + 1];
     const int32_t R = counters[5];
     if (R > RS_CAP) {
-        if (threadIdx.x == 0) atomicOr(err, 8);
+        // too many members for one workgroup: the host ranks them with the
+        // radix path after the iteration's sync; until then the iteration's
+        // remaining kernels see the flag and do nothing
+        if (threadIdx.x == 0) *deferred = 1;
         return;
     }
     uint32_t cap = 1;
@@ -1172,7 +1192,8 @@ __global__ void k_rank_apply(int32_t R, const uint64_t* keys, const int64_t* pre
 constexpr int CK_BLOCK = 1024;
 __global__ void __launch_bounds__(CK_BLOCK)
     k_cookie(int32_t k, int64_t width, const int64_t* off, const int64_t* sizes, int64_t max_dest,
-             int64_t* evict_adjust, int64_t* evict_start, int64_t* acc) {
+             int64_t* evict_adjust, int64_t* evict_start, int64_t* acc, const int32_t* deferred) {
+    if (*deferred) return;
     __shared__ int64_t ws[CK_BLOCK / WAVE + 1];
     int64_t run = 0, run2 = 0;
     for (int32_t p0 = 0; p0 < k; p0 += CK_BLOCK) {
@@ -1215,7 +1236,9 @@ __device__ inline int32_t chunk_of(const int64_t* es, int32_t lo, int32_t k, int
 // jet_refiner.hpp:519-544 without the re-draws, which are queued.
 __global__ void k_strong_dest(const vid_t* moves, const int32_t* mc, vid_t nmax, const part_t* part, const wgt_t* vw,
                               const int64_t* evict_adjust, const int64_t* evict_start, int32_t k, int64_t* score,
-                              part_t* dest, vid_t* redraw, int32_t* rcount, int32_t rmax) {
+                              part_t* dest, vid_t* redraw, int32_t* rcount, int32_t rmax,
+                              const int32_t* deferred) {
+    if (*deferred) return;
     const int32_t ns = mc[2], total = mc[2] + mc[3];
     FOR_THREAD(x, total) {
         vid_t v = list_at(moves, ns, nmax, x);
@@ -1240,8 +1263,10 @@ __global__ void k_strong_dest(const vid_t* moves, const int32_t* mc, vid_t nmax,
 constexpr int RD_MAX = 8192;
 __global__ void __launch_bounds__(256) k_strong_redraw(vid_t* redraw, const int32_t* rcount, const part_t* part,
                                                        const wgt_t* vw, const int64_t* evict_start, int32_t k,
-                                                       const int64_t* score, part_t* dest, int64_t* acc, int32_t* err) {
+                                                       const int64_t* score, part_t* dest, int64_t* acc, int32_t* err,
+                                                       const int32_t* deferred) {
     __shared__ vid_t ids[RD_MAX];
+    if (*deferred) return;
     int32_t r = rcount[0];
     if (r > RD_MAX) {
         if (threadIdx.x == 0) atomicOr(err, 4);
@@ -1353,6 +1378,7 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
     tier_mid_ = a.alloc<vid_t>(N);
     tier_big_ = a.alloc<vid_t>(N);
     hub_ = a.alloc<uint8_t>(N);
+    pull_mid_ = a.alloc<vid_t>(N);
     hub_dirty_ = a.alloc<uint8_t>(N);
     moved_ = a.alloc<uint8_t>(N);
     counters_ = a.alloc<int32_t>(32);
@@ -1380,15 +1406,17 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
 
 void Refiner::build_tiers(const Level& L) {
     DGraph g = L.view();
-    fill_i32(counters_ + 16, 0, 3, s_);
-    k_conn_tiers<<<grid_for(g.n, 256, 8192), 256, 0, s_>>>(g, p_.k, tier_mid_, tier_big_, counters_ + 16, hub_);
+    fill_i32(counters_ + 16, 0, 4, s_);
+    k_conn_tiers<<<grid_for(g.n, 256, 8192), 256, 0, s_>>>(g, p_.k, tier_mid_, tier_big_, pull_mid_, counters_ + 16,
+                                                            hub_);
     k_max_degree<<<grid_for(g.n, 256, 2048), 256, 0, s_>>>(g, counters_ + 18);
     JET_LAUNCH_CHECK();
-    int32_t c[3];
-    read_i32_words(counters_ + 16, 3, c, s_);
+    int32_t c[4];
+    read_i32_words(counters_ + 16, 4, c, s_);
     ntier_mid_ = c[0];
     ntier_big_ = c[1];
     has_big_ = c[2] > BIG_DEG;
+    npull_mid_ = c[3];
 }
 
 void Refiner::init_conn(const Level& L, const part_t* part, const int32_t* mark) {
@@ -1404,12 +1432,13 @@ void Refiner::init_conn(const Level& L, const part_t* part, const int32_t* mark)
         fill_i32(dest_cache_, NULL_PART, g.n, s_);
         fill_i32(lock_, 0, g.n, s_);
     }
-    k_conn_build_small<<<grid_for(g.n, SB_BLOCK, 8192), SB_BLOCK, 0, s_>>>(g, part, p_.k, conn_off_, conn_, conn_size_,
-                                                                           mk, dest_cache_, gate);
+    // the gated rebuild after moves usually exits at once: keep its grid small
+    k_conn_build_small<<<grid_for(g.n, SB_BLOCK, mark ? 1024 : 8192), SB_BLOCK, 0, s_>>>(
+        g, part, p_.k, conn_off_, conn_, conn_size_, mk, dest_cache_, gate);
     JET_LAUNCH_CHECK();
     if (ntier_mid_ > 0) {
         const size_t lds = 2 * sizeof(int2) * (size_t)WB_WAVES * std::min(p_.k, WAVE_TABLE_MAX);
-        k_conn_build_wave<<<grid_for(ntier_mid_, WB_WAVES, 8192), WB_WAVES * WAVE, lds, s_>>>(
+        k_conn_build_wave<<<grid_for(ntier_mid_, WB_WAVES, mark ? 1024 : 8192), WB_WAVES * WAVE, lds, s_>>>(
             g, part, p_.k, conn_off_, conn_, conn_size_, mk, dest_cache_, tier_mid_, ntier_mid_, gate);
         JET_LAUNCH_CHECK();
     }
@@ -1447,24 +1476,30 @@ void Refiner::jet_lp(const Level& L, const part_t* part, double fr) {
 }
 
 // Ranks the members of flagged buckets (counter 5 of them) and applies the
-// eviction decisions.
-void Refiner::rank_flagged(const Level& L, const part_t* part, const int64_t* sizes, int64_t width, int key_bits,
-                           bool strong) {
+// eviction decisions. The single-workgroup ranker is launched without reading
+// the count on the host; when the count exceeds its capacity it raises the
+// deferral flag (misc_[2]), the iteration's remaining kernels do nothing, and
+// rank_deferred() finishes the step with the radix path after the sync.
+void Refiner::rank_flagged(const Level& L, const part_t* part, const int64_t* sizes, int64_t width, bool strong) {
+    DGraph g = L.view();
+    int32_t* c = cnt();
+    int64_t smax = size_max_;
+    if (strong)
+        k_rank_small<true><<<1, RS_BLOCK, 0, s_>>>(cand_, c, bid_, part, buckets_, sizes, smax, width, g.vw, g.rowptr,
+                                                  evict_adjust_, score_, moves_, c, nmax_, misc_ + 2);
+    else
+        k_rank_small<false><<<1, RS_BLOCK, 0, s_>>>(cand_, c, bid_, part, buckets_, sizes, smax, width, g.vw,
+                                                   g.rowptr, evict_adjust_, score_, moves_, c, nmax_, misc_ + 2);
+    JET_LAUNCH_CHECK();
+}
+
+void Refiner::rank_radix(const Level& L, const part_t* part, const int64_t* sizes, int64_t width, int key_bits,
+                         bool strong) {
     DGraph g = L.view();
     int32_t* c = cnt();
     int64_t smax = size_max_;
     int32_t R = read_scalar(c + 5, s_);
     if (R == 0) return;
-    if (R <= RS_CAP) {
-        if (strong)
-            k_rank_small<true><<<1, RS_BLOCK, 0, s_>>>(cand_, c, bid_, part, buckets_, sizes, smax, width, g.vw,
-                                                      g.rowptr, evict_adjust_, score_, moves_, c, nmax_, misc_ + 1);
-        else
-            k_rank_small<false><<<1, RS_BLOCK, 0, s_>>>(cand_, c, bid_, part, buckets_, sizes, smax, width, g.vw,
-                                                       g.rowptr, evict_adjust_, score_, moves_, c, nmax_, misc_ + 1);
-        JET_LAUNCH_CHECK();
-        return;
-    }
     ArenaScope sc(a_);
     uint64_t* keys = a_.alloc<uint64_t>(R);
     uint64_t* ktmp = a_.alloc<uint64_t>(R);
@@ -1484,23 +1519,58 @@ void Refiner::rank_flagged(const Level& L, const part_t* part, const int64_t* si
     JET_LAUNCH_CHECK();
 }
 
+Refiner::RebalanceShape Refiner::rebalance_shape(vid_t n, bool strong) const {
+    RebalanceShape r;
+    const int32_t k = p_.k;
+    r.sections = sections_for(n, k);
+    r.width = (int64_t)MAX_BUCKETS * r.sections;
+    // weak: gain bucket 50 is not capped (jet_refiner.hpp:349-351) and spills
+    // past the last part's minibuckets, so `sections` more are kept and scanned
+    r.nbk = strong ? (int64_t)k * r.width + 1 : (int64_t)k * r.width + r.sections + 1;
+    const int64_t smax = size_max_;
+    if (strong) {
+        r.max_dest = std::max<int64_t>((int64_t)opt_ + 1, (int64_t)(smax * 0.99));
+    } else {
+        r.max_dest = (int64_t)(smax * 0.99);
+        if (r.max_dest < smax - 100) r.max_dest = smax - 100;
+    }
+    r.key_bits = 1;
+    while ((int64_t(1) << r.key_bits) <= r.nbk) r.key_bits++;
+    return r;
+}
+
+// rebalance_strong's destination step (jet_refiner.hpp:480-544), after ranking.
+void Refiner::strong_tail(const Level& L, const part_t* part, const int64_t* sizes, const RebalanceShape& r) {
+    DGraph g = L.view();
+    int32_t* c = cnt();
+    const int32_t k = p_.k;
+    int nb = grid_for(g.n, 256, 8192);
+    k_cookie<<<1, CK_BLOCK, 0, s_>>>(k, r.width, buckets_, sizes, r.max_dest, evict_adjust_, evict_start_, acc_,
+                                     misc_ + 2);
+    k_strong_dest<<<nb, 256, 0, s_>>>(moves_, c, nmax_, part, g.vw, evict_adjust_, evict_start_, k, score_, dest_,
+                                      redraw_, c + 4, RD_MAX, misc_ + 2);
+    k_strong_redraw<<<1, 256, 0, s_>>>(redraw_, c + 4, part, g.vw, evict_start_, k, score_, dest_, acc_, misc_ + 1,
+                                       misc_ + 2);
+    JET_LAUNCH_CHECK();
+}
+
+// Finishes a step whose ranking was deferred: clears the flag, ranks with the
+// radix path and runs the strong destination step.
+void Refiner::rank_deferred(const Level& L, const part_t* part, const int64_t* sizes, bool strong) {
+    fill_i32(misc_ + 2, 0, 1, s_);
+    RebalanceShape r = rebalance_shape(L.n, strong);
+    rank_radix(L, part, sizes, r.width, r.key_bits, strong);
+    if (strong) strong_tail(L, part, sizes, r);
+}
+
 void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes, bool strong) {
     DGraph g = L.view();
     const int32_t k = p_.k;
     int32_t* c = cnt();
-    int32_t sections = sections_for(g.n, k);
-    int64_t width = (int64_t)MAX_BUCKETS * sections;
-    // weak: gain bucket 50 is not capped (jet_refiner.hpp:349-351) and spills
-    // past the last part's minibuckets, so `sections` more are kept and scanned
-    int64_t nbk = strong ? (int64_t)k * width + 1 : (int64_t)k * width + sections + 1;
-    int64_t smax = size_max_;
-    int64_t max_dest;
-    if (strong) {
-        max_dest = std::max<int64_t>((int64_t)opt_ + 1, (int64_t)(smax * 0.99));
-    } else {
-        max_dest = (int64_t)(smax * 0.99);
-        if (max_dest < smax - 100) max_dest = smax - 100;
-    }
+    const RebalanceShape r = rebalance_shape(g.n, strong);
+    const int32_t sections = r.sections;
+    const int64_t width = r.width, nbk = r.nbk, max_dest = r.max_dest;
+    const int64_t smax = size_max_;
     k_reset_buckets<<<grid_for(nbk + 1, 256, 2048), 256, 0, s_>>>(buckets_, bflag_, nbk + 1, cnt_next(), sizes, k,
                                                                     max_dest, strong ? 1 : 0, undersized_, misc_, acc_,
                                                                     evict_adjust_);
@@ -1524,40 +1594,33 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
                                                           c, nmax_);
     k_collect<<<cb, LS_WAVES * WAVE, 0, s_>>>(g.n, bid_, bflag_, cand_, c);
     JET_LAUNCH_CHECK();
-    int bits = 1;
-    while ((int64_t(1) << bits) <= nbk) bits++;
-    rank_flagged(L, part, sizes, width, bits, strong);
-    if (strong) {
-        k_cookie<<<1, CK_BLOCK, 0, s_>>>(k, width, buckets_, sizes, max_dest, evict_adjust_, evict_start_, acc_);
-        k_strong_dest<<<nb, 256, 0, s_>>>(moves_, c, nmax_, part, g.vw, evict_adjust_, evict_start_, k, score_, dest_,
-                                          redraw_, c + 4, RD_MAX);
-        k_strong_redraw<<<1, 256, 0, s_>>>(redraw_, c + 4, part, g.vw, evict_start_, k, score_, dest_, acc_,
-                                           misc_ + 1);
-        JET_LAUNCH_CHECK();
-    }
+    rank_flagged(L, part, sizes, width, strong);
+    if (strong) strong_tail(L, part, sizes, r);
 }
 
-void Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_t& cut_change, int64_t& max_size) {
+bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_t& cut_change, int64_t& max_size) {
     DGraph g = L.view();
     const int32_t k = p_.k;
     int32_t* c = cnt();
-    // moves are at most n/10 on the push path; a modest grid keeps the
-    // completion ticket of k_moves_finish cheap
-    int nb = grid_for(g.n, 256, 512);
+    // a small grid-stride grid: the moves are usually few, and every extra
+    // workgroup costs a flush of its part-size deltas (k_moves_apply) and a
+    // serialised completion ticket (k_moves_finish)
+    int nb = grid_for(g.n, 256, 128);
     k_moves_apply<<<nb, 256, 0, s_>>>(g, moves_, c, nmax_, g.vw, part, dest_, dest_cache_, conn_off_, conn_,
-                                      conn_size_, reinterpret_cast<unsigned long long*>(sizes), k, acc_, moved_);
+                                      conn_size_, reinterpret_cast<unsigned long long*>(sizes), k, acc_, moved_,
+                                      misc_ + 2);
     JET_LAUNCH_CHECK();
     int G = group_for_avg(g);
-    int nbs = grid_for((int64_t)g.n * G, 256, 4096);
-    int nbb = grid_for((int64_t)g.n * WAVE, 256, 2048);
+    int nbs = grid_for((int64_t)g.n * G, 256, 1024);
+    int nbb = grid_for((int64_t)g.n * WAVE, 256, 512);
     for (int add = 0; add < 2; add++) {
 #define JET_UPD(GG)                                                                                                   \
     if (add) k_update_small<true, GG><<<nbs, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,       \
                                                           conn_size_, dest_cache_, k, misc_ + 1, swap_bit_,      \
-                                                          hub_, hub_dirty_);                                     \
+                                                          hub_, hub_dirty_, misc_ + 2);                         \
     else k_update_small<false, GG><<<nbs, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,          \
                                                         conn_size_, dest_cache_, k, misc_ + 1, swap_bit_,        \
-                                                        hub_, hub_dirty_);
+                                                        hub_, hub_dirty_, misc_ + 2);
         switch (G) {
             case 4: JET_UPD(4) break;
             case 8: JET_UPD(8) break;
@@ -1569,32 +1632,45 @@ void Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
         if (has_big_) {
             if (add) k_update_big<true><<<nbb, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,
                                                              conn_size_, dest_cache_, k, misc_ + 1, swap_bit_, hub_,
-                                                             hub_dirty_);
+                                                             hub_dirty_, misc_ + 2);
             else k_update_big<false><<<nbb, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,
                                                            conn_size_, dest_cache_, k, misc_ + 1, swap_bit_, hub_,
-                                                           hub_dirty_);
+                                                           hub_dirty_, misc_ + 2);
         }
+        // pulled rows: big-tier rows by a 256-thread workgroup, rows of
+        // degree (BIG_DEG, HEAVY_DEG_REFINE] by a single-wave workgroup
+        const size_t lds = sizeof(int32_t) * (size_t)k;
         if (ntier_big_ > 0) {
-            const size_t lds = sizeof(int32_t) * (size_t)k;
             const int hb = std::min(ntier_big_, 2048);
-            if (add) k_update_hubs<true><<<hb, HB_BLOCK, lds, s_>>>(g, tier_big_, ntier_big_, c, moved_, hub_dirty_,
+            if (add) k_update_hubs<true, 256><<<hb, 256, lds, s_>>>(g, tier_big_, ntier_big_, c, moved_, hub_dirty_,
                                                                     part, dest_, conn_off_, conn_, conn_size_,
-                                                                    dest_cache_, k, misc_ + 1);
-            else k_update_hubs<false><<<hb, HB_BLOCK, lds, s_>>>(g, tier_big_, ntier_big_, c, moved_, hub_dirty_,
+                                                                    dest_cache_, k, misc_ + 1, misc_ + 2);
+            else k_update_hubs<false, 256><<<hb, 256, lds, s_>>>(g, tier_big_, ntier_big_, c, moved_, hub_dirty_,
                                                                  part, dest_, conn_off_, conn_, conn_size_,
-                                                                 dest_cache_, k, misc_ + 1);
+                                                                 dest_cache_, k, misc_ + 1, misc_ + 2);
+        }
+        if (npull_mid_ > 0) {
+            const int hb = std::min(npull_mid_, 8192);
+            if (add) k_update_hubs<true, WAVE><<<hb, WAVE, lds, s_>>>(g, pull_mid_, npull_mid_, c, moved_, hub_dirty_,
+                                                                      part, dest_, conn_off_, conn_, conn_size_,
+                                                                      dest_cache_, k, misc_ + 1, misc_ + 2);
+            else k_update_hubs<false, WAVE><<<hb, WAVE, lds, s_>>>(g, pull_mid_, npull_mid_, c, moved_, hub_dirty_,
+                                                                   part, dest_, conn_off_, conn_, conn_size_,
+                                                                   dest_cache_, k, misc_ + 1, misc_ + 2);
         }
         JET_LAUNCH_CHECK();
     }
     init_conn(L, part, swap_bit_);  // rebuild marked rows (gated on the device)
     k_moves_finish<<<nb, 256, 0, s_>>>(moves_, c, nmax_, part, dest_, conn_off_, conn_, conn_size_, sizes, k, acc_,
-                                       res_dev_, moved_);
+                                       res_dev_, moved_, misc_ + 2);
     JET_LAUNCH_CHECK();
     JET_HIP(hipStreamSynchronize(s_));
     volatile int64_t* res = res_host_;
+    if (res[2]) return true;  // ranking deferred: nothing applied, counters kept
     cut_change = res[0];
     max_size = res[1];
     par_ ^= 1;
+    return false;
 }
 
 LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
@@ -1645,16 +1721,22 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
         int count = 0;
         while (count++ <= 11) {
             rec.iterations++;
+            bool strong = false;
             if (curr_imb <= imb_max) {
                 jet_lp(L, cur_part_, fr);
                 balance_counter = 0;
                 rec.lp_iterations++;
             } else {
-                rebalance(L, cur_part_, curr_sizes_, balance_counter >= 2);
+                strong = balance_counter >= 2;
+                rebalance(L, cur_part_, curr_sizes_, strong);
                 balance_counter++;
             }
             int64_t cc = 0, mx = 0;
-            perform_moves(L, cur_part_, curr_sizes_, cc, mx);
+            if (perform_moves(L, cur_part_, curr_sizes_, cc, mx)) {
+                rank_deferred(L, cur_part_, curr_sizes_, strong);
+                if (perform_moves(L, cur_part_, curr_sizes_, cc, mx))
+                    throw DeviceError("rebalancing step deferred twice");
+            }
             curr_imb = mx > opt_ ? mx - opt_ : 0;
             curr_cut -= cc;
             bool take = false;
@@ -1703,7 +1785,10 @@ void Refiner::single_step(const Level& L, const part_t* part, int kind, double f
     build_tiers(L);
     init_conn(L, cur_part_, nullptr);
     if (kind == 0) jet_lp(L, cur_part_, fr);
-    else rebalance(L, cur_part_, curr_sizes_, kind == 2);
+    else {
+        rebalance(L, cur_part_, curr_sizes_, kind == 2);
+        if (read_scalar(misc_ + 2, s_)) rank_deferred(L, cur_part_, curr_sizes_, kind == 2);
+    }
     int32_t c4[4];
     JET_HIP(hipMemcpyAsync(c4, cnt(), sizeof(c4), hipMemcpyDeviceToHost, s_));
     JET_HIP(hipStreamSynchronize(s_));
