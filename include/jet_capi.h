@@ -204,6 +204,48 @@ int jet_generate_stencil3d(int64_t nx, int64_t ny, int64_t nz, uint64_t seed, in
 int jet_generate_rmat(int32_t scale, int32_t ef, double a, double b, double c, uint64_t seed, int64_t* rowptr,
                       int32_t* col, int64_t* m_out);
 
+/* ---- Files and quality metrics of the reference's drivers (app/io.hpp,
+ * app/part_eval.cpp, src/part_stat.hpp). */
+
+/* A CSR graph in host memory allocated by the library (jet_free_graph). */
+typedef struct jet_host_graph_t {
+    int64_t n;
+    int64_t m; /* stored entries: both directions */
+    int64_t* row_offsets;
+    int32_t* col;
+    int32_t* ew; /* NULL when the file has no edge weights */
+    int32_t uniform_ew;
+} jet_host_graph_t;
+
+/* app/io.hpp:96-205: METIS graph file (header "n m [fmt [ncon]]", '%'
+ * comments, 1-based neighbours, fmt 1 = edge weights). Vertex weights and
+ * ncon are JET_EUNSUPPORTED, as in the reference. */
+int jet_load_metis(const char* path, jet_host_graph_t* out);
+void jet_free_graph(jet_host_graph_t* g);
+
+/* app/io.hpp:50-81: config file of 4 values (coarsening_alg, num_parts,
+ * num_iter, max_imb_ratio) and an optional 5th (ultra_settings); the other
+ * fields get config_t's defaults. */
+int jet_load_config(const char* path, jet_config_t* cfg);
+
+/* app/io.hpp:207-235: one part id per line. */
+int jet_write_partition(const char* path, int64_t n, const int32_t* part);
+int jet_read_partition(const char* path, int64_t n, int32_t* part);
+
+/* src/part_stat.hpp:77-239 and app/part_eval.cpp:46-115, computed on the GPU. */
+typedef struct jet_quality_t {
+    int64_t edge_cut;     /* undirected */
+    int64_t comm_size;    /* sum over vertices of the other parts among their neighbours */
+    int64_t max_part_cut; /* largest cut weight incident to one part */
+    int64_t largest_part;
+    int64_t smallest_part;
+    double max_imb; /* largest / ceil(total / k) */
+    double min_imb; /* smallest / ceil(total / k) */
+} jet_quality_t;
+
+int jet_partition_quality(int64_t n, const int64_t* row_offsets, const int32_t* col, const int32_t* ew,
+                          const int32_t* vw, const int32_t* part, int32_t k, int32_t ptr_space, jet_quality_t* out);
+
 /* ---- Batched and served partitioning (one GPU, several jobs in flight).
  *
  * A context owns `workers` GPU workers, each with its own HIP stream and
@@ -256,11 +298,13 @@ int jet_partition_batch(jet_ctx_t* ctx, int32_t count, const jet_config_t* cfgs,
                         const int32_t* col, const int32_t* ew, const int32_t* vw, const int32_t* uniform_ew,
                         int32_t ptr_space, int32_t* part_out, int64_t* cuts_out, jet_stats_t* stats);
 
-/* ---- Kernel timing for measurement tools: when enabled, the library records
- * HIP events around every launch of the named kernel family on its stream
- * and accumulates launch count, total milliseconds and algorithmic bytes.
- * Families: "lp_select", "afterburner", "conn_build", "update", "contract",
- * "hec", "project". */
+/* ---- Kernel timing for measurement tools. jet_prof_enable("lp_select")
+ * times every later launch of that kernel family on the device's own clock
+ * (earliest workgroup start to latest workgroup end, per launch);
+ * "lp_select:count" instead counts the items its algorithmic bytes follow
+ * from (a separate run, since counting adds work to the kernel); "none"
+ * stops. Families: "lp_select". jet_prof_read returns the launches, their
+ * total milliseconds (timing) and their algorithmic bytes (counting). */
 int jet_prof_enable(const char* family);
 int jet_prof_read(int64_t* launches, double* total_ms, double* bytes);
 

@@ -19,17 +19,148 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
 #include <queue>
 #include <thread>
 #include <vector>
 
+#include <sys/mman.h>
+
 namespace {
+
+// Blocks of 2 MiB or more for BigVec: mapped with transparent huge pages
+// requested, and kept after release for reuse by later arrays of a similar
+// size (best fit within 2x), since first-touch page faults, not arithmetic,
+// dominated bisections of graphs with tens of millions of edges. Emptied at
+// the end of every jet_host_initpart call.
+class BlockPool {
+  public:
+    static constexpr size_t HUGE_BYTES = size_t(2) << 20;
+    void* take(size_t bytes, size_t& got) {
+        bytes = (bytes + HUGE_BYTES - 1) / HUGE_BYTES * HUGE_BYTES;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            auto it = free_.lower_bound(bytes);
+            if (it != free_.end() && it->first <= 2 * bytes) {
+                got = it->first;
+                void* p = it->second;
+                free_.erase(it);
+                return p;
+            }
+        }
+        void* m = mmap(nullptr, bytes, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+        if (m == MAP_FAILED) throw std::bad_alloc();
+        (void)madvise(m, bytes, MADV_HUGEPAGE);
+        got = bytes;
+        return m;
+    }
+    void give(void* p, size_t bytes) {
+        std::lock_guard<std::mutex> lk(mu_);
+        free_.emplace(bytes, p);
+    }
+    void trim() {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto& kv : free_) munmap(kv.second, kv.first);
+        free_.clear();
+    }
+
+  private:
+    std::mutex mu_;
+    std::multimap<size_t, void*> free_;
+};
+
+BlockPool& block_pool() {
+    static BlockPool* p = new BlockPool();
+    return *p;
+}
+
+// Owned array for the large per-graph arrays: no zero fill on resize; blocks
+// of 2 MiB or more come from the block pool.
+template <class T>
+class BigVec {
+  public:
+    BigVec() = default;
+    BigVec(const BigVec&) = delete;
+    BigVec& operator=(const BigVec&) = delete;
+    BigVec(BigVec&& o) noexcept { swap(o); }
+    BigVec& operator=(BigVec&& o) noexcept {
+        BigVec t(std::move(o));
+        swap(t);
+        return *this;
+    }
+    ~BigVec() { release(); }
+
+    size_t size() const { return n_; }
+    bool empty() const { return n_ == 0; }
+    T* data() { return p_; }
+    const T* data() const { return p_; }
+    T& operator[](size_t i) { return p_[i]; }
+    const T& operator[](size_t i) const { return p_[i]; }
+    T* begin() { return p_; }
+    T* end() { return p_ + n_; }
+    const T* begin() const { return p_; }
+    const T* end() const { return p_ + n_; }
+
+    // Contents beyond the old size are unspecified.
+    void resize(size_t n) {
+        if (n > cap_) grow(n);
+        n_ = n;
+    }
+    void assign(size_t n, T v) {
+        resize(n);
+        std::fill(p_, p_ + n, v);
+    }
+    void assign(const T* b, const T* e) {
+        resize((size_t)(e - b));
+        std::memcpy(p_, b, sizeof(T) * n_);
+    }
+
+  private:
+    static constexpr size_t HUGE_BYTES = BlockPool::HUGE_BYTES;
+    void swap(BigVec& o) {
+        std::swap(p_, o.p_);
+        std::swap(n_, o.n_);
+        std::swap(cap_, o.cap_);
+        std::swap(mapped_, o.mapped_);
+    }
+    void grow(size_t cap) {
+        size_t bytes = sizeof(T) * cap;
+        T* q = nullptr;
+        bool mapped = false;
+        if (bytes >= HUGE_BYTES) {
+            size_t got = 0;
+            q = static_cast<T*>(block_pool().take(bytes, got));
+            mapped = true;
+            cap = got / sizeof(T);
+        } else {
+            q = static_cast<T*>(std::malloc(bytes > 0 ? bytes : 1));
+            if (!q) throw std::bad_alloc();
+        }
+        if (n_ > 0) std::memcpy(q, p_, sizeof(T) * n_);
+        release();
+        p_ = q;
+        cap_ = cap;
+        mapped_ = mapped;
+    }
+    void release() {
+        if (!p_) return;
+        if (mapped_) block_pool().give(p_, sizeof(T) * cap_);
+        else std::free(p_);
+        p_ = nullptr;
+        cap_ = 0;
+    }
+    T* p_ = nullptr;
+    size_t n_ = 0, cap_ = 0;
+    bool mapped_ = false;
+};
 
 struct Sub {
     int32_t n = 0;
-    std::vector<int64_t> xadj;
-    std::vector<int32_t> adj, adjw, vw;
-    std::vector<int32_t> gid;  // local -> global vertex id
+    BigVec<int64_t> xadj;
+    BigVec<int32_t> adj, adjw, vw;
+    BigVec<int32_t> gid;  // local -> global vertex id
     int64_t total_w = 0;
 };
 
@@ -483,24 +614,37 @@ static std::vector<int8_t> bisect_multilevel(const Sub& s, double frac, double u
 static Sub induced(const Sub& s, const std::vector<int8_t>& side, int8_t which) {
     Sub r;
     std::vector<int32_t> loc(s.n, -1);
+    int32_t nr = 0;
     for (int32_t v = 0; v < s.n; v++)
-        if (side[v] == which) {
-            loc[v] = r.n++;
-            r.gid.push_back(s.gid[v]);
-            r.vw.push_back(s.vw[v]);
-            r.total_w += s.vw[v];
-        }
-    r.xadj.assign(1, 0);
-    for (int32_t v = 0; v < s.n; v++) {
-        if (side[v] != which) continue;
+        if (side[v] == which) loc[v] = nr++;
+    r.n = nr;
+    r.gid.resize(nr);
+    r.vw.resize(nr);
+    r.xadj.resize((size_t)nr + 1);
+    r.xadj[0] = 0;
+    for (int32_t v = 0; v < s.n; v++) {  // pass 1: kept edges per kept vertex
+        if (loc[v] < 0) continue;
+        int64_t c = 0;
+        for (int64_t j = s.xadj[v]; j < s.xadj[v + 1]; j++) c += loc[s.adj[j]] >= 0;
+        r.xadj[loc[v] + 1] = c;
+        r.gid[loc[v]] = s.gid[v];
+        r.vw[loc[v]] = s.vw[v];
+        r.total_w += s.vw[v];
+    }
+    for (int32_t i = 0; i < nr; i++) r.xadj[i + 1] += r.xadj[i];
+    r.adj.resize(r.xadj[nr]);
+    r.adjw.resize(r.xadj[nr]);
+    for (int32_t v = 0; v < s.n; v++) {  // pass 2: fill
+        if (loc[v] < 0) continue;
+        int64_t o = r.xadj[loc[v]];
         for (int64_t j = s.xadj[v]; j < s.xadj[v + 1]; j++) {
-            int32_t u = s.adj[j];
-            if (loc[u] >= 0) {
-                r.adj.push_back(loc[u]);
-                r.adjw.push_back(s.adjw[j]);
+            int32_t u = loc[s.adj[j]];
+            if (u >= 0) {
+                r.adj[o] = u;
+                r.adjw[o] = s.adjw[j];
+                o++;
             }
         }
-        r.xadj.push_back((int64_t)r.adj.size());
     }
     return r;
 }
@@ -572,5 +716,7 @@ extern "C" int jet_host_initpart(int32_t n, const int64_t* xadj, const int32_t* 
     while ((1 << depth) < k) depth++;
     double ub = depth > 0 ? std::pow(imb, 1.0 / depth) : imb;
     recurse(s, k, 0, ub, seed, part, host_threads());
+    s = Sub();
+    block_pool().trim();
     return 0;
 }

@@ -120,6 +120,15 @@ static T read_one(Context& C, const T* p, int space) {
     return read_scalar(p, C.stream);
 }
 
+static void check_k(int32_t k) {
+    if (k < 1 || k > JET_MAX_PARTS) throw InvalidInput("k must be in [1, 8192]");
+}
+
+// Part ids index per-part arrays on the device: reject ids outside [0, k).
+static void check_parts(Context& C, int64_t n, const part_t* dev_part, int32_t k) {
+    if (!parts_in_range(n, dev_part, k, C.scratch, C.stream)) throw InvalidInput("part ids must be in [0, k)");
+}
+
 static void check_space(int space) {
     if (space != JET_PTR_HOST && space != JET_PTR_DEVICE) throw InvalidInput("ptr_space must be JET_PTR_HOST or JET_PTR_DEVICE");
 }
@@ -583,10 +592,12 @@ int jet_conn_tables(int64_t n, const int64_t* row_offsets, const int32_t* col, c
                     int32_t* out_val, int32_t* out_size) {
     return guard_impl([&] {
         if (ptr_space != JET_PTR_HOST) throw InvalidInput("jet_conn_tables takes host pointers");
+        check_k(k);
         Context& C = context();
         CallScope call(C);
         Level L = stage_graph(C, n, row_offsets, nullptr, col, ew, nullptr, nullptr, nullptr, ptr_space, false);
         const part_t* dp = to_device(C, part, n, ptr_space);
+        check_parts(C, n, dp, k);
         RefineParams rp;
         rp.k = k;
         int64_t cap = conn_capacity(L.n, L.rowptr, k, C.scratch, C.stream);
@@ -607,10 +618,12 @@ int jet_refine_step(int32_t kind, int64_t n, const int64_t* row_offsets, const i
     return guard_impl([&] {
         if (ptr_space != JET_PTR_HOST) throw InvalidInput("jet_refine_step takes host pointers");
         if (kind < 0 || kind > 2) throw InvalidInput("kind must be 0, 1 or 2");
+        check_k(k);
         Context& C = context();
         CallScope call(C);
         Level L = stage_graph(C, n, row_offsets, nullptr, col, ew, nullptr, vw, nullptr, ptr_space, false);
         const part_t* dp = to_device(C, part, n, ptr_space);
+        check_parts(C, n, dp, k);
         RefineParams rp;
         rp.k = k;
         rp.imb = imb;
@@ -628,6 +641,7 @@ int jet_refine_level(int64_t n, const int64_t* row_offsets, const int32_t* col, 
                      int32_t k, double imb, double tol, int32_t ultra, int32_t uniform, int32_t ptr_space,
                      int32_t* part_io, int64_t* rec_out) {
     return guard_impl([&] {
+        check_k(k);
         Context& C = context();
         CallScope call(C);
         Level L = stage_graph(C, n, row_offsets, nullptr, col, ew, nullptr, vw, nullptr, ptr_space, uniform != 0);
@@ -635,6 +649,7 @@ int jet_refine_level(int64_t n, const int64_t* row_offsets, const int32_t* col, 
         if (ptr_space == JET_PTR_DEVICE)
             JET_HIP(hipMemcpyAsync(dp, part_io, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, C.stream));
         else JET_HIP(hipMemcpyAsync(dp, part_io, sizeof(int32_t) * n, hipMemcpyHostToDevice, C.stream));
+        check_parts(C, n, dp, k);
         RefineParams rp;
         rp.k = k;
         rp.imb = imb;
@@ -707,9 +722,49 @@ int jet_part_sizes(int64_t n, const int32_t* vw, const int32_t* part, int32_t k,
         }
         g.vw = dvw;
         const part_t* dp = to_device(C, part, n, ptr_space);
+        check_parts(C, n, dp, k);
         int64_t* ds = ptr_space == JET_PTR_DEVICE ? sizes_out : C.persist.alloc<int64_t>(k);
         part_sizes(g, dp, k, ds, C.stream);
         from_device(C, sizes_out, ds, k, ptr_space);
+    });
+}
+
+int jet_partition_quality(int64_t n, const int64_t* row_offsets, const int32_t* col, const int32_t* ew,
+                          const int32_t* vw, const int32_t* part, int32_t k, int32_t ptr_space, jet_quality_t* out) {
+    return guard_impl([&] {
+        if (!out) throw InvalidInput("out is NULL");
+        check_k(k);
+        Context& C = context();
+        CallScope call(C);
+        Level L = stage_graph(C, n, row_offsets, nullptr, col, ew, nullptr, vw, nullptr, ptr_space, false);
+        const part_t* dp = to_device(C, part, n, ptr_space);
+        check_parts(C, n, dp, k);
+        const int64_t cut2 = total_cut2(L.view(), dp, C.scratch, C.stream);
+        int64_t* ds = C.persist.alloc<int64_t>(k);
+        part_sizes(L.view(), dp, k, ds, C.stream);
+        std::vector<int64_t> hs(k);
+        JET_HIP(hipMemcpyAsync(hs.data(), ds, sizeof(int64_t) * k, hipMemcpyDeviceToHost, C.stream));
+        RefineParams rp;
+        rp.k = k;
+        int64_t cap = conn_capacity(L.n, L.rowptr, k, C.scratch, C.stream);
+        Refiner ref(C.scratch, C.stream, L.n, cap, rp);
+        int64_t comm = 0;
+        std::vector<int64_t> cpp;
+        ref.quality(L, dp, comm, cpp);
+        int64_t total = 0, largest = 0, smallest = INT64_MAX;
+        for (int64_t v : hs) {
+            total += v;
+            largest = std::max(largest, v);
+            smallest = std::min(smallest, v);
+        }
+        const double opt = (double)((total + k - 1) / k);
+        out->edge_cut = cut2 / 2;
+        out->comm_size = comm;
+        out->max_part_cut = cpp.empty() ? 0 : *std::max_element(cpp.begin(), cpp.end());
+        out->largest_part = largest;
+        out->smallest_part = smallest;
+        out->max_imb = opt > 0 ? (double)largest / opt : 0.0;
+        out->min_imb = opt > 0 ? (double)smallest / opt : 0.0;
     });
 }
 

@@ -14,6 +14,8 @@ void widen_offsets(int64_t n1, const int32_t* in, int64_t* out, hipStream_t s);
 int32_t narrow_weights(int64_t cnt, const int64_t* in, int32_t* out, Arena& a, hipStream_t s);
 // int64 columns to int32; returns 2 when a column is outside [0, n).
 int32_t narrow_columns(int64_t cnt, const int64_t* in, int64_t n, int32_t* out, Arena& a, hipStream_t s);
+// Whether every part id is in [0, k).
+bool parts_in_range(int64_t n, const int32_t* part, int32_t k, Arena& a, hipStream_t s);
 int64_t conn_capacity(vid_t n, const eoff_t* rowptr, int32_t k, Arena& a, hipStream_t s);
 void gain_bucket_batch(int64_t cnt, const int64_t* g, const int64_t* w, int32_t* out, hipStream_t s);
 

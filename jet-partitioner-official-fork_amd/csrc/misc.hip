@@ -118,6 +118,25 @@ int32_t narrow_columns(int64_t cnt, const int64_t* in, int64_t n, int32_t* out, 
     return read_scalar(err, s);
 }
 
+__global__ void k_check_parts(int64_t n, const int32_t* part, int32_t k, int32_t* err) {
+    int32_t e = 0;
+    FOR_THREAD(i, n) {
+        int32_t p = part[i];
+        if (p < 0 || p >= k) e = 1;
+    }
+    if (e) atomicOr(err, 1);
+}
+
+bool parts_in_range(int64_t n, const int32_t* part, int32_t k, Arena& a, hipStream_t s) {
+    if (n <= 0) return true;
+    ArenaScope sc(a);
+    int32_t* err = a.alloc<int32_t>(1);
+    fill_i32(err, 0, 1, s);
+    k_check_parts<<<grid_for(n, 256, 4096), 256, 0, s>>>(n, part, k, err);
+    JET_LAUNCH_CHECK();
+    return read_scalar(err, s) == 0;
+}
+
 __global__ void k_conn_capacity(vid_t n, const eoff_t* rowptr, int32_t k, unsigned long long* out) {
     __shared__ int64_t red[256 / WAVE];
     int64_t c = 0;

@@ -44,6 +44,9 @@ class Refiner {
     void single_step(const Level& L, const part_t* part, int kind, double filter_ratio, std::vector<int32_t>& moves,
                      std::vector<int32_t>& dest);
 
+    // Communication volume and cut weight per part of `part` (part_stat.hpp).
+    void quality(const Level& L, const part_t* part, int64_t& comm_size, std::vector<int64_t>& cut_per_part);
+
     // Connectivity tables for `part` (init_conn_data), copied to the host.
     void conn_tables(const Level& L, const part_t* part, std::vector<int64_t>& off, std::vector<int32_t>& slot_part,
                      std::vector<int32_t>& slot_val, std::vector<int32_t>& size);
@@ -107,11 +110,11 @@ class Refiner {
     part_t* cur_part_;
     vid_t* tier_mid_;
     vid_t* tier_big_;
-    // rows of degree > BIG_DEG ("hubs": tier_big_ and pull_mid_) take their
-    // update_small changes through k_update_hubs: flags per vertex for hub,
-    // hub with a moved neighbour, and moved in this iteration
+    // rows of degree > BIG_DEG (tier_big_ and pull_mid_) take their
+    // update_small changes through k_pull_hubs, the others through
+    // k_pull_small: flags per vertex for such a row, and for moved in this
+    // iteration
     uint8_t* hub_;
-    uint8_t* hub_dirty_;
     uint8_t* moved_;
     vid_t* pull_mid_;  // mid-tier rows of degree > BIG_DEG
     int32_t ntier_mid_ = 0, ntier_big_ = 0, npull_mid_ = 0;

@@ -635,18 +635,6 @@ __device__ inline void conn_add(int2* tab, int32_t* vsize, int32_t cap, part_t b
     atomicAdd(&tab[po].y, w);
 }
 
-template <bool ADD>
-__device__ inline void update_edge(vid_t v, int32_t w, part_t q, const eoff_t* conn_off, int2* conn,
-                                   int32_t* conn_size, part_t* dest_cache, int32_t k, int32_t* err) {
-    int2* tab = conn + conn_off[v];
-    if (ADD) {
-        dest_cache[v] = NULL_PART;
-        conn_add(tab, &conn_size[v], (int32_t)(conn_off[v + 1] - conn_off[v]), q, w, err);
-    } else {
-        conn_sub(tab, ld_rx(&conn_size[v]), q, w, k, err);
-    }
-}
-
 // perform_moves' choice between update_small (push each moved vertex's
 // contribution into its neighbours' tables) and update_large (rebuild the
 // tables of every neighbour of a moved vertex): rebuild when more than n/10
@@ -656,104 +644,104 @@ __device__ inline void update_edge(vid_t v, int32_t w, part_t q, const eoff_t* c
 // choice follows the reference exactly.
 __device__ inline bool update_by_rebuild(const int32_t* mc, vid_t n) { return mc[2] + mc[3] > n / 10; }
 
-// ADD == false: subtract from the old part (held in dest), or, when the tables
-// are rebuilt instead, mark the neighbours; ADD == true: add to the new part
-// (held in part). G lanes stride the edges of one move.
-template <bool ADD, int G>
-__global__ void k_update_small(DGraph g, const vid_t* moves, int32_t* mc, vid_t nmax, const part_t* part,
-                               const part_t* dest, const eoff_t* conn_off, int2* conn, int32_t* conn_size,
-                               part_t* dest_cache, int32_t k, int32_t* err, int32_t* mark, const uint8_t* hub,
-                               uint8_t* hub_dirty, const int32_t* deferred) {
+// Marks the neighbours of the moved vertices: update_small and update_large
+// (jet_refiner.hpp:743-888) both change exactly these rows. Small moves by G
+// lanes, big moves by a wave (k_mark_big). In the rebuild case the first
+// workgroup also raises the gate of the rebuild kernels (counter 6).
+template <int G>
+__global__ void k_mark_small(DGraph g, const vid_t* moves, int32_t* mc, int32_t* mark, const int32_t* deferred) {
     if (*deferred) return;
-    const bool rebuild = update_by_rebuild(mc, g.n);
-    if (ADD && rebuild) return;
-    if (!ADD && rebuild && blockIdx.x == 0 && threadIdx.x == 0) mc[6] = 1;  // gates the rebuild
+    if (blockIdx.x == 0 && threadIdx.x == 0 && update_by_rebuild(mc, g.n)) mc[6] = 1;
     const int32_t ns = mc[2];
     FOR_GROUP(x, ns) {
         vid_t i = moves[x];
-        if (!ADD && rebuild) {
-            for (eoff_t j = g.rowptr[i] + sub_; j < g.rowptr[i + 1]; j += G) mark[g.col[j]] = 1;
-            continue;
-        }
-        part_t q = ADD ? part[i] : dest[i];
         for (eoff_t j = g.rowptr[i] + sub_; j < g.rowptr[i + 1]; j += G) {
-            vid_t v = g.col[j];
-            if (hub[v]) {  // updated by k_update_hubs
-                if (!ADD) hub_dirty[v] = 1;
-                continue;
-            }
-            update_edge<ADD>(v, g.ew[j], q, conn_off, conn, conn_size, dest_cache, k, err);
+            vid_t u = g.col[j];
+            if (!mark[u]) mark[u] = 1;
         }
     }
 }
 
-template <bool ADD>
-__global__ void k_update_big(DGraph g, const vid_t* moves, const int32_t* mc, vid_t nmax, const part_t* part,
-                             const part_t* dest, const eoff_t* conn_off, int2* conn, int32_t* conn_size,
-                             part_t* dest_cache, int32_t k, int32_t* err, int32_t* mark, const uint8_t* hub,
-                             uint8_t* hub_dirty, const int32_t* deferred) {
+__global__ void k_mark_big(DGraph g, const vid_t* moves, const int32_t* mc, vid_t nmax, int32_t* mark,
+                           const int32_t* deferred) {
     if (*deferred) return;
-    const bool rebuild = update_by_rebuild(mc, g.n);
-    if (ADD && rebuild) return;
     const int32_t nb = mc[3];
     const int lane = threadIdx.x & (WAVE - 1);
     const int64_t wstride = ((int64_t)gridDim.x * blockDim.x) / WAVE;
     for (int64_t x = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE; x < nb; x += wstride) {
         vid_t i = moves[nmax - 1 - x];
-        if (!ADD && rebuild) {
-            for (eoff_t j = g.rowptr[i] + lane; j < g.rowptr[i + 1]; j += WAVE) mark[g.col[j]] = 1;
-            continue;
-        }
-        part_t q = ADD ? part[i] : dest[i];
         for (eoff_t j = g.rowptr[i] + lane; j < g.rowptr[i + 1]; j += WAVE) {
-            vid_t v = g.col[j];
-            if (hub[v]) {  // updated by k_update_hubs
-                if (!ADD) hub_dirty[v] = 1;
-                continue;
-            }
-            update_edge<ADD>(v, g.ew[j], q, conn_off, conn, conn_size, dest_cache, k, err);
+            vid_t u = g.col[j];
+            if (!mark[u]) mark[u] = 1;
         }
     }
 }
 
-// update_small for rows of degree > BIG_DEG ("hubs": the workgroup-built tier
-// and the pull_mid rows), one workgroup per row with a moved neighbour: the row's moved neighbours are found by a scan of
-// its adjacency and their weights summed per part in LDS, then each part's
-// total is applied to the table once, with the subtraction phase before the
-// addition phase as for the other rows. The resulting table equals the one
-// per-edge pushes leave: a part is reclaimed when its connectivity reaches 0
-// in a table of fewer than k slots (jet_refiner.hpp:806-810), new parts take
-// free or reclaimed slots and then overflow slots (:828-884). Pushing instead
-// would land every moved neighbour's atomics on the same few hub slots.
-template <bool ADD, int BLOCK>
+// update_small applied per row instead of per moved vertex: each marked row
+// takes its moved neighbours' weights off their old parts (dest) and then
+// puts them on their new parts (part). A row has one owner here, so this
+// leaves the table that per-edge pushes leave (subtractions before additions;
+// a part is reclaimed when its connectivity reaches 0 in a table of fewer
+// than k slots, jet_refiner.hpp:806-810; new parts take free or reclaimed
+// slots, then overflow slots, :828-884), without many moved neighbours'
+// atomics meeting on the same slots. Rows of degree <= BIG_DEG: one thread
+// each (k_pull_small); larger rows: one workgroup with per-part sums in LDS
+// (k_pull_hubs).
+__global__ void k_pull_small(DGraph g, const int32_t* mc, int32_t* mark, const uint8_t* hub, const uint8_t* moved,
+                             const part_t* part, const part_t* dest, const eoff_t* conn_off, int2* conn,
+                             int32_t* conn_size, part_t* dest_cache, int32_t k, int32_t* err, const int32_t* deferred) {
+    if (*deferred || update_by_rebuild(mc, g.n)) return;
+    FOR_THREAD(u, g.n) {
+        if (!mark[u] || hub[u]) continue;
+        const eoff_t rs = g.rowptr[u], re = g.rowptr[u + 1];
+        int2* tab = conn + conn_off[u];
+        const int32_t cap = (int32_t)(conn_off[u + 1] - conn_off[u]);
+        const int32_t vsz = conn_size[u];
+        for (eoff_t j = rs; j < re; j++) {
+            vid_t v = g.col[j];
+            if (moved[v]) conn_sub(tab, vsz, dest[v], g.ew[j], k, err);
+        }
+        for (eoff_t j = rs; j < re; j++) {
+            vid_t v = g.col[j];
+            if (moved[v]) conn_add(tab, &conn_size[u], cap, part[v], g.ew[j], err);
+        }
+        dest_cache[u] = NULL_PART;
+        mark[u] = 0;
+    }
+}
+
+template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
-    k_update_hubs(DGraph g, const vid_t* list, int32_t nlist, const int32_t* mc, const uint8_t* moved,
-                  uint8_t* hub_dirty, const part_t* part, const part_t* dest, const eoff_t* conn_off, int2* conn,
-                  int32_t* conn_size, part_t* dest_cache, int32_t k, int32_t* err, const int32_t* deferred) {
+    k_pull_hubs(DGraph g, const vid_t* list, int32_t nlist, const int32_t* mc, int32_t* mark, const uint8_t* moved,
+                const part_t* part, const part_t* dest, const eoff_t* conn_off, int2* conn, int32_t* conn_size,
+                part_t* dest_cache, int32_t k, int32_t* err, const int32_t* deferred) {
     if (*deferred || update_by_rebuild(mc, g.n)) return;
     extern __shared__ int32_t delta[];  // k
     for (int32_t r = blockIdx.x; r < nlist; r += gridDim.x) {
         const vid_t u = list[r];
-        if (!hub_dirty[u]) continue;
-        for (int32_t p = threadIdx.x; p < k; p += BLOCK) delta[p] = 0;
-        __syncthreads();
-        for (eoff_t j = g.rowptr[u] + threadIdx.x; j < g.rowptr[u + 1]; j += BLOCK) {
-            vid_t v = g.col[j];
-            if (moved[v]) atomicAdd(&delta[ADD ? part[v] : dest[v]], g.ew[j]);
-        }
-        __syncthreads();
+        if (!mark[u]) continue;
         int2* tab = conn + conn_off[u];
         const int32_t cap = (int32_t)(conn_off[u + 1] - conn_off[u]);
         const int32_t vsz = conn_size[u];
-        for (int32_t p = threadIdx.x; p < k; p += BLOCK) {
-            const int32_t w = delta[p];
-            if (w == 0) continue;
-            if (ADD) conn_add(tab, &conn_size[u], cap, p, w, err);
-            else conn_sub(tab, vsz, p, w, k, err);
+        for (int phase = 0; phase < 2; phase++) {  // 0: old parts off, 1: new parts on
+            for (int32_t p = threadIdx.x; p < k; p += BLOCK) delta[p] = 0;
+            __syncthreads();
+            for (eoff_t j = g.rowptr[u] + threadIdx.x; j < g.rowptr[u + 1]; j += BLOCK) {
+                vid_t v = g.col[j];
+                if (moved[v]) atomicAdd(&delta[phase ? part[v] : dest[v]], g.ew[j]);
+            }
+            __syncthreads();
+            for (int32_t p = threadIdx.x; p < k; p += BLOCK) {
+                const int32_t w = delta[p];
+                if (w == 0) continue;
+                if (phase) conn_add(tab, &conn_size[u], cap, p, w, err);
+                else conn_sub(tab, vsz, p, w, k, err);
+            }
+            __syncthreads();
         }
-        if (ADD && threadIdx.x == 0) {
+        if (threadIdx.x == 0) {
             dest_cache[u] = NULL_PART;
-            hub_dirty[u] = 0;
+            mark[u] = 0;
         }
         __syncthreads();
     }
@@ -1379,7 +1367,6 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
     tier_big_ = a.alloc<vid_t>(N);
     hub_ = a.alloc<uint8_t>(N);
     pull_mid_ = a.alloc<vid_t>(N);
-    hub_dirty_ = a.alloc<uint8_t>(N);
     moved_ = a.alloc<uint8_t>(N);
     counters_ = a.alloc<int32_t>(32);
     acc_ = a.alloc<int64_t>(8);
@@ -1394,7 +1381,6 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
     buckets_ = a.alloc<int64_t>(nbuckets_);
     bflag_ = a.alloc<int32_t>(nbuckets_);
     fill_i32(swap_bit_, 0, N, s);
-    JET_HIP(hipMemsetAsync(hub_dirty_, 0, (size_t)N, s));
     JET_HIP(hipMemsetAsync(moved_, 0, (size_t)N, s));
     fill_i32(counters_, 0, 32, s);
     fill_i64(acc_, 0, 8, s);
@@ -1574,7 +1560,6 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
     k_reset_buckets<<<grid_for(nbk + 1, 256, 2048), 256, 0, s_>>>(buckets_, bflag_, nbk + 1, cnt_next(), sizes, k,
                                                                     max_dest, strong ? 1 : 0, undersized_, misc_, acc_,
                                                                     evict_adjust_);
-    int nb = grid_for(g.n, 256, 8192);
     int nsel = grid_for(g.n, 256, 1024);  // each workgroup amortises its LDS bucket map
     if (strong)
         k_strong_select<<<nsel, 256, 0, s_>>>(g, part, sizes, smax, opt_, max_dest, acc_, conn_off_, conn_, conn_size_,
@@ -1613,53 +1598,27 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
     int G = group_for_avg(g);
     int nbs = grid_for((int64_t)g.n * G, 256, 1024);
     int nbb = grid_for((int64_t)g.n * WAVE, 256, 512);
-    for (int add = 0; add < 2; add++) {
-#define JET_UPD(GG)                                                                                                   \
-    if (add) k_update_small<true, GG><<<nbs, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,       \
-                                                          conn_size_, dest_cache_, k, misc_ + 1, swap_bit_,      \
-                                                          hub_, hub_dirty_, misc_ + 2);                         \
-    else k_update_small<false, GG><<<nbs, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,          \
-                                                        conn_size_, dest_cache_, k, misc_ + 1, swap_bit_,        \
-                                                        hub_, hub_dirty_, misc_ + 2);
-        switch (G) {
-            case 4: JET_UPD(4) break;
-            case 8: JET_UPD(8) break;
-            case 16: JET_UPD(16) break;
-            case 32: JET_UPD(32) break;
-            default: JET_UPD(64) break;
-        }
-#undef JET_UPD
-        if (has_big_) {
-            if (add) k_update_big<true><<<nbb, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,
-                                                             conn_size_, dest_cache_, k, misc_ + 1, swap_bit_, hub_,
-                                                             hub_dirty_, misc_ + 2);
-            else k_update_big<false><<<nbb, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,
-                                                           conn_size_, dest_cache_, k, misc_ + 1, swap_bit_, hub_,
-                                                           hub_dirty_, misc_ + 2);
-        }
-        // pulled rows: big-tier rows by a 256-thread workgroup, rows of
-        // degree (BIG_DEG, HEAVY_DEG_REFINE] by a single-wave workgroup
-        const size_t lds = sizeof(int32_t) * (size_t)k;
-        if (ntier_big_ > 0) {
-            const int hb = std::min(ntier_big_, 2048);
-            if (add) k_update_hubs<true, 256><<<hb, 256, lds, s_>>>(g, tier_big_, ntier_big_, c, moved_, hub_dirty_,
-                                                                    part, dest_, conn_off_, conn_, conn_size_,
-                                                                    dest_cache_, k, misc_ + 1, misc_ + 2);
-            else k_update_hubs<false, 256><<<hb, 256, lds, s_>>>(g, tier_big_, ntier_big_, c, moved_, hub_dirty_,
-                                                                 part, dest_, conn_off_, conn_, conn_size_,
-                                                                 dest_cache_, k, misc_ + 1, misc_ + 2);
-        }
-        if (npull_mid_ > 0) {
-            const int hb = std::min(npull_mid_, 8192);
-            if (add) k_update_hubs<true, WAVE><<<hb, WAVE, lds, s_>>>(g, pull_mid_, npull_mid_, c, moved_, hub_dirty_,
-                                                                      part, dest_, conn_off_, conn_, conn_size_,
-                                                                      dest_cache_, k, misc_ + 1, misc_ + 2);
-            else k_update_hubs<false, WAVE><<<hb, WAVE, lds, s_>>>(g, pull_mid_, npull_mid_, c, moved_, hub_dirty_,
-                                                                   part, dest_, conn_off_, conn_, conn_size_,
-                                                                   dest_cache_, k, misc_ + 1, misc_ + 2);
-        }
-        JET_LAUNCH_CHECK();
+    switch (G) {
+        case 4: k_mark_small<4><<<nbs, 256, 0, s_>>>(g, moves_, c, swap_bit_, misc_ + 2); break;
+        case 8: k_mark_small<8><<<nbs, 256, 0, s_>>>(g, moves_, c, swap_bit_, misc_ + 2); break;
+        case 16: k_mark_small<16><<<nbs, 256, 0, s_>>>(g, moves_, c, swap_bit_, misc_ + 2); break;
+        case 32: k_mark_small<32><<<nbs, 256, 0, s_>>>(g, moves_, c, swap_bit_, misc_ + 2); break;
+        default: k_mark_small<64><<<nbs, 256, 0, s_>>>(g, moves_, c, swap_bit_, misc_ + 2); break;
     }
+    if (has_big_) k_mark_big<<<nbb, 256, 0, s_>>>(g, moves_, c, nmax_, swap_bit_, misc_ + 2);
+    k_pull_small<<<grid_for(g.n, 256, 2048), 256, 0, s_>>>(g, c, swap_bit_, hub_, moved_, part, dest_, conn_off_,
+                                                           conn_, conn_size_, dest_cache_, k, misc_ + 1, misc_ + 2);
+    const size_t lds = sizeof(int32_t) * (size_t)k;
+    if (ntier_big_ > 0)
+        k_pull_hubs<256><<<std::min(ntier_big_, 2048), 256, lds, s_>>>(g, tier_big_, ntier_big_, c, swap_bit_, moved_,
+                                                                       part, dest_, conn_off_, conn_, conn_size_,
+                                                                       dest_cache_, k, misc_ + 1, misc_ + 2);
+    if (npull_mid_ > 0)
+        k_pull_hubs<WAVE><<<std::min(npull_mid_, 8192), WAVE, lds, s_>>>(g, pull_mid_, npull_mid_, c, swap_bit_,
+                                                                         moved_, part, dest_, conn_off_, conn_,
+                                                                         conn_size_, dest_cache_, k, misc_ + 1,
+                                                                         misc_ + 2);
+    JET_LAUNCH_CHECK();
     init_conn(L, part, swap_bit_);  // rebuild marked rows (gated on the device)
     k_moves_finish<<<nb, 256, 0, s_>>>(moves_, c, nmax_, part, dest_, conn_off_, conn_, conn_size_, sizes, k, acc_,
                                        res_dev_, moved_, misc_ + 2);
@@ -1817,6 +1776,48 @@ void Refiner::single_step(const Level& L, const part_t* part, int kind, double f
         moves[t] = v[idx[t]];
         dest[t] = d[idx[t]];
     }
+}
+
+// part_stat.hpp:131-144, 186-239 from the connectivity tables: the
+// communication volume (per vertex, the parts other than its own that its
+// neighbours are in) and the cut weight incident to each part.
+__global__ void k_quality(DGraph g, const part_t* part, const eoff_t* conn_off, const int2* conn,
+                          const int32_t* conn_size, unsigned long long* comm, unsigned long long* cut_per_part) {
+    __shared__ int64_t red[256 / WAVE];
+    int64_t c = 0;
+    FOR_THREAD(i, g.n) {
+        const part_t p = part[i];
+        const int2* tab = conn + conn_off[i];
+        const int32_t sz = conn_size[i];
+        int64_t ext = 0;
+        for (int32_t s = 0; s < sz; s++) {
+            int2 e = tab[s];
+            if (e.x > NULL_PART && e.x != p && e.y > 0) {
+                c++;
+                ext += e.y;
+            }
+        }
+        if (ext) atomicAdd(&cut_per_part[p], (unsigned long long)ext);
+    }
+    c = block_reduce_sum<int64_t, 256>(c, red);
+    if (threadIdx.x == 0 && c) atomicAdd(comm, (unsigned long long)c);
+}
+
+void Refiner::quality(const Level& L, const part_t* part, int64_t& comm_size, std::vector<int64_t>& cut_per_part) {
+    DGraph g = L.view();
+    copy_i32(cur_part_, part, g.n, s_);
+    build_tiers(L);
+    init_conn(L, cur_part_, nullptr);
+    ArenaScope sc(a_);
+    unsigned long long* d = a_.alloc<unsigned long long>(1 + p_.k);
+    fill_i64(reinterpret_cast<int64_t*>(d), 0, 1 + p_.k, s_);
+    k_quality<<<grid_for(g.n, 256, 4096), 256, 0, s_>>>(g, cur_part_, conn_off_, conn_, conn_size_, d, d + 1);
+    JET_LAUNCH_CHECK();
+    std::vector<int64_t> h(1 + p_.k);
+    JET_HIP(hipMemcpyAsync(h.data(), d, sizeof(int64_t) * h.size(), hipMemcpyDeviceToHost, s_));
+    JET_HIP(hipStreamSynchronize(s_));
+    comm_size = h[0];
+    cut_per_part.assign(h.begin() + 1, h.end());
 }
 
 void Refiner::conn_tables(const Level& L, const part_t* part, std::vector<int64_t>& off,

@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "libjet_amd.so")
+# JET_AMD_LIB selects another build of the library (for comparing builds).
+LIB_PATH = os.environ.get("JET_AMD_LIB") or os.path.join(os.path.dirname(_HERE), "libjet_amd.so")
 
 JET_OK, JET_EINVAL, JET_ENOMEM, JET_EINIT, JET_EDEVICE, JET_EUNSUPPORTED = 0, -1, -2, -3, -4, -5
 JET_EAGAIN = 1
@@ -52,6 +53,20 @@ class Stats(ctypes.Structure):
                 ("num_levels", ctypes.c_int32), ("attempts", ctypes.c_int32), ("finest_imb_ratio", ctypes.c_double),
                 ("finest_edge_cut", ctypes.c_int64), ("largest_part", ctypes.c_int64),
                 ("smallest_part", ctypes.c_int64), ("levels", LevelStats * MAX_LEVELS)]
+
+
+class HostGraph(ctypes.Structure):
+    """jet_host_graph_t (library-allocated CSR in host memory)."""
+    _fields_ = [("n", ctypes.c_int64), ("m", ctypes.c_int64), ("row_offsets", ctypes.POINTER(ctypes.c_int64)),
+                ("col", ctypes.POINTER(ctypes.c_int32)), ("ew", ctypes.POINTER(ctypes.c_int32)),
+                ("uniform_ew", ctypes.c_int32)]
+
+
+class Quality(ctypes.Structure):
+    """jet_quality_t (src/part_stat.hpp, app/part_eval.cpp)."""
+    _fields_ = [("edge_cut", ctypes.c_int64), ("comm_size", ctypes.c_int64), ("max_part_cut", ctypes.c_int64),
+                ("largest_part", ctypes.c_int64), ("smallest_part", ctypes.c_int64), ("max_imb", ctypes.c_double),
+                ("min_imb", ctypes.c_double)]
 
 
 class JobTimes(ctypes.Structure):
@@ -102,13 +117,19 @@ def lib():
             "jet_submit": [VP, c.POINTER(Config), i64, VP, VP, VP, VP, i32, i32, u64, VP, c.POINTER(i64)],
             "jet_poll": [VP, i64, i32, c.POINTER(i64), c.POINTER(Stats), c.POINTER(JobTimes)],
             "jet_partition_batch": [VP, i32, c.POINTER(Config), VP, VP, VP, VP, VP, VP, VP, VP, i32, VP, VP, VP],
+            "jet_load_metis": [c.c_char_p, c.POINTER(HostGraph)],
+            "jet_free_graph": [c.POINTER(HostGraph)],
+            "jet_load_config": [c.c_char_p, c.POINTER(Config)],
+            "jet_write_partition": [c.c_char_p, i64, VP],
+            "jet_read_partition": [c.c_char_p, i64, VP],
+            "jet_partition_quality": [i64, VP, VP, VP, VP, VP, i32, i32, c.POINTER(Quality)],
             "jet_prof_enable": [c.c_char_p],
             "jet_prof_read": [c.POINTER(i64), c.POINTER(dbl), c.POINTER(dbl)],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
             f.argtypes = args
-            f.restype = None if name == "jet_config_default" else c.c_int
+            f.restype = None if name in ("jet_config_default", "jet_free_graph") else c.c_int
         _lib = L
     return _lib
 
@@ -450,3 +471,47 @@ class Context:
                                          _p(vw), _p(uni), PTR_HOST, _p(part), _p(cuts), st))
         parts = [part[voff[i]:voff[i + 1]] for i in range(count)]
         return (parts, cuts, list(st)) if stats else (parts, cuts)
+
+
+# ---------------------------------------------------------------- files, quality
+def load_metis(path):
+    """app/io.hpp METIS graph file -> CSR dict {rowptr, col, ew, uniform}."""
+    hg = HostGraph()
+    _check(lib().jet_load_metis(os.fsencode(path), ctypes.byref(hg)))
+    try:
+        n, m = hg.n, hg.m
+        g = {"rowptr": np.ctypeslib.as_array(hg.row_offsets, shape=(n + 1,)).copy(),
+             "col": np.ctypeslib.as_array(hg.col, shape=(max(m, 1),))[:m].copy(),
+             "uniform": bool(hg.uniform_ew)}
+        g["ew"] = (np.ctypeslib.as_array(hg.ew, shape=(max(m, 1),))[:m].copy() if hg.ew
+                   else np.ones(m, dtype=np.int32))
+        return g
+    finally:
+        lib().jet_free_graph(ctypes.byref(hg))
+
+
+def load_config(path):
+    """app/io.hpp config file -> Config."""
+    cfg = Config()
+    _check(lib().jet_load_config(os.fsencode(path), ctypes.byref(cfg)))
+    return cfg
+
+
+def write_partition(path, part):
+    p = _c(part, np.int32)
+    _check(lib().jet_write_partition(os.fsencode(path), len(p), _p(p)))
+
+
+def read_partition(path, n):
+    out = np.empty(n, dtype=np.int32)
+    _check(lib().jet_read_partition(os.fsencode(path), n, _p(out)))
+    return out
+
+
+def partition_quality(g, part, k):
+    """jet_partition_quality on host arrays -> Quality."""
+    rp, col, ew, vw = _graph(g)
+    q = Quality()
+    _check(lib().jet_partition_quality(len(rp) - 1, _p(rp), _p(col), _p(ew), _p(vw), _p(_c(part, np.int32)), k,
+                                       PTR_HOST, ctypes.byref(q)))
+    return q

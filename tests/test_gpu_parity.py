@@ -1,5 +1,6 @@
 """GPU parity: every hot-path stage through the C-ABI against the CPU oracle
 on the same inputs, bit-exact (integer work), then end-to-end partitions."""
+import os
 import subprocess
 import sys
 
@@ -286,3 +287,59 @@ def test_jet_h_nine_entry_points(tmp_path):
     part, cut = jet_amd.partition_csr(g, jet_amd.Config(num_parts=8, coarsening_alg=1), seed=5, uniform=False)
     for ln in lines:
         assert int(ln[1]) == cut and int(ln[2]) == _fnv(part)
+
+
+def _numpy_quality(g, part, k):
+    rp, col, ew = g["rowptr"], g["col"], g["ew"]
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    ext = part[rows] != part[col]
+    cut = int(ew[ext].sum()) // 2
+    cut_pp = np.bincount(part[rows][ext], weights=ew[ext], minlength=k).astype(np.int64)
+    pairs = np.unique(np.stack([rows[ext], part[col][ext]]), axis=1)
+    sizes = np.bincount(part, minlength=k)
+    opt = (len(part) + k - 1) // k
+    return cut, pairs.shape[1], int(cut_pp.max()), sizes.max() / opt, sizes.min() / opt
+
+
+@pytest.mark.parametrize("k", [2, 16, 300])
+def test_partition_quality_metrics(k):
+    g = graphs.with_weights(graphs.rmat(12, 8, seed=6), seed=6, wmax=7)
+    part = np.random.default_rng(k).integers(0, k, len(g["rowptr"]) - 1).astype(np.int32)
+    q = jet_amd.partition_quality(g, part, k)
+    cut, comm, mpc, mx, mn = _numpy_quality(g, part, k)
+    assert (q.edge_cut, q.comm_size, q.max_part_cut) == (cut, comm, mpc)
+    assert abs(q.max_imb - mx) < 1e-12 and abs(q.min_imb - mn) < 1e-12
+
+
+def test_part_ids_out_of_range_rejected():
+    g = graphs.grid2d(16, 16)
+    part = np.zeros(256, dtype=np.int32)
+    part[7] = 4
+    for call in (lambda: jet_amd.part_sizes(np.ones(256, dtype=np.int32), part, 4),
+                 lambda: jet_amd.partition_quality(g, part, 4),
+                 lambda: jet_amd.conn_tables(g, part, 4),
+                 lambda: jet_amd.refine_level(g, part, 4, 1.03)):
+        with pytest.raises(jet_amd.JetError) as e:
+            call()
+        assert e.value.code == jet_amd.JET_EINVAL
+
+
+def test_cli_jet_and_pstat(tmp_path):
+    from test_io_cpu import write_metis
+    pkg = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "jet-partitioner-official-fork_amd")
+    g = graphs.with_weights(graphs.grid2d(64, 48), seed=2, wmax=4)
+    write_metis(tmp_path / "g.graph", g, True)
+    (tmp_path / "c.txt").write_text("1\n8\n3\n1.03\n")
+    r = subprocess.run([os.path.join(pkg, "jet"), str(tmp_path / "g.graph"), str(tmp_path / "c.txt"),
+                        str(tmp_path / "p.txt"), str(tmp_path / "m.json")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    best = min(jet_amd.partition_csr(g, jet_amd.Config(num_parts=8, coarsening_alg=1), seed=s)[1] for s in range(3))
+    assert f"min edgecut found is {best}" in r.stdout
+    part = jet_amd.read_partition(str(tmp_path / "p.txt"), len(g["rowptr"]) - 1)
+    assert jet_amd.edge_cut2(g, part) // 2 == best
+    assert len((tmp_path / "m.json").read_text().splitlines()) == 3
+    r = subprocess.run([os.path.join(pkg, "pstat"), str(tmp_path / "g.graph"), str(tmp_path / "p.txt"), "8"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    cut, comm, mpc, mx, mn = _numpy_quality(g, part, 8)
+    assert f"Cutsize: {cut}" in r.stdout and f"Comm size: {comm}" in r.stdout and f"Max part cut: {mpc}" in r.stdout
