@@ -9,6 +9,7 @@ undirected edges), edge weights uniform in [1, 10], k = 64, imbalance 1.03,
 HEC coarsening. Prints one JSON line.
 """
 import argparse
+import hashlib
 import ctypes
 import json
 import os
@@ -28,8 +29,27 @@ WORKLOADS = {
     "stencil3d-128": dict(desc="3-D 7-pt stencil 128^3, w in [1,10], k=64, imb 1.03, HEC", k=64, imb=1.03, alg=1),
     "grid2d-256": dict(desc="2-D grid 256x256, unit weights, k=8, imb 1.03, 2-hop", k=8, imb=1.03, alg=0),
     "rmat-22": dict(desc="R-MAT scale 22 ef 16 (.57,.19,.19), unit weights, k=256, imb 1.10, HEC", k=256, imb=1.10,
-                    alg=1),
+                    alg=1, cpu_sample="rmat-20", golden="rmat22_k256_golden.json"),
 }
+
+# The serial oracle needs minutes for R-MAT scale 22; its CPU baseline is timed
+# on the same generator at scale 20 (same edge factor, k, imbalance, seed).
+SAMPLES = {"rmat-20": "R-MAT scale 20 ef 16 (.57,.19,.19), k=256, imb 1.10, HEC, seed 0"}
+
+
+def counted_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    counter summary (two rocprofv3 --pmc passes, FETCH_SIZE then WRITE_SIZE,
+    each scaled by the same counter over a plain copy of known size), or None
+    when no summary exists for that pair."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("HBM bytes per launch:"):
+                return int(line.split(":")[1])
+    return None
 
 
 def make_graph(name):
@@ -39,6 +59,8 @@ def make_graph(name):
         return jet_amd.grid2d(256, 256)
     if name == "rmat-22":
         return jet_amd.rmat(22, 16, seed=0)
+    if name == "rmat-20":
+        return jet_amd.rmat(20, 16, seed=0)
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -281,6 +303,9 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrival rate per second")
     ap.add_argument("--pool", type=int, default=16, help="serve: distinct graphs requests draw from")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM budget for admitted jobs")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 1 GiB device copy (reads 1 GiB, writes 1 GiB) three times, for scaling "
+                         "the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--prof", default="lp_select", help="kernel family timed with HIP events")
@@ -295,6 +320,14 @@ def main():
 
     if not torch.cuda.is_available():
         raise SystemExit("no GPU visible")
+    if args.calibrate_copy:
+        a = torch.ones(1 << 28, dtype=torch.int32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.add(a, 0, out=b)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 << 28, "copy_bytes_written": 4 << 28, "launches": 3}), flush=True)
+        return
     if args.workload == "batch":
         return run_batch(args)
     if args.workload == "serve":
@@ -343,15 +376,26 @@ def main():
 
     roofline = None
     if prof_ok:
+        L = jet_amd.lib()
         launches, tot_ms, nbytes = ctypes.c_int64(0), ctypes.c_double(0), ctypes.c_double(0)
-        if jet_amd.lib().jet_prof_read(ctypes.byref(launches), ctypes.byref(tot_ms), ctypes.byref(nbytes)) == 0 \
-                and launches.value > 0 and tot_ms.value > 0:
-            avg_s = tot_ms.value / 1e3 / launches.value
+        ok = L.jet_prof_read(ctypes.byref(launches), ctypes.byref(tot_ms), ctypes.byref(nbytes)) == 0 \
+            and launches.value > 0 and tot_ms.value > 0
+        t_launches, t_ms = launches.value, tot_ms.value
+        # algorithmic bytes: one more (untimed) partition in counting mode;
+        # the computation is deterministic, so its launches are the timed ones
+        if ok and L.jet_prof_enable((args.prof + ":count").encode()) == 0:
+            step()
+            ok = L.jet_prof_read(ctypes.byref(launches), ctypes.byref(tot_ms), ctypes.byref(nbytes)) == 0 \
+                and launches.value > 0
+        L.jet_prof_enable(b"none")
+        if ok:
+            avg_s = t_ms / 1e3 / t_launches
             per_launch = nbytes.value / launches.value
             achieved = per_launch / avg_s / 1e9
             roofline = {"kernel": args.prof, "bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0,
-                        "unit": "GB/s", "frac": round(achieved / 8000.0, 4), "traffic": None,
-                        "launches": launches.value, "avg_us": round(avg_s * 1e6, 2),
+                        "unit": "GB/s", "frac": round(achieved / 8000.0, 4),
+                        "traffic": counted_traffic(args.prof, args.workload),
+                        "launches": t_launches, "avg_us": round(avg_s * 1e6, 2),
                         "bytes_per_launch": round(per_launch)}
 
     part = d_part.cpu().numpy()
@@ -361,18 +405,38 @@ def main():
 
     cpu = None
     cut_ratio = None
+    if "golden" in wl:
+        # the oracle's partition of this exact graph, recorded once
+        with open(os.path.join(ROOT, "tests", "golden", wl["golden"])) as f:
+            gold = json.load(f)
+        cut_ratio = round(cut / gold["cut"], 6)
+        same = hashlib.sha256(part.astype("<i4").tobytes()).hexdigest() == gold["part_sha256"]
+        print(f"# oracle golden cut {gold['cut']}; gpu partition identical to it: {same}", file=sys.stderr,
+              flush=True)
     if not args.no_cpu_baseline:
         import oracle
 
-        t1 = time.perf_counter()
-        opart, ocut, _ = oracle.partition(g, wl["k"], wl["imb"], alg=wl["alg"], seed=args.seed)
-        cdt = time.perf_counter() - t1
-        cpu = {"value": round(edges / cdt, 1), "unit": "edges/s", "cores": 1, "kind": "port",
-               "sample": f"one full partition of the same {args.workload} graph, same seed, serial oracle "
-                         f"({cdt:.2f}s)"}
-        cut_ratio = round(cut / ocut, 6) if ocut else None
-        print(f"# cpu oracle {cdt:.2f}s cut {ocut}; gpu/oracle partitions identical: "
-              f"{bool(np.array_equal(part, opart))}", file=sys.stderr, flush=True)
+        if "cpu_sample" in wl:
+            sname = wl["cpu_sample"]
+            sg = make_graph(sname)
+            sedges = len(sg["col"]) // 2
+            print(f"# cpu oracle on the {sname} sample ({sedges} edges)", file=sys.stderr, flush=True)
+            t1 = time.perf_counter()
+            oracle.partition(sg, wl["k"], wl["imb"], alg=wl["alg"], seed=args.seed)
+            cdt = time.perf_counter() - t1
+            cpu = {"value": round(sedges / cdt, 1), "unit": "edges/s", "cores": 1, "kind": "port",
+                   "sample": f"one full partition of {SAMPLES[sname]} ({sedges} edges), serial oracle ({cdt:.2f}s)"}
+        else:
+            print("# cpu oracle on the full graph", file=sys.stderr, flush=True)
+            t1 = time.perf_counter()
+            opart, ocut, _ = oracle.partition(g, wl["k"], wl["imb"], alg=wl["alg"], seed=args.seed)
+            cdt = time.perf_counter() - t1
+            cpu = {"value": round(edges / cdt, 1), "unit": "edges/s", "cores": 1, "kind": "port",
+                   "sample": f"one full partition of the same {args.workload} graph, same seed, serial oracle "
+                             f"({cdt:.2f}s)"}
+            cut_ratio = round(cut / ocut, 6) if ocut else None
+            print(f"# cpu oracle {cdt:.2f}s cut {ocut}; gpu/oracle partitions identical: "
+                  f"{bool(np.array_equal(part, opart))}", file=sys.stderr, flush=True)
 
     out = {
         "metric": "edges/s partitioned (1 GPU) at cut <=1.01x ref",

@@ -132,4 +132,44 @@ __host__ __device__ inline int32_t gain_bucket(int64_t gx, int64_t vwgt) {
     return gain_type;
 }
 
+// gain_bucket without the loops. x -> (float)(x / 1.5) (and x * 1.5) is
+// monotone, so the number of loop steps from g is the number of thresholds g
+// passes: GB_UP[j] is the largest float that reaches <= 1 within j divisions,
+// GB_DOWN[j] the smallest that reaches >= 1 within j multiplications. Only the
+// first 26 / 23 steps matter: more are clamped. The quotient skips the double
+// division when the weight is 1 (the quotient is then the gain itself).
+constexpr float GB_UP[26] = {
+    0x1.0000000000000p+0f, 0x1.8000000000000p+0f, 0x1.2000000000000p+1f, 0x1.b000000000000p+1f,
+    0x1.4400000000000p+2f, 0x1.e600000000000p+2f, 0x1.6c80000000000p+3f, 0x1.1160000000000p+4f,
+    0x1.9a10000000000p+4f, 0x1.338c000000000p+5f, 0x1.cd52000000000p+5f, 0x1.59fd800000000p+6f,
+    0x1.037e200000000p+7f, 0x1.853d300000000p+7f, 0x1.23ede40000000p+8f, 0x1.b5e4d60000000p+8f,
+    0x1.486ba00000000p+9f, 0x1.eca1700000000p+9f, 0x1.7179140000000p+10f, 0x1.151ace0000000p+11f,
+    0x1.9fa8360000000p+11f, 0x1.37be280000000p+12f, 0x1.d39d3c0000000p+12f, 0x1.5eb5ec0000000p+13f,
+    0x1.0708700000000p+14f, 0x1.8a8ca80000000p+14f};
+constexpr float GB_DOWN[23] = {
+    0x1.0000000000000p+0f, 0x1.5555560000000p-1f, 0x1.c71c720000000p-2f, 0x1.2f684c0000000p-2f,
+    0x1.948b100000000p-3f, 0x1.0db20a0000000p-3f, 0x1.67980e0000000p-4f, 0x1.df75680000000p-5f,
+    0x1.3fa39a0000000p-5f, 0x1.aa2f780000000p-6f, 0x1.1c1fa60000000p-6f, 0x1.7ad4de0000000p-7f,
+    0x1.f91bd20000000p-8f, 0x1.50bd380000000p-8f, 0x1.c0fc4a0000000p-9f, 0x1.2b52dc0000000p-9f,
+    0x1.8f19240000000p-10f, 0x1.0a10c20000000p-10f, 0x1.62c1020000000p-11f, 0x1.d901580000000p-12f,
+    0x1.3b563a0000000p-12f, 0x1.a472f80000000p-13f, 0x1.184ca60000000p-13f};
+
+__host__ __device__ inline int32_t gain_bucket_fast(int64_t gx, int64_t vwgt) {
+    const float a = (float)gx;
+    const float b = (float)vwgt;
+    float gain = b == 1.0f ? a : (float)((double)a / (double)b);
+    if (gain > 0.0f) return 0;
+    if (gain == 0.0f) return 1;
+    gain = -gain;
+    int32_t c = 0;
+    if (gain < 1.0f) {
+#pragma unroll
+        for (int j = 0; j < 23; j++) c += gain < GB_DOWN[j] ? 1 : 0;
+        return MID_BUCKET - c < 2 ? 2 : MID_BUCKET - c;
+    }
+#pragma unroll
+    for (int j = 0; j < 26; j++) c += gain > GB_UP[j] ? 1 : 0;
+    return MID_BUCKET + c > MAX_BUCKETS ? MAX_BUCKETS - 1 : MID_BUCKET + c;
+}
+
 }  // namespace jet

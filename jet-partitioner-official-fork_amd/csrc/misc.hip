@@ -160,7 +160,7 @@ int64_t conn_capacity(vid_t n, const eoff_t* rowptr, int32_t k, Arena& a, hipStr
 }
 
 __global__ void k_gain_bucket(int64_t cnt, const int64_t* g, const int64_t* w, int32_t* out) {
-    FOR_THREAD(i, cnt) out[i] = gain_bucket(g[i], w[i]);
+    FOR_THREAD(i, cnt) out[i] = gain_bucket_fast(g[i], w[i]);
 }
 
 void gain_bucket_batch(int64_t cnt, const int64_t* g, const int64_t* w, int32_t* out, hipStream_t s) {

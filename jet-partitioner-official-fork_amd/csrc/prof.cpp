@@ -73,8 +73,9 @@ bool prof_read(int64_t& launches, double& ms, double& bytes) {
         JET_HIP(hipMemcpy(c, P.dev, sizeof(c), hipMemcpyDeviceToHost));
         if (P.family == PF_LP_SELECT) {
             // per cache miss: conn_off 8 + conn_size 4 + part 4 + gain write 4 +
-            // cache write 4; per table slot 8; per candidate 4 (list write)
-            bytes += 24.0 * (double)c[0] + 8.0 * (double)c[1] + 4.0 * (double)c[2];
+            // cache write 4; per table slot 8; per candidate: gain read 4,
+            // pre-gain write 4, row offsets 16, list write 4
+            bytes += 24.0 * (double)c[0] + 8.0 * (double)c[1] + 28.0 * (double)c[2];
         }
         return true;
     }

@@ -110,11 +110,11 @@ class Refiner {
     part_t* cur_part_;
     vid_t* tier_mid_;
     vid_t* tier_big_;
-    // rows of degree > BIG_DEG (tier_big_ and pull_mid_) take their
-    // update_small changes through k_pull_hubs, the others through
-    // k_pull_small: flags per vertex for such a row, and for moved in this
-    // iteration
+    // rows of degree > BIG_DEG ("hubs": tier_big_ and pull_mid_) take their
+    // update_small changes through k_update_hubs: flags per vertex for hub,
+    // hub with a moved neighbour, and moved in this iteration
     uint8_t* hub_;
+    uint8_t* hub_dirty_;
     uint8_t* moved_;
     vid_t* pull_mid_;  // mid-tier rows of degree > BIG_DEG
     int32_t ntier_mid_ = 0, ntier_big_ = 0, npull_mid_ = 0;
@@ -134,6 +134,8 @@ class Refiner {
     vid_t* redraw_;
     int64_t* buckets_;
     int32_t* bflag_;
+    int64_t* bcopies_ = nullptr;  // ncopies_ - 1 more bucket arrays, kept zero between uses
+    int32_t ncopies_ = 1;
     int64_t nbuckets_ = 0;
     int64_t* res_host_ = nullptr;  // thread mailbox: (cut change, largest part) of the last iteration
     int64_t* res_dev_ = nullptr;

@@ -347,12 +347,26 @@ __global__ void __launch_bounds__(BB_BLOCK)
     }
 }
 
+// Table slots in groups of four independent loads; f(slot) sees the slots in
+// order.
+template <class F>
+__device__ inline void for_slots(const int2* tab, int32_t sz, F&& f) {
+    for (int32_t s0 = 0; s0 < sz; s0 += 4) {
+        int2 e[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) e[u] = s0 + u < sz ? tab[s0 + u] : make_int2(NULL_PART, 0);
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (s0 + u < sz) f(e[u]);
+    }
+}
+
 // ---------------------------------------------------------------- jet_lp
 // jet_refiner.hpp:211-268: destination selection, viable-move filter.
 // Each wave owns a contiguous chunk of LS_CHW vertices; candidates are staged
 // in LDS and published with one atomic per workgroup.
 constexpr int LS_WAVES = 4;
-constexpr int LS_CHW = 512;
+constexpr int LS_CHW = 256;
 
 __global__ void __launch_bounds__(LS_WAVES* WAVE)
     k_lp_select(DGraph g, const part_t* part, const eoff_t* conn_off, const int2* conn, const int32_t* conn_size,
@@ -368,22 +382,32 @@ __global__ void __launch_bounds__(LS_WAVES* WAVE)
     if (blockIdx.x == 0 && threadIdx.x < 8) next_counters[threadIdx.x] = 0;
     const int lane = threadIdx.x & (WAVE - 1);
     const int64_t wbase = ((int64_t)blockIdx.x * LS_WAVES + threadIdx.x / WAVE) * LS_CHW;
-    for (int32_t off = 0; off < LS_CHW; off += WAVE) {
-        int64_t i = wbase + off + lane;
+    // the cached destinations and locks of all LS_CHW / WAVE passes are
+    // loaded up front, so the passes do not wait for them one after another
+    constexpr int NPASS = LS_CHW / WAVE;
+    part_t cached[NPASS];
+    int32_t locks[NPASS];
+#pragma unroll
+    for (int u = 0; u < NPASS; u++) {
+        const int64_t i = wbase + u * WAVE + lane;
+        cached[u] = i < g.n ? dest_cache[i] : NO_MOVE;
+        locks[u] = i < g.n ? lock[i] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < NPASS; u++) {
+        const int64_t i = wbase + u * WAVE + lane;
         bool is_cand = false;
         eoff_t deg = 0;
         if (i < g.n) {
-            part_t best = dest_cache[i];
+            part_t best = cached[u];
             if (best == NULL_PART) {
                 part_t p = part[i];
                 best = NO_MOVE;
                 int32_t b_conn = 0, p_conn = 0;
-                const int2* tab = conn + conn_off[i];
-                int32_t sz = conn_size[i];
+                const int32_t sz = conn_size[i];
                 n_miss++;
                 n_slot += sz;
-                for (int32_t s = 0; s < sz; s++) {
-                    int2 e = tab[s];
+                for_slots(conn + conn_off[i], sz, [&](int2 e) {
                     if (e.y > b_conn && e.x != p) {
                         best = e.x;
                         b_conn = e.y;
@@ -392,7 +416,7 @@ __global__ void __launch_bounds__(LS_WAVES* WAVE)
                     } else if (e.y > 0 && e.x == p) {
                         p_conn = e.y;
                     }
-                }
+                });
                 int32_t gsv = 0;
                 if (best != NO_MOVE) {
                     if (b_conn >= p_conn || ((double)(p_conn - b_conn) < floor(fr * (double)p_conn))) gsv = b_conn - p_conn;
@@ -401,14 +425,17 @@ __global__ void __launch_bounds__(LS_WAVES* WAVE)
                 gain_p[i] = gsv;
                 dest_cache[i] = best;
             }
-            dest[i] = best;
-            is_cand = best != NO_MOVE && lock[i] == 0;
+            // a vertex that is not a candidate this iteration gets NO_MOVE as
+            // its destination: the afterburner skips such neighbours before
+            // reading their pre-gain, which is then only written for candidates
+            const int32_t locked = locks[u];
+            is_cand = best != NO_MOVE && locked == 0;
+            dest[i] = is_cand ? best : NO_MOVE;
             if (is_cand) {
                 n_cand++;
                 pregain[i] = gain_p[i];
                 deg = g.rowptr[i + 1] - g.rowptr[i];
-            } else {
-                pregain[i] = GAIN_MIN;
+            } else if (locked) {
                 lock[i] = 0;
             }
         }
@@ -430,12 +457,15 @@ __global__ void __launch_bounds__(LS_WAVES* WAVE)
 
 // jet_refiner.hpp:275-307 (afterburner): gain correction assuming every
 // higher-priority neighbour (larger pre-gain, or equal and smaller id) moves.
+// Non-candidates (destination NO_MOVE) have the lowest priority, as the
+// reference's minimum pre-gain gives them.
 __device__ inline int64_t afterburn_edge(vid_t v, vid_t i, int32_t ig, int32_t w, part_t p, part_t best,
                                          const part_t* part, const part_t* dest, const int32_t* pregain) {
+    part_t vp = dest[v];
+    if (vp == NO_MOVE) return 0;
     int32_t vg = pregain[v];
     int64_t ch = 0;
     if (vg > ig || (vg == ig && v < i)) {
-        part_t vp = dest[v];
         if (vp == p) ch -= w;
         else if (vp == best) ch += w;
         vp = part[v];
@@ -635,6 +665,18 @@ __device__ inline void conn_add(int2* tab, int32_t* vsize, int32_t cap, part_t b
     atomicAdd(&tab[po].y, w);
 }
 
+template <bool ADD>
+__device__ inline void update_edge(vid_t v, int32_t w, part_t q, const eoff_t* conn_off, int2* conn,
+                                   int32_t* conn_size, part_t* dest_cache, int32_t k, int32_t* err) {
+    int2* tab = conn + conn_off[v];
+    if (ADD) {
+        dest_cache[v] = NULL_PART;
+        conn_add(tab, &conn_size[v], (int32_t)(conn_off[v + 1] - conn_off[v]), q, w, err);
+    } else {
+        conn_sub(tab, ld_rx(&conn_size[v]), q, w, k, err);
+    }
+}
+
 // perform_moves' choice between update_small (push each moved vertex's
 // contribution into its neighbours' tables) and update_large (rebuild the
 // tables of every neighbour of a moved vertex): rebuild when more than n/10
@@ -644,104 +686,104 @@ __device__ inline void conn_add(int2* tab, int32_t* vsize, int32_t cap, part_t b
 // choice follows the reference exactly.
 __device__ inline bool update_by_rebuild(const int32_t* mc, vid_t n) { return mc[2] + mc[3] > n / 10; }
 
-// Marks the neighbours of the moved vertices: update_small and update_large
-// (jet_refiner.hpp:743-888) both change exactly these rows. Small moves by G
-// lanes, big moves by a wave (k_mark_big). In the rebuild case the first
-// workgroup also raises the gate of the rebuild kernels (counter 6).
-template <int G>
-__global__ void k_mark_small(DGraph g, const vid_t* moves, int32_t* mc, int32_t* mark, const int32_t* deferred) {
+// ADD == false: subtract from the old part (held in dest), or, when the tables
+// are rebuilt instead, mark the neighbours; ADD == true: add to the new part
+// (held in part). G lanes stride the edges of one move.
+template <bool ADD, int G>
+__global__ void k_update_small(DGraph g, const vid_t* moves, int32_t* mc, vid_t nmax, const part_t* part,
+                               const part_t* dest, const eoff_t* conn_off, int2* conn, int32_t* conn_size,
+                               part_t* dest_cache, int32_t k, int32_t* err, int32_t* mark, const uint8_t* hub,
+                               uint8_t* hub_dirty, const int32_t* deferred) {
     if (*deferred) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && update_by_rebuild(mc, g.n)) mc[6] = 1;
+    const bool rebuild = update_by_rebuild(mc, g.n);
+    if (ADD && rebuild) return;
+    if (!ADD && rebuild && blockIdx.x == 0 && threadIdx.x == 0) mc[6] = 1;  // gates the rebuild
     const int32_t ns = mc[2];
     FOR_GROUP(x, ns) {
         vid_t i = moves[x];
+        if (!ADD && rebuild) {
+            for (eoff_t j = g.rowptr[i] + sub_; j < g.rowptr[i + 1]; j += G) mark[g.col[j]] = 1;
+            continue;
+        }
+        part_t q = ADD ? part[i] : dest[i];
         for (eoff_t j = g.rowptr[i] + sub_; j < g.rowptr[i + 1]; j += G) {
-            vid_t u = g.col[j];
-            if (!mark[u]) mark[u] = 1;
+            vid_t v = g.col[j];
+            if (hub[v]) {  // updated by k_update_hubs
+                if (!ADD) hub_dirty[v] = 1;
+                continue;
+            }
+            update_edge<ADD>(v, g.ew[j], q, conn_off, conn, conn_size, dest_cache, k, err);
         }
     }
 }
 
-__global__ void k_mark_big(DGraph g, const vid_t* moves, const int32_t* mc, vid_t nmax, int32_t* mark,
-                           const int32_t* deferred) {
+template <bool ADD>
+__global__ void k_update_big(DGraph g, const vid_t* moves, const int32_t* mc, vid_t nmax, const part_t* part,
+                             const part_t* dest, const eoff_t* conn_off, int2* conn, int32_t* conn_size,
+                             part_t* dest_cache, int32_t k, int32_t* err, int32_t* mark, const uint8_t* hub,
+                             uint8_t* hub_dirty, const int32_t* deferred) {
     if (*deferred) return;
+    const bool rebuild = update_by_rebuild(mc, g.n);
+    if (ADD && rebuild) return;
     const int32_t nb = mc[3];
     const int lane = threadIdx.x & (WAVE - 1);
     const int64_t wstride = ((int64_t)gridDim.x * blockDim.x) / WAVE;
     for (int64_t x = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE; x < nb; x += wstride) {
         vid_t i = moves[nmax - 1 - x];
+        if (!ADD && rebuild) {
+            for (eoff_t j = g.rowptr[i] + lane; j < g.rowptr[i + 1]; j += WAVE) mark[g.col[j]] = 1;
+            continue;
+        }
+        part_t q = ADD ? part[i] : dest[i];
         for (eoff_t j = g.rowptr[i] + lane; j < g.rowptr[i + 1]; j += WAVE) {
-            vid_t u = g.col[j];
-            if (!mark[u]) mark[u] = 1;
+            vid_t v = g.col[j];
+            if (hub[v]) {  // updated by k_update_hubs
+                if (!ADD) hub_dirty[v] = 1;
+                continue;
+            }
+            update_edge<ADD>(v, g.ew[j], q, conn_off, conn, conn_size, dest_cache, k, err);
         }
     }
 }
 
-// update_small applied per row instead of per moved vertex: each marked row
-// takes its moved neighbours' weights off their old parts (dest) and then
-// puts them on their new parts (part). A row has one owner here, so this
-// leaves the table that per-edge pushes leave (subtractions before additions;
-// a part is reclaimed when its connectivity reaches 0 in a table of fewer
-// than k slots, jet_refiner.hpp:806-810; new parts take free or reclaimed
-// slots, then overflow slots, :828-884), without many moved neighbours'
-// atomics meeting on the same slots. Rows of degree <= BIG_DEG: one thread
-// each (k_pull_small); larger rows: one workgroup with per-part sums in LDS
-// (k_pull_hubs).
-__global__ void k_pull_small(DGraph g, const int32_t* mc, int32_t* mark, const uint8_t* hub, const uint8_t* moved,
-                             const part_t* part, const part_t* dest, const eoff_t* conn_off, int2* conn,
-                             int32_t* conn_size, part_t* dest_cache, int32_t k, int32_t* err, const int32_t* deferred) {
-    if (*deferred || update_by_rebuild(mc, g.n)) return;
-    FOR_THREAD(u, g.n) {
-        if (!mark[u] || hub[u]) continue;
-        const eoff_t rs = g.rowptr[u], re = g.rowptr[u + 1];
-        int2* tab = conn + conn_off[u];
-        const int32_t cap = (int32_t)(conn_off[u + 1] - conn_off[u]);
-        const int32_t vsz = conn_size[u];
-        for (eoff_t j = rs; j < re; j++) {
-            vid_t v = g.col[j];
-            if (moved[v]) conn_sub(tab, vsz, dest[v], g.ew[j], k, err);
-        }
-        for (eoff_t j = rs; j < re; j++) {
-            vid_t v = g.col[j];
-            if (moved[v]) conn_add(tab, &conn_size[u], cap, part[v], g.ew[j], err);
-        }
-        dest_cache[u] = NULL_PART;
-        mark[u] = 0;
-    }
-}
-
-template <int BLOCK>
+// update_small for rows of degree > BIG_DEG ("hubs": the workgroup-built tier
+// and the pull_mid rows), one workgroup per row with a moved neighbour: the row's moved neighbours are found by a scan of
+// its adjacency and their weights summed per part in LDS, then each part's
+// total is applied to the table once, with the subtraction phase before the
+// addition phase as for the other rows. The resulting table equals the one
+// per-edge pushes leave: a part is reclaimed when its connectivity reaches 0
+// in a table of fewer than k slots (jet_refiner.hpp:806-810), new parts take
+// free or reclaimed slots and then overflow slots (:828-884). Pushing instead
+// would land every moved neighbour's atomics on the same few hub slots.
+template <bool ADD, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
-    k_pull_hubs(DGraph g, const vid_t* list, int32_t nlist, const int32_t* mc, int32_t* mark, const uint8_t* moved,
-                const part_t* part, const part_t* dest, const eoff_t* conn_off, int2* conn, int32_t* conn_size,
-                part_t* dest_cache, int32_t k, int32_t* err, const int32_t* deferred) {
+    k_update_hubs(DGraph g, const vid_t* list, int32_t nlist, const int32_t* mc, const uint8_t* moved,
+                  uint8_t* hub_dirty, const part_t* part, const part_t* dest, const eoff_t* conn_off, int2* conn,
+                  int32_t* conn_size, part_t* dest_cache, int32_t k, int32_t* err, const int32_t* deferred) {
     if (*deferred || update_by_rebuild(mc, g.n)) return;
     extern __shared__ int32_t delta[];  // k
     for (int32_t r = blockIdx.x; r < nlist; r += gridDim.x) {
         const vid_t u = list[r];
-        if (!mark[u]) continue;
+        if (!hub_dirty[u]) continue;
+        for (int32_t p = threadIdx.x; p < k; p += BLOCK) delta[p] = 0;
+        __syncthreads();
+        for (eoff_t j = g.rowptr[u] + threadIdx.x; j < g.rowptr[u + 1]; j += BLOCK) {
+            vid_t v = g.col[j];
+            if (moved[v]) atomicAdd(&delta[ADD ? part[v] : dest[v]], g.ew[j]);
+        }
+        __syncthreads();
         int2* tab = conn + conn_off[u];
         const int32_t cap = (int32_t)(conn_off[u + 1] - conn_off[u]);
         const int32_t vsz = conn_size[u];
-        for (int phase = 0; phase < 2; phase++) {  // 0: old parts off, 1: new parts on
-            for (int32_t p = threadIdx.x; p < k; p += BLOCK) delta[p] = 0;
-            __syncthreads();
-            for (eoff_t j = g.rowptr[u] + threadIdx.x; j < g.rowptr[u + 1]; j += BLOCK) {
-                vid_t v = g.col[j];
-                if (moved[v]) atomicAdd(&delta[phase ? part[v] : dest[v]], g.ew[j]);
-            }
-            __syncthreads();
-            for (int32_t p = threadIdx.x; p < k; p += BLOCK) {
-                const int32_t w = delta[p];
-                if (w == 0) continue;
-                if (phase) conn_add(tab, &conn_size[u], cap, p, w, err);
-                else conn_sub(tab, vsz, p, w, k, err);
-            }
-            __syncthreads();
+        for (int32_t p = threadIdx.x; p < k; p += BLOCK) {
+            const int32_t w = delta[p];
+            if (w == 0) continue;
+            if (ADD) conn_add(tab, &conn_size[u], cap, p, w, err);
+            else conn_sub(tab, vsz, p, w, k, err);
         }
-        if (threadIdx.x == 0) {
+        if (ADD && threadIdx.x == 0) {
             dest_cache[u] = NULL_PART;
-            mark[u] = 0;
+            hub_dirty[u] = 0;
         }
         __syncthreads();
     }
@@ -845,8 +887,9 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// Per-workgroup LDS map of bucket sums (consecutive vertices share few
-// buckets); overflow goes straight to global atomics.
+// Per-workgroup LDS map of bucket sums (a workgroup takes a contiguous range
+// of vertices, and consecutive vertices share few buckets); overflow goes
+// straight to global atomics.
 constexpr int BH_SLOTS = 1024;
 
 struct BucketMap {
@@ -861,21 +904,25 @@ struct BucketMap {
     }
     __device__ void add(int32_t b, unsigned long long w, unsigned long long* global) {
         uint32_t h = ((uint32_t)b * 0x9E3779B1u) >> 22;
+        int32_t slot = -1;
         for (int probe = 0; probe < 16; probe++) {
             int32_t k = key[h];
             if (k == -1) {
                 int32_t old = atomicCAS(&key[h], -1, b);
                 if (old == -1 || old == b) {
-                    atomicAdd(&val[h], w);
-                    return;
+                    slot = (int32_t)h;
+                    break;
                 }
             } else if (k == b) {
-                atomicAdd(&val[h], w);
-                return;
+                slot = (int32_t)h;
+                break;
             }
             h = (h + 1) & (BH_SLOTS - 1);
         }
-        atomicAdd(&global[b], w);
+        // distinct scopes keep the LDS add and the global fallback apart (one
+        // flat atomic for both would serve the common LDS case slowly)
+        if (slot >= 0) __hip_atomic_fetch_add(&val[slot], w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else atomicAdd(&global[b], w);
     }
     __device__ void flush(unsigned long long* global) {
         __syncthreads();
@@ -884,30 +931,85 @@ struct BucketMap {
     }
 };
 
+// The select kernels' workgroups flush their bucket maps into `copies`
+// copies of the bucket array (workgroup b into copy b mod copies), so that
+// fewer workgroups' atomics meet on one line of it; k_sum_bucket_copies folds
+// copies 1.. into copy 0 (the bucket array itself) and clears them.
+constexpr int BUCKET_COPIES_MAX = 16;
+constexpr int64_t BUCKET_COPIES_BYTES = 64ll << 20;
+
+__device__ inline unsigned long long* bucket_copy(unsigned long long* buckets, unsigned long long* extra,
+                                                  int32_t copies, int64_t stride) {
+    const int32_t c = (int32_t)(blockIdx.x % (uint32_t)copies);
+    return c == 0 ? buckets : extra + (int64_t)(c - 1) * stride;
+}
+
+__global__ void k_sum_bucket_copies(int64_t* buckets, int64_t* extra, int32_t copies, int64_t nb, int64_t stride) {
+    FOR_THREAD(j, nb) {
+        int64_t sum = buckets[j];
+        for (int32_t c0 = 0; c0 < copies - 1; c0 += 8) {
+            int64_t v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = c0 + u < copies - 1 ? extra[(int64_t)(c0 + u) * stride + j] : 0;
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                sum += v[u];
+                if (v[u]) extra[(int64_t)(c0 + u) * stride + j] = 0;
+            }
+        }
+        buckets[j] = sum;
+    }
+}
+
+// Part sizes of the select kernels: staged in LDS when k <= SEL_LDS_K (each
+// table slot's destination check is then an LDS read, not a dependent global
+// load), read from global memory otherwise.
+constexpr int SEL_LDS_K = 1024;
+
+template <bool LDS>
+struct PartSizes {
+    const int64_t* global;
+    int64_t* lds;
+    __device__ void stage(int32_t k) {
+        if (LDS)
+            for (int32_t t = threadIdx.x; t < k; t += blockDim.x) lds[t] = global[t];
+    }
+    __device__ int64_t operator[](int32_t p) const { return LDS ? lds[p] : global[p]; }
+};
+
 // jet_refiner.hpp:592-643: destination and bucket of each vertex of an
 // oversized part.
-__global__ void k_weak_select(DGraph g, const part_t* part, const int64_t* sizes, int64_t size_max, int32_t opt,
-                              int64_t max_dest, const part_t* undersized, const int32_t* misc,
-                              const eoff_t* conn_off, const int2* conn, const int32_t* conn_size, int32_t sections,
-                              part_t* dest, int32_t* bid, unsigned long long* buckets) {
+template <bool LDS>
+__global__ void __launch_bounds__(256)
+    k_weak_select(DGraph g, const part_t* part, const int64_t* sizes, int32_t k, int64_t size_max, int32_t opt,
+                  int64_t max_dest, const part_t* undersized, const int32_t* misc, const eoff_t* conn_off,
+                  const int2* conn, const int32_t* conn_size, int32_t sections, part_t* dest, int32_t* bid,
+                  unsigned long long* buckets_base, unsigned long long* extra, int32_t copies, int64_t stride) {
+    unsigned long long* buckets = bucket_copy(buckets_base, extra, copies, stride);
     __shared__ int32_t bk[BH_SLOTS];
     __shared__ unsigned long long bv[BH_SLOTS];
+    __shared__ int64_t ssz[LDS ? SEL_LDS_K : 1];
+    PartSizes<LDS> ps{sizes, ssz};
+    ps.stage(k);
     BucketMap bm{bk, bv};
     bm.init();
     const int32_t tu = misc[0];
-    FOR_THREAD(i, g.n) {
-        part_t p = part[i];
+    FOR_BLOCK_RANGE(i, g.n) {
+        const part_t p = part[i];
         part_t d = p;
         int32_t b = -1;
-        int64_t sp = sizes[p];
-        if (sp > size_max && (double)g.vw[i] < 1.5 * (double)(sp - opt)) {
+        const int64_t sp = ps[p];
+        // the loads a vertex of an oversized part needs are issued together
+        const bool over = sp > size_max;
+        const wgt_t w = over ? g.vw[i] : 0;
+        const eoff_t off = over ? conn_off[i] : 0;
+        const int32_t sz = over ? conn_size[i] : 0;
+        const part_t spare = over && tu > 0 ? undersized[(uint32_t)i % (uint32_t)tu] : p;
+        if (over && (double)w < 1.5 * (double)(sp - opt)) {
             part_t best = p;
             int32_t gain = 0, p_gain = 0;
-            const int2* tab = conn + conn_off[i];
-            int32_t sz = conn_size[i];
-            for (int32_t s = 0; s < sz; s++) {
-                int2 e = tab[s];
-                if (e.x > NULL_PART && sizes[e.x] < max_dest) {
+            for_slots(conn + off, sz, [&](int2 e) {
+                if (e.x > NULL_PART && ps[e.x] < max_dest) {
                     if (e.y > gain) {
                         best = e.x;
                         gain = e.y;
@@ -916,19 +1018,19 @@ __global__ void k_weak_select(DGraph g, const part_t* part, const int64_t* sizes
                     }
                 }
                 if (e.x == p) p_gain = e.y;
-            }
+            });
             int64_t sv = 0;
             if (gain > 0) {
                 d = best;
                 sv = (int64_t)gain - p_gain;
             } else if (tu > 0) {
-                d = undersized[i % tu];
+                d = spare;
                 sv = -(int64_t)p_gain;
             }
             if (d != p) {
-                int32_t gt = gain_bucket(sv, g.vw[i]);
-                b = (MAX_BUCKETS * p + gt) * sections + (int32_t)(i % sections);
-                bm.add(b, (unsigned long long)g.vw[i], buckets);
+                int32_t gt = gain_bucket_fast(sv, w);
+                b = (MAX_BUCKETS * p + gt) * sections + (int32_t)((uint32_t)i % (uint32_t)sections);
+                bm.add(b, (unsigned long long)w, buckets);
             }
         }
         dest[i] = d;
@@ -938,41 +1040,47 @@ __global__ void k_weak_select(DGraph g, const part_t* part, const int64_t* sizes
 }
 
 // jet_refiner.hpp:449-481.
-__global__ void k_strong_select(DGraph g, const part_t* part, const int64_t* sizes, int64_t size_max, int32_t opt,
-                                int64_t max_dest, const int64_t* acc, const eoff_t* conn_off, const int2* conn,
-                                const int32_t* conn_size, int32_t sections, int32_t* bid,
-                                unsigned long long* buckets) {
+template <bool LDS>
+__global__ void __launch_bounds__(256)
+    k_strong_select(DGraph g, const part_t* part, const int64_t* sizes, int32_t k, int64_t size_max, int32_t opt,
+                    int64_t max_dest, const int64_t* acc, const eoff_t* conn_off, const int2* conn,
+                    const int32_t* conn_size, int32_t sections, int32_t* bid, unsigned long long* buckets_base,
+                    unsigned long long* extra, int32_t copies, int64_t stride) {
+    unsigned long long* buckets = bucket_copy(buckets_base, extra, copies, stride);
     __shared__ int32_t bk[BH_SLOTS];
     __shared__ unsigned long long bv[BH_SLOTS];
+    __shared__ int64_t ssz[LDS ? SEL_LDS_K : 1];
+    PartSizes<LDS> ps{sizes, ssz};
+    ps.stage(k);
     BucketMap bm{bk, bv};
     bm.init();
     const int64_t max_vwgt = acc[3];
-    FOR_THREAD(i, g.n) {
-        part_t p = part[i];
+    FOR_BLOCK_RANGE(i, g.n) {
+        const part_t p = part[i];
         int32_t b = -1;
-        int64_t sp = sizes[p];
-        int64_t w = g.vw[i];
+        const int64_t sp = ps[p];
+        const bool over = sp > size_max;
+        const int64_t w = over ? g.vw[i] : 0;
+        const eoff_t off = over ? conn_off[i] : 0;
+        const int32_t sz = over ? conn_size[i] : 0;
         bool vw_ok = max_vwgt > 0 && w <= 2 * max_vwgt;
-        if (sp > size_max && vw_ok && w < 2 * (sp - opt)) {
+        if (over && vw_ok && w < 2 * (sp - opt)) {
             uint64_t tk = 0, tg = 0;
             int64_t p_gain = 0;
-            const int2* tab = conn + conn_off[i];
-            int32_t sz = conn_size[i];
-            for (int32_t s = 0; s < sz; s++) {
-                int2 e = tab[s];
+            for_slots(conn + off, sz, [&](int2 e) {
                 if (e.x == p) {
                     p_gain = e.y;
-                } else if (e.x > NULL_PART && sizes[e.x] < max_dest) {
+                } else if (e.x > NULL_PART && ps[e.x] < max_dest) {
                     tg += (uint64_t)e.y;
                     tk += 1;
                 }
-            }
+            });
             if (tk == 0) tk = 1;
             int64_t gain = (int64_t)(tg / tk) - p_gain;
             int64_t vwgt = w < sp - size_max ? w : sp - size_max;
-            int32_t gt = gain_bucket(gain, vwgt);
+            int32_t gt = gain_bucket_fast(gain, vwgt);
             if (gt < MAX_BUCKETS) {
-                b = (MAX_BUCKETS * p + gt) * sections + (int32_t)(i % sections) + 1;
+                b = (MAX_BUCKETS * p + gt) * sections + (int32_t)((uint32_t)i % (uint32_t)sections) + 1;
                 bm.add(b, (unsigned long long)w, buckets);
             }
         }
@@ -1366,6 +1474,7 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
     tier_mid_ = a.alloc<vid_t>(N);
     tier_big_ = a.alloc<vid_t>(N);
     hub_ = a.alloc<uint8_t>(N);
+    hub_dirty_ = a.alloc<uint8_t>(N);
     pull_mid_ = a.alloc<vid_t>(N);
     moved_ = a.alloc<uint8_t>(N);
     counters_ = a.alloc<int32_t>(32);
@@ -1379,8 +1488,13 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
     redraw_ = a.alloc<vid_t>(RD_MAX);
     nbuckets_ = (int64_t)MAX_BUCKETS * p.k * sections_for(n, p.k) + MAX_SECTIONS + 4;
     buckets_ = a.alloc<int64_t>(nbuckets_);
+    ncopies_ = (int32_t)std::max<int64_t>(
+        1, std::min<int64_t>(BUCKET_COPIES_MAX, BUCKET_COPIES_BYTES / ((int64_t)sizeof(int64_t) * nbuckets_)));
+    bcopies_ = a.alloc<int64_t>(std::max<int64_t>(1, nbuckets_ * (ncopies_ - 1)));
+    if (ncopies_ > 1) fill_i64(bcopies_, 0, nbuckets_ * (ncopies_ - 1), s);
     bflag_ = a.alloc<int32_t>(nbuckets_);
     fill_i32(swap_bit_, 0, N, s);
+    JET_HIP(hipMemsetAsync(hub_dirty_, 0, (size_t)N, s));
     JET_HIP(hipMemsetAsync(moved_, 0, (size_t)N, s));
     fill_i32(counters_, 0, 32, s);
     fill_i64(acc_, 0, 8, s);
@@ -1445,8 +1559,8 @@ void Refiner::jet_lp(const Level& L, const part_t* part, double fr) {
                                                 dest_, pregain_, cand_, c, nmax_, cnt_next(),
                                                 prof_counters(PF_LP_SELECT), prof_span(PF_LP_SELECT));
     JET_LAUNCH_CHECK();
-    // per vertex: cache read 4, destination write 4, lock read 4, pre-gain write 4
-    prof_account(PF_LP_SELECT, 16.0 * g.n);
+    // per vertex: cache read 4, destination write 4, lock read 4
+    prof_account(PF_LP_SELECT, 12.0 * g.n);
     unsigned ab = (unsigned)std::max<int64_t>(256, ceil_div((int64_t)g.n, (int64_t)AB_WAVES * AB_CHW));
     switch (group_for_avg(g)) {
         case 4: k_afterburner_small<4><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_); break;
@@ -1560,14 +1674,33 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
     k_reset_buckets<<<grid_for(nbk + 1, 256, 2048), 256, 0, s_>>>(buckets_, bflag_, nbk + 1, cnt_next(), sizes, k,
                                                                     max_dest, strong ? 1 : 0, undersized_, misc_, acc_,
                                                                     evict_adjust_);
-    int nsel = grid_for(g.n, 256, 1024);  // each workgroup amortises its LDS bucket map
-    if (strong)
-        k_strong_select<<<nsel, 256, 0, s_>>>(g, part, sizes, smax, opt_, max_dest, acc_, conn_off_, conn_, conn_size_,
-                                            sections, bid_, reinterpret_cast<unsigned long long*>(buckets_));
-    else
-        k_weak_select<<<nsel, 256, 0, s_>>>(g, part, sizes, smax, opt_, max_dest, undersized_, misc_, conn_off_, conn_,
-                                          conn_size_, sections, dest_, bid_,
-                                          reinterpret_cast<unsigned long long*>(buckets_));
+    // contiguous ranges of at least 512 vertices per workgroup: vertices of
+    // one oversized part share few buckets, and the serial per-vertex chain
+    // of a heavy range stays short
+    int nsel = grid_for(g.n, 512, 4096);
+    unsigned long long* bk = reinterpret_cast<unsigned long long*>(buckets_);
+    unsigned long long* bc = reinterpret_cast<unsigned long long*>(bcopies_);
+    const bool lds = k <= SEL_LDS_K;
+    if (strong) {
+        if (lds)
+            k_strong_select<true><<<nsel, 256, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, acc_, conn_off_, conn_,
+                                                        conn_size_, sections, bid_, bk, bc, ncopies_, nbuckets_);
+        else
+            k_strong_select<false><<<nsel, 256, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, acc_, conn_off_,
+                                                         conn_, conn_size_, sections, bid_, bk, bc, ncopies_, nbuckets_);
+    } else {
+        if (lds)
+            k_weak_select<true><<<nsel, 256, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, undersized_, misc_,
+                                                      conn_off_, conn_, conn_size_, sections, dest_, bid_, bk, bc,
+                                                      ncopies_, nbuckets_);
+        else
+            k_weak_select<false><<<nsel, 256, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, undersized_, misc_,
+                                                       conn_off_, conn_, conn_size_, sections, dest_, bid_, bk, bc,
+                                                       ncopies_, nbuckets_);
+    }
+    if (nsel > 1 && ncopies_ > 1)
+        k_sum_bucket_copies<<<grid_for(nbk + 1, 256, 2048), 256, 0, s_>>>(buckets_, bcopies_, ncopies_, nbk + 1,
+                                                                          nbuckets_);
     JET_LAUNCH_CHECK();
     excl_scan_i64_i64(buckets_, buckets_, nbk, a_, s_);
     int cb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_WAVES * LS_CHW));
@@ -1598,27 +1731,53 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
     int G = group_for_avg(g);
     int nbs = grid_for((int64_t)g.n * G, 256, 1024);
     int nbb = grid_for((int64_t)g.n * WAVE, 256, 512);
-    switch (G) {
-        case 4: k_mark_small<4><<<nbs, 256, 0, s_>>>(g, moves_, c, swap_bit_, misc_ + 2); break;
-        case 8: k_mark_small<8><<<nbs, 256, 0, s_>>>(g, moves_, c, swap_bit_, misc_ + 2); break;
-        case 16: k_mark_small<16><<<nbs, 256, 0, s_>>>(g, moves_, c, swap_bit_, misc_ + 2); break;
-        case 32: k_mark_small<32><<<nbs, 256, 0, s_>>>(g, moves_, c, swap_bit_, misc_ + 2); break;
-        default: k_mark_small<64><<<nbs, 256, 0, s_>>>(g, moves_, c, swap_bit_, misc_ + 2); break;
+    for (int add = 0; add < 2; add++) {
+#define JET_UPD(GG)                                                                                                   \
+    if (add) k_update_small<true, GG><<<nbs, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,       \
+                                                          conn_size_, dest_cache_, k, misc_ + 1, swap_bit_,      \
+                                                          hub_, hub_dirty_, misc_ + 2);                         \
+    else k_update_small<false, GG><<<nbs, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,          \
+                                                        conn_size_, dest_cache_, k, misc_ + 1, swap_bit_,        \
+                                                        hub_, hub_dirty_, misc_ + 2);
+        switch (G) {
+            case 4: JET_UPD(4) break;
+            case 8: JET_UPD(8) break;
+            case 16: JET_UPD(16) break;
+            case 32: JET_UPD(32) break;
+            default: JET_UPD(64) break;
+        }
+#undef JET_UPD
+        if (has_big_) {
+            if (add) k_update_big<true><<<nbb, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,
+                                                             conn_size_, dest_cache_, k, misc_ + 1, swap_bit_, hub_,
+                                                             hub_dirty_, misc_ + 2);
+            else k_update_big<false><<<nbb, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,
+                                                           conn_size_, dest_cache_, k, misc_ + 1, swap_bit_, hub_,
+                                                           hub_dirty_, misc_ + 2);
+        }
+        // pulled rows: big-tier rows by a 256-thread workgroup, rows of
+        // degree (BIG_DEG, HEAVY_DEG_REFINE] by a single-wave workgroup
+        const size_t lds = sizeof(int32_t) * (size_t)k;
+        if (ntier_big_ > 0) {
+            const int hb = std::min(ntier_big_, 2048);
+            if (add) k_update_hubs<true, 256><<<hb, 256, lds, s_>>>(g, tier_big_, ntier_big_, c, moved_, hub_dirty_,
+                                                                    part, dest_, conn_off_, conn_, conn_size_,
+                                                                    dest_cache_, k, misc_ + 1, misc_ + 2);
+            else k_update_hubs<false, 256><<<hb, 256, lds, s_>>>(g, tier_big_, ntier_big_, c, moved_, hub_dirty_,
+                                                                 part, dest_, conn_off_, conn_, conn_size_,
+                                                                 dest_cache_, k, misc_ + 1, misc_ + 2);
+        }
+        if (npull_mid_ > 0) {
+            const int hb = std::min(npull_mid_, 8192);
+            if (add) k_update_hubs<true, WAVE><<<hb, WAVE, lds, s_>>>(g, pull_mid_, npull_mid_, c, moved_, hub_dirty_,
+                                                                      part, dest_, conn_off_, conn_, conn_size_,
+                                                                      dest_cache_, k, misc_ + 1, misc_ + 2);
+            else k_update_hubs<false, WAVE><<<hb, WAVE, lds, s_>>>(g, pull_mid_, npull_mid_, c, moved_, hub_dirty_,
+                                                                   part, dest_, conn_off_, conn_, conn_size_,
+                                                                   dest_cache_, k, misc_ + 1, misc_ + 2);
+        }
+        JET_LAUNCH_CHECK();
     }
-    if (has_big_) k_mark_big<<<nbb, 256, 0, s_>>>(g, moves_, c, nmax_, swap_bit_, misc_ + 2);
-    k_pull_small<<<grid_for(g.n, 256, 2048), 256, 0, s_>>>(g, c, swap_bit_, hub_, moved_, part, dest_, conn_off_,
-                                                           conn_, conn_size_, dest_cache_, k, misc_ + 1, misc_ + 2);
-    const size_t lds = sizeof(int32_t) * (size_t)k;
-    if (ntier_big_ > 0)
-        k_pull_hubs<256><<<std::min(ntier_big_, 2048), 256, lds, s_>>>(g, tier_big_, ntier_big_, c, swap_bit_, moved_,
-                                                                       part, dest_, conn_off_, conn_, conn_size_,
-                                                                       dest_cache_, k, misc_ + 1, misc_ + 2);
-    if (npull_mid_ > 0)
-        k_pull_hubs<WAVE><<<std::min(npull_mid_, 8192), WAVE, lds, s_>>>(g, pull_mid_, npull_mid_, c, swap_bit_,
-                                                                         moved_, part, dest_, conn_off_, conn_,
-                                                                         conn_size_, dest_cache_, k, misc_ + 1,
-                                                                         misc_ + 2);
-    JET_LAUNCH_CHECK();
     init_conn(L, part, swap_bit_);  // rebuild marked rows (gated on the device)
     k_moves_finish<<<nb, 256, 0, s_>>>(moves_, c, nmax_, part, dest_, conn_off_, conn_, conn_size_, sizes, k, acc_,
                                        res_dev_, moved_, misc_ + 2);

@@ -11,6 +11,15 @@ namespace jet {
     for (int64_t IDX_ = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; IDX_ < (LEN_);         \
          IDX_ += (int64_t)gridDim.x * blockDim.x)
 
+// One contiguous range of items per workgroup (ranges rounded to blockDim),
+// one item per thread and pass: consecutive items share a workgroup, so a
+// workgroup-local aggregation sees fewer distinct keys than a grid stride.
+#define FOR_BLOCK_RANGE(IDX_, LEN_)                                                                  \
+    const int64_t per_block_ =                                                                       \
+        (((int64_t)(LEN_) + gridDim.x - 1) / gridDim.x + blockDim.x - 1) / blockDim.x * blockDim.x;   \
+    const int64_t range_end_ = min((int64_t)(LEN_), (int64_t)(blockIdx.x + 1) * per_block_);         \
+    for (int64_t IDX_ = (int64_t)blockIdx.x * per_block_ + threadIdx.x; IDX_ < range_end_; IDX_ += blockDim.x)
+
 // Grid-stride loop, one item per group of G lanes (G a power of two <= 64);
 // defines sub_ = lane index inside the group.
 #define FOR_GROUP(IDX_, LEN_)                                                                 \

@@ -108,3 +108,18 @@ def test_batch_context_without_gpu_is_a_clean_error():
     with pytest.raises(jet_amd.JetError) as e:
         jet_amd.Context(workers=2)
     assert e.value.code == jet_amd.JET_EDEVICE
+
+
+def test_loop_free_gain_bucket_equals_the_loop(tmp_path):
+    """The kernels' loop-free gain_bucket (threshold tables) against the
+    literal loop of jet_refiner.hpp:326-356, both built for the host from the
+    device header: dense integers, every threshold's neighbouring floats and
+    random 64-bit pairs. A one-ulp change to any threshold fails it."""
+    src = os.path.join(ROOT, "tests", "cpp", "gain_bucket_check.cpp")
+    exe = str(tmp_path / "gain_bucket_check")
+    csrc = os.path.join(ROOT, "jet-partitioner-official-fork_amd", "csrc")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-I" + csrc, src,
+                    "-o", exe], check=True, capture_output=True, text=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout
+    assert int(r.stdout.split()[-1]) > 7_000_000
