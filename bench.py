@@ -52,6 +52,24 @@ def counted_traffic(kernel, workload):
     return None
 
 
+def copy_bandwidth(torch):
+    """This device's plain-copy bandwidth (read + write bytes / time) over a
+    1 GiB int32 copy, best of three, timed with events on torch's stream."""
+    a = torch.ones(1 << 28, dtype=torch.int32, device="cuda:0")
+    b = torch.empty_like(a)
+    best = None
+    for _ in range(3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        t = e0.elapsed_time(e1) / 1e3
+        best = t if best is None else min(best, t)
+    del a, b
+    return round(2 * (4 << 28) / best / 1e9, 1)
+
+
 def make_graph(name):
     if name == "stencil3d-128":
         return jet_amd.stencil3d(128, 128, 128, seed=0, wmax=10)
@@ -337,8 +355,10 @@ def main():
     g = make_graph(args.workload)
     n = len(g["rowptr"]) - 1
     m = len(g["col"])
-    print(f"# workload {args.workload}: n={n} m={m} (generated in {time.time() - t_gen:.1f}s)", file=sys.stderr,
-          flush=True)
+    deg = np.diff(g["rowptr"])
+    max_deg, isolated = int(deg.max()), int((deg == 0).sum())
+    print(f"# workload {args.workload}: n={n} m={m} max degree {max_deg} isolated {isolated} "
+          f"(generated in {time.time() - t_gen:.1f}s)", file=sys.stderr, flush=True)
     dev = torch.device("cuda:0")
     d_rp = torch.from_numpy(g["rowptr"]).to(dev)
     d_col = torch.from_numpy(g["col"]).to(dev)
@@ -393,6 +413,7 @@ def main():
             per_launch = nbytes.value / launches.value
             achieved = per_launch / avg_s / 1e9
             roofline = {"kernel": args.prof, "bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0,
+                        "copy_GBps": copy_bandwidth(torch),
                         "unit": "GB/s", "frac": round(achieved / 8000.0, 4),
                         "traffic": counted_traffic(args.prof, args.workload),
                         "launches": t_launches, "avg_us": round(avg_s * 1e6, 2),
@@ -452,6 +473,7 @@ def main():
         "dtype": "int32",
         "data": "synthetic",
         "config": {"workload": args.workload, "description": wl["desc"], "vertices": n, "undirected_edges": edges,
+                   "max_degree": max_deg, "isolated_vertices": isolated,
                    "k": wl["k"], "imbalance": wl["imb"], "coarsening_alg": wl["alg"], "seed": args.seed},
         "edge_cut": cut,
         "cut_vs_oracle": cut_ratio,

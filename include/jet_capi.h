@@ -39,8 +39,8 @@ extern "C" {
 /* config_t (header/jet_config.h:43-52): same fields, same meaning.
  * coarsening_alg: 0 = 2-hop matching (MtMetis), 1 = HEC, 2 = pure matching,
  * other = 2-hop matching (partitioner.hpp:75-87). num_iter: attempts, best
- * edge cut kept by strict < (app/driver.cpp:110-130). dump_coarse is not
- * supported and must be 0. */
+ * edge cut kept by strict < (app/driver.cpp:110-130). dump_coarse: write the
+ * coarse hierarchy during the call (see jet_load_hierarchy below). */
 typedef struct jet_config_t {
     int32_t coarsening_alg;
     int32_t num_iter;
@@ -231,6 +231,44 @@ int jet_load_config(const char* path, jet_config_t* cfg);
 /* app/io.hpp:207-235: one part id per line. */
 int jet_write_partition(const char* path, int64_t n, const int32_t* part);
 int jet_read_partition(const char* path, int64_t n, int32_t* part);
+
+/* Coarse-hierarchy dump and replay (src/binary_dump.hpp:61-97,
+ * app/import_coarse.cpp:49-145, uncoarsen.hpp:94-111).
+ *
+ * With config.dump_coarse set, a plain partition call writes two files in the
+ * working directory before it refines the first level (from the coarsest)
+ * whose partition is balanced: coarse_part.out (that level's partition, raw
+ * int32) and coarse_graphs.out (int32 level count, then per level, finest
+ * first: n, m, row offsets[n+1], columns[m], edge weights[m], vertex
+ * weights[n] and, from the second level on, the map from the previous level).
+ * The widths are the reference graph type's for the entry point used:
+ * jet_partition_csr32 writes JET_DUMP_32, jet_partition_csr and
+ * jet_partition_csr_big JET_DUMP_BIG, jet_partition_csr_biggest
+ * JET_DUMP_BIGGEST. Each attempt dumps; the last one's files remain.
+ * dump_coarse is rejected for jobs of a jet_ctx. */
+#define JET_DUMP_32 0      /* matrix_t: 4-byte ordinals, offsets and weights */
+#define JET_DUMP_BIG 1     /* big_matrix_t: 4-byte ordinals, 8-byte offsets and weights */
+#define JET_DUMP_BIGGEST 2 /* biggest_matrix_t: 8-byte ordinals, offsets and weights */
+
+typedef struct jet_hierarchy jet_hierarchy_t;
+
+/* import_coarse.cpp:49-110: reads and checks a hierarchy file into host memory. */
+int jet_load_hierarchy(const char* path, int32_t kind, jet_hierarchy_t** out);
+int32_t jet_hierarchy_levels(const jet_hierarchy_t* h);
+int64_t jet_hierarchy_vertices(const jet_hierarchy_t* h, int32_t level); /* level 0 = finest; -1 if out of range */
+int64_t jet_hierarchy_edges(const jet_hierarchy_t* h, int32_t level);
+void jet_free_hierarchy(jet_hierarchy_t* h);
+
+/* import_coarse.cpp:112-121: n raw int32 part ids. */
+int jet_load_coarse_part(const char* path, int64_t n, int32_t* part);
+
+/* import_coarse.cpp:123-145 (uncoarsen.hpp:127-155) on the GPU: refines the
+ * loaded hierarchy from its coarsest level, whose partition is coarse_part
+ * (host, n of the last level), to the finest; part_out (host) receives the
+ * finest level's partition and cut_out its edge cut. Refinement draws no
+ * random numbers, so no seed is taken. dump_coarse must be 0. */
+int jet_uncoarsen_hierarchy(const jet_config_t* cfg, const jet_hierarchy_t* h, const int32_t* coarse_part,
+                            int32_t* part_out, int64_t* cut_out, jet_stats_t* stats);
 
 /* src/part_stat.hpp:77-239 and app/part_eval.cpp:46-115, computed on the GPU. */
 typedef struct jet_quality_t {

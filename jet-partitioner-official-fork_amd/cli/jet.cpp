@@ -2,8 +2,10 @@
 //   jet <metis_graph_file> <config_file> [partition_output_file] [metrics_file]
 // Builds: jet (refine_tolerance 0.999), jet4 (0.9999), jet2 (0.99), jet_host and
 // jet_serial (the reference's host-memory variants; here every variant runs on
-// the GPU). Attempt i uses seed JET_SEED + i (default 0); the best cut is kept
-// by strict <. The metrics file, when given, receives one JSON line per attempt.
+// the GPU), and jet_export (dump_coarse set: writes coarse_graphs.out and
+// coarse_part.out for jet_import, with the reference driver's int32 graph
+// type). Attempt i uses seed JET_SEED + i (default 0); the best cut is kept by
+// strict <. The metrics file, when given, receives one JSON line per attempt.
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -34,6 +36,9 @@ int main(int argc, char** argv) {
         return -1;
     }
     cfg.refine_tolerance = JET_REFINE_TOL;
+#ifdef JET_EXPORT
+    cfg.dump_coarse = 1;
+#endif
     jet_host_graph_t g;
     if (jet_load_metis(argv[1], &g) != JET_OK) {
         std::fprintf(stderr, "FATAL ERROR: %s\n", jet_last_error());
@@ -52,8 +57,16 @@ int main(int argc, char** argv) {
     for (int i = 0; i < attempts; i++) {
         int64_t cut = 0;
         jet_stats_t st;
+#ifdef JET_EXPORT
+        // the reference driver's graph type (int32 offsets and weights) sets
+        // the dump's widths
+        std::vector<int32_t> rp32(g.row_offsets, g.row_offsets + g.n + 1);
+        int rc = jet_partition_csr32(&one, g.n, rp32.data(), g.col, g.ew, nullptr, g.uniform_ew, JET_PTR_HOST,
+                                     seed + (unsigned long long)i, part.data(), &cut, &st);
+#else
         int rc = jet_partition_csr(&one, g.n, g.row_offsets, g.col, g.ew, nullptr, g.uniform_ew, JET_PTR_HOST,
                                    seed + (unsigned long long)i, part.data(), &cut, &st);
+#endif
         if (rc != JET_OK) {
             std::fprintf(stderr, "FATAL ERROR: %s\n", jet_last_error());
             jet_free_graph(&g);

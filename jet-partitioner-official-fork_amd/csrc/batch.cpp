@@ -128,6 +128,11 @@ int check_job_args(const jet_config_t* cfg, int64_t n, const int64_t* rp, const 
         jet::set_last_error("ptr_space must be JET_PTR_HOST or JET_PTR_DEVICE");
         return JET_EINVAL;
     }
+    if (cfg->dump_coarse) {
+        // concurrent jobs would write the same two files
+        jet::set_last_error("dump_coarse is not supported for jobs of a jet_ctx");
+        return JET_EINVAL;
+    }
     return JET_OK;
 }
 
@@ -293,6 +298,11 @@ int jet_partition_batch(jet_ctx_t* X, int32_t count, const jet_config_t* cfgs, c
             if (n < 1 || m < 0) {
                 first_rc = JET_EINVAL;
                 first_err = "graph " + std::to_string(g) + ": empty vertex range or negative edge range";
+                break;
+            }
+            if (cfgs[g].dump_coarse) {
+                first_rc = JET_EINVAL;
+                first_err = "graph " + std::to_string(g) + ": dump_coarse is not supported for jobs of a jet_ctx";
                 break;
             }
             Job* j = new Job();

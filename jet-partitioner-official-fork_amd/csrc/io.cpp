@@ -6,10 +6,12 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <vector>
 
+#include "hierarchy.h"
 #include "jet_capi.h"
 
 namespace jet {
@@ -171,6 +173,101 @@ int jet_read_partition(const char* path, int64_t n, int32_t* part) {
     if (!f.is_open()) return fail(JET_EINVAL, std::string("could not open partition file ") + path);
     for (int64_t i = 0; i < n; i++)
         if (!(f >> part[i])) return fail(JET_EINVAL, "partition file has fewer than n entries");
+    return JET_OK;
+}
+
+// import_coarse.cpp:49-110: the hierarchy file. Values are read at the widths
+// of the reference graph type that wrote them and checked: sizes against the
+// file, offsets monotone from 0, columns and map entries in range, weights in
+// [1, 2^31).
+int jet_load_hierarchy(const char* path, int32_t kind, jet_hierarchy_t** out) {
+    if (!path || !out) return fail(JET_EINVAL, "path and out are required");
+    *out = nullptr;
+    if (kind != JET_DUMP_32 && kind != JET_DUMP_BIG && kind != JET_DUMP_BIGGEST)
+        return fail(JET_EINVAL, "kind must be JET_DUMP_32, JET_DUMP_BIG or JET_DUMP_BIGGEST");
+    const int ob = kind == JET_DUMP_BIGGEST ? 8 : 4;  // ordinal bytes
+    const int eb = kind == JET_DUMP_32 ? 4 : 8;       // offset and weight bytes
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(JET_EINVAL, std::string("could not open hierarchy file ") + path);
+    auto rd = [&](int bytes, int64_t count, std::vector<int64_t>& v) -> bool {
+        v.resize((size_t)count);
+        if (bytes == 8) return count == 0 || std::fread(v.data(), 8, (size_t)count, f) == (size_t)count;
+        std::vector<int32_t> t((size_t)count);
+        if (count > 0 && std::fread(t.data(), 4, (size_t)count, f) != (size_t)count) return false;
+        for (int64_t i = 0; i < count; i++) v[(size_t)i] = t[(size_t)i];
+        return true;
+    };
+    auto bad = [&](const std::string& msg) {
+        std::fclose(f);
+        return fail(JET_EINVAL, "hierarchy file " + std::string(path) + ": " + msg);
+    };
+    int32_t count = 0;
+    if (std::fread(&count, 4, 1, f) != 1 || count < 1) return bad("missing or non-positive level count");
+    std::unique_ptr<jet_hierarchy_t> h(new jet_hierarchy_t());
+    std::vector<int64_t> t;
+    int64_t prev_n = 0;
+    for (int32_t l = 0; l < count; l++) {
+        jet_hierarchy::Level L;
+        std::vector<int64_t> hdr;
+        if (!rd(ob, 1, hdr)) return bad("truncated at level " + std::to_string(l));
+        L.n = hdr[0];
+        if (!rd(eb, 1, hdr)) return bad("truncated at level " + std::to_string(l));
+        L.m = hdr[0];
+        if (L.n < 1 || L.n > 2147483645LL || L.m < 0) return bad("level " + std::to_string(l) + ": bad n or m");
+        if (!rd(eb, L.n + 1, L.rowptr)) return bad("truncated row offsets at level " + std::to_string(l));
+        if (L.rowptr[0] != 0 || L.rowptr[(size_t)L.n] != L.m) return bad("row offsets do not span m");
+        for (int64_t i = 0; i < L.n; i++)
+            if (L.rowptr[(size_t)i + 1] < L.rowptr[(size_t)i]) return bad("row offsets not monotone");
+        auto narrow = [&](int bytes, int64_t cnt, std::vector<int32_t>& dst, int64_t lo, int64_t hi) -> int {
+            if (!rd(bytes, cnt, t)) return 1;
+            dst.resize((size_t)cnt);
+            for (int64_t i = 0; i < cnt; i++) {
+                const int64_t v = t[(size_t)i];
+                if (v < lo || v > hi) return 2;
+                dst[(size_t)i] = (int32_t)v;
+            }
+            return 0;
+        };
+        int e = narrow(ob, L.m, L.col, 0, L.n - 1);
+        if (e) return bad(e == 1 ? "truncated columns" : "column out of range");
+        e = narrow(eb, L.m, L.ew, 1, 2147483647LL);
+        if (e) return bad(e == 1 ? "truncated edge weights" : "edge weight outside [1, 2^31)");
+        e = narrow(eb, L.n, L.vw, 1, 2147483647LL);
+        if (e) return bad(e == 1 ? "truncated vertex weights" : "vertex weight outside [1, 2^31)");
+        if (l > 0) {
+            e = narrow(ob, prev_n, L.map, 0, L.n - 1);
+            if (e) return bad(e == 1 ? "truncated map" : "map entry out of range");
+        }
+        prev_n = L.n;
+        h->levels.push_back(std::move(L));
+    }
+    std::fclose(f);
+    *out = h.release();
+    return JET_OK;
+}
+
+int32_t jet_hierarchy_levels(const jet_hierarchy_t* h) { return h ? (int32_t)h->levels.size() : 0; }
+
+int64_t jet_hierarchy_vertices(const jet_hierarchy_t* h, int32_t level) {
+    if (!h || level < 0 || level >= (int32_t)h->levels.size()) return -1;
+    return h->levels[(size_t)level].n;
+}
+
+int64_t jet_hierarchy_edges(const jet_hierarchy_t* h, int32_t level) {
+    if (!h || level < 0 || level >= (int32_t)h->levels.size()) return -1;
+    return h->levels[(size_t)level].m;
+}
+
+void jet_free_hierarchy(jet_hierarchy_t* h) { delete h; }
+
+// import_coarse.cpp:112-121 and binary_dump.hpp:89-95: n raw int32 part ids.
+int jet_load_coarse_part(const char* path, int64_t n, int32_t* part) {
+    if (!path || !part || n < 1) return fail(JET_EINVAL, "path, part and n >= 1 are required");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(JET_EINVAL, std::string("could not open coarse partition file ") + path);
+    const size_t got = std::fread(part, 4, (size_t)n, f);
+    std::fclose(f);
+    if (got != (size_t)n) return fail(JET_EINVAL, "coarse partition file has fewer than n entries");
     return JET_OK;
 }
 

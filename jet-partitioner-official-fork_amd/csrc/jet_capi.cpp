@@ -20,6 +20,7 @@
 #include "arena.h"
 #include "coarsen.h"
 #include "engine.h"
+#include "hierarchy.h"
 #include "contract.h"
 #include "jet_capi.h"
 #include "misc.h"
@@ -216,9 +217,186 @@ static int32_t coarsen_level(const Level& cur, int32_t alg, uint64_t seed, int32
     return coarsen_match(g, cur.uniform, seed, level, alg != 2, map, a, s);
 }
 
+// binary_dump.hpp:61-97. Widths of the reference's graph type for the
+// entry point used: ordinal, offset and weight bytes (matrix_t 4/4/4,
+// big_matrix_t 4/8/8, biggest_matrix_t 8/8/8).
+struct DumpWidths {
+    int ord, off, val;
+};
+
+static void put(FILE* f, int64_t v, int bytes) {
+    if (bytes == 4) {
+        int32_t x = (int32_t)v;
+        std::fwrite(&x, 4, 1, f);
+    } else {
+        std::fwrite(&v, 8, 1, f);
+    }
+}
+
+template <class T>
+static void put_array(FILE* f, const std::vector<T>& a, int bytes) {
+    if ((int)sizeof(T) == bytes) {
+        if (!a.empty()) std::fwrite(a.data(), sizeof(T), a.size(), f);
+        return;
+    }
+    const size_t chunk = 1 << 16;
+    std::vector<char> buf(chunk * bytes);
+    for (size_t i = 0; i < a.size(); i += chunk) {
+        const size_t c = std::min(chunk, a.size() - i);
+        for (size_t j = 0; j < c; j++) {
+            const int64_t v = (int64_t)a[i + j];
+            if (bytes == 4) {
+                int32_t x = (int32_t)v;
+                std::memcpy(buf.data() + 4 * j, &x, 4);
+            } else {
+                std::memcpy(buf.data() + 8 * j, &v, 8);
+            }
+        }
+        std::fwrite(buf.data(), bytes, c, f);
+    }
+}
+
+template <class T>
+static std::vector<T> to_host(Context& C, const T* d, int64_t count) {
+    std::vector<T> h((size_t)count);
+    if (count > 0) JET_HIP(hipMemcpyAsync(h.data(), d, sizeof(T) * count, hipMemcpyDeviceToHost, C.stream));
+    JET_HIP(hipStreamSynchronize(C.stream));
+    return h;
+}
+
+// Writes coarse_part.out (the partition of level `last`, int32) and
+// coarse_graphs.out (levels 0..last, finest first, each with the map from the
+// level before it) in the working directory, as dump_coarse does.
+static void dump_hierarchy(Context& C, const std::vector<Level>& levels, size_t last, const part_t* part,
+                           const DumpWidths& w) {
+    {
+        FILE* f = std::fopen("coarse_part.out", "wb");
+        if (!f) throw InvalidInput("dump_coarse: cannot write coarse_part.out");
+        std::vector<part_t> h = to_host(C, part, levels[last].n);
+        put_array(f, h, 4);
+        std::fclose(f);
+    }
+    FILE* f = std::fopen("coarse_graphs.out", "wb");
+    if (!f) throw InvalidInput("dump_coarse: cannot write coarse_graphs.out");
+    int32_t count = (int32_t)(last + 1);
+    std::fwrite(&count, sizeof(int32_t), 1, f);
+    for (size_t i = 0; i <= last; i++) {
+        const Level& L = levels[i];
+        put(f, L.n, w.ord);
+        put(f, L.m, w.off);
+        put_array(f, to_host(C, L.rowptr, (int64_t)L.n + 1), w.off);
+        put_array(f, to_host(C, L.col, L.m), w.ord);
+        put_array(f, to_host(C, L.ew, L.m), w.val);
+        put_array(f, to_host(C, L.vw, L.n), w.val);
+        if (i > 0) {
+            const vid_t prev_n = levels[i - 1].n;
+            std::vector<vid_t> map;
+            if (L.map) {
+                map = to_host(C, L.map, prev_n);
+            } else {
+                map.resize(prev_n);
+                for (vid_t v = 0; v < prev_n; v++) map[v] = v;
+            }
+            put_array(f, map, w.ord);
+        }
+    }
+    const bool ok = std::ferror(f) == 0;
+    std::fclose(f);
+    if (!ok) throw InvalidInput("dump_coarse: write to coarse_graphs.out failed");
+}
+
+// uncoarsener::multilevel_jet and uncoarsen (uncoarsen.hpp:88-155): refine
+// each level from the coarsest (whose partition is `cur`) to the finest and
+// project onto the next finer one; the finest result lands in part_out. With
+// `dump`, the hierarchy is written before refining the first level whose
+// partition is balanced, and the refinement clock restarts there.
+static void uncoarsen_levels(Context& C, const jet_config_t& cfg, const std::vector<Level>& levels, part_t* cur,
+                             part_t* part_out, AttemptResult& R, const DumpWidths* dump, clk::time_point t_start) {
+    hipStream_t s = C.stream;
+    const int32_t k = cfg.num_parts;
+    const Level& fine = levels.front();
+    RefineParams rp;
+    rp.k = k;
+    rp.imb = cfg.max_imb_ratio;
+    rp.tol = cfg.refine_tolerance;
+    rp.ultra = cfg.ultra_settings != 0;
+    {
+        ArenaScope rscope(C.scratch);
+        int64_t cap = conn_capacity(fine.n, fine.rowptr, k, C.scratch, s);
+        Refiner ref(C.scratch, s, fine.n, cap, rp);
+        // A level that shares its graph with the level just refined (the
+        // coarsening fixed point) starts from that level's output partition
+        // and refine_data. When that level changed neither, this level
+        // repeats the same deterministic computation from the same state, so
+        // its record is the same and the partition stays as it is.
+        bool unchanged = false;
+        bool dumped = dump == nullptr;
+        LevelRecord last;
+        for (size_t L = levels.size(); L-- > 0;) {
+            if (!dumped) {
+                // uncoarsen.hpp:96-111: the part sizes of `cur` on this level
+                // (equal to the refined sizes carried from the coarser level)
+                std::vector<int64_t> hs;
+                {
+                    ArenaScope ds(C.scratch);
+                    int64_t* d = C.scratch.alloc<int64_t>(k);
+                    part_sizes(levels[L].view(), cur, k, d, s);
+                    hs = to_host(C, d, k);
+                }
+                int64_t mx = 0, tot = 0;
+                for (int64_t v : hs) {
+                    mx = std::max(mx, v);
+                    tot += v;
+                }
+                const int64_t opt = (tot + k - 1) / k;
+                if ((double)mx / (double)opt <= cfg.max_imb_ratio) {
+                    dump_hierarchy(C, levels, L, cur, *dump);
+                    dumped = true;
+                    t_start = clk::now();
+                }
+            }
+            const bool same_graph = L + 1 < levels.size() && levels[L + 1].map == nullptr;
+            LevelRecord rec;
+            if (same_graph && unchanged) {
+                rec = last;
+                rec.seconds = 0;
+            } else {
+                rec = ref.refine(levels[L], cur);
+                unchanged = !ref.changed_last();
+                last = rec;
+            }
+            if (cfg.verbose && L + 1 == levels.size())
+                std::printf("Initial %lld %.6f %d\n", (long long)(rec.cut2 / 2), 0.0, (int)levels[L].n);
+            R.recs.push_back(rec);
+            if (L > 0) {
+                if (levels[L].map == nullptr && L > 1) continue;  // identity level: same buffer
+                part_t* fp = (L == 1) ? part_out : C.persist.alloc<part_t>(levels[L - 1].n);
+                if (levels[L].map == nullptr)
+                    JET_HIP(hipMemcpyAsync(fp, cur, sizeof(part_t) * levels[L].n, hipMemcpyDeviceToDevice, s));
+                else
+                    project(levels[L - 1].n, levels[L].map, cur, fp, s);
+                cur = fp;
+            }
+        }
+        R.cut = ref.cut2() / 2;
+        R.total = ref.total_size();
+        R.opt = ref.opt();
+        std::vector<int64_t> hs(k);
+        JET_HIP(hipMemcpyAsync(hs.data(), ref.best_sizes_dev(), sizeof(int64_t) * k, hipMemcpyDeviceToHost, s));
+        JET_HIP(hipStreamSynchronize(s));
+        R.largest = 0;
+        R.smallest = R.total;
+        for (int64_t v : hs) {
+            R.largest = std::max(R.largest, v);
+            R.smallest = std::min(R.smallest, v);
+        }
+    }
+    R.t_refine = secs(t_start, clk::now());
+}
+
 // partitioner.hpp:61-125. part_out: device buffer of fine.n entries.
 static AttemptResult partition_once(Context& C, const jet_config_t& cfg, const Level& fine, uint64_t seed,
-                                    part_t* part_out) {
+                                    part_t* part_out, const DumpWidths* dump) {
     hipStream_t s = C.stream;
     AttemptResult R;
     auto t0 = clk::now();
@@ -299,62 +477,8 @@ static AttemptResult partition_once(Context& C, const jet_config_t& cfg, const L
     R.t_init = secs(t1, t2);
 
     // uncoarsening (uncoarsen.hpp:88-125)
-    RefineParams rp;
-    rp.k = k;
-    rp.imb = cfg.max_imb_ratio;
-    rp.tol = cfg.refine_tolerance;
-    rp.ultra = cfg.ultra_settings != 0;
-    {
-        ArenaScope rscope(C.scratch);
-        int64_t cap = conn_capacity(fine.n, fine.rowptr, k, C.scratch, s);
-        Refiner ref(C.scratch, s, fine.n, cap, rp);
-        // A level that shares its graph with the level just refined (the
-        // coarsening fixed point) starts from that level's output partition
-        // and refine_data. When that level changed neither, this level
-        // repeats the same deterministic computation from the same state, so
-        // its record is the same and the partition stays as it is.
-        bool unchanged = false;
-        LevelRecord last;
-        for (size_t L = levels.size(); L-- > 0;) {
-            const bool same_graph = L + 1 < levels.size() && levels[L + 1].map == nullptr;
-            LevelRecord rec;
-            if (same_graph && unchanged) {
-                rec = last;
-                rec.seconds = 0;
-            } else {
-                rec = ref.refine(levels[L], cur);
-                unchanged = !ref.changed_last();
-                last = rec;
-            }
-            if (cfg.verbose && L + 1 == levels.size())
-                std::printf("Initial %lld %.6f %d\n", (long long)(rec.cut2 / 2), 0.0, (int)levels[L].n);
-            R.recs.push_back(rec);
-            if (L > 0) {
-                if (levels[L].map == nullptr && L > 1) continue;  // identity level: same buffer
-                part_t* fp = (L == 1) ? part_out : C.persist.alloc<part_t>(levels[L - 1].n);
-                if (levels[L].map == nullptr)
-                    JET_HIP(hipMemcpyAsync(fp, cur, sizeof(part_t) * levels[L].n, hipMemcpyDeviceToDevice, s));
-                else
-                    project(levels[L - 1].n, levels[L].map, cur, fp, s);
-                cur = fp;
-            }
-        }
-        R.cut = ref.cut2() / 2;
-        R.total = ref.total_size();
-        R.opt = ref.opt();
-        std::vector<int64_t> hs(k);
-        JET_HIP(hipMemcpyAsync(hs.data(), ref.best_sizes_dev(), sizeof(int64_t) * k, hipMemcpyDeviceToHost, s));
-        JET_HIP(hipStreamSynchronize(s));
-        R.largest = 0;
-        R.smallest = R.total;
-        for (int64_t v : hs) {
-            R.largest = std::max(R.largest, v);
-            R.smallest = std::min(R.smallest, v);
-        }
-    }
-    auto t3 = clk::now();
-    R.t_refine = secs(t2, t3);
-    R.t_total = secs(t0, t3);
+    uncoarsen_levels(C, cfg, levels, cur, part_out, R, dump, t2);
+    R.t_total = secs(t0, clk::now());
     C.persist.release(level_mark);
     return R;
 }
@@ -392,7 +516,6 @@ static void check_config(const jet_config_t* cfg) {
     if (!cfg) throw InvalidInput("config is NULL");
     if (cfg->num_parts < 1 || cfg->num_parts > JET_MAX_PARTS) throw InvalidInput("num_parts must be in [1, 8192]");
     if (!(cfg->max_imb_ratio >= 1.0)) throw InvalidInput("max_imb_ratio must be >= 1");
-    if (cfg->dump_coarse) throw InvalidInput("dump_coarse is not supported");
 }
 
 // Attempt loop of app/driver.cpp:110-130: best cut by strict <, seeds seed + i.
@@ -403,6 +526,10 @@ static void partition_entry(const jet_config_t* cfg, int64_t n, const int64_t* r
                             Context* ctx = nullptr) {
     check_config(cfg);
     if (!part_out || !cut_out) throw InvalidInput("part_out and cut_out are required");
+    if (cfg->dump_coarse && ctx) throw InvalidInput("dump_coarse is not supported for jobs of a jet_ctx");
+    // the reference's graph type for this entry point sets the dump's widths
+    const DumpWidths widths{col64 ? 8 : 4, rp32 ? 4 : 8, rp32 ? 4 : 8};
+    const DumpWidths* dump = cfg->dump_coarse ? &widths : nullptr;
     Context& C = ctx ? *ctx : context();
     CallScope call(C);
     Level fine = stage_graph(C, n, rp64, rp32, col, ew32, ew64, vw32, vw64, space, uniform_ew != 0, true, col64);
@@ -412,7 +539,7 @@ static void partition_entry(const jet_config_t* cfg, int64_t n, const int64_t* r
     AttemptResult best_r;
     bool have = false;
     for (int32_t i = 0; i < attempts; i++) {
-        AttemptResult r = partition_once(C, *cfg, fine, seed + (uint64_t)i, trial);
+        AttemptResult r = partition_once(C, *cfg, fine, seed + (uint64_t)i, trial, dump);
         if (!have || r.cut < best_r.cut) {
             if (trial != best) copy_i32(best, trial, n, C.stream);
             best_r = r;
@@ -726,6 +853,42 @@ int jet_part_sizes(int64_t n, const int32_t* vw, const int32_t* part, int32_t k,
         int64_t* ds = ptr_space == JET_PTR_DEVICE ? sizes_out : C.persist.alloc<int64_t>(k);
         part_sizes(g, dp, k, ds, C.stream);
         from_device(C, sizes_out, ds, k, ptr_space);
+    });
+}
+
+int jet_uncoarsen_hierarchy(const jet_config_t* cfg, const jet_hierarchy_t* h, const int32_t* coarse_part,
+                            int32_t* part_out, int64_t* cut_out, jet_stats_t* stats) {
+    return guard_impl([&] {
+        check_config(cfg);
+        if (cfg->dump_coarse) throw InvalidInput("dump_coarse must be 0 when replaying a hierarchy");
+        if (!h || h->levels.empty() || !coarse_part || !part_out || !cut_out)
+            throw InvalidInput("hierarchy, coarse_part, part_out and cut_out are required");
+        Context& C = context();
+        CallScope call(C);
+        auto t0 = clk::now();
+        std::vector<Level> levels;
+        for (size_t l = 0; l < h->levels.size(); l++) {
+            const jet_hierarchy::Level& H = h->levels[l];
+            if (l > 0 && (int64_t)H.map.size() != h->levels[l - 1].n) throw InvalidInput("map length mismatch");
+            Level L = stage_graph(C, H.n, H.rowptr.data(), nullptr, H.col.data(), H.ew.data(), nullptr, H.vw.data(),
+                                  nullptr, JET_PTR_HOST, false);
+            if (l > 0) L.map = to_device(C, H.map.data(), (int64_t)H.map.size(), JET_PTR_HOST);
+            levels.push_back(L);
+        }
+        const Level& cs = levels.back();
+        part_t* cur = C.persist.alloc<part_t>(cs.n);
+        JET_HIP(hipMemcpyAsync(cur, coarse_part, sizeof(int32_t) * cs.n, hipMemcpyHostToDevice, C.stream));
+        check_parts(C, cs.n, cur, cfg->num_parts);
+        const vid_t fine_n = levels.front().n;
+        part_t* fine_part = levels.size() == 1 ? cur : C.persist.alloc<part_t>(fine_n);
+        AttemptResult R;
+        auto t1 = clk::now();
+        R.t_init = secs(t0, t1);
+        uncoarsen_levels(C, *cfg, levels, cur, fine_part, R, nullptr, t1);
+        R.t_total = secs(t0, clk::now());
+        from_device(C, part_out, fine_part, fine_n, JET_PTR_HOST);
+        *cut_out = R.cut;
+        fill_stats(R, 1, cfg->num_parts, stats);
     });
 }
 

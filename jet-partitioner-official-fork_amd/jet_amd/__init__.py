@@ -123,13 +123,23 @@ def lib():
             "jet_write_partition": [c.c_char_p, i64, VP],
             "jet_read_partition": [c.c_char_p, i64, VP],
             "jet_partition_quality": [i64, VP, VP, VP, VP, VP, i32, i32, c.POINTER(Quality)],
+            "jet_load_hierarchy": [c.c_char_p, i32, c.POINTER(VP)],
+            "jet_hierarchy_levels": [VP],
+            "jet_hierarchy_vertices": [VP, i32],
+            "jet_hierarchy_edges": [VP, i32],
+            "jet_free_hierarchy": [VP],
+            "jet_load_coarse_part": [c.c_char_p, i64, VP],
+            "jet_uncoarsen_hierarchy": [c.POINTER(Config), VP, VP, VP, c.POINTER(i64), c.POINTER(Stats)],
             "jet_prof_enable": [c.c_char_p],
             "jet_prof_read": [c.POINTER(i64), c.POINTER(dbl), c.POINTER(dbl)],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
             f.argtypes = args
-            f.restype = None if name in ("jet_config_default", "jet_free_graph") else c.c_int
+            f.restype = None if name in ("jet_config_default", "jet_free_graph", "jet_free_hierarchy") else c.c_int
+        L.jet_hierarchy_levels.restype = i32
+        L.jet_hierarchy_vertices.restype = i64
+        L.jet_hierarchy_edges.restype = i64
         _lib = L
     return _lib
 
@@ -189,26 +199,28 @@ def partition_device(n, rowptr_ptr, col_ptr, ew_ptr, vw_ptr, part_ptr, cfg, seed
 # Reference entry-point names (header/jet.h:46-107). They return
 # (part, edge_cut, stats) for a CSR dict {rowptr, col, ew, vw}.
 def _partition_named(g, config, uniform_ew, seed, big):
+    # converted arrays are bound to names so that they outlive the call
     rp = np.ascontiguousarray(g["rowptr"])
     n = len(rp) - 1
     part = np.empty(n, dtype=np.int32)
     cut = ctypes.c_int64(0)
     st = Stats()
     if big == "biggest":
-        rc = lib().jet_partition_csr_biggest(ctypes.byref(config), n, _p(_c(rp, np.int64)), _p(_c(g["col"], np.int64)),
-                                             _p(_c(g.get("ew"), np.int64)), _p(_c(g.get("vw"), np.int64)),
+        rp64, col, ew, vw = (_c(rp, np.int64), _c(g["col"], np.int64), _c(g.get("ew"), np.int64),
+                             _c(g.get("vw"), np.int64))
+        rc = lib().jet_partition_csr_biggest(ctypes.byref(config), n, _p(rp64), _p(col), _p(ew), _p(vw),
                                              int(uniform_ew), PTR_HOST, seed, _p(part), ctypes.byref(cut),
                                              ctypes.byref(st))
     elif big:
-        ew = _c(g.get("ew"), np.int64)
-        vw = _c(g.get("vw"), np.int64)
-        rc = lib().jet_partition_csr_big(ctypes.byref(config), n, _p(_c(rp, np.int64)), _p(_c(g["col"], np.int32)),
-                                         _p(ew), _p(vw), int(uniform_ew), PTR_HOST, seed, _p(part),
-                                         ctypes.byref(cut), ctypes.byref(st))
+        rp64, col, ew, vw = (_c(rp, np.int64), _c(g["col"], np.int32), _c(g.get("ew"), np.int64),
+                             _c(g.get("vw"), np.int64))
+        rc = lib().jet_partition_csr_big(ctypes.byref(config), n, _p(rp64), _p(col), _p(ew), _p(vw),
+                                         int(uniform_ew), PTR_HOST, seed, _p(part), ctypes.byref(cut),
+                                         ctypes.byref(st))
     else:
-        rp32 = _c(rp, np.int32)
-        rc = lib().jet_partition_csr32(ctypes.byref(config), n, _p(rp32), _p(_c(g["col"], np.int32)),
-                                       _p(_c(g.get("ew"), np.int32)), _p(_c(g.get("vw"), np.int32)), int(uniform_ew),
+        rp32, col, ew, vw = (_c(rp, np.int32), _c(g["col"], np.int32), _c(g.get("ew"), np.int32),
+                             _c(g.get("vw"), np.int32))
+        rc = lib().jet_partition_csr32(ctypes.byref(config), n, _p(rp32), _p(col), _p(ew), _p(vw), int(uniform_ew),
                                        PTR_HOST, seed, _p(part), ctypes.byref(cut), ctypes.byref(st))
     _check(rc)
     return part, cut.value, st
@@ -276,7 +288,8 @@ def contract(g, mp, nc):
     oew = np.empty(max(m, 1), dtype=np.int32)
     ovw = np.empty(max(nc, 1), dtype=np.int32)
     mc = ctypes.c_int64(0)
-    _check(lib().jet_contract(n, _p(rp), _p(col), _p(ew), _p(vw), _p(_c(mp, np.int32)), nc, PTR_HOST, _p(orp),
+    mp = _c(mp, np.int32)
+    _check(lib().jet_contract(n, _p(rp), _p(col), _p(ew), _p(vw), _p(mp), nc, PTR_HOST, _p(orp),
                               _p(ocol), _p(oew), _p(ovw), ctypes.byref(mc)))
     return {"rowptr": orp, "col": ocol[:mc.value].copy(), "ew": oew[:mc.value].copy(), "vw": ovw[:nc].copy()}
 
@@ -290,7 +303,8 @@ def conn_tables(g, part, k):
     sp = np.empty(max(cap, 1), dtype=np.int32)
     sv = np.empty(max(cap, 1), dtype=np.int32)
     sz = np.empty(n, dtype=np.int32)
-    _check(lib().jet_conn_tables(n, _p(rp), _p(col), _p(ew), _p(_c(part, np.int32)), k, PTR_HOST, _p(off), _p(sp),
+    part = _c(part, np.int32)
+    _check(lib().jet_conn_tables(n, _p(rp), _p(col), _p(ew), _p(part), k, PTR_HOST, _p(off), _p(sp),
                                  _p(sv), _p(sz)))
     return off, sp[:cap], sv[:cap], sz
 
@@ -301,7 +315,8 @@ def refine_step(kind, g, part, k, imb, filter_ratio=0.75):
     mv = np.empty(max(n, 1), dtype=np.int32)
     de = np.empty(max(n, 1), dtype=np.int32)
     cnt = ctypes.c_int32(0)
-    _check(lib().jet_refine_step(kind, n, _p(rp), _p(col), _p(ew), _p(vw), _p(_c(part, np.int32)), k, imb,
+    part = _c(part, np.int32)
+    _check(lib().jet_refine_step(kind, n, _p(rp), _p(col), _p(ew), _p(vw), _p(part), k, imb,
                                  filter_ratio, PTR_HOST, _p(mv), _p(de), ctypes.byref(cnt)))
     return mv[:cnt.value].copy(), de[:cnt.value].copy()
 
@@ -319,21 +334,24 @@ def refine_level(g, part, k, imb, tol=0.999, ultra=False, uniform=False):
 def project(mp, coarse):
     mp = _c(mp, np.int32)
     out = np.empty(len(mp), dtype=np.int32)
-    _check(lib().jet_project(len(mp), _p(mp), _p(_c(coarse, np.int32)), PTR_HOST, _p(out)))
+    coarse = _c(coarse, np.int32)
+    _check(lib().jet_project(len(mp), _p(mp), _p(coarse), PTR_HOST, _p(out)))
     return out
 
 
 def edge_cut2(g, part):
     rp, col, ew, _ = _graph(g)
     out = ctypes.c_int64(0)
-    _check(lib().jet_edge_cut(len(rp) - 1, _p(rp), _p(col), _p(ew), _p(_c(part, np.int32)), PTR_HOST,
+    part = _c(part, np.int32)
+    _check(lib().jet_edge_cut(len(rp) - 1, _p(rp), _p(col), _p(ew), _p(part), PTR_HOST,
                               ctypes.byref(out)))
     return out.value
 
 
 def part_sizes(vw, part, k):
     out = np.empty(k, dtype=np.int64)
-    _check(lib().jet_part_sizes(len(part), _p(_c(vw, np.int32)), _p(_c(part, np.int32)), k, PTR_HOST, _p(out)))
+    vw, part = _c(vw, np.int32), _c(part, np.int32)
+    _check(lib().jet_part_sizes(len(part), _p(vw), _p(part), k, PTR_HOST, _p(out)))
     return out
 
 
@@ -512,6 +530,59 @@ def partition_quality(g, part, k):
     """jet_partition_quality on host arrays -> Quality."""
     rp, col, ew, vw = _graph(g)
     q = Quality()
-    _check(lib().jet_partition_quality(len(rp) - 1, _p(rp), _p(col), _p(ew), _p(vw), _p(_c(part, np.int32)), k,
+    part = _c(part, np.int32)
+    _check(lib().jet_partition_quality(len(rp) - 1, _p(rp), _p(col), _p(ew), _p(vw), _p(part), k,
                                        PTR_HOST, ctypes.byref(q)))
     return q
+
+
+# ---------------------------------------------------------------- dump / replay
+DUMP_32, DUMP_BIG, DUMP_BIGGEST = 0, 1, 2
+
+
+class Hierarchy:
+    """A coarse hierarchy file (dump_coarse's coarse_graphs.out) loaded into
+    host memory by jet_load_hierarchy; `kind` gives the widths it was written
+    with (DUMP_32, DUMP_BIG, DUMP_BIGGEST)."""
+
+    def __init__(self, path, kind=DUMP_32):
+        self._h = ctypes.c_void_p()
+        _check(lib().jet_load_hierarchy(os.fsencode(path), kind, ctypes.byref(self._h)))
+
+    @property
+    def levels(self):
+        return lib().jet_hierarchy_levels(self._h)
+
+    def vertices(self, level):
+        return lib().jet_hierarchy_vertices(self._h, level)
+
+    def edges(self, level):
+        return lib().jet_hierarchy_edges(self._h, level)
+
+    def close(self):
+        if self._h:
+            lib().jet_free_hierarchy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def uncoarsen(self, cfg, coarse_part, stats=False):
+        """jet_uncoarsen_hierarchy: refine from the coarsest level's partition
+        to the finest; returns (part, cut) or (part, cut, Stats)."""
+        cp = _c(coarse_part, np.int32)
+        part = np.empty(self.vertices(0), dtype=np.int32)
+        cut = ctypes.c_int64()
+        st = Stats()
+        _check(lib().jet_uncoarsen_hierarchy(ctypes.byref(cfg), self._h, _p(cp), _p(part), ctypes.byref(cut),
+                                             ctypes.byref(st)))
+        return (part, cut.value, st) if stats else (part, cut.value)
+
+
+def load_coarse_part(path, n):
+    out = np.empty(n, dtype=np.int32)
+    _check(lib().jet_load_coarse_part(os.fsencode(path), n, _p(out)))
+    return out

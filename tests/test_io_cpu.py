@@ -103,3 +103,114 @@ def test_cli_usage_and_no_device(tmp_path):
                            capture_output=True, text=True)
         assert r.returncode != 0 and "no HIP device" in r.stderr
         assert "vertices: 64; edges: 112" in r.stdout
+
+
+# ---------------------------------------------------------------- coarse-hierarchy files
+WIDTHS = {jet_amd.DUMP_32: (np.int32, np.int32, np.int32), jet_amd.DUMP_BIG: (np.int32, np.int64, np.int64),
+          jet_amd.DUMP_BIGGEST: (np.int64, np.int64, np.int64)}
+
+
+def write_hierarchy(path, levels, kind):
+    """binary_dump.hpp:61-97 layout: int32 count, then per level n, m, row
+    offsets, columns, edge weights, vertex weights and (after the first) the
+    map from the previous level, at the widths of `kind`."""
+    ot, et, vt = WIDTHS[kind]
+    with open(path, "wb") as f:
+        f.write(np.int32(len(levels)).tobytes())
+        for i, L in enumerate(levels):
+            n = len(L["rowptr"]) - 1
+            f.write(np.array([n], dtype=ot).tobytes())
+            f.write(np.array([len(L["col"])], dtype=et).tobytes())
+            f.write(np.asarray(L["rowptr"], dtype=et).tobytes())
+            f.write(np.asarray(L["col"], dtype=ot).tobytes())
+            f.write(np.asarray(L["ew"], dtype=vt).tobytes())
+            f.write(np.asarray(L["vw"], dtype=vt).tobytes())
+            if i > 0:
+                f.write(np.asarray(L["map"], dtype=ot).tobytes())
+
+
+def two_levels():
+    """An 8x8 grid and its contraction by 2x2 blocks into a 4x4 grid."""
+    fine = graphs.grid2d(8, 8)
+    fine["vw"] = np.ones(64, dtype=np.int32)
+    mp = np.array([(y // 2) * 4 + x // 2 for y in range(8) for x in range(8)], dtype=np.int32)
+    coarse = graphs.grid2d(4, 4)
+    coarse["ew"] = np.full(len(coarse["col"]), 2, dtype=np.int32)
+    coarse["vw"] = np.full(16, 4, dtype=np.int32)
+    coarse["map"] = mp
+    return [fine, coarse]
+
+
+@pytest.mark.parametrize("kind", sorted(WIDTHS))
+def test_hierarchy_file_loads_at_each_width(tmp_path, kind):
+    lv = two_levels()
+    write_hierarchy(tmp_path / "h.out", lv, kind)
+    h = jet_amd.Hierarchy(str(tmp_path / "h.out"), kind)
+    assert h.levels == 2
+    assert (h.vertices(0), h.edges(0)) == (64, len(lv[0]["col"]))
+    assert (h.vertices(1), h.edges(1)) == (16, len(lv[1]["col"]))
+    assert h.vertices(2) == -1
+    h.close()
+
+
+@pytest.mark.parametrize("case", ["truncated", "map_range", "col_range", "weight", "offsets", "count"])
+def test_hierarchy_file_errors(tmp_path, case):
+    lv = two_levels()
+    kind = jet_amd.DUMP_32
+    if case == "map_range":
+        lv[1]["map"] = lv[1]["map"].copy()
+        lv[1]["map"][5] = 16
+    elif case == "col_range":
+        lv[0]["col"] = lv[0]["col"].copy()
+        lv[0]["col"][3] = 64
+    elif case == "weight":
+        lv[1]["vw"] = lv[1]["vw"].copy()
+        lv[1]["vw"][0] = 0
+    elif case == "offsets":
+        lv[0]["rowptr"] = lv[0]["rowptr"].copy()
+        lv[0]["rowptr"][10], lv[0]["rowptr"][11] = lv[0]["rowptr"][11], lv[0]["rowptr"][10]
+    write_hierarchy(tmp_path / "h.out", lv, kind)
+    if case == "truncated":
+        data = (tmp_path / "h.out").read_bytes()
+        (tmp_path / "h.out").write_bytes(data[:-7])
+    if case == "count":
+        (tmp_path / "h.out").write_bytes(np.int32(0).tobytes())
+    with pytest.raises(jet_amd.JetError) as e:
+        jet_amd.Hierarchy(str(tmp_path / "h.out"), kind)
+    assert e.value.code == jet_amd.JET_EINVAL
+    with pytest.raises(jet_amd.JetError):
+        jet_amd.Hierarchy(str(tmp_path / "missing.out"), kind)
+    with pytest.raises(jet_amd.JetError):
+        jet_amd.Hierarchy(str(tmp_path / "h.out"), 7)
+
+
+def test_coarse_part_file(tmp_path):
+    p = np.arange(10, dtype=np.int32) % 3
+    (tmp_path / "coarse_part.out").write_bytes(p.tobytes())
+    assert np.array_equal(jet_amd.load_coarse_part(str(tmp_path / "coarse_part.out"), 10), p)
+    with pytest.raises(jet_amd.JetError):
+        jet_amd.load_coarse_part(str(tmp_path / "coarse_part.out"), 11)
+
+
+def test_replay_without_device_is_a_clean_error(tmp_path):
+    if jet_amd.device_count() > 0:
+        pytest.skip("a GPU is present")
+    write_hierarchy(tmp_path / "h.out", two_levels(), jet_amd.DUMP_32)
+    h = jet_amd.Hierarchy(str(tmp_path / "h.out"))
+    with pytest.raises(jet_amd.JetError) as e:
+        h.uncoarsen(jet_amd.Config(num_parts=2), np.zeros(16, dtype=np.int32))
+    assert e.value.code == jet_amd.JET_EDEVICE
+    cfg = jet_amd.Config(num_parts=2)
+    cfg.dump_coarse = 1
+    with pytest.raises(jet_amd.JetError) as e:
+        h.uncoarsen(cfg, np.zeros(16, dtype=np.int32))
+    assert e.value.code == jet_amd.JET_EINVAL
+
+
+def test_import_cli_usage_and_missing_files(tmp_path):
+    r = subprocess.run([os.path.join(PKG, "jet_import")], capture_output=True, text=True)
+    assert r.returncode != 0 and "Usage:" in r.stderr
+    (tmp_path / "c.txt").write_text("1 4 1 1.03\n")
+    r = subprocess.run([os.path.join(PKG, "jet_import"), str(tmp_path / "c.txt")], capture_output=True, text=True,
+                       cwd=tmp_path)
+    assert r.returncode != 0 and "coarse_graphs.out" in r.stderr
