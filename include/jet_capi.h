@@ -284,6 +284,13 @@ typedef struct jet_quality_t {
 int jet_partition_quality(int64_t n, const int64_t* row_offsets, const int32_t* col, const int32_t* ew,
                           const int32_t* vw, const int32_t* part, int32_t k, int32_t ptr_space, jet_quality_t* out);
 
+/* src/part_stat.hpp:116-129 (cut_heatmap), on the GPU: heat_out[p * k + q] =
+ * total weight of the edges from vertices of part p to vertices of part q,
+ * q != p (each undirected cut edge counted once in each direction; the
+ * diagonal is 0). k * k entries, in ptr_space. */
+int jet_cut_heatmap(int64_t n, const int64_t* row_offsets, const int32_t* col, const int32_t* ew, const int32_t* part,
+                    int32_t k, int32_t ptr_space, int64_t* heat_out);
+
 /* ---- Batched and served partitioning (one GPU, several jobs in flight).
  *
  * A context owns `workers` GPU workers, each with its own HIP stream and

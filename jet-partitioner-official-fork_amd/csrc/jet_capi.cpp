@@ -931,6 +931,27 @@ int jet_partition_quality(int64_t n, const int64_t* row_offsets, const int32_t* 
     });
 }
 
+int jet_cut_heatmap(int64_t n, const int64_t* row_offsets, const int32_t* col, const int32_t* ew, const int32_t* part,
+                    int32_t k, int32_t ptr_space, int64_t* heat_out) {
+    return guard_impl([&] {
+        if (!heat_out) throw InvalidInput("heat_out is NULL");
+        check_k(k);
+        Context& C = context();
+        CallScope call(C);
+        Level L = stage_graph(C, n, row_offsets, nullptr, col, ew, nullptr, nullptr, nullptr, ptr_space, false);
+        const part_t* dp = to_device(C, part, n, ptr_space);
+        check_parts(C, n, dp, k);
+        const int64_t kk = (int64_t)k * k;
+        int64_t* dh = ptr_space == JET_PTR_DEVICE ? heat_out : C.persist.alloc<int64_t>(kk);
+        RefineParams rp;
+        rp.k = k;
+        int64_t cap = conn_capacity(L.n, L.rowptr, k, C.scratch, C.stream);
+        Refiner ref(C.scratch, C.stream, L.n, cap, rp);
+        ref.heatmap(L, dp, dh);
+        from_device(C, heat_out, dh, kk, ptr_space);
+    });
+}
+
 int jet_gain_bucket_batch(int64_t count, const int64_t* gains, const int64_t* vwgts, int32_t* out) {
     return guard_impl([&] {
         if (count < 0) throw InvalidInput("count < 0");

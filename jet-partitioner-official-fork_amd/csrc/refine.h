@@ -46,6 +46,8 @@ class Refiner {
 
     // Communication volume and cut weight per part of `part` (part_stat.hpp).
     void quality(const Level& L, const part_t* part, int64_t& comm_size, std::vector<int64_t>& cut_per_part);
+    // cut weight between every ordered pair of parts, k * k entries (device)
+    void heatmap(const Level& L, const part_t* part, int64_t* heat);
 
     // Connectivity tables for `part` (init_conn_data), copied to the host.
     void conn_tables(const Level& L, const part_t* part, std::vector<int64_t>& off, std::vector<int32_t>& slot_part,
@@ -133,7 +135,7 @@ class Refiner {
     int32_t* misc_;  // [0] undersized count, [1] error bits, [2] ranking deferred
     vid_t* redraw_;
     int64_t* buckets_;
-    int32_t* bflag_;
+    int64_t* pub_ = nullptr;      // per-group tickets and cut-change partials of the publishing kernel
     int64_t* bcopies_ = nullptr;  // ncopies_ - 1 more bucket arrays, kept zero between uses
     int32_t ncopies_ = 1;
     int64_t nbuckets_ = 0;

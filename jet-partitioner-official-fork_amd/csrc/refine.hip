@@ -572,7 +572,7 @@ __global__ void k_moves_apply(DGraph g, const vid_t* moves, int32_t* mc, vid_t n
         }
         part[v] = b;
         dest[v] = p;
-        moved[v] = 1;
+        if (moved) moved[v] = 1;
     }
     cc = block_reduce_sum<int64_t, 256>(cc, red);
     if (threadIdx.x == 0 && cc) atomicAdd((unsigned long long*)&acc[0], (unsigned long long)cc);
@@ -686,28 +686,88 @@ __device__ inline void update_edge(vid_t v, int32_t w, part_t q, const eoff_t* c
 // choice follows the reference exactly.
 __device__ inline bool update_by_rebuild(const int32_t* mc, vid_t n) { return mc[2] + mc[3] > n / 10; }
 
+// cut change part 2 and the largest part (jet_refiner.hpp:937-948), done by
+// the subtraction pass, the last kernel whose work the results need. For a
+// moved vertex v the reference adds conn(v, new part) - conn(v, old part) from
+// the updated tables; that is the sum over v's edges (v, u) of
+// w * ([part(u) == new part of v] - [part(u) == old part of v]) with the
+// parts after the moves, which needs no table. Block 0 takes the largest
+// part; the last workgroup to finish writes (cut change, largest part, not
+// deferred) to the host mailbox. A deferred step reports only the deferral.
+// The subtraction pass adds each workgroup's part-2 sum into one of
+// PUB_GROUPS partials (workgroup b into partial b mod PUB_GROUPS), 128 bytes
+// apart, so that no line takes more than a sixteenth of the grid's atomics.
+constexpr int PUB_GROUPS = 16;
+constexpr int PUB_STRIDE = 16;  // int64 words: 128 bytes
+
+__device__ inline void add_cut_part2(int64_t* partials, int64_t c2, int64_t* red) {
+    c2 = block_reduce_sum<int64_t, 256>(c2, red);
+    if (threadIdx.x == 0 && c2)
+        atomicAdd((unsigned long long*)&partials[(int64_t)(blockIdx.x % PUB_GROUPS) * PUB_STRIDE],
+                  (unsigned long long)c2);
+}
+
+// One workgroup, after the step's kernels: folds the partials, takes the
+// largest part and writes (cut change, largest part, deferred) to the host
+// mailbox, then clears the accumulators. A single workgroup needs no
+// completion ticket, and the kernel boundary before it orders the other
+// kernels' atomics.
+__global__ void __launch_bounds__(256)
+    k_publish(int64_t* acc, int64_t* partials, const int64_t* sizes, int32_t k, int64_t* host_out,
+              const int32_t* deferred) {
+    __shared__ int64_t red[256 / WAVE];
+    if (*deferred) {
+        if (threadIdx.x == 0) {
+            host_out[2] = 1;
+            __threadfence_system();
+        }
+        return;
+    }
+    int64_t mx = INT64_MIN;
+    for (int t = threadIdx.x; t < k; t += blockDim.x) mx = sizes[t] > mx ? sizes[t] : mx;
+    mx = block_reduce_max<int64_t, 256>(mx, red);
+    int64_t c2 = threadIdx.x < PUB_GROUPS ? partials[(int64_t)threadIdx.x * PUB_STRIDE] : 0;
+    c2 = block_reduce_sum<int64_t, 256>(c2, red);
+    if (threadIdx.x < PUB_GROUPS) partials[(int64_t)threadIdx.x * PUB_STRIDE] = 0;
+    if (threadIdx.x == 0) {
+        host_out[0] = acc[0] + c2;
+        host_out[1] = mx;
+        host_out[2] = 0;
+        __threadfence_system();
+        acc[0] = 0;
+    }
+}
+
 // ADD == false: subtract from the old part (held in dest), or, when the tables
 // are rebuilt instead, mark the neighbours; ADD == true: add to the new part
 // (held in part). G lanes stride the edges of one move.
 template <bool ADD, int G>
-__global__ void k_update_small(DGraph g, const vid_t* moves, int32_t* mc, vid_t nmax, const part_t* part,
-                               const part_t* dest, const eoff_t* conn_off, int2* conn, int32_t* conn_size,
-                               part_t* dest_cache, int32_t k, int32_t* err, int32_t* mark, const uint8_t* hub,
-                               uint8_t* hub_dirty, const int32_t* deferred) {
+__global__ void __launch_bounds__(256)
+    k_update_small(DGraph g, const vid_t* moves, int32_t* mc, vid_t nmax, const part_t* part, const part_t* dest,
+                   const eoff_t* conn_off, int2* conn, int32_t* conn_size, part_t* dest_cache, int32_t k,
+                   int32_t* err, int32_t* mark, const uint8_t* hub, uint8_t* hub_dirty, const int32_t* deferred,
+                   int64_t* partials) {
+    __shared__ int64_t red[256 / WAVE];
     if (*deferred) return;
     const bool rebuild = update_by_rebuild(mc, g.n);
     if (ADD && rebuild) return;
     if (!ADD && rebuild && blockIdx.x == 0 && threadIdx.x == 0) mc[6] = 1;  // gates the rebuild
     const int32_t ns = mc[2];
+    int64_t c2 = 0;
     FOR_GROUP(x, ns) {
         vid_t i = moves[x];
-        if (!ADD && rebuild) {
-            for (eoff_t j = g.rowptr[i] + sub_; j < g.rowptr[i + 1]; j += G) mark[g.col[j]] = 1;
-            continue;
-        }
-        part_t q = ADD ? part[i] : dest[i];
+        const part_t pnew = part[i], pold = dest[i];
+        const part_t q = ADD ? pnew : pold;
         for (eoff_t j = g.rowptr[i] + sub_; j < g.rowptr[i + 1]; j += G) {
             vid_t v = g.col[j];
+            if (!ADD) {
+                const part_t pv = part[v];
+                c2 += (int64_t)g.ew[j] * ((pv == pnew ? 1 : 0) - (pv == pold ? 1 : 0));
+                if (rebuild) {
+                    mark[v] = 1;
+                    continue;
+                }
+            }
             if (hub[v]) {  // updated by k_update_hubs
                 if (!ADD) hub_dirty[v] = 1;
                 continue;
@@ -715,28 +775,37 @@ __global__ void k_update_small(DGraph g, const vid_t* moves, int32_t* mc, vid_t 
             update_edge<ADD>(v, g.ew[j], q, conn_off, conn, conn_size, dest_cache, k, err);
         }
     }
+    if (!ADD) add_cut_part2(partials, c2, red);
 }
 
 template <bool ADD>
-__global__ void k_update_big(DGraph g, const vid_t* moves, const int32_t* mc, vid_t nmax, const part_t* part,
-                             const part_t* dest, const eoff_t* conn_off, int2* conn, int32_t* conn_size,
-                             part_t* dest_cache, int32_t k, int32_t* err, int32_t* mark, const uint8_t* hub,
-                             uint8_t* hub_dirty, const int32_t* deferred) {
+__global__ void __launch_bounds__(256)
+    k_update_big(DGraph g, const vid_t* moves, const int32_t* mc, vid_t nmax, const part_t* part, const part_t* dest,
+                 const eoff_t* conn_off, int2* conn, int32_t* conn_size, part_t* dest_cache, int32_t k, int32_t* err,
+                 int32_t* mark, const uint8_t* hub, uint8_t* hub_dirty, const int32_t* deferred,
+                 int64_t* partials) {
+    __shared__ int64_t red[256 / WAVE];
     if (*deferred) return;
     const bool rebuild = update_by_rebuild(mc, g.n);
     if (ADD && rebuild) return;
     const int32_t nb = mc[3];
     const int lane = threadIdx.x & (WAVE - 1);
     const int64_t wstride = ((int64_t)gridDim.x * blockDim.x) / WAVE;
+    int64_t c2 = 0;
     for (int64_t x = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE; x < nb; x += wstride) {
         vid_t i = moves[nmax - 1 - x];
-        if (!ADD && rebuild) {
-            for (eoff_t j = g.rowptr[i] + lane; j < g.rowptr[i + 1]; j += WAVE) mark[g.col[j]] = 1;
-            continue;
-        }
-        part_t q = ADD ? part[i] : dest[i];
+        const part_t pnew = part[i], pold = dest[i];
+        const part_t q = ADD ? pnew : pold;
         for (eoff_t j = g.rowptr[i] + lane; j < g.rowptr[i + 1]; j += WAVE) {
             vid_t v = g.col[j];
+            if (!ADD) {
+                const part_t pv = part[v];
+                c2 += (int64_t)g.ew[j] * ((pv == pnew ? 1 : 0) - (pv == pold ? 1 : 0));
+                if (rebuild) {
+                    mark[v] = 1;
+                    continue;
+                }
+            }
             if (hub[v]) {  // updated by k_update_hubs
                 if (!ADD) hub_dirty[v] = 1;
                 continue;
@@ -744,6 +813,7 @@ __global__ void k_update_big(DGraph g, const vid_t* moves, const int32_t* mc, vi
             update_edge<ADD>(v, g.ew[j], q, conn_off, conn, conn_size, dest_cache, k, err);
         }
     }
+    if (!ADD) add_cut_part2(partials, c2, red);
 }
 
 // update_small for rows of degree > BIG_DEG ("hubs": the workgroup-built tier
@@ -789,54 +859,13 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
-// cut change part 2 and the largest part (jet_refiner.hpp:937-948).
-__global__ void k_moves_finish(const vid_t* moves, const int32_t* mc, vid_t nmax, const part_t* part,
-                               const part_t* dest, const eoff_t* conn_off, const int2* conn, const int32_t* conn_size,
-                               const int64_t* sizes, int32_t k, int64_t* acc, int64_t* host_out, uint8_t* moved,
-                               const int32_t* deferred) {
-    __shared__ int64_t red[256 / WAVE];
-    if (*deferred) {  // nothing was applied: report the deferral
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            host_out[2] = 1;
-            __threadfence_system();
-        }
-        return;
-    }
+// Clears the moved flags of this step's moves (only levels with hub rows,
+// whose pulls read them, set them).
+__global__ void k_clear_moved(const vid_t* moves, const int32_t* mc, vid_t nmax, uint8_t* moved,
+                              const int32_t* deferred) {
+    if (*deferred) return;
     const int32_t ns = mc[2], total = mc[2] + mc[3];
-    int64_t cc = 0;
-    FOR_THREAD(x, total) {
-        vid_t v = list_at(moves, ns, nmax, x);
-        moved[v] = 0;
-        const int2* tab = conn + conn_off[v];
-        int32_t sz = conn_size[v];
-        cc += (int64_t)lookup(tab, sz, part[v]) - lookup(tab, sz, dest[v]);
-    }
-    cc = block_reduce_sum<int64_t, 256>(cc, red);
-    if (threadIdx.x == 0 && cc) atomicAdd((unsigned long long*)&acc[1], (unsigned long long)cc);
-    if (blockIdx.x == 0) {
-        int64_t mx = INT64_MIN;
-        for (int t = threadIdx.x; t < k; t += blockDim.x) mx = sizes[t] > mx ? sizes[t] : mx;
-        mx = block_reduce_max<int64_t, 256>(mx, red);
-        if (threadIdx.x == 0) acc[2] = mx;
-    }
-    // the last workgroup to finish publishes (cut change, largest part)
-    if (threadIdx.x == 0) {
-        __threadfence();
-        unsigned int ticket = atomicAdd(reinterpret_cast<unsigned int*>(&acc[4]), 1u);
-        if (ticket == gridDim.x - 1) {
-            __threadfence();
-            int64_t c1 = __hip_atomic_load(&acc[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            int64_t c2 = __hip_atomic_load(&acc[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            int64_t m2 = __hip_atomic_load(&acc[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            host_out[0] = c1 + c2;
-            host_out[1] = m2;
-            host_out[2] = 0;
-            __threadfence_system();
-            acc[0] = 0;
-            acc[1] = 0;
-            acc[4] = 0;
-        }
-    }
+    FOR_THREAD(x, total) moved[list_at(moves, ns, nmax, x)] = 0;
 }
 
 // ---------------------------------------------------------------- rebalancing
@@ -853,13 +882,10 @@ __global__ void k_moves_finish(const vid_t* moves, const int32_t* mc, vid_t nmax
 // the largest receiver capacity (strong, :440-448) and clears the eviction
 // adjustments.
 __global__ void __launch_bounds__(256)
-    k_reset_buckets(int64_t* buckets, int32_t* bflag, int64_t nb, int32_t* next_counters, const int64_t* sizes,
+    k_reset_buckets(int64_t* buckets, int64_t nb, int32_t* next_counters, const int64_t* sizes,
                     int32_t k, int64_t max_dest, int strong, part_t* undersized, int32_t* misc, int64_t* acc,
                     int64_t* evict_adjust) {
-    FOR_THREAD(i, nb) {
-        buckets[i] = 0;
-        bflag[i] = 0;
-    }
+    FOR_THREAD(i, nb) buckets[i] = 0;
     if (blockIdx.x != 0) return;
     if (threadIdx.x < 8) next_counters[threadIdx.x] = 0;
     __shared__ int32_t ws[256 / WAVE + 1];
@@ -1090,19 +1116,24 @@ __global__ void __launch_bounds__(256)
 }
 
 // off = exclusive scan of the bucket sums. WEAK: evicts whole buckets below the
-// limit and flags straddling ones; STRONG: flags every bucket below the limit.
+// limit and sends the members of straddling ones to the ranking list; STRONG:
+// sends the members of every bucket below the limit there. All members of a
+// bucket share its part and offsets, so each decides for itself.
 template <bool STRONG>
 __global__ void __launch_bounds__(LS_WAVES* WAVE)
     k_classify(DGraph g, const part_t* part, const int32_t* bid, const int64_t* off, const int64_t* sizes,
-               int64_t size_max, int64_t width, int32_t* bflag, vid_t* moves, int32_t* mcounters, vid_t nmax) {
+               int64_t size_max, int64_t width, vid_t* moves, int32_t* counters, vid_t nmax, vid_t* rlist) {
     __shared__ vid_t sbuf[LS_WAVES * LS_CHW];
+    __shared__ vid_t rbuf[LS_WAVES * LS_CHW];
     __shared__ int32_t sh[2 * LS_WAVES + 2];
+    __shared__ int32_t rsh[2 * LS_WAVES + 2];
     StagedLists<LS_WAVES, LS_CHW> st(sbuf);
+    StagedLists<LS_WAVES, LS_CHW> rt(rbuf);
     const int lane = threadIdx.x & (WAVE - 1);
     const int64_t wbase = ((int64_t)blockIdx.x * LS_WAVES + threadIdx.x / WAVE) * LS_CHW;
     for (int32_t o = 0; o < LS_CHW; o += WAVE) {
         int64_t i = wbase + o + lane;
-        bool ev = false;
+        bool ev = false, rank = false;
         eoff_t deg = 0;
         if (i < g.n) {
             int32_t b = bid[i];
@@ -1112,38 +1143,20 @@ __global__ void __launch_bounds__(LS_WAVES* WAVE)
                 int64_t limit = sizes[p] - size_max;
                 int64_t bs = off[b + 1] - off[b];
                 if (STRONG) {
-                    if (base < limit) bflag[b] = 1;
+                    rank = base < limit;
                 } else if (base + bs <= limit) {
                     ev = true;
                     deg = g.rowptr[i + 1] - g.rowptr[i];
-                } else if (base < limit) {
-                    bflag[b] = 1;
+                } else {
+                    rank = base < limit;
                 }
             }
         }
         st.push(ev && deg <= BIG_DEG, ev && deg > BIG_DEG, (vid_t)i);
+        rt.push(rank, false, (vid_t)i);
     }
-    st.flush(moves, &mcounters[2], &mcounters[3], nmax, sh);
-}
-
-// Members of flagged buckets -> rlist (counter in counters[5]).
-__global__ void __launch_bounds__(LS_WAVES* WAVE)
-    k_collect(vid_t n, const int32_t* bid, const int32_t* bflag, vid_t* rlist, int32_t* counters) {
-    __shared__ vid_t sbuf[LS_WAVES * LS_CHW];
-    __shared__ int32_t sh[2 * LS_WAVES + 2];
-    StagedLists<LS_WAVES, LS_CHW> st(sbuf);
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int64_t wbase = ((int64_t)blockIdx.x * LS_WAVES + threadIdx.x / WAVE) * LS_CHW;
-    for (int32_t o = 0; o < LS_CHW; o += WAVE) {
-        int64_t i = wbase + o + lane;
-        bool take = false;
-        if (i < n) {
-            int32_t b = bid[i];
-            take = b >= 0 && bflag[b] != 0;
-        }
-        st.push(take, false, (vid_t)i);
-    }
-    st.flush(rlist, &counters[5], &counters[7], n, sh);
+    st.flush(moves, &counters[2], &counters[3], nmax, sh);
+    rt.flush(rlist, &counters[5], &counters[7], g.n, rsh);
 }
 
 // Decision for one ranked vertex: pos = weight of same-bucket members with a
@@ -1492,12 +1505,13 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
         1, std::min<int64_t>(BUCKET_COPIES_MAX, BUCKET_COPIES_BYTES / ((int64_t)sizeof(int64_t) * nbuckets_)));
     bcopies_ = a.alloc<int64_t>(std::max<int64_t>(1, nbuckets_ * (ncopies_ - 1)));
     if (ncopies_ > 1) fill_i64(bcopies_, 0, nbuckets_ * (ncopies_ - 1), s);
-    bflag_ = a.alloc<int32_t>(nbuckets_);
     fill_i32(swap_bit_, 0, N, s);
     JET_HIP(hipMemsetAsync(hub_dirty_, 0, (size_t)N, s));
     JET_HIP(hipMemsetAsync(moved_, 0, (size_t)N, s));
     fill_i32(counters_, 0, 32, s);
     fill_i64(acc_, 0, 8, s);
+    pub_ = a.alloc<int64_t>((int64_t)PUB_GROUPS * PUB_STRIDE);
+    fill_i64(pub_, 0, (int64_t)PUB_GROUPS * PUB_STRIDE, s);
     // (cut change, largest part) of each iteration land in mailbox slots 8, 9
     Mailbox& mb = thread_mailbox();
     res_host_ = mb.host + 8;
@@ -1671,7 +1685,7 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
     const int32_t sections = r.sections;
     const int64_t width = r.width, nbk = r.nbk, max_dest = r.max_dest;
     const int64_t smax = size_max_;
-    k_reset_buckets<<<grid_for(nbk + 1, 256, 2048), 256, 0, s_>>>(buckets_, bflag_, nbk + 1, cnt_next(), sizes, k,
+    k_reset_buckets<<<grid_for(nbk + 1, 256, 2048), 256, 0, s_>>>(buckets_, nbk + 1, cnt_next(), sizes, k,
                                                                     max_dest, strong ? 1 : 0, undersized_, misc_, acc_,
                                                                     evict_adjust_);
     // contiguous ranges of at least 512 vertices per workgroup: vertices of
@@ -1680,37 +1694,39 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
     int nsel = grid_for(g.n, 512, 4096);
     unsigned long long* bk = reinterpret_cast<unsigned long long*>(buckets_);
     unsigned long long* bc = reinterpret_cast<unsigned long long*>(bcopies_);
+    // about 256 workgroups per copy: fewer workgroups meet on a line without
+    // copies, and a copy costs a pass of the fold
+    const int32_t copies = std::max(1, std::min(ncopies_, nsel / 256));
     const bool lds = k <= SEL_LDS_K;
     if (strong) {
         if (lds)
             k_strong_select<true><<<nsel, 256, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, acc_, conn_off_, conn_,
-                                                        conn_size_, sections, bid_, bk, bc, ncopies_, nbuckets_);
+                                                        conn_size_, sections, bid_, bk, bc, copies, nbuckets_);
         else
             k_strong_select<false><<<nsel, 256, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, acc_, conn_off_,
-                                                         conn_, conn_size_, sections, bid_, bk, bc, ncopies_, nbuckets_);
+                                                         conn_, conn_size_, sections, bid_, bk, bc, copies, nbuckets_);
     } else {
         if (lds)
             k_weak_select<true><<<nsel, 256, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, undersized_, misc_,
                                                       conn_off_, conn_, conn_size_, sections, dest_, bid_, bk, bc,
-                                                      ncopies_, nbuckets_);
+                                                      copies, nbuckets_);
         else
             k_weak_select<false><<<nsel, 256, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, undersized_, misc_,
                                                        conn_off_, conn_, conn_size_, sections, dest_, bid_, bk, bc,
-                                                       ncopies_, nbuckets_);
+                                                       copies, nbuckets_);
     }
-    if (nsel > 1 && ncopies_ > 1)
-        k_sum_bucket_copies<<<grid_for(nbk + 1, 256, 2048), 256, 0, s_>>>(buckets_, bcopies_, ncopies_, nbk + 1,
+    if (copies > 1)
+        k_sum_bucket_copies<<<grid_for(nbk + 1, 256, 2048), 256, 0, s_>>>(buckets_, bcopies_, copies, nbk + 1,
                                                                           nbuckets_);
     JET_LAUNCH_CHECK();
     excl_scan_i64_i64(buckets_, buckets_, nbk, a_, s_);
     int cb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_WAVES * LS_CHW));
     if (strong)
-        k_classify<true><<<cb, LS_WAVES * WAVE, 0, s_>>>(g, part, bid_, buckets_, sizes, smax, width, bflag_, moves_, c,
-                                                         nmax_);
+        k_classify<true><<<cb, LS_WAVES * WAVE, 0, s_>>>(g, part, bid_, buckets_, sizes, smax, width, moves_, c, nmax_,
+                                                         cand_);
     else
-        k_classify<false><<<cb, LS_WAVES * WAVE, 0, s_>>>(g, part, bid_, buckets_, sizes, smax, width, bflag_, moves_,
-                                                          c, nmax_);
-    k_collect<<<cb, LS_WAVES * WAVE, 0, s_>>>(g.n, bid_, bflag_, cand_, c);
+        k_classify<false><<<cb, LS_WAVES * WAVE, 0, s_>>>(g, part, bid_, buckets_, sizes, smax, width, moves_, c,
+                                                          nmax_, cand_);
     JET_LAUNCH_CHECK();
     rank_flagged(L, part, sizes, width, strong);
     if (strong) strong_tail(L, part, sizes, r);
@@ -1720,13 +1736,14 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
     DGraph g = L.view();
     const int32_t k = p_.k;
     int32_t* c = cnt();
+    // the moved flags feed only the hub pulls
+    const bool hubs = ntier_big_ > 0 || npull_mid_ > 0;
     // a small grid-stride grid: the moves are usually few, and every extra
-    // workgroup costs a flush of its part-size deltas (k_moves_apply) and a
-    // serialised completion ticket (k_moves_finish)
+    // workgroup costs a flush of its part-size deltas
     int nb = grid_for(g.n, 256, 128);
     k_moves_apply<<<nb, 256, 0, s_>>>(g, moves_, c, nmax_, g.vw, part, dest_, dest_cache_, conn_off_, conn_,
-                                      conn_size_, reinterpret_cast<unsigned long long*>(sizes), k, acc_, moved_,
-                                      misc_ + 2);
+                                      conn_size_, reinterpret_cast<unsigned long long*>(sizes), k, acc_,
+                                      hubs ? moved_ : nullptr, misc_ + 2);
     JET_LAUNCH_CHECK();
     int G = group_for_avg(g);
     int nbs = grid_for((int64_t)g.n * G, 256, 1024);
@@ -1735,10 +1752,10 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
 #define JET_UPD(GG)                                                                                                   \
     if (add) k_update_small<true, GG><<<nbs, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,       \
                                                           conn_size_, dest_cache_, k, misc_ + 1, swap_bit_,      \
-                                                          hub_, hub_dirty_, misc_ + 2);                         \
+                                                          hub_, hub_dirty_, misc_ + 2, pub_);                   \
     else k_update_small<false, GG><<<nbs, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,          \
                                                         conn_size_, dest_cache_, k, misc_ + 1, swap_bit_,        \
-                                                        hub_, hub_dirty_, misc_ + 2);
+                                                        hub_, hub_dirty_, misc_ + 2, pub_);
         switch (G) {
             case 4: JET_UPD(4) break;
             case 8: JET_UPD(8) break;
@@ -1750,10 +1767,10 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
         if (has_big_) {
             if (add) k_update_big<true><<<nbb, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,
                                                              conn_size_, dest_cache_, k, misc_ + 1, swap_bit_, hub_,
-                                                             hub_dirty_, misc_ + 2);
+                                                             hub_dirty_, misc_ + 2, pub_);
             else k_update_big<false><<<nbb, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,
                                                            conn_size_, dest_cache_, k, misc_ + 1, swap_bit_, hub_,
-                                                           hub_dirty_, misc_ + 2);
+                                                           hub_dirty_, misc_ + 2, pub_);
         }
         // pulled rows: big-tier rows by a 256-thread workgroup, rows of
         // degree (BIG_DEG, HEAVY_DEG_REFINE] by a single-wave workgroup
@@ -1778,9 +1795,12 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
         }
         JET_LAUNCH_CHECK();
     }
+    if (hubs) {
+        k_clear_moved<<<nb, 256, 0, s_>>>(moves_, c, nmax_, moved_, misc_ + 2);
+        JET_LAUNCH_CHECK();
+    }
     init_conn(L, part, swap_bit_);  // rebuild marked rows (gated on the device)
-    k_moves_finish<<<nb, 256, 0, s_>>>(moves_, c, nmax_, part, dest_, conn_off_, conn_, conn_size_, sizes, k, acc_,
-                                       res_dev_, moved_, misc_ + 2);
+    k_publish<<<1, 256, 0, s_>>>(acc_, pub_, sizes, k, res_dev_, misc_ + 2);
     JET_LAUNCH_CHECK();
     JET_HIP(hipStreamSynchronize(s_));
     volatile int64_t* res = res_host_;
@@ -1977,6 +1997,53 @@ void Refiner::quality(const Level& L, const part_t* part, int64_t& comm_size, st
     JET_HIP(hipStreamSynchronize(s_));
     comm_size = h[0];
     cut_per_part.assign(h.begin() + 1, h.end());
+}
+
+// part_stat.hpp:116-129 from the connectivity tables: heat[p * k + q] = cut
+// weight from vertices of part p to part q (q != p). With k * k <= HEAT_LDS
+// each workgroup sums into LDS first.
+constexpr int HEAT_LDS = 4096;
+
+template <bool LDS>
+__global__ void __launch_bounds__(256)
+    k_heatmap(DGraph g, const part_t* part, const eoff_t* conn_off, const int2* conn, const int32_t* conn_size,
+              int32_t k, unsigned long long* heat) {
+    __shared__ unsigned long long lh[LDS ? HEAT_LDS : 1];
+    if (LDS) {
+        for (int t = threadIdx.x; t < k * k; t += blockDim.x) lh[t] = 0;
+        __syncthreads();
+    }
+    FOR_BLOCK_RANGE(i, g.n) {
+        const part_t p = part[i];
+        for_slots(conn + conn_off[i], conn_size[i], [&](int2 e) {
+            if (e.x > NULL_PART && e.x != p && e.y > 0) {
+                const int64_t at = (int64_t)p * k + e.x;
+                if (LDS) atomicAdd(&lh[at], (unsigned long long)e.y);
+                else atomicAdd(&heat[at], (unsigned long long)e.y);
+            }
+        });
+    }
+    if (LDS) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < k * k; t += blockDim.x)
+            if (lh[t]) atomicAdd(&heat[t], lh[t]);
+    }
+}
+
+void Refiner::heatmap(const Level& L, const part_t* part, int64_t* heat) {
+    DGraph g = L.view();
+    copy_i32(cur_part_, part, g.n, s_);
+    build_tiers(L);
+    init_conn(L, cur_part_, nullptr);
+    const int64_t kk = (int64_t)p_.k * p_.k;
+    fill_i64(heat, 0, kk, s_);
+    unsigned long long* h = reinterpret_cast<unsigned long long*>(heat);
+    if (kk <= HEAT_LDS)
+        k_heatmap<true><<<grid_for(g.n, 2048, 1024), 256, 0, s_>>>(g, cur_part_, conn_off_, conn_, conn_size_, p_.k, h);
+    else
+        k_heatmap<false><<<grid_for(g.n, 256, 8192), 256, 0, s_>>>(g, cur_part_, conn_off_, conn_, conn_size_, p_.k,
+                                                                   h);
+    JET_LAUNCH_CHECK();
 }
 
 void Refiner::conn_tables(const Level& L, const part_t* part, std::vector<int64_t>& off,

@@ -123,6 +123,7 @@ def lib():
             "jet_write_partition": [c.c_char_p, i64, VP],
             "jet_read_partition": [c.c_char_p, i64, VP],
             "jet_partition_quality": [i64, VP, VP, VP, VP, VP, i32, i32, c.POINTER(Quality)],
+            "jet_cut_heatmap": [i64, VP, VP, VP, VP, i32, i32, VP],
             "jet_load_hierarchy": [c.c_char_p, i32, c.POINTER(VP)],
             "jet_hierarchy_levels": [VP],
             "jet_hierarchy_vertices": [VP, i32],
@@ -534,6 +535,15 @@ def partition_quality(g, part, k):
     _check(lib().jet_partition_quality(len(rp) - 1, _p(rp), _p(col), _p(ew), _p(vw), _p(part), k,
                                        PTR_HOST, ctypes.byref(q)))
     return q
+
+
+def cut_heatmap(g, part, k):
+    """jet_cut_heatmap on host arrays -> int64 [k, k]."""
+    rp, col, ew, _ = _graph(g)
+    part = _c(part, np.int32)
+    out = np.empty(k * k, dtype=np.int64)
+    _check(lib().jet_cut_heatmap(len(rp) - 1, _p(rp), _p(col), _p(ew), _p(part), k, PTR_HOST, _p(out)))
+    return out.reshape(k, k)
 
 
 # ---------------------------------------------------------------- dump / replay

@@ -311,12 +311,32 @@ def test_partition_quality_metrics(k):
     assert abs(q.max_imb - mx) < 1e-12 and abs(q.min_imb - mn) < 1e-12
 
 
+@pytest.mark.parametrize("k", [1, 8, 64, 100])
+def test_cut_heatmap(k):
+    """part_stat.hpp:116-129: heat[p, q] = weight of the edges from part p to
+    part q != p, against a bincount over the edge list; row sums give the
+    max part cut. k = 8 and 64 take the LDS path, k = 100 the global one."""
+    g = graphs.with_weights(graphs.rmat(12, 8, seed=9), seed=4, wmax=9)
+    n = len(g["rowptr"]) - 1
+    part = np.random.default_rng(k + 1).integers(0, k, n).astype(np.int32)
+    heat = jet_amd.cut_heatmap(g, part, k)
+    src = np.repeat(np.arange(n), np.diff(g["rowptr"]))
+    pu, pv = part[src], part[g["col"]]
+    cut = pu != pv
+    exp = np.bincount((pu[cut].astype(np.int64) * k + pv[cut]), weights=g["ew"][cut], minlength=k * k)
+    assert np.array_equal(heat.ravel(), exp.astype(np.int64))
+    assert np.array_equal(heat, heat.T) and not np.diag(heat).any()
+    assert heat.sum(axis=1).max() == jet_amd.partition_quality(g, part, k).max_part_cut
+    assert heat.sum() == 2 * jet_amd.partition_quality(g, part, k).edge_cut
+
+
 def test_part_ids_out_of_range_rejected():
     g = graphs.grid2d(16, 16)
     part = np.zeros(256, dtype=np.int32)
     part[7] = 4
     for call in (lambda: jet_amd.part_sizes(np.ones(256, dtype=np.int32), part, 4),
                  lambda: jet_amd.partition_quality(g, part, 4),
+                 lambda: jet_amd.cut_heatmap(g, part, 4),
                  lambda: jet_amd.conn_tables(g, part, 4),
                  lambda: jet_amd.refine_level(g, part, 4, 1.03)):
         with pytest.raises(jet_amd.JetError) as e:
