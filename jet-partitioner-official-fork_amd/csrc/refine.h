@@ -81,6 +81,8 @@ class Refiner {
     void strong_tail(const Level& L, const part_t* part, const int64_t* sizes, const RebalanceShape& r);
     void rank_deferred(const Level& L, const part_t* part, const int64_t* sizes, bool strong);
     // Returns true when the step's ranking was deferred (nothing applied).
+    // rebuilds the rows an update_large step marked, if any
+    void flush_rebuild(const Level& L);
     bool perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_t& cut_change, int64_t& max_size);
     // Two counter sets alternate between iterations: the first kernel of an
     // iteration clears the other set, so no fill launch is needed.
@@ -135,6 +137,7 @@ class Refiner {
     int32_t* misc_;  // [0] undersized count, [1] error bits, [2] ranking deferred
     vid_t* redraw_;
     int64_t* buckets_;
+    bool rebuild_pending_ = false;  // update_large marked rows still to rebuild
     int64_t* pub_ = nullptr;      // per-group tickets and cut-change partials of the publishing kernel
     int64_t* bcopies_ = nullptr;  // ncopies_ - 1 more bucket arrays, kept zero between uses
     int32_t ncopies_ = 1;

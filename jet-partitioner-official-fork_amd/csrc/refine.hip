@@ -708,13 +708,13 @@ __device__ inline void add_cut_part2(int64_t* partials, int64_t c2, int64_t* red
 }
 
 // One workgroup, after the step's kernels: folds the partials, takes the
-// largest part and writes (cut change, largest part, deferred) to the host
-// mailbox, then clears the accumulators. A single workgroup needs no
+// largest part and writes (cut change, largest part, deferred, rebuild) to
+// the host mailbox, then clears the accumulators. A single workgroup needs no
 // completion ticket, and the kernel boundary before it orders the other
 // kernels' atomics.
 __global__ void __launch_bounds__(256)
     k_publish(int64_t* acc, int64_t* partials, const int64_t* sizes, int32_t k, int64_t* host_out,
-              const int32_t* deferred) {
+              const int32_t* deferred, const int32_t* mc) {
     __shared__ int64_t red[256 / WAVE];
     if (*deferred) {
         if (threadIdx.x == 0) {
@@ -733,6 +733,7 @@ __global__ void __launch_bounds__(256)
         host_out[0] = acc[0] + c2;
         host_out[1] = mx;
         host_out[2] = 0;
+        host_out[3] = mc[6];  // the tables are to be rebuilt (update_large)
         __threadfence_system();
         acc[0] = 0;
     }
@@ -1536,7 +1537,6 @@ void Refiner::build_tiers(const Level& L) {
 void Refiner::init_conn(const Level& L, const part_t* part, const int32_t* mark) {
     DGraph g = L.view();
     int32_t* mk = const_cast<int32_t*>(mark);
-    const int32_t* gate = mark ? cnt() + 6 : nullptr;
     if (!mark) {
         ArenaScope sc(a_);
         int32_t* caps = a_.alloc<int32_t>((int64_t)g.n + 1);
@@ -1546,21 +1546,20 @@ void Refiner::init_conn(const Level& L, const part_t* part, const int32_t* mark)
         fill_i32(dest_cache_, NULL_PART, g.n, s_);
         fill_i32(lock_, 0, g.n, s_);
     }
-    // the gated rebuild after moves usually exits at once: keep its grid small
-    k_conn_build_small<<<grid_for(g.n, SB_BLOCK, mark ? 1024 : 8192), SB_BLOCK, 0, s_>>>(
-        g, part, p_.k, conn_off_, conn_, conn_size_, mk, dest_cache_, gate);
+    k_conn_build_small<<<grid_for(g.n, SB_BLOCK, 8192), SB_BLOCK, 0, s_>>>(g, part, p_.k, conn_off_, conn_,
+                                                                          conn_size_, mk, dest_cache_, nullptr);
     JET_LAUNCH_CHECK();
     if (ntier_mid_ > 0) {
         const size_t lds = 2 * sizeof(int2) * (size_t)WB_WAVES * std::min(p_.k, WAVE_TABLE_MAX);
-        k_conn_build_wave<<<grid_for(ntier_mid_, WB_WAVES, mark ? 1024 : 8192), WB_WAVES * WAVE, lds, s_>>>(
-            g, part, p_.k, conn_off_, conn_, conn_size_, mk, dest_cache_, tier_mid_, ntier_mid_, gate);
+        k_conn_build_wave<<<grid_for(ntier_mid_, WB_WAVES, 8192), WB_WAVES * WAVE, lds, s_>>>(
+            g, part, p_.k, conn_off_, conn_, conn_size_, mk, dest_cache_, tier_mid_, ntier_mid_, nullptr);
         JET_LAUNCH_CHECK();
     }
     if (ntier_big_ > 0) {
         // a row's capacity is min(deg, k) <= k: two tables of k slots suffice
         const size_t lds = 2 * sizeof(int2) * (size_t)p_.k;
         k_conn_build_block<<<std::min(ntier_big_, 2048), BB_BLOCK, lds, s_>>>(
-            g, part, p_.k, conn_off_, conn_, conn_size_, mk, dest_cache_, tier_big_, ntier_big_, gate);
+            g, part, p_.k, conn_off_, conn_, conn_size_, mk, dest_cache_, tier_big_, ntier_big_, nullptr);
         JET_LAUNCH_CHECK();
     }
 }
@@ -1799,16 +1798,25 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
         k_clear_moved<<<nb, 256, 0, s_>>>(moves_, c, nmax_, moved_, misc_ + 2);
         JET_LAUNCH_CHECK();
     }
-    init_conn(L, part, swap_bit_);  // rebuild marked rows (gated on the device)
-    k_publish<<<1, 256, 0, s_>>>(acc_, pub_, sizes, k, res_dev_, misc_ + 2);
+    k_publish<<<1, 256, 0, s_>>>(acc_, pub_, sizes, k, res_dev_, misc_ + 2, c);
     JET_LAUNCH_CHECK();
     JET_HIP(hipStreamSynchronize(s_));
     volatile int64_t* res = res_host_;
     if (res[2]) return true;  // ranking deferred: nothing applied, counters kept
     cut_change = res[0];
     max_size = res[1];
+    // update_large: the marked rows are rebuilt before the tables are next
+    // read (rebuild_pending), so a step that pushed its updates launches no
+    // rebuild kernels
+    rebuild_pending_ = res[3] != 0;
     par_ ^= 1;
     return false;
+}
+
+void Refiner::flush_rebuild(const Level& L) {
+    if (!rebuild_pending_) return;
+    init_conn(L, cur_part_, swap_bit_);
+    rebuild_pending_ = false;
 }
 
 LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
@@ -1860,6 +1868,7 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
         while (count++ <= 11) {
             rec.iterations++;
             bool strong = false;
+            flush_rebuild(L);
             if (curr_imb <= imb_max) {
                 jet_lp(L, cur_part_, fr);
                 balance_counter = 0;
@@ -1894,6 +1903,7 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
             }
         }
     }
+    flush_rebuild(L);
     int32_t err = read_scalar(misc_ + 1, s_);
     if (err) throw DeviceError("connectivity table invariant violated (code " + std::to_string(err) + ")");
     rec.cut2 = cut_;
