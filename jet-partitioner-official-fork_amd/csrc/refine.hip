@@ -374,17 +374,23 @@ __global__ void __launch_bounds__(LS_WAVES* WAVE)
                 vid_t* cand, int32_t* counters, vid_t nmax, int32_t* next_counters, unsigned long long* pc,
                 unsigned long long* span) {
     prof_span_begin(span);
+    constexpr int NPASS = LS_CHW / WAVE;
     __shared__ vid_t sbuf[LS_WAVES * LS_CHW];
     __shared__ int32_t sh[2 * LS_WAVES + 2];
     __shared__ int64_t pred[LS_WAVES];
+    // wave-private: the slots (u * WAVE + lane) of the wave's cache misses,
+    // and each miss's new destination and gain
+    __shared__ int16_t mslot[LS_WAVES][LS_CHW];
+    __shared__ part_t mbest[LS_WAVES][LS_CHW];
+    __shared__ int32_t mgain[LS_WAVES][LS_CHW];
     int64_t n_miss = 0, n_slot = 0, n_cand = 0;
     StagedLists<LS_WAVES, LS_CHW> st(sbuf);
     if (blockIdx.x == 0 && threadIdx.x < 8) next_counters[threadIdx.x] = 0;
     const int lane = threadIdx.x & (WAVE - 1);
-    const int64_t wbase = ((int64_t)blockIdx.x * LS_WAVES + threadIdx.x / WAVE) * LS_CHW;
-    // the cached destinations and locks of all LS_CHW / WAVE passes are
-    // loaded up front, so the passes do not wait for them one after another
-    constexpr int NPASS = LS_CHW / WAVE;
+    const int wv = threadIdx.x / WAVE;
+    const int64_t wbase = ((int64_t)blockIdx.x * LS_WAVES + wv) * LS_CHW;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    // 1. the cached destinations and locks of all passes, loaded together
     part_t cached[NPASS];
     int32_t locks[NPASS];
 #pragma unroll
@@ -393,53 +399,81 @@ __global__ void __launch_bounds__(LS_WAVES* WAVE)
         cached[u] = i < g.n ? dest_cache[i] : NO_MOVE;
         locks[u] = i < g.n ? lock[i] : 0;
     }
+    // 2. the wave's misses, compacted, so that every lane evaluates one
+    // table at a time instead of each pass waiting on its own few misses
+    int32_t nm = 0;
+#pragma unroll
+    for (int u = 0; u < NPASS; u++) {
+        const bool miss = cached[u] == NULL_PART;
+        const uint64_t mk = __ballot(miss);
+        if (miss) mslot[wv][nm + (int32_t)__popcll(mk & lt)] = (int16_t)(u * WAVE + lane);
+        nm += (int32_t)__popcll(mk);
+    }
+    wave_sync();
+    for (int32_t m = lane; m < nm; m += WAVE) {
+        const int32_t slot = mslot[wv][m];
+        const int64_t i = wbase + slot;
+        const part_t p = part[i];
+        part_t best = NO_MOVE;
+        int32_t b_conn = 0, p_conn = 0;
+        const int32_t sz = conn_size[i];
+        n_miss++;
+        n_slot += sz;
+        for_slots(conn + conn_off[i], sz, [&](int2 e) {
+            if (e.y > b_conn && e.x != p) {
+                best = e.x;
+                b_conn = e.y;
+            } else if (e.y == b_conn && b_conn > 0 && e.x != p && e.x < best) {
+                best = e.x;
+            } else if (e.y > 0 && e.x == p) {
+                p_conn = e.y;
+            }
+        });
+        int32_t gsv = 0;
+        if (best != NO_MOVE) {
+            if (b_conn >= p_conn || ((double)(p_conn - b_conn) < floor(fr * (double)p_conn))) gsv = b_conn - p_conn;
+            else best = NO_MOVE;
+        }
+        gain_p[i] = gsv;
+        dest_cache[i] = best;
+        mbest[wv][slot] = best;
+        mgain[wv][slot] = gsv;
+    }
+    wave_sync();
+    // 3. destinations; a vertex that is not a candidate this iteration gets
+    // NO_MOVE: the afterburner skips such neighbours before reading their
+    // pre-gain, which is then only written for candidates. The candidates'
+    // gains and row offsets of all passes are loaded together.
+    bool is_cand[NPASS];
+    int32_t gain[NPASS];
+    eoff_t r0[NPASS], r1[NPASS];
 #pragma unroll
     for (int u = 0; u < NPASS; u++) {
         const int64_t i = wbase + u * WAVE + lane;
-        bool is_cand = false;
-        eoff_t deg = 0;
+        const bool miss = cached[u] == NULL_PART;
+        const part_t best = miss ? mbest[wv][u * WAVE + lane] : cached[u];
+        is_cand[u] = i < g.n && best != NO_MOVE && locks[u] == 0;
         if (i < g.n) {
-            part_t best = cached[u];
-            if (best == NULL_PART) {
-                part_t p = part[i];
-                best = NO_MOVE;
-                int32_t b_conn = 0, p_conn = 0;
-                const int32_t sz = conn_size[i];
-                n_miss++;
-                n_slot += sz;
-                for_slots(conn + conn_off[i], sz, [&](int2 e) {
-                    if (e.y > b_conn && e.x != p) {
-                        best = e.x;
-                        b_conn = e.y;
-                    } else if (e.y == b_conn && b_conn > 0 && e.x != p && e.x < best) {
-                        best = e.x;
-                    } else if (e.y > 0 && e.x == p) {
-                        p_conn = e.y;
-                    }
-                });
-                int32_t gsv = 0;
-                if (best != NO_MOVE) {
-                    if (b_conn >= p_conn || ((double)(p_conn - b_conn) < floor(fr * (double)p_conn))) gsv = b_conn - p_conn;
-                    else best = NO_MOVE;
-                }
-                gain_p[i] = gsv;
-                dest_cache[i] = best;
-            }
-            // a vertex that is not a candidate this iteration gets NO_MOVE as
-            // its destination: the afterburner skips such neighbours before
-            // reading their pre-gain, which is then only written for candidates
-            const int32_t locked = locks[u];
-            is_cand = best != NO_MOVE && locked == 0;
-            dest[i] = is_cand ? best : NO_MOVE;
-            if (is_cand) {
-                n_cand++;
-                pregain[i] = gain_p[i];
-                deg = g.rowptr[i + 1] - g.rowptr[i];
-            } else if (locked) {
-                lock[i] = 0;
-            }
+            dest[i] = is_cand[u] ? best : NO_MOVE;
+            if (!is_cand[u] && locks[u]) lock[i] = 0;
         }
-        st.push(is_cand && deg <= BIG_DEG, is_cand && deg > BIG_DEG, (vid_t)i);
+        gain[u] = 0;
+        r0[u] = r1[u] = 0;
+        if (is_cand[u]) {
+            gain[u] = miss ? mgain[wv][u * WAVE + lane] : gain_p[i];
+            r0[u] = g.rowptr[i];
+            r1[u] = g.rowptr[i + 1];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NPASS; u++) {
+        const int64_t i = wbase + u * WAVE + lane;
+        const eoff_t deg = r1[u] - r0[u];
+        if (is_cand[u]) {
+            n_cand++;
+            pregain[i] = gain[u];
+        }
+        st.push(is_cand[u] && deg <= BIG_DEG, is_cand[u] && deg > BIG_DEG, (vid_t)i);
     }
     st.flush(cand, &counters[0], &counters[1], nmax, sh);
     if (pc) {
