@@ -21,6 +21,9 @@ constexpr int MAILBOX_SLOTS = 64;
 struct Mailbox {
     int64_t* host = nullptr;
     int64_t* dev = nullptr;
+    // last sequence word a kernel was asked to write here: unique per
+    // mailbox, so a stale word never matches a new wait
+    int64_t seq = 0;
     ~Mailbox();
 };
 Mailbox& thread_mailbox();

@@ -81,6 +81,7 @@ class Refiner {
     void strong_tail(const Level& L, const part_t* part, const int64_t* sizes, const RebalanceShape& r);
     void rank_deferred(const Level& L, const part_t* part, const int64_t* sizes, bool strong);
     // Returns true when the step's ranking was deferred (nothing applied).
+    void wait_published(volatile int64_t* res, int64_t seq);
     // rebuilds the rows an update_large step marked, if any
     void flush_rebuild(const Level& L);
     bool perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_t& cut_change, int64_t& max_size);
@@ -138,6 +139,7 @@ class Refiner {
     vid_t* redraw_;
     int64_t* buckets_;
     bool rebuild_pending_ = false;  // update_large marked rows still to rebuild
+    Mailbox* mailbox_ = nullptr;    // the creating thread's mailbox (results and sequence words)
     int64_t* pub_ = nullptr;      // per-group tickets and cut-change partials of the publishing kernel
     int64_t* bcopies_ = nullptr;  // ncopies_ - 1 more bucket arrays, kept zero between uses
     int32_t ncopies_ = 1;

@@ -748,11 +748,13 @@ __device__ inline void add_cut_part2(int64_t* partials, int64_t c2, int64_t* red
 // kernels' atomics.
 __global__ void __launch_bounds__(256)
     k_publish(int64_t* acc, int64_t* partials, const int64_t* sizes, int32_t k, int64_t* host_out,
-              const int32_t* deferred, const int32_t* mc) {
+              const int32_t* deferred, const int32_t* mc, int64_t seq) {
     __shared__ int64_t red[256 / WAVE];
     if (*deferred) {
         if (threadIdx.x == 0) {
             host_out[2] = 1;
+            __threadfence_system();
+            host_out[5] = seq;  // the host waits for this word
             __threadfence_system();
         }
         return;
@@ -768,6 +770,8 @@ __global__ void __launch_bounds__(256)
         host_out[1] = mx;
         host_out[2] = 0;
         host_out[3] = mc[6];  // the tables are to be rebuilt (update_large)
+        __threadfence_system();
+        host_out[5] = seq;  // written after the results: the host waits for it
         __threadfence_system();
         acc[0] = 0;
     }
@@ -1549,6 +1553,7 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
     fill_i64(pub_, 0, (int64_t)PUB_GROUPS * PUB_STRIDE, s);
     // (cut change, largest part) of each iteration land in mailbox slots 8, 9
     Mailbox& mb = thread_mailbox();
+    mailbox_ = &mb;
     res_host_ = mb.host + 8;
     res_dev_ = mb.dev + 8;
 }
@@ -1765,6 +1770,27 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
     if (strong) strong_tail(L, part, sizes, r);
 }
 
+// Waits for k_publish's sequence word in the mapped mailbox: a poll of host
+// memory returns within a microsecond or two of the write, where a stream
+// synchronisation sleeps and wakes. Every few thousand polls the stream is
+// queried, so an error or a stream that finished without the word ends the
+// wait.
+void Refiner::wait_published(volatile int64_t* res, int64_t seq) {
+    for (uint64_t spins = 1;; spins++) {
+        if (res[5] == seq) return;
+        if (spins % 4096 == 0) {
+            hipError_t e = hipStreamQuery(s_);
+            if (e == hipSuccess) {
+                if (res[5] == seq) return;
+                throw DeviceError("refinement step finished without publishing its results");
+            }
+            if (e != hipErrorNotReady) {
+                JET_HIP(e);
+            }
+        }
+    }
+}
+
 bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_t& cut_change, int64_t& max_size) {
     DGraph g = L.view();
     const int32_t k = p_.k;
@@ -1832,10 +1858,11 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
         k_clear_moved<<<nb, 256, 0, s_>>>(moves_, c, nmax_, moved_, misc_ + 2);
         JET_LAUNCH_CHECK();
     }
-    k_publish<<<1, 256, 0, s_>>>(acc_, pub_, sizes, k, res_dev_, misc_ + 2, c);
+    const int64_t seq = ++mailbox_->seq;
+    k_publish<<<1, 256, 0, s_>>>(acc_, pub_, sizes, k, res_dev_, misc_ + 2, c, seq);
     JET_LAUNCH_CHECK();
-    JET_HIP(hipStreamSynchronize(s_));
     volatile int64_t* res = res_host_;
+    wait_published(res, seq);
     if (res[2]) return true;  // ranking deferred: nothing applied, counters kept
     cut_change = res[0];
     max_size = res[1];
