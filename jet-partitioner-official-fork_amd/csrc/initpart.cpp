@@ -352,8 +352,14 @@ struct Bisector {
 };
 
 constexpr int JET_HOST_THREADS_MAX = 16;
-// Below this many adjacency entries a subproblem is not worth a thread.
+// Below this many adjacency entries a subproblem is not worth a thread for
+// its induced subgraphs or its coarsening; trials of one bisection and the
+// two halves of a recursion get threads from much smaller sizes, since their
+// work per thread (whole trials, whole subtrees) is larger. On a 557-vertex,
+// 7430-entry graph split 64 ways the latter cut the time about four-fold.
 constexpr int64_t PAR_MIN_EDGES = 200000;
+constexpr int64_t PAR_MIN_TRIAL_EDGES = 2000;
+constexpr int64_t PAR_MIN_RECURSE_EDGES = 500;
 
 struct Trial {
     std::vector<int8_t> side;
@@ -407,7 +413,7 @@ static std::vector<int8_t> bisect_flat(const Sub& s, double frac, double ub, uin
     int trials = n > 20000 ? 2 : (n > 2000 ? 4 : 8);
     int passes = n > 20000 ? 4 : 8;
     std::vector<Trial> res(trials);
-    if (par > 1 && (int64_t)s.adj.size() >= PAR_MIN_EDGES) {
+    if (par > 1 && (int64_t)s.adj.size() >= PAR_MIN_TRIAL_EDGES) {
         for (int t0 = 0; t0 < trials; t0 += par) {
             std::vector<std::thread> th;
             int t1 = std::min(trials, t0 + par);
@@ -670,7 +676,7 @@ static void recurse(const Sub& s, int32_t k, int32_t first, double ub, uint64_t 
     }
     side.clear();
     side.shrink_to_fit();
-    if (par > 1 && (int64_t)(a.adj.size() + b.adj.size()) >= PAR_MIN_EDGES) {
+    if (par > 1 && (int64_t)(a.adj.size() + b.adj.size()) >= PAR_MIN_RECURSE_EDGES) {
         int pa = par / 2;
         std::thread th([&]() { recurse(a, k0, first, ub, seed, part, pa); });
         recurse(b, k1, first + k0, ub, seed, part, par - pa);
