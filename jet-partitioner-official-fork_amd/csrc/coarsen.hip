@@ -32,9 +32,11 @@ __device__ inline T group_sum(T x) {
 }
 
 // ---------------------------------------------------------------- heavy list
-__global__ void k_heavy_list(DGraph g, vid_t* list, int32_t* cnt) {
-    FOR_THREAD(i, g.n) {
-        int32_t p = wave_append(cnt, g.rowptr[i + 1] - g.rowptr[i] > HEAVY_DEG);
+__global__ void __launch_bounds__(256) k_heavy_list(DGraph g, vid_t* list, int32_t* cnt) {
+    __shared__ int32_t sh[256 / WAVE + 1];
+    FOR_BLOCK_UNIFORM(i, g.n) {
+        const bool heavy = i < g.n && g.rowptr[i + 1] - g.rowptr[i] > HEAVY_DEG;
+        int32_t p = block_append<256>(cnt, heavy, sh);
         if (p >= 0) list[p] = (vid_t)i;
     }
 }
@@ -206,10 +208,17 @@ __global__ void k_mc_resolve(const vid_t* wl, int32_t len, int32_t round, int sw
     }
 }
 
-__global__ void k_mc_next(const vid_t* wl, int32_t len, vid_t n, const vid_t* vcmap, vid_t* next, int32_t* cnt) {
-    FOR_THREAD(x, len) {
-        vid_t u = wl ? wl[x] : (vid_t)x;
-        int32_t p = wave_append(cnt, vcmap[u] >= n);
+__global__ void __launch_bounds__(256)
+    k_mc_next(const vid_t* wl, int32_t len, vid_t n, const vid_t* vcmap, vid_t* next, int32_t* cnt) {
+    __shared__ int32_t sh[256 / WAVE + 1];
+    FOR_BLOCK_UNIFORM(x, len) {
+        vid_t u = 0;
+        bool keep = false;
+        if (x < len) {
+            u = wl ? wl[x] : (vid_t)x;
+            keep = vcmap[u] >= n;
+        }
+        int32_t p = block_append<256>(cnt, keep, sh);
         if (p >= 0) next[p] = u;
     }
 }
@@ -441,11 +450,17 @@ __global__ void k_commit_reset(const vid_t* wl, int32_t len, vid_t* vcmap) {
     }
 }
 
-__global__ void k_match_next(const vid_t* wl, int32_t len, const vid_t* vcmap, const vid_t* hn, vid_t* next,
-                             int32_t* cnt) {
-    FOR_THREAD(x, len) {
-        vid_t u = wl ? wl[x] : (vid_t)x;
-        int32_t p = wave_append(cnt, vcmap[u] == ORD_MAX && hn[u] != ORD_MAX);
+__global__ void __launch_bounds__(256) k_match_next(const vid_t* wl, int32_t len, const vid_t* vcmap, const vid_t* hn,
+                                                    vid_t* next, int32_t* cnt) {
+    __shared__ int32_t sh[256 / WAVE + 1];
+    FOR_BLOCK_UNIFORM(x, len) {
+        vid_t u = 0;
+        bool keep = false;
+        if (x < len) {
+            u = wl ? wl[x] : (vid_t)x;
+            keep = vcmap[u] == ORD_MAX && hn[u] != ORD_MAX;
+        }
+        int32_t p = block_append<256>(cnt, keep, sh);
         if (p >= 0) next[p] = u;
     }
 }

@@ -40,12 +40,15 @@ __global__ void k_members(vid_t n, const vid_t* map, const int32_t* moff, int32_
     }
 }
 
-__global__ void k_tiers(vid_t nc, const eoff_t* hoff, vid_t* l1, vid_t* l2, vid_t* l3, int32_t* cnt) {
-    FOR_THREAD(c, nc) {
-        eoff_t hs = hoff[c + 1] - hoff[c];
-        int32_t a = wave_append(&cnt[0], hs <= T1_MAX);
-        int32_t b = wave_append(&cnt[1], hs > T1_MAX && hs <= T2_MAX);
-        int32_t d = wave_append(&cnt[2], hs > T2_MAX);
+__global__ void __launch_bounds__(256)
+    k_tiers(vid_t nc, const eoff_t* hoff, vid_t* l1, vid_t* l2, vid_t* l3, int32_t* cnt) {
+    __shared__ int32_t sh[256 / WAVE + 1];
+    FOR_BLOCK_UNIFORM(c, nc) {
+        const bool in = c < nc;
+        const eoff_t hs = in ? hoff[c + 1] - hoff[c] : 0;
+        int32_t a = block_append<256>(&cnt[0], in && hs <= T1_MAX, sh);
+        int32_t b = block_append<256>(&cnt[1], in && hs > T1_MAX && hs <= T2_MAX, sh);
+        int32_t d = block_append<256>(&cnt[2], in && hs > T2_MAX, sh);
         if (a >= 0) l1[a] = (vid_t)c;
         if (b >= 0) l2[b] = (vid_t)c;
         if (d >= 0) l3[d] = (vid_t)c;

@@ -134,18 +134,20 @@ __global__ void k_conn_caps(DGraph g, int32_t k, int32_t* caps) {
 // Table-build tiers (mid: one wave per row, big: one workgroup per row) and
 // the rows whose update_small changes are pulled (degree > BIG_DEG): big-tier
 // rows and pull_mid rows; hub[] flags both kinds.
-__global__ void k_conn_tiers(DGraph g, int32_t k, vid_t* mid, vid_t* big, vid_t* pull_mid, int32_t* cnt,
-                             uint8_t* hub) {
-    FOR_THREAD(i, g.n) {
-        int64_t d = g.rowptr[i + 1] - g.rowptr[i];
+__global__ void __launch_bounds__(256) k_conn_tiers(DGraph g, int32_t k, vid_t* mid, vid_t* big, vid_t* pull_mid,
+                                                    int32_t* cnt, uint8_t* hub) {
+    __shared__ int32_t sh[256 / WAVE + 1];
+    FOR_BLOCK_UNIFORM(i, g.n) {
+        const bool in = i < g.n;
+        int64_t d = in ? g.rowptr[i + 1] - g.rowptr[i] : 0;
         int64_t cap = d > k ? k : d;
         bool is_mid = d > SMALL_DEG && d <= HEAVY_DEG_REFINE && cap <= WAVE_TABLE_MAX;
         bool is_big = d > SMALL_DEG && !is_mid;
         bool is_pmid = is_mid && d > BIG_DEG;
-        hub[i] = (is_big || is_pmid) ? 1 : 0;
-        int32_t a = wave_append(&cnt[0], is_mid);
-        int32_t b = wave_append(&cnt[1], is_big);
-        int32_t c = wave_append(&cnt[3], is_pmid);
+        if (in) hub[i] = (is_big || is_pmid) ? 1 : 0;
+        int32_t a = block_append<256>(&cnt[0], is_mid, sh);
+        int32_t b = block_append<256>(&cnt[1], is_big, sh);
+        int32_t c = block_append<256>(&cnt[3], is_pmid, sh);
         if (a >= 0) mid[a] = (vid_t)i;
         if (b >= 0) big[b] = (vid_t)i;
         if (c >= 0) pull_mid[c] = (vid_t)i;
@@ -1488,14 +1490,15 @@ static int group_for_avg(const DGraph& g) {
     return avg <= 4 ? 4 : avg <= 8 ? 8 : avg <= 16 ? 16 : avg <= 32 ? 32 : 64;
 }
 
-__global__ void k_max_degree(DGraph g, int32_t* out) {
+__global__ void __launch_bounds__(256) k_max_degree(DGraph g, int32_t* out) {
+    __shared__ int32_t red[256 / WAVE];
     int32_t mx = 0;
     FOR_THREAD(i, g.n) {
         int64_t d = g.rowptr[i + 1] - g.rowptr[i];
         if (d > mx) mx = (int32_t)(d > ORD_MAX ? ORD_MAX : d);
     }
-    mx = wave_reduce_max(mx);
-    if ((threadIdx.x & (WAVE - 1)) == 0 && mx) atomicMax(out, mx);
+    mx = block_reduce_max<int32_t, 256>(mx, red);
+    if (threadIdx.x == 0 && mx) atomicMax(out, mx);
 }
 
 Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefineParams& p) : a_(a), s_(s), p_(p), nmax_(n) {
@@ -1563,7 +1566,7 @@ void Refiner::build_tiers(const Level& L) {
     fill_i32(counters_ + 16, 0, 4, s_);
     k_conn_tiers<<<grid_for(g.n, 256, 8192), 256, 0, s_>>>(g, p_.k, tier_mid_, tier_big_, pull_mid_, counters_ + 16,
                                                             hub_);
-    k_max_degree<<<grid_for(g.n, 256, 2048), 256, 0, s_>>>(g, counters_ + 18);
+    k_max_degree<<<grid_for(g.n, 256, 512), 256, 0, s_>>>(g, counters_ + 18);
     JET_LAUNCH_CHECK();
     int32_t c[4];
     read_i32_words(counters_ + 16, 4, c, s_);

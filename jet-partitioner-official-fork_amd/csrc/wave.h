@@ -46,6 +46,40 @@ __device__ inline int32_t wave_append(int32_t* counter, bool pred) {
     return pred ? base + (int32_t)__popcll(mask & ((1ull << lane) - 1ull)) : -1;
 }
 
+// Workgroup-aggregated append: one atomic per workgroup and call instead of
+// one per wave (thousands of waves adding to one counter serialise on it).
+// Every thread of the workgroup must make the same sequence of calls; sh
+// holds BLOCK / WAVE + 1 ints. Returns the slot of each pred lane, else -1.
+template <int BLOCK>
+__device__ inline int32_t block_append(int32_t* counter, bool pred, int32_t* sh) {
+    constexpr int WAVES = BLOCK / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int w = threadIdx.x / WAVE;
+    const uint64_t mask = __ballot(pred);
+    if (lane == 0) sh[w] = (int32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t t = 0;
+        for (int q = 0; q < WAVES; q++) {
+            const int32_t c = sh[q];
+            sh[q] = t;
+            t += c;
+        }
+        sh[WAVES] = t ? atomicAdd(counter, t) : 0;
+    }
+    __syncthreads();
+    const int32_t r = pred ? sh[WAVES] + sh[w] + (int32_t)__popcll(mask & ((1ull << lane) - 1ull)) : -1;
+    __syncthreads();
+    return r;
+}
+
+// Grid-stride loop whose trip count is the same for every thread of a
+// workgroup (IDX_ may run past LEN_: check it), so the body may call
+// block_append.
+#define FOR_BLOCK_UNIFORM(IDX_, LEN_)                                                              \
+    for (int64_t base_ = (int64_t)blockIdx.x * blockDim.x, IDX_ = base_ + threadIdx.x; base_ < (LEN_); \
+         base_ += (int64_t)gridDim.x * blockDim.x, IDX_ = base_ + threadIdx.x)
+
 // Unordered two-list append with wave-private LDS staging and one global
 // atomic per workgroup and list. Each wave stages up to CHW items (front list
 // grows up, back list grows down inside its region); flush() publishes them:
