@@ -497,9 +497,10 @@ __global__ void __launch_bounds__(LS_WAVES* WAVE)
 // reference's minimum pre-gain gives them.
 __device__ inline int64_t afterburn_edge(vid_t v, vid_t i, int32_t ig, int32_t w, part_t p, part_t best,
                                          const part_t* part, const part_t* dest, const int32_t* pregain) {
+    // both loaded at once; the pre-gain of a non-candidate is stale and unused
     part_t vp = dest[v];
-    if (vp == NO_MOVE) return 0;
     int32_t vg = pregain[v];
+    if (vp == NO_MOVE) return 0;
     int64_t ch = 0;
     if (vg > ig || (vg == ig && v < i)) {
         if (vp == p) ch -= w;
@@ -743,39 +744,48 @@ __device__ inline void add_cut_part2(int64_t* partials, int64_t c2, int64_t* red
                   (unsigned long long)c2);
 }
 
-// One workgroup, after the step's kernels: folds the partials, takes the
-// largest part and writes (cut change, largest part, deferred, rebuild) to
-// the host mailbox, then clears the accumulators. A single workgroup needs no
-// completion ticket, and the kernel boundary before it orders the other
-// kernels' atomics.
-__global__ void __launch_bounds__(256)
-    k_publish(int64_t* acc, int64_t* partials, const int64_t* sizes, int32_t k, int64_t* host_out,
-              const int32_t* deferred, const int32_t* mc, int64_t seq) {
-    __shared__ int64_t red[256 / WAVE];
-    if (*deferred) {
+// Run by one workgroup (256 threads) after the subtraction pass: folds the
+// partials, takes the largest part and writes (cut change, largest part,
+// deferred, rebuild) to the host mailbox, then the sequence word the host
+// waits for, and clears the accumulators. One workgroup needs no completion
+// ticket (a ticket in a many-workgroup kernel needs an agent-scope release
+// fence per workgroup, which writes back L2), and the kernel boundary before
+// it orders the other kernels' atomics.
+struct Publish {
+    int64_t* acc;
+    int64_t* partials;
+    const int64_t* sizes;
+    int32_t k;
+    int64_t* host_out;
+    const int32_t* mc;
+    int64_t seq;
+};
+
+__device__ inline void publish_step(const Publish& pb, bool deferred, int64_t* red) {
+    if (deferred) {
         if (threadIdx.x == 0) {
-            host_out[2] = 1;
+            pb.host_out[2] = 1;
             __threadfence_system();
-            host_out[5] = seq;  // the host waits for this word
+            pb.host_out[5] = pb.seq;  // the host waits for this word
             __threadfence_system();
         }
         return;
     }
     int64_t mx = INT64_MIN;
-    for (int t = threadIdx.x; t < k; t += blockDim.x) mx = sizes[t] > mx ? sizes[t] : mx;
+    for (int t = threadIdx.x; t < pb.k; t += blockDim.x) mx = pb.sizes[t] > mx ? pb.sizes[t] : mx;
     mx = block_reduce_max<int64_t, 256>(mx, red);
-    int64_t c2 = threadIdx.x < PUB_GROUPS ? partials[(int64_t)threadIdx.x * PUB_STRIDE] : 0;
+    int64_t c2 = threadIdx.x < PUB_GROUPS ? pb.partials[(int64_t)threadIdx.x * PUB_STRIDE] : 0;
     c2 = block_reduce_sum<int64_t, 256>(c2, red);
-    if (threadIdx.x < PUB_GROUPS) partials[(int64_t)threadIdx.x * PUB_STRIDE] = 0;
+    if (threadIdx.x < PUB_GROUPS) pb.partials[(int64_t)threadIdx.x * PUB_STRIDE] = 0;
     if (threadIdx.x == 0) {
-        host_out[0] = acc[0] + c2;
-        host_out[1] = mx;
-        host_out[2] = 0;
-        host_out[3] = mc[6];  // the tables are to be rebuilt (update_large)
+        pb.host_out[0] = pb.acc[0] + c2;
+        pb.host_out[1] = mx;
+        pb.host_out[2] = 0;
+        pb.host_out[3] = pb.mc[6];  // the tables are to be rebuilt (update_large)
         __threadfence_system();
-        host_out[5] = seq;  // written after the results: the host waits for it
+        pb.host_out[5] = pb.seq;  // written after the results: the host waits for it
         __threadfence_system();
-        acc[0] = 0;
+        pb.acc[0] = 0;
     }
 }
 
@@ -787,8 +797,11 @@ __global__ void __launch_bounds__(256)
     k_update_small(DGraph g, const vid_t* moves, int32_t* mc, vid_t nmax, const part_t* part, const part_t* dest,
                    const eoff_t* conn_off, int2* conn, int32_t* conn_size, part_t* dest_cache, int32_t k,
                    int32_t* err, int32_t* mark, const uint8_t* hub, uint8_t* hub_dirty, const int32_t* deferred,
-                   int64_t* partials) {
+                   int64_t* partials, Publish pb) {
     __shared__ int64_t red[256 / WAVE];
+    // the addition pass's workgroup 0 publishes the step (the subtraction
+    // pass, which completes the results, has finished)
+    if (ADD && blockIdx.x == 0) publish_step(pb, *deferred != 0, red);
     if (*deferred) return;
     const bool rebuild = update_by_rebuild(mc, g.n);
     if (ADD && rebuild) return;
@@ -1773,7 +1786,7 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
     if (strong) strong_tail(L, part, sizes, r);
 }
 
-// Waits for k_publish's sequence word in the mapped mailbox: a poll of host
+// Waits for the step's sequence word in the mapped mailbox: a poll of host
 // memory returns within a microsecond or two of the write, where a stream
 // synchronisation sleeps and wakes. Every few thousand polls the stream is
 // queried, so an error or a stream that finished without the word ends the
@@ -1809,15 +1822,17 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
     JET_LAUNCH_CHECK();
     int G = group_for_avg(g);
     int nbs = grid_for((int64_t)g.n * G, 256, 1024);
+    const int64_t seq = ++mailbox_->seq;
+    const Publish pb{acc_, pub_, sizes, k, res_dev_, c, seq};
     int nbb = grid_for((int64_t)g.n * WAVE, 256, 512);
     for (int add = 0; add < 2; add++) {
 #define JET_UPD(GG)                                                                                                   \
     if (add) k_update_small<true, GG><<<nbs, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,       \
                                                           conn_size_, dest_cache_, k, misc_ + 1, swap_bit_,      \
-                                                          hub_, hub_dirty_, misc_ + 2, pub_);                   \
+                                                          hub_, hub_dirty_, misc_ + 2, pub_, pb);               \
     else k_update_small<false, GG><<<nbs, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,          \
                                                         conn_size_, dest_cache_, k, misc_ + 1, swap_bit_,        \
-                                                        hub_, hub_dirty_, misc_ + 2, pub_);
+                                                        hub_, hub_dirty_, misc_ + 2, pub_, pb);
         switch (G) {
             case 4: JET_UPD(4) break;
             case 8: JET_UPD(8) break;
@@ -1861,9 +1876,6 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
         k_clear_moved<<<nb, 256, 0, s_>>>(moves_, c, nmax_, moved_, misc_ + 2);
         JET_LAUNCH_CHECK();
     }
-    const int64_t seq = ++mailbox_->seq;
-    k_publish<<<1, 256, 0, s_>>>(acc_, pub_, sizes, k, res_dev_, misc_ + 2, c, seq);
-    JET_LAUNCH_CHECK();
     volatile int64_t* res = res_host_;
     wait_published(res, seq);
     if (res[2]) return true;  // ranking deferred: nothing applied, counters kept
