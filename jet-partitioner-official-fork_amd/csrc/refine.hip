@@ -135,13 +135,13 @@ __global__ void k_conn_caps(DGraph g, int32_t k, int32_t* caps) {
 // the rows whose update_small changes are pulled (degree > BIG_DEG): big-tier
 // rows and pull_mid rows; hub[] flags both kinds.
 __global__ void __launch_bounds__(256) k_conn_tiers(DGraph g, int32_t k, vid_t* mid, vid_t* big, vid_t* pull_mid,
-                                                    int32_t* cnt, uint8_t* hub) {
+                                                    int32_t* cnt, uint8_t* hub, int32_t small_max) {
     __shared__ int32_t sh[256 / WAVE + 1];
     FOR_BLOCK_UNIFORM(i, g.n) {
         const bool in = i < g.n;
         int64_t d = in ? g.rowptr[i + 1] - g.rowptr[i] : 0;
         int64_t cap = d > k ? k : d;
-        bool is_mid = d > SMALL_DEG && d <= HEAVY_DEG_REFINE && cap <= WAVE_TABLE_MAX;
+        bool is_mid = d > small_max && d <= HEAVY_DEG_REFINE && cap <= WAVE_TABLE_MAX;
         bool is_big = d > SMALL_DEG && !is_mid;
         bool is_pmid = is_mid && d > BIG_DEG;
         if (in) hub[i] = (is_big || is_pmid) ? 1 : 0;
@@ -162,17 +162,19 @@ __device__ inline int32_t shrink_size(int32_t used, int32_t cap) {
 }
 
 constexpr int SB_BLOCK = 256;
+// Levels up to this many vertices build every table with a wave per row.
+constexpr vid_t SMALL_LEVEL_N = 32768;
 
 __global__ void __launch_bounds__(SB_BLOCK)
     k_conn_build_small(DGraph g, const part_t* part, int32_t k, const eoff_t* conn_off, int2* conn,
-                       int32_t* conn_size, int32_t* mark, part_t* dest_cache, const int32_t* gate) {
+                       int32_t* conn_size, int32_t* mark, part_t* dest_cache, const int32_t* gate, int32_t small_max) {
     if (gate && gate[0] == 0) return;
     __shared__ int2 tab[SMALL_DEG][SB_BLOCK];
     const int t = threadIdx.x;
     FOR_THREAD(i, g.n) {
         eoff_t rs = g.rowptr[i], re = g.rowptr[i + 1];
         int32_t deg = (int32_t)(re - rs);
-        if (deg > SMALL_DEG) continue;
+        if (deg > small_max) continue;
         if (mark) {
             if (!mark[i]) continue;
             mark[i] = 0;
@@ -369,28 +371,31 @@ __device__ inline void for_slots(const int2* tab, int32_t sz, F&& f) {
 // in LDS and published with one atomic per workgroup.
 constexpr int LS_WAVES = 4;
 constexpr int LS_CHW = 256;
+// levels up to LS_ONE_PASS_WAVES * WAVE vertices run one pass per wave
+constexpr int LS_ONE_PASS_WAVES = 8192;
 
+template <int CHW>
 __global__ void __launch_bounds__(LS_WAVES* WAVE)
     k_lp_select(DGraph g, const part_t* part, const eoff_t* conn_off, const int2* conn, const int32_t* conn_size,
                 part_t* dest_cache, int32_t* gain_p, int32_t* lock, double fr, part_t* dest, int32_t* pregain,
                 vid_t* cand, int32_t* counters, vid_t nmax, int32_t* next_counters, unsigned long long* pc,
                 unsigned long long* span) {
     prof_span_begin(span);
-    constexpr int NPASS = LS_CHW / WAVE;
-    __shared__ vid_t sbuf[LS_WAVES * LS_CHW];
+    constexpr int NPASS = CHW / WAVE;
+    __shared__ vid_t sbuf[LS_WAVES * CHW];
     __shared__ int32_t sh[2 * LS_WAVES + 2];
     __shared__ int64_t pred[LS_WAVES];
     // wave-private: the slots (u * WAVE + lane) of the wave's cache misses,
     // and each miss's new destination and gain
-    __shared__ int16_t mslot[LS_WAVES][LS_CHW];
-    __shared__ part_t mbest[LS_WAVES][LS_CHW];
-    __shared__ int32_t mgain[LS_WAVES][LS_CHW];
+    __shared__ int16_t mslot[LS_WAVES][CHW];
+    __shared__ part_t mbest[LS_WAVES][CHW];
+    __shared__ int32_t mgain[LS_WAVES][CHW];
     int64_t n_miss = 0, n_slot = 0, n_cand = 0;
-    StagedLists<LS_WAVES, LS_CHW> st(sbuf);
+    StagedLists<LS_WAVES, CHW> st(sbuf);
     if (blockIdx.x == 0 && threadIdx.x < 8) next_counters[threadIdx.x] = 0;
     const int lane = threadIdx.x & (WAVE - 1);
     const int wv = threadIdx.x / WAVE;
-    const int64_t wbase = ((int64_t)blockIdx.x * LS_WAVES + wv) * LS_CHW;
+    const int64_t wbase = ((int64_t)blockIdx.x * LS_WAVES + wv) * CHW;
     const uint64_t lt = (1ull << lane) - 1ull;
     // 1. the cached destinations and locks of all passes, loaded together
     part_t cached[NPASS];
@@ -1577,8 +1582,11 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
 void Refiner::build_tiers(const Level& L) {
     DGraph g = L.view();
     fill_i32(counters_ + 16, 0, 4, s_);
+    // on small levels every row with edges takes the wave tier: a thread per
+    // row walks its edges one after another, a wave inserts them all at once
+    small_max_ = g.n <= SMALL_LEVEL_N ? 0 : SMALL_DEG;
     k_conn_tiers<<<grid_for(g.n, 256, 8192), 256, 0, s_>>>(g, p_.k, tier_mid_, tier_big_, pull_mid_, counters_ + 16,
-                                                            hub_);
+                                                            hub_, small_max_);
     k_max_degree<<<grid_for(g.n, 256, 512), 256, 0, s_>>>(g, counters_ + 18);
     JET_LAUNCH_CHECK();
     int32_t c[4];
@@ -1602,7 +1610,8 @@ void Refiner::init_conn(const Level& L, const part_t* part, const int32_t* mark)
         fill_i32(lock_, 0, g.n, s_);
     }
     k_conn_build_small<<<grid_for(g.n, SB_BLOCK, 8192), SB_BLOCK, 0, s_>>>(g, part, p_.k, conn_off_, conn_,
-                                                                          conn_size_, mk, dest_cache_, nullptr);
+                                                                          conn_size_, mk, dest_cache_, nullptr,
+                                                                          small_max_);
     JET_LAUNCH_CHECK();
     if (ntier_mid_ > 0) {
         const size_t lds = 2 * sizeof(int2) * (size_t)WB_WAVES * std::min(p_.k, WAVE_TABLE_MAX);
@@ -1622,10 +1631,20 @@ void Refiner::init_conn(const Level& L, const part_t* part, const int32_t* mark)
 void Refiner::jet_lp(const Level& L, const part_t* part, double fr) {
     DGraph g = L.view();
     int32_t* c = cnt();
-    int nb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_WAVES * LS_CHW));
-    k_lp_select<<<nb, LS_WAVES * WAVE, 0, s_>>>(g, part, conn_off_, conn_, conn_size_, dest_cache_, gain_p_, lock_, fr,
-                                                dest_, pregain_, cand_, c, nmax_, cnt_next(),
-                                                prof_counters(PF_LP_SELECT), prof_span(PF_LP_SELECT));
+    // a pass per wave on levels that fill the GPU with single-pass waves,
+    // four passes (fewer workgroups, list appends) above that
+    if (g.n <= (int64_t)LS_ONE_PASS_WAVES * WAVE) {
+        int nb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_WAVES * WAVE));
+        k_lp_select<WAVE><<<nb, LS_WAVES * WAVE, 0, s_>>>(g, part, conn_off_, conn_, conn_size_, dest_cache_, gain_p_,
+                                                          lock_, fr, dest_, pregain_, cand_, c, nmax_, cnt_next(),
+                                                          prof_counters(PF_LP_SELECT), prof_span(PF_LP_SELECT));
+    } else {
+        int nb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_WAVES * LS_CHW));
+        k_lp_select<LS_CHW><<<nb, LS_WAVES * WAVE, 0, s_>>>(g, part, conn_off_, conn_, conn_size_, dest_cache_,
+                                                            gain_p_, lock_, fr, dest_, pregain_, cand_, c, nmax_,
+                                                            cnt_next(), prof_counters(PF_LP_SELECT),
+                                                            prof_span(PF_LP_SELECT));
+    }
     JET_LAUNCH_CHECK();
     // per vertex: cache read 4, destination write 4, lock read 4
     prof_account(PF_LP_SELECT, 12.0 * g.n);
