@@ -1761,10 +1761,10 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
     k_reset_buckets<<<grid_for(nbk + 1, 256, 2048), 256, 0, s_>>>(buckets_, nbk + 1, cnt_next(), sizes, k,
                                                                     max_dest, strong ? 1 : 0, undersized_, misc_, acc_,
                                                                     evict_adjust_);
-    // contiguous ranges of at least 512 vertices per workgroup: vertices of
-    // one oversized part share few buckets, and the serial per-vertex chain
-    // of a heavy range stays short
-    int nsel = grid_for(g.n, 512, 4096);
+    // contiguous ranges of at least 512 vertices per workgroup (256 on small
+    // levels, one per thread): vertices of one oversized part share few
+    // buckets, and the serial per-vertex chain of a heavy range stays short
+    int nsel = grid_for(g.n, g.n <= SMALL_LEVEL_N ? 256 : 512, 4096);
     unsigned long long* bk = reinterpret_cast<unsigned long long*>(buckets_);
     unsigned long long* bc = reinterpret_cast<unsigned long long*>(bcopies_);
     // about 256 workgroups per copy: fewer workgroups meet on a line without
