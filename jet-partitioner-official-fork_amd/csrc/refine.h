@@ -129,7 +129,7 @@ class Refiner {
     // bucket members, [6] update_large chosen
     int32_t* counters_;
     int par_ = 0;
-    int64_t* acc_;  // [0] cut change 1, [1] cut change 2, [2] max part, [3] strong capacity / re-draw counter, [4] ticket
+    int64_t* acc_;  // [0] cut change part 1, [3] strong capacity / re-draw counter
     int64_t* curr_sizes_;
     int64_t* best_sizes_;
     int64_t* evict_adjust_;
@@ -141,7 +141,7 @@ class Refiner {
     int32_t small_max_ = 0;         // rows of degree <= this take the thread-per-row table build
     bool rebuild_pending_ = false;  // update_large marked rows still to rebuild
     Mailbox* mailbox_ = nullptr;    // the creating thread's mailbox (results and sequence words)
-    int64_t* pub_ = nullptr;      // per-group tickets and cut-change partials of the publishing kernel
+    int64_t* pub_ = nullptr;      // cut change part 2: PUB_GROUPS partials, 128 bytes apart
     int64_t* bcopies_ = nullptr;  // ncopies_ - 1 more bucket arrays, kept zero between uses
     int32_t ncopies_ = 1;
     int64_t nbuckets_ = 0;
