@@ -116,11 +116,14 @@ class Refiner {
     vid_t* tier_mid_;
     vid_t* tier_big_;
     // rows of degree > BIG_DEG ("hubs": tier_big_ and pull_mid_) take their
-    // update_small changes through k_update_hubs: flags per vertex for hub,
-    // hub with a moved neighbour, and moved in this iteration
+    // update_small changes through k_update_hubs: flags per vertex for hub
+    // and hub with a moved neighbour
     uint8_t* hub_;
     uint8_t* hub_dirty_;
-    uint8_t* moved_;
+    // stamp_[v] == gen_: v is in the current step's move list (the move-list
+    // producers write it; every selection step takes a new generation)
+    int32_t* stamp_;
+    int32_t gen_ = 0;
     vid_t* pull_mid_;  // mid-tier rows of degree > BIG_DEG
     int32_t ntier_mid_ = 0, ntier_big_ = 0, npull_mid_ = 0;
     bool has_big_ = false;
@@ -129,7 +132,7 @@ class Refiner {
     // bucket members, [6] update_large chosen
     int32_t* counters_;
     int par_ = 0;
-    int64_t* acc_;  // [0] cut change part 1, [3] strong capacity / re-draw counter
+    int64_t* acc_;  // [3] strong capacity / re-draw counter
     int64_t* curr_sizes_;
     int64_t* best_sizes_;
     int64_t* evict_adjust_;

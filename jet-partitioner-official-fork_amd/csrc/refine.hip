@@ -525,7 +525,8 @@ constexpr int AB_CHW = 256;
 template <int G>
 __global__ void __launch_bounds__(AB_WAVES* WAVE)
     k_afterburner_small(DGraph g, const vid_t* cand, const int32_t* counters, const part_t* part, const part_t* dest,
-                        const int32_t* pregain, int32_t* lock, vid_t* moves, int32_t* mcounters, vid_t nmax) {
+                        const int32_t* pregain, int32_t* lock, vid_t* moves, int32_t* mcounters, vid_t nmax,
+                        int32_t* stamp, int32_t gen) {
     __shared__ vid_t sbuf[AB_WAVES * AB_CHW];
     __shared__ int32_t sh[2 * AB_WAVES + 2];
     StagedLists<AB_WAVES, AB_CHW> st(sbuf);
@@ -557,12 +558,12 @@ __global__ void __launch_bounds__(AB_WAVES* WAVE)
         }
         st.push(keep && sub == 0, false, i);
     }
-    st.flush(moves, &mcounters[2], &mcounters[3], nmax, sh);
+    st.flush_stamped(moves, &mcounters[2], &mcounters[3], nmax, sh, stamp, gen);
 }
 
 __global__ void k_afterburner_big(DGraph g, const vid_t* cand, const int32_t* counters, const part_t* part,
                                   const part_t* dest, const int32_t* pregain, int32_t* lock, vid_t* moves,
-                                  int32_t* mcounters, vid_t nmax) {
+                                  int32_t* mcounters, vid_t nmax, int32_t* stamp, int32_t gen) {
     const int32_t nb = counters[1];
     const int lane = threadIdx.x & (WAVE - 1);
     const int64_t wstride = ((int64_t)gridDim.x * blockDim.x) / WAVE;
@@ -577,6 +578,7 @@ __global__ void k_afterburner_big(DGraph g, const vid_t* cand, const int32_t* co
         change = wave_reduce_sum(change);
         if (lane == 0 && (int64_t)ig + change >= 0) {
             lock[i] = 1;
+            stamp[i] = gen;
             if (re - rs > BIG_DEG) moves[nmax - 1 - atomicAdd(&mcounters[3], 1)] = i;
             else moves[atomicAdd(&mcounters[2], 1)] = i;
         }
@@ -584,49 +586,6 @@ __global__ void k_afterburner_big(DGraph g, const vid_t* cand, const int32_t* co
 }
 
 // ---------------------------------------------------------------- perform_moves
-// jet_refiner.hpp:905-955.
-__global__ void k_moves_apply(DGraph g, const vid_t* moves, int32_t* mc, vid_t nmax, const wgt_t* vw, part_t* part,
-                              part_t* dest, part_t* dest_cache, const eoff_t* conn_off, const int2* conn,
-                              const int32_t* conn_size, unsigned long long* sizes, int32_t k, int64_t* acc,
-                              uint8_t* moved, const int32_t* deferred) {
-    if (*deferred) return;
-    __shared__ int64_t red[256 / WAVE];
-    __shared__ unsigned long long ls[LDS_K];
-    const bool lds = k <= LDS_K;
-    if (lds) {
-        for (int t = threadIdx.x; t < k; t += blockDim.x) ls[t] = 0;
-        __syncthreads();
-    }
-    const int32_t ns = mc[2], total = mc[2] + mc[3];
-    int64_t cc = 0;
-    FOR_THREAD(x, total) {
-        vid_t v = list_at(moves, ns, nmax, x);
-        part_t p = part[v], b = dest[v];
-        const int2* tab = conn + conn_off[v];
-        int32_t sz = conn_size[v];
-        cc += (int64_t)lookup(tab, sz, b) - lookup(tab, sz, p);
-        dest_cache[v] = NULL_PART;
-        unsigned long long w = (unsigned long long)vw[v];
-        if (lds) {
-            atomicAdd(&ls[p], (unsigned long long)(-(long long)w));
-            atomicAdd(&ls[b], w);
-        } else {
-            atomicAdd(&sizes[p], (unsigned long long)(-(long long)w));
-            atomicAdd(&sizes[b], w);
-        }
-        part[v] = b;
-        dest[v] = p;
-        if (moved) moved[v] = 1;
-    }
-    cc = block_reduce_sum<int64_t, 256>(cc, red);
-    if (threadIdx.x == 0 && cc) atomicAdd((unsigned long long*)&acc[0], (unsigned long long)cc);
-    if (lds) {
-        __syncthreads();
-        for (int t = threadIdx.x; t < k; t += blockDim.x)
-            if (ls[t]) atomicAdd(&sizes[t], ls[t]);
-    }
-}
-
 // update_small subtract (jet_refiner.hpp:785-812).
 __device__ inline void conn_sub(int2* tab, int32_t vsz, part_t p, int32_t w, int32_t k, int32_t* err) {
     int32_t po = p % vsz;
@@ -730,17 +689,17 @@ __device__ inline void update_edge(vid_t v, int32_t w, part_t q, const eoff_t* c
 // choice follows the reference exactly.
 __device__ inline bool update_by_rebuild(const int32_t* mc, vid_t n) { return mc[2] + mc[3] > n / 10; }
 
-// cut change part 2 and the largest part (jet_refiner.hpp:937-948), done by
-// the subtraction pass, the last kernel whose work the results need. For a
-// moved vertex v the reference adds conn(v, new part) - conn(v, old part) from
-// the updated tables; that is the sum over v's edges (v, u) of
-// w * ([part(u) == new part of v] - [part(u) == old part of v]) with the
-// parts after the moves, which needs no table. Block 0 takes the largest
-// part; the last workgroup to finish writes (cut change, largest part, not
-// deferred) to the host mailbox. A deferred step reports only the deferral.
-// The subtraction pass adds each workgroup's part-2 sum into one of
+// The cut change and the largest part (jet_refiner.hpp:937-948). For a moved
+// vertex v the reference adds conn(v, new part) - conn(v, old part) from its
+// table before the updates and again after them; each is the sum over v's
+// edges (v, u) of w * ([part(u) == new part of v] - [part(u) == old part of v]),
+// with the parts before and after the moves, so the first pass computes both
+// from the parts and needs no table. It adds each workgroup's sum into one of
 // PUB_GROUPS partials (workgroup b into partial b mod PUB_GROUPS), 128 bytes
 // apart, so that no line takes more than a sixteenth of the grid's atomics.
+// Workgroup 0 of the second pass folds them, takes the largest part and
+// writes (cut change, largest part, deferred, rebuild) to the host mailbox. A
+// deferred step reports only the deferral.
 constexpr int PUB_GROUPS = 16;
 constexpr int PUB_STRIDE = 16;  // int64 words: 128 bytes
 
@@ -751,7 +710,7 @@ __device__ inline void add_cut_part2(int64_t* partials, int64_t c2, int64_t* red
                   (unsigned long long)c2);
 }
 
-// Run by one workgroup (256 threads) after the subtraction pass: folds the
+// Run by one workgroup (256 threads) after the first pass: folds the
 // partials, takes the largest part and writes (cut change, largest part,
 // deferred, rebuild) to the host mailbox, then the sequence word the host
 // waits for, and clears the accumulators. One workgroup needs no completion
@@ -759,7 +718,6 @@ __device__ inline void add_cut_part2(int64_t* partials, int64_t c2, int64_t* red
 // fence per workgroup, which writes back L2), and the kernel boundary before
 // it orders the other kernels' atomics.
 struct Publish {
-    int64_t* acc;
     int64_t* partials;
     const int64_t* sizes;
     int32_t k;
@@ -785,45 +743,76 @@ __device__ inline void publish_step(const Publish& pb, bool deferred, int64_t* r
     c2 = block_reduce_sum<int64_t, 256>(c2, red);
     if (threadIdx.x < PUB_GROUPS) pb.partials[(int64_t)threadIdx.x * PUB_STRIDE] = 0;
     if (threadIdx.x == 0) {
-        pb.host_out[0] = pb.acc[0] + c2;
+        pb.host_out[0] = c2;
         pb.host_out[1] = mx;
         pb.host_out[2] = 0;
         pb.host_out[3] = pb.mc[6];  // the tables are to be rebuilt (update_large)
         __threadfence_system();
         pb.host_out[5] = pb.seq;  // written after the results: the host waits for it
         __threadfence_system();
-        pb.acc[0] = 0;
     }
 }
 
-// ADD == false: subtract from the old part (held in dest), or, when the tables
-// are rebuilt instead, mark the neighbours; ADD == true: add to the new part
-// (held in part). G lanes stride the edges of one move.
+// The moves of a step, in two passes (jet_refiner.hpp:905-955 and
+// update_small/update_large, :785-884). A move v goes from part[v] to dest[v];
+// the producers of the move list stamp each listed vertex with the step's
+// generation, so the new part of any vertex u is dest[u] when stamp[u] == gen
+// and part[u] otherwise, before part[] changes.
+// ADD == false (first pass): part-size deltas of the moves, the moved
+// vertices' cached destinations cleared, the whole cut change (for each edge
+// (v, u) of a moved v: w * ([p(u) == new] - [p(u) == old]) with the parts
+// before and after the step, the reference's table lookups before and after
+// the updates), and the old part's connectivity subtracted from each
+// neighbour's table, or, when the tables are rebuilt instead (update_large),
+// the neighbours marked.
+// ADD == true (second pass): part[v] = dest[v] and the new part's
+// connectivity added (unless rebuilding); workgroup 0 publishes the step.
+// G lanes stride the edges of one move.
 template <bool ADD, int G>
 __global__ void __launch_bounds__(256)
-    k_update_small(DGraph g, const vid_t* moves, int32_t* mc, vid_t nmax, const part_t* part, const part_t* dest,
+    k_update_small(DGraph g, const vid_t* moves, int32_t* mc, vid_t nmax, part_t* part, const part_t* dest,
                    const eoff_t* conn_off, int2* conn, int32_t* conn_size, part_t* dest_cache, int32_t k,
                    int32_t* err, int32_t* mark, const uint8_t* hub, uint8_t* hub_dirty, const int32_t* deferred,
-                   int64_t* partials, Publish pb) {
+                   const int32_t* stamp, int32_t gen, unsigned long long* sizes, int64_t* partials, Publish pb) {
     __shared__ int64_t red[256 / WAVE];
-    // the addition pass's workgroup 0 publishes the step (the subtraction
-    // pass, which completes the results, has finished)
+    __shared__ unsigned long long ls[ADD ? 1 : LDS_K];
+    // the second pass's workgroup 0 publishes the step (the first pass, which
+    // completes the results, has finished)
     if (ADD && blockIdx.x == 0) publish_step(pb, *deferred != 0, red);
     if (*deferred) return;
     const bool rebuild = update_by_rebuild(mc, g.n);
-    if (ADD && rebuild) return;
-    if (!ADD && rebuild && blockIdx.x == 0 && threadIdx.x == 0) mc[6] = 1;  // gates the rebuild
+    if (!ADD && rebuild && blockIdx.x == 0 && threadIdx.x == 0) mc[6] = 1;  // the tables are to be rebuilt
+    const bool lds = !ADD && k <= LDS_K;
+    if (lds) {
+        for (int t = threadIdx.x; t < k; t += blockDim.x) ls[t] = 0;
+        __syncthreads();
+    }
     const int32_t ns = mc[2];
-    int64_t c2 = 0;
+    int64_t cc = 0;
     FOR_GROUP(x, ns) {
         vid_t i = moves[x];
-        const part_t pnew = part[i], pold = dest[i];
-        const part_t q = ADD ? pnew : pold;
+        const part_t pold = part[i], pnew = dest[i];
+        if (ADD) {
+            if (sub_ == 0) part[i] = pnew;
+            if (rebuild) continue;
+        } else if (sub_ == 0) {
+            dest_cache[i] = NULL_PART;
+            const unsigned long long w = (unsigned long long)g.vw[i];
+            if (lds) {
+                atomicAdd(&ls[pold], (unsigned long long)(-(long long)w));
+                atomicAdd(&ls[pnew], w);
+            } else {
+                atomicAdd(&sizes[pold], (unsigned long long)(-(long long)w));
+                atomicAdd(&sizes[pnew], w);
+            }
+        }
         for (eoff_t j = g.rowptr[i] + sub_; j < g.rowptr[i + 1]; j += G) {
             vid_t v = g.col[j];
             if (!ADD) {
                 const part_t pv = part[v];
-                c2 += (int64_t)g.ew[j] * ((pv == pnew ? 1 : 0) - (pv == pold ? 1 : 0));
+                const part_t nv = stamp[v] == gen ? dest[v] : pv;
+                cc += (int64_t)g.ew[j] *
+                      ((pv == pnew ? 1 : 0) - (pv == pold ? 1 : 0) + (nv == pnew ? 1 : 0) - (nv == pold ? 1 : 0));
                 if (rebuild) {
                     mark[v] = 1;
                     continue;
@@ -833,35 +822,52 @@ __global__ void __launch_bounds__(256)
                 if (!ADD) hub_dirty[v] = 1;
                 continue;
             }
-            update_edge<ADD>(v, g.ew[j], q, conn_off, conn, conn_size, dest_cache, k, err);
+            update_edge<ADD>(v, g.ew[j], ADD ? pnew : pold, conn_off, conn, conn_size, dest_cache, k, err);
         }
     }
-    if (!ADD) add_cut_part2(partials, c2, red);
+    if (!ADD) {
+        add_cut_part2(partials, cc, red);
+        if (lds) {
+            __syncthreads();
+            for (int t = threadIdx.x; t < k; t += blockDim.x)
+                if (ls[t]) atomicAdd(&sizes[t], ls[t]);
+        }
+    }
 }
 
 template <bool ADD>
 __global__ void __launch_bounds__(256)
-    k_update_big(DGraph g, const vid_t* moves, const int32_t* mc, vid_t nmax, const part_t* part, const part_t* dest,
+    k_update_big(DGraph g, const vid_t* moves, const int32_t* mc, vid_t nmax, part_t* part, const part_t* dest,
                  const eoff_t* conn_off, int2* conn, int32_t* conn_size, part_t* dest_cache, int32_t k, int32_t* err,
-                 int32_t* mark, const uint8_t* hub, uint8_t* hub_dirty, const int32_t* deferred,
-                 int64_t* partials) {
+                 int32_t* mark, const uint8_t* hub, uint8_t* hub_dirty, const int32_t* deferred, const int32_t* stamp,
+                 int32_t gen, unsigned long long* sizes, int64_t* partials) {
     __shared__ int64_t red[256 / WAVE];
     if (*deferred) return;
     const bool rebuild = update_by_rebuild(mc, g.n);
-    if (ADD && rebuild) return;
     const int32_t nb = mc[3];
     const int lane = threadIdx.x & (WAVE - 1);
     const int64_t wstride = ((int64_t)gridDim.x * blockDim.x) / WAVE;
-    int64_t c2 = 0;
+    int64_t cc = 0;
     for (int64_t x = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE; x < nb; x += wstride) {
         vid_t i = moves[nmax - 1 - x];
-        const part_t pnew = part[i], pold = dest[i];
-        const part_t q = ADD ? pnew : pold;
-        for (eoff_t j = g.rowptr[i] + lane; j < g.rowptr[i + 1]; j += WAVE) {
+        const part_t pold = part[i], pnew = dest[i];
+        if (ADD) {
+            if (lane == 0) part[i] = pnew;
+            if (rebuild) continue;
+        } else if (lane == 0) {
+            dest_cache[i] = NULL_PART;
+            const unsigned long long w = (unsigned long long)g.vw[i];
+            atomicAdd(&sizes[pold], (unsigned long long)(-(long long)w));
+            atomicAdd(&sizes[pnew], w);
+        }
+        eoff_t rs = g.rowptr[i], re = g.rowptr[i + 1];
+        for (eoff_t j = rs + lane; j < re; j += WAVE) {
             vid_t v = g.col[j];
             if (!ADD) {
                 const part_t pv = part[v];
-                c2 += (int64_t)g.ew[j] * ((pv == pnew ? 1 : 0) - (pv == pold ? 1 : 0));
+                const part_t nv = stamp[v] == gen ? dest[v] : pv;
+                cc += (int64_t)g.ew[j] *
+                      ((pv == pnew ? 1 : 0) - (pv == pold ? 1 : 0) + (nv == pnew ? 1 : 0) - (nv == pold ? 1 : 0));
                 if (rebuild) {
                     mark[v] = 1;
                     continue;
@@ -871,10 +877,10 @@ __global__ void __launch_bounds__(256)
                 if (!ADD) hub_dirty[v] = 1;
                 continue;
             }
-            update_edge<ADD>(v, g.ew[j], q, conn_off, conn, conn_size, dest_cache, k, err);
+            update_edge<ADD>(v, g.ew[j], ADD ? pnew : pold, conn_off, conn, conn_size, dest_cache, k, err);
         }
     }
-    if (!ADD) add_cut_part2(partials, c2, red);
+    if (!ADD) add_cut_part2(partials, cc, red);
 }
 
 // update_small for rows of degree > BIG_DEG ("hubs": the workgroup-built tier
@@ -888,7 +894,7 @@ __global__ void __launch_bounds__(256)
 // would land every moved neighbour's atomics on the same few hub slots.
 template <bool ADD, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
-    k_update_hubs(DGraph g, const vid_t* list, int32_t nlist, const int32_t* mc, const uint8_t* moved,
+    k_update_hubs(DGraph g, const vid_t* list, int32_t nlist, const int32_t* mc, const int32_t* stamp, int32_t gen,
                   uint8_t* hub_dirty, const part_t* part, const part_t* dest, const eoff_t* conn_off, int2* conn,
                   int32_t* conn_size, part_t* dest_cache, int32_t k, int32_t* err, const int32_t* deferred) {
     if (*deferred || update_by_rebuild(mc, g.n)) return;
@@ -900,7 +906,8 @@ __global__ void __launch_bounds__(BLOCK)
         __syncthreads();
         for (eoff_t j = g.rowptr[u] + threadIdx.x; j < g.rowptr[u + 1]; j += BLOCK) {
             vid_t v = g.col[j];
-            if (moved[v]) atomicAdd(&delta[ADD ? part[v] : dest[v]], g.ew[j]);
+            // between the passes part[v] is still the old part; the new one is dest[v]
+            if (stamp[v] == gen) atomicAdd(&delta[ADD ? dest[v] : part[v]], g.ew[j]);
         }
         __syncthreads();
         int2* tab = conn + conn_off[u];
@@ -920,14 +927,6 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
-// Clears the moved flags of this step's moves (only levels with hub rows,
-// whose pulls read them, set them).
-__global__ void k_clear_moved(const vid_t* moves, const int32_t* mc, vid_t nmax, uint8_t* moved,
-                              const int32_t* deferred) {
-    if (*deferred) return;
-    const int32_t ns = mc[2], total = mc[2] + mc[3];
-    FOR_THREAD(x, total) moved[list_at(moves, ns, nmax, x)] = 0;
-}
 
 // ---------------------------------------------------------------- rebalancing
 // Rebalancer minibuckets (jet_refiner.hpp:359-412, :449-481, :632-643).
@@ -1183,7 +1182,8 @@ __global__ void __launch_bounds__(256)
 template <bool STRONG>
 __global__ void __launch_bounds__(LS_WAVES* WAVE)
     k_classify(DGraph g, const part_t* part, const int32_t* bid, const int64_t* off, const int64_t* sizes,
-               int64_t size_max, int64_t width, vid_t* moves, int32_t* counters, vid_t nmax, vid_t* rlist) {
+               int64_t size_max, int64_t width, vid_t* moves, int32_t* counters, vid_t nmax, vid_t* rlist,
+               int32_t* stamp, int32_t gen) {
     __shared__ vid_t sbuf[LS_WAVES * LS_CHW];
     __shared__ vid_t rbuf[LS_WAVES * LS_CHW];
     __shared__ int32_t sh[2 * LS_WAVES + 2];
@@ -1216,7 +1216,7 @@ __global__ void __launch_bounds__(LS_WAVES* WAVE)
         st.push(ev && deg <= BIG_DEG, ev && deg > BIG_DEG, (vid_t)i);
         rt.push(rank, false, (vid_t)i);
     }
-    st.flush(moves, &counters[2], &counters[3], nmax, sh);
+    st.flush_stamped(moves, &counters[2], &counters[3], nmax, sh, stamp, gen);
     rt.flush(rlist, &counters[5], &counters[7], g.n, rsh);
 }
 
@@ -1226,7 +1226,7 @@ template <bool STRONG>
 __device__ inline void rank_decide(vid_t v, int32_t b, int64_t pos, const part_t* part, const int64_t* off,
                                    const int64_t* sizes, int64_t size_max, int64_t width, const wgt_t* vw,
                                    const eoff_t* rowptr, int64_t* evict_adjust, int64_t* score, vid_t* moves,
-                                   int32_t* mcounters, vid_t nmax) {
+                                   int32_t* mcounters, vid_t nmax, int32_t* stamp, int32_t gen) {
     part_t p = part[v];
     int64_t base = off[b] - off[(int64_t)p * width];
     int64_t limit = sizes[p] - size_max;
@@ -1242,6 +1242,7 @@ __device__ inline void rank_decide(vid_t v, int32_t b, int64_t pos, const part_t
         ev = base + pos < limit;
     }
     if (ev) {
+        stamp[v] = gen;
         if (rowptr[v + 1] - rowptr[v] > BIG_DEG) moves[nmax - 1 - atomicAdd(&mcounters[3], 1)] = v;
         else moves[atomicAdd(&mcounters[2], 1)] = v;
     }
@@ -1257,7 +1258,7 @@ __global__ void __launch_bounds__(RS_BLOCK)
     k_rank_small(const vid_t* rlist, const int32_t* counters, const int32_t* bid, const part_t* part,
                  const int64_t* off, const int64_t* sizes, int64_t size_max, int64_t width, const wgt_t* vw,
                  const eoff_t* rowptr, int64_t* evict_adjust, int64_t* score, vid_t* moves, int32_t* mcounters,
-                 vid_t nmax, int32_t* deferred) {
+                 vid_t nmax, int32_t* deferred, int32_t* stamp, int32_t gen) {
     __shared__ uint64_t keys[RS_CAP];
     __shared__ int64_t pre[RS_CAP];
     __shared__ int64_t ws[RS_BLOCK / WAVE + 1];
@@ -1320,7 +1321,7 @@ __global__ void __launch_bounds__(RS_BLOCK)
             else hi = mid;
         }
         rank_decide<STRONG>((vid_t)(key & 0xffffffffull), b, pre[t] - pre[lo], part, off, sizes, size_max, width, vw,
-                            rowptr, evict_adjust, score, moves, mcounters, nmax);
+                            rowptr, evict_adjust, score, moves, mcounters, nmax, stamp, gen);
     }
 }
 
@@ -1340,7 +1341,7 @@ template <bool STRONG>
 __global__ void k_rank_apply(int32_t R, const uint64_t* keys, const int64_t* pre, const part_t* part,
                              const int64_t* off, const int64_t* sizes, int64_t size_max, int64_t width,
                              const wgt_t* vw, const eoff_t* rowptr, int64_t* evict_adjust, int64_t* score,
-                             vid_t* moves, int32_t* mcounters, vid_t nmax) {
+                             vid_t* moves, int32_t* mcounters, vid_t nmax, int32_t* stamp, int32_t gen) {
     FOR_THREAD(t, R) {
         uint64_t key = keys[t];
         int32_t b = (int32_t)(key >> 32);
@@ -1352,7 +1353,7 @@ __global__ void k_rank_apply(int32_t R, const uint64_t* keys, const int64_t* pre
             else hi = mid;
         }
         rank_decide<STRONG>((vid_t)(key & 0xffffffffull), b, pre[t] - pre[lo], part, off, sizes, size_max, width, vw,
-                            rowptr, evict_adjust, score, moves, mcounters, nmax);
+                            rowptr, evict_adjust, score, moves, mcounters, nmax, stamp, gen);
     }
 }
 
@@ -1551,7 +1552,7 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
     hub_ = a.alloc<uint8_t>(N);
     hub_dirty_ = a.alloc<uint8_t>(N);
     pull_mid_ = a.alloc<vid_t>(N);
-    moved_ = a.alloc<uint8_t>(N);
+    stamp_ = a.alloc<int32_t>(N);
     counters_ = a.alloc<int32_t>(32);
     acc_ = a.alloc<int64_t>(8);
     curr_sizes_ = a.alloc<int64_t>(p.k);
@@ -1569,7 +1570,7 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
     if (ncopies_ > 1) fill_i64(bcopies_, 0, nbuckets_ * (ncopies_ - 1), s);
     fill_i32(swap_bit_, 0, N, s);
     JET_HIP(hipMemsetAsync(hub_dirty_, 0, (size_t)N, s));
-    JET_HIP(hipMemsetAsync(moved_, 0, (size_t)N, s));
+    fill_i32(stamp_, -1, N, s);
     fill_i32(counters_, 0, 32, s);
     fill_i64(acc_, 0, 8, s);
     pub_ = a.alloc<int64_t>((int64_t)PUB_GROUPS * PUB_STRIDE);
@@ -1633,6 +1634,7 @@ void Refiner::init_conn(const Level& L, const part_t* part, const int32_t* mark)
 void Refiner::jet_lp(const Level& L, const part_t* part, double fr) {
     DGraph g = L.view();
     int32_t* c = cnt();
+    gen_++;  // this step's moves are stamped with a new generation
     // a pass per wave on levels that fill the GPU with single-pass waves,
     // eight passes above that: fewer workgroups, so fewer appends to the
     // candidate counters (measured: 14.2 us per launch against 18.2 with
@@ -1654,15 +1656,16 @@ void Refiner::jet_lp(const Level& L, const part_t* part, double fr) {
     prof_account(PF_LP_SELECT, 12.0 * g.n);
     unsigned ab = (unsigned)std::max<int64_t>(256, ceil_div((int64_t)g.n, (int64_t)AB_WAVES * AB_CHW));
     switch (group_for_avg(g)) {
-        case 4: k_afterburner_small<4><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_); break;
-        case 8: k_afterburner_small<8><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_); break;
-        case 16: k_afterburner_small<16><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_); break;
-        case 32: k_afterburner_small<32><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_); break;
-        default: k_afterburner_small<64><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_); break;
+        case 4: k_afterburner_small<4><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_, stamp_, gen_); break;
+        case 8: k_afterburner_small<8><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_, stamp_, gen_); break;
+        case 16: k_afterburner_small<16><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_, stamp_, gen_); break;
+        case 32: k_afterburner_small<32><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_, stamp_, gen_); break;
+        default: k_afterburner_small<64><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_, stamp_, gen_); break;
     }
     if (has_big_)
         k_afterburner_big<<<grid_for((int64_t)g.n * WAVE, 256, 2048), 256, 0, s_>>>(g, cand_, c, part, dest_, pregain_,
-                                                                                   lock_, moves_, c, nmax_);
+                                                                                   lock_, moves_, c, nmax_, stamp_,
+                                                                                   gen_);
     JET_LAUNCH_CHECK();
 }
 
@@ -1677,10 +1680,10 @@ void Refiner::rank_flagged(const Level& L, const part_t* part, const int64_t* si
     int64_t smax = size_max_;
     if (strong)
         k_rank_small<true><<<1, RS_BLOCK, 0, s_>>>(cand_, c, bid_, part, buckets_, sizes, smax, width, g.vw, g.rowptr,
-                                                  evict_adjust_, score_, moves_, c, nmax_, misc_ + 2);
+                                                  evict_adjust_, score_, moves_, c, nmax_, misc_ + 2, stamp_, gen_);
     else
         k_rank_small<false><<<1, RS_BLOCK, 0, s_>>>(cand_, c, bid_, part, buckets_, sizes, smax, width, g.vw,
-                                                   g.rowptr, evict_adjust_, score_, moves_, c, nmax_, misc_ + 2);
+                                                   g.rowptr, evict_adjust_, score_, moves_, c, nmax_, misc_ + 2, stamp_, gen_);
     JET_LAUNCH_CHECK();
 }
 
@@ -1703,10 +1706,10 @@ void Refiner::rank_radix(const Level& L, const part_t* part, const int64_t* size
     excl_scan_i64_i64(prefix_, prefix_, R, a_, s_);
     if (strong)
         k_rank_apply<true><<<rb, 256, 0, s_>>>(R, keys, prefix_, part, buckets_, sizes, smax, width, g.vw, g.rowptr,
-                                              evict_adjust_, score_, moves_, c, nmax_);
+                                              evict_adjust_, score_, moves_, c, nmax_, stamp_, gen_);
     else
         k_rank_apply<false><<<rb, 256, 0, s_>>>(R, keys, prefix_, part, buckets_, sizes, smax, width, g.vw, g.rowptr,
-                                               evict_adjust_, score_, moves_, c, nmax_);
+                                               evict_adjust_, score_, moves_, c, nmax_, stamp_, gen_);
     JET_LAUNCH_CHECK();
 }
 
@@ -1762,6 +1765,7 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
     const int32_t sections = r.sections;
     const int64_t width = r.width, nbk = r.nbk, max_dest = r.max_dest;
     const int64_t smax = size_max_;
+    gen_++;  // this step's moves are stamped with a new generation
     k_reset_buckets<<<grid_for(nbk + 1, 256, 2048), 256, 0, s_>>>(buckets_, nbk + 1, cnt_next(), sizes, k,
                                                                     max_dest, strong ? 1 : 0, undersized_, misc_, acc_,
                                                                     evict_adjust_);
@@ -1800,10 +1804,10 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
     int cb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_WAVES * LS_CHW));
     if (strong)
         k_classify<true><<<cb, LS_WAVES * WAVE, 0, s_>>>(g, part, bid_, buckets_, sizes, smax, width, moves_, c, nmax_,
-                                                         cand_);
+                                                         cand_, stamp_, gen_);
     else
         k_classify<false><<<cb, LS_WAVES * WAVE, 0, s_>>>(g, part, bid_, buckets_, sizes, smax, width, moves_, c,
-                                                          nmax_, cand_);
+                                                          nmax_, cand_, stamp_, gen_);
     JET_LAUNCH_CHECK();
     rank_flagged(L, part, sizes, width, strong);
     if (strong) strong_tail(L, part, sizes, r);
@@ -1834,28 +1838,21 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
     DGraph g = L.view();
     const int32_t k = p_.k;
     int32_t* c = cnt();
-    // the moved flags feed only the hub pulls
-    const bool hubs = ntier_big_ > 0 || npull_mid_ > 0;
-    // a small grid-stride grid: the moves are usually few, and every extra
-    // workgroup costs a flush of its part-size deltas
-    int nb = grid_for(g.n, 256, 128);
-    k_moves_apply<<<nb, 256, 0, s_>>>(g, moves_, c, nmax_, g.vw, part, dest_, dest_cache_, conn_off_, conn_,
-                                      conn_size_, reinterpret_cast<unsigned long long*>(sizes), k, acc_,
-                                      hubs ? moved_ : nullptr, misc_ + 2);
-    JET_LAUNCH_CHECK();
+    unsigned long long* usz = reinterpret_cast<unsigned long long*>(sizes);
     int G = group_for_avg(g);
     int nbs = grid_for((int64_t)g.n * G, 256, 1024);
     const int64_t seq = ++mailbox_->seq;
-    const Publish pb{acc_, pub_, sizes, k, res_dev_, c, seq};
+    const Publish pb{pub_, sizes, k, res_dev_, c, seq};
     int nbb = grid_for((int64_t)g.n * WAVE, 256, 512);
     for (int add = 0; add < 2; add++) {
 #define JET_UPD(GG)                                                                                                   \
     if (add) k_update_small<true, GG><<<nbs, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,       \
                                                           conn_size_, dest_cache_, k, misc_ + 1, swap_bit_,      \
-                                                          hub_, hub_dirty_, misc_ + 2, pub_, pb);               \
+                                                          hub_, hub_dirty_, misc_ + 2, stamp_, gen_, usz, pub_,  \
+                                                          pb);                                                   \
     else k_update_small<false, GG><<<nbs, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,          \
                                                         conn_size_, dest_cache_, k, misc_ + 1, swap_bit_,        \
-                                                        hub_, hub_dirty_, misc_ + 2, pub_, pb);
+                                                        hub_, hub_dirty_, misc_ + 2, stamp_, gen_, usz, pub_, pb);
         switch (G) {
             case 4: JET_UPD(4) break;
             case 8: JET_UPD(8) break;
@@ -1867,36 +1864,33 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
         if (has_big_) {
             if (add) k_update_big<true><<<nbb, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,
                                                              conn_size_, dest_cache_, k, misc_ + 1, swap_bit_, hub_,
-                                                             hub_dirty_, misc_ + 2, pub_);
+                                                             hub_dirty_, misc_ + 2, stamp_, gen_, usz, pub_);
             else k_update_big<false><<<nbb, 256, 0, s_>>>(g, moves_, c, nmax_, part, dest_, conn_off_, conn_,
                                                            conn_size_, dest_cache_, k, misc_ + 1, swap_bit_, hub_,
-                                                           hub_dirty_, misc_ + 2, pub_);
+                                                           hub_dirty_, misc_ + 2, stamp_, gen_, usz, pub_);
         }
         // pulled rows: big-tier rows by a 256-thread workgroup, rows of
         // degree (BIG_DEG, HEAVY_DEG_REFINE] by a single-wave workgroup
         const size_t lds = sizeof(int32_t) * (size_t)k;
         if (ntier_big_ > 0) {
             const int hb = std::min(ntier_big_, 2048);
-            if (add) k_update_hubs<true, 256><<<hb, 256, lds, s_>>>(g, tier_big_, ntier_big_, c, moved_, hub_dirty_,
-                                                                    part, dest_, conn_off_, conn_, conn_size_,
-                                                                    dest_cache_, k, misc_ + 1, misc_ + 2);
-            else k_update_hubs<false, 256><<<hb, 256, lds, s_>>>(g, tier_big_, ntier_big_, c, moved_, hub_dirty_,
-                                                                 part, dest_, conn_off_, conn_, conn_size_,
-                                                                 dest_cache_, k, misc_ + 1, misc_ + 2);
+            if (add) k_update_hubs<true, 256><<<hb, 256, lds, s_>>>(g, tier_big_, ntier_big_, c, stamp_, gen_,
+                                                                    hub_dirty_, part, dest_, conn_off_, conn_,
+                                                                    conn_size_, dest_cache_, k, misc_ + 1, misc_ + 2);
+            else k_update_hubs<false, 256><<<hb, 256, lds, s_>>>(g, tier_big_, ntier_big_, c, stamp_, gen_,
+                                                                 hub_dirty_, part, dest_, conn_off_, conn_,
+                                                                 conn_size_, dest_cache_, k, misc_ + 1, misc_ + 2);
         }
         if (npull_mid_ > 0) {
             const int hb = std::min(npull_mid_, 8192);
-            if (add) k_update_hubs<true, WAVE><<<hb, WAVE, lds, s_>>>(g, pull_mid_, npull_mid_, c, moved_, hub_dirty_,
-                                                                      part, dest_, conn_off_, conn_, conn_size_,
-                                                                      dest_cache_, k, misc_ + 1, misc_ + 2);
-            else k_update_hubs<false, WAVE><<<hb, WAVE, lds, s_>>>(g, pull_mid_, npull_mid_, c, moved_, hub_dirty_,
-                                                                   part, dest_, conn_off_, conn_, conn_size_,
-                                                                   dest_cache_, k, misc_ + 1, misc_ + 2);
+            if (add) k_update_hubs<true, WAVE><<<hb, WAVE, lds, s_>>>(g, pull_mid_, npull_mid_, c, stamp_, gen_,
+                                                                      hub_dirty_, part, dest_, conn_off_, conn_,
+                                                                      conn_size_, dest_cache_, k, misc_ + 1,
+                                                                      misc_ + 2);
+            else k_update_hubs<false, WAVE><<<hb, WAVE, lds, s_>>>(g, pull_mid_, npull_mid_, c, stamp_, gen_,
+                                                                   hub_dirty_, part, dest_, conn_off_, conn_,
+                                                                   conn_size_, dest_cache_, k, misc_ + 1, misc_ + 2);
         }
-        JET_LAUNCH_CHECK();
-    }
-    if (hubs) {
-        k_clear_moved<<<nb, 256, 0, s_>>>(moves_, c, nmax_, moved_, misc_ + 2);
         JET_LAUNCH_CHECK();
     }
     volatile int64_t* res = res_host_;

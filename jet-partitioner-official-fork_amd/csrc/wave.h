@@ -124,6 +124,14 @@ struct StagedLists {
         for (int32_t t = lane; t < nf; t += WAVE) list[bf + t] = buf[t];
         for (int32_t t = lane; t < nb; t += WAVE) list[nmax - 1 - (bb + t)] = buf[CHW - 1 - t];
     }
+    // flush() that also writes stamp[v] = gen for every listed v
+    __device__ inline void flush_stamped(vid_t* list, int32_t* cnt_front, int32_t* cnt_back, vid_t nmax, int32_t* sh,
+                                         int32_t* stamp, int32_t gen) {
+        flush(list, cnt_front, cnt_back, nmax, sh);
+        const int lane = threadIdx.x & (WAVE - 1);
+        for (int32_t t = lane; t < nf; t += WAVE) stamp[buf[t]] = gen;
+        for (int32_t t = lane; t < nb; t += WAVE) stamp[buf[CHW - 1 - t]] = gen;
+    }
 };
 
 template <class T>
