@@ -586,16 +586,35 @@ __global__ void k_afterburner_big(DGraph g, const vid_t* cand, const int32_t* co
 }
 
 // ---------------------------------------------------------------- perform_moves
-// update_small subtract (jet_refiner.hpp:785-812).
+__device__ inline int32_t next_slot(int32_t s, int32_t vsz) { return s + 1 == vsz ? 0 : s + 1; }
+
+// update_small subtract (jet_refiner.hpp:785-812). The linear probe for p
+// reads four slots per round trip; during the subtraction pass the slot that
+// holds p does not move (only parts whose count reaches 0 are reclaimed), so
+// the slot found is the one the one-at-a-time probe finds.
 __device__ inline void conn_sub(int2* tab, int32_t vsz, part_t p, int32_t w, int32_t k, int32_t* err) {
     int32_t po = p % vsz;
-    int32_t guard = 0;
-    while (ld_rx(&tab[po].x) != p) {
-        po = (po + 1) % vsz;
-        if (++guard > vsz) {
+    for (int32_t probed = 0;; probed += 4) {
+        if (probed > vsz) {
             atomicOr(err, 1);
             return;
         }
+        const int32_t a = po, b = next_slot(a, vsz), c = next_slot(b, vsz), d = next_slot(c, vsz);
+        const int32_t xa = ld_rx(&tab[a].x), xb = ld_rx(&tab[b].x), xc = ld_rx(&tab[c].x), xd = ld_rx(&tab[d].x);
+        if (xa == p) break;
+        if (xb == p) {
+            po = b;
+            break;
+        }
+        if (xc == p) {
+            po = c;
+            break;
+        }
+        if (xd == p) {
+            po = d;
+            break;
+        }
+        po = next_slot(d, vsz);
     }
     int32_t x = atomicAdd(&tab[po].y, -w);
     if (vsz < k && x == w) __hip_atomic_store(&tab[po].x, HASH_RECLAIM, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -606,16 +625,28 @@ __device__ inline void conn_add(int2* tab, int32_t* vsize, int32_t cap, part_t b
     int32_t vsz = ld_rx(vsize);
     int32_t po = best % vsz;
     bool success = false;
-    for (int32_t q = 0; q < vsz; q++) {
-        int32_t pi = (best + q) % vsz;
-        int32_t e = ld_rx(&tab[pi].x);
-        if (e == best) {
-            success = true;
-            po = pi;
-            break;
-        } else if (e == NULL_PART) {
-            break;
+    // the search for best or an empty slot, four slots per round trip (the
+    // values are those a one-at-a-time probe could read: other threads only
+    // fill empty or reclaimed slots meanwhile)
+    for (int32_t q = 0, s0 = po; q < vsz; q += 4) {
+        int32_t sl[4], e[4];
+        sl[0] = s0;
+        for (int j = 1; j < 4; j++) sl[j] = next_slot(sl[j - 1], vsz);
+        for (int j = 0; j < 4; j++) e[j] = ld_rx(&tab[sl[j]].x);
+        bool stop = false;
+        for (int j = 0; j < 4 && !stop; j++) {
+            if (q + j >= vsz) {
+                stop = true;
+            } else if (e[j] == best) {
+                success = true;
+                po = sl[j];
+                stop = true;
+            } else if (e[j] == NULL_PART) {
+                stop = true;
+            }
         }
+        if (stop) break;
+        s0 = next_slot(sl[3], vsz);
     }
     int32_t count = 0;
     while (!success && count < vsz) {
@@ -666,18 +697,6 @@ __device__ inline void conn_add(int2* tab, int32_t* vsize, int32_t cap, part_t b
         }
     }
     atomicAdd(&tab[po].y, w);
-}
-
-template <bool ADD>
-__device__ inline void update_edge(vid_t v, int32_t w, part_t q, const eoff_t* conn_off, int2* conn,
-                                   int32_t* conn_size, part_t* dest_cache, int32_t k, int32_t* err) {
-    int2* tab = conn + conn_off[v];
-    if (ADD) {
-        dest_cache[v] = NULL_PART;
-        conn_add(tab, &conn_size[v], (int32_t)(conn_off[v + 1] - conn_off[v]), q, w, err);
-    } else {
-        conn_sub(tab, ld_rx(&conn_size[v]), q, w, k, err);
-    }
 }
 
 // perform_moves' choice between update_small (push each moved vertex's
@@ -807,22 +826,37 @@ __global__ void __launch_bounds__(256)
             }
         }
         for (eoff_t j = g.rowptr[i] + sub_; j < g.rowptr[i + 1]; j += G) {
-            vid_t v = g.col[j];
+            const vid_t v = g.col[j];
+            const int32_t w = g.ew[j];
             if (!ADD) {
-                const part_t pv = part[v];
-                const part_t nv = stamp[v] == gen ? dest[v] : pv;
-                cc += (int64_t)g.ew[j] *
-                      ((pv == pnew ? 1 : 0) - (pv == pold ? 1 : 0) + (nv == pnew ? 1 : 0) - (nv == pold ? 1 : 0));
+                // loaded together: the neighbour's parts and table
+                const part_t pv = part[v], dv = dest[v];
+                const bool sv = stamp[v] == gen;
                 if (rebuild) {
+                    const part_t nv = sv ? dv : pv;
+                    cc += (int64_t)w * ((pv == pnew ? 1 : 0) - (pv == pold ? 1 : 0) + (nv == pnew ? 1 : 0) -
+                                        (nv == pold ? 1 : 0));
                     mark[v] = 1;
                     continue;
                 }
+                const uint8_t hv = hub[v];
+                const eoff_t co = conn_off[v];
+                const int32_t cs = ld_rx(&conn_size[v]);
+                const part_t nv = sv ? dv : pv;
+                cc += (int64_t)w *
+                      ((pv == pnew ? 1 : 0) - (pv == pold ? 1 : 0) + (nv == pnew ? 1 : 0) - (nv == pold ? 1 : 0));
+                if (hv) {  // updated by k_update_hubs
+                    hub_dirty[v] = 1;
+                    continue;
+                }
+                conn_sub(conn + co, cs, pold, w, k, err);
+            } else {
+                const uint8_t hv = hub[v];
+                const eoff_t co = conn_off[v], ce = conn_off[v + 1];
+                if (hv) continue;  // updated by k_update_hubs
+                dest_cache[v] = NULL_PART;
+                conn_add(conn + co, &conn_size[v], (int32_t)(ce - co), pnew, w, err);
             }
-            if (hub[v]) {  // updated by k_update_hubs
-                if (!ADD) hub_dirty[v] = 1;
-                continue;
-            }
-            update_edge<ADD>(v, g.ew[j], ADD ? pnew : pold, conn_off, conn, conn_size, dest_cache, k, err);
         }
     }
     if (!ADD) {
@@ -862,22 +896,37 @@ __global__ void __launch_bounds__(256)
         }
         eoff_t rs = g.rowptr[i], re = g.rowptr[i + 1];
         for (eoff_t j = rs + lane; j < re; j += WAVE) {
-            vid_t v = g.col[j];
+            const vid_t v = g.col[j];
+            const int32_t w = g.ew[j];
             if (!ADD) {
-                const part_t pv = part[v];
-                const part_t nv = stamp[v] == gen ? dest[v] : pv;
-                cc += (int64_t)g.ew[j] *
-                      ((pv == pnew ? 1 : 0) - (pv == pold ? 1 : 0) + (nv == pnew ? 1 : 0) - (nv == pold ? 1 : 0));
+                // loaded together: the neighbour's parts and table
+                const part_t pv = part[v], dv = dest[v];
+                const bool sv = stamp[v] == gen;
                 if (rebuild) {
+                    const part_t nv = sv ? dv : pv;
+                    cc += (int64_t)w * ((pv == pnew ? 1 : 0) - (pv == pold ? 1 : 0) + (nv == pnew ? 1 : 0) -
+                                        (nv == pold ? 1 : 0));
                     mark[v] = 1;
                     continue;
                 }
+                const uint8_t hv = hub[v];
+                const eoff_t co = conn_off[v];
+                const int32_t cs = ld_rx(&conn_size[v]);
+                const part_t nv = sv ? dv : pv;
+                cc += (int64_t)w *
+                      ((pv == pnew ? 1 : 0) - (pv == pold ? 1 : 0) + (nv == pnew ? 1 : 0) - (nv == pold ? 1 : 0));
+                if (hv) {  // updated by k_update_hubs
+                    hub_dirty[v] = 1;
+                    continue;
+                }
+                conn_sub(conn + co, cs, pold, w, k, err);
+            } else {
+                const uint8_t hv = hub[v];
+                const eoff_t co = conn_off[v], ce = conn_off[v + 1];
+                if (hv) continue;  // updated by k_update_hubs
+                dest_cache[v] = NULL_PART;
+                conn_add(conn + co, &conn_size[v], (int32_t)(ce - co), pnew, w, err);
             }
-            if (hub[v]) {  // updated by k_update_hubs
-                if (!ADD) hub_dirty[v] = 1;
-                continue;
-            }
-            update_edge<ADD>(v, g.ew[j], ADD ? pnew : pold, conn_off, conn, conn_size, dest_cache, k, err);
         }
     }
     if (!ADD) add_cut_part2(partials, cc, red);
@@ -1430,19 +1479,30 @@ __global__ void k_strong_dest(const vid_t* moves, const int32_t* mc, vid_t nmax,
 }
 
 // Re-draws in increasing vertex id, one after another (the counter starts at
-// the total eviction weight).
+// the total eviction weight). The serial walk reads only LDS: the chunk
+// starts (k + 1 <= RD_ES_MAX) and, a batch of 256 at a time, the re-drawn
+// vertices' scores and weights are staged there first.
 constexpr int RD_MAX = 8192;
+constexpr int RD_ES_MAX = 2049;
 __global__ void __launch_bounds__(256) k_strong_redraw(vid_t* redraw, const int32_t* rcount, const part_t* part,
                                                        const wgt_t* vw, const int64_t* evict_start, int32_t k,
                                                        const int64_t* score, part_t* dest, int64_t* acc, int32_t* err,
                                                        const int32_t* deferred) {
     __shared__ vid_t ids[RD_MAX];
+    __shared__ int64_t es_lds[RD_ES_MAX];
+    __shared__ int64_t bscore[256];
+    __shared__ wgt_t bw[256];
     if (*deferred) return;
     int32_t r = rcount[0];
     if (r > RD_MAX) {
         if (threadIdx.x == 0) atomicOr(err, 4);
         return;
     }
+    if (r == 0) return;
+    const bool es_in_lds = k + 1 <= RD_ES_MAX;
+    if (es_in_lds)
+        for (int32_t t = threadIdx.x; t <= k; t += blockDim.x) es_lds[t] = evict_start[t];
+    const int64_t* es = es_in_lds ? es_lds : evict_start;
     uint32_t cap = 1;
     while ((int32_t)cap < r) cap <<= 1;
     for (uint32_t t = threadIdx.x; t < cap; t += blockDim.x) ids[t] = (int32_t)t < r ? redraw[t] : ORD_MAX;
@@ -1462,27 +1522,39 @@ __global__ void __launch_bounds__(256) k_strong_redraw(vid_t* redraw, const int3
         }
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    int64_t counter = acc[3];
-    for (int32_t x = 0; x < r; x++) {
-        vid_t v = ids[x];
-        int64_t s = score[v];
-        int32_t p = 0;
-        bool done = false;
-        while (p < k) {
-            p = chunk_of(evict_start, p, k, s);
-            if (p < k && vw[v] / 2 <= evict_start[p + 1] - s) {
-                dest[v] = p;
-                done = true;
-                break;
-            } else if (p < k) {
-                s = counter;
-                counter += vw[v];
+    int64_t counter = acc[3];  // carried by thread 0
+    for (int32_t base = 0; base < r; base += 256) {
+        const int32_t nb = r - base < 256 ? r - base : 256;
+        if ((int32_t)threadIdx.x < nb) {
+            const vid_t v = ids[base + threadIdx.x];
+            bscore[threadIdx.x] = score[v];
+            bw[threadIdx.x] = vw[v];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int32_t x = 0; x < nb; x++) {
+                const vid_t v = ids[base + x];
+                const wgt_t wv = bw[x];
+                int64_t sc = bscore[x];
+                int32_t p = 0;
+                bool done = false;
+                while (p < k) {
+                    p = chunk_of(es, p, k, sc);
+                    if (p < k && wv / 2 <= es[p + 1] - sc) {
+                        dest[v] = p;
+                        done = true;
+                        break;
+                    } else if (p < k) {
+                        sc = counter;
+                        counter += wv;
+                    }
+                }
+                if (!done) dest[v] = part[v];
             }
         }
-        if (!done) dest[v] = part[v];
+        __syncthreads();
     }
-    acc[3] = counter;
+    if (threadIdx.x == 0) acc[3] = counter;
 }
 
 __global__ void k_collect_moves(const vid_t* moves, const int32_t* mc, vid_t nmax, const part_t* dest, vid_t* out_v,
