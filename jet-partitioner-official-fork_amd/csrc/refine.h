@@ -26,6 +26,8 @@ struct RefineParams {
     bool ultra = false;
 };
 
+struct MoveRec;
+
 // Holds the refiner's scratch (sized by the finest level, jet_refiner.hpp:134-181)
 // and the refine_data persisting across levels (jet_refiner.hpp:100-106).
 class Refiner {
@@ -107,6 +109,7 @@ class Refiner {
     int32_t* bid_;
     vid_t* cand_;
     vid_t* moves_;
+    MoveRec* mrec_;  // per listed move: row, vertex and new part (first pass to second)
     int32_t* swap_bit_;
     uint32_t* sval_;
     uint32_t* vtmp_;
