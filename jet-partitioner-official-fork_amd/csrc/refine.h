@@ -144,6 +144,8 @@ class Refiner {
     int32_t* misc_;  // [0] undersized count, [1] error bits, [2] ranking deferred
     vid_t* redraw_;
     int64_t* buckets_;
+    int64_t* boff_;        // the scan of a small bucket array, written by the classification kernel
+    const int64_t* off_ = nullptr;  // the current rebalance step's scanned buckets (buckets_ or boff_)
     int32_t small_max_ = 0;         // rows of degree <= this take the thread-per-row table build
     bool rebuild_pending_ = false;  // update_large marked rows still to rebuild
     Mailbox* mailbox_ = nullptr;    // the creating thread's mailbox (results and sequence words)

@@ -31,7 +31,9 @@ constexpr int32_t LDS_K = 2048;         // part-size deltas aggregated in LDS up
 constexpr int32_t HEAVY_DEG_REFINE = 2048;  // rows above this degree are built by a workgroup
 
 // ---------------------------------------------------------------- helpers
-__device__ inline int32_t ld_rx(const int32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline int32_t ld_rx(const int32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __device__ inline int32_t lookup(const int2* tab, int32_t size, part_t target) {
     for (int32_t q = 0; q < size; q++) {
@@ -329,7 +331,8 @@ __global__ void __launch_bounds__(BB_BLOCK)
         }
         if (threadIdx.x == 0) used = 0;
         __syncthreads();
-        for (eoff_t j = rs + threadIdx.x; j < re; j += BB_BLOCK) lds_table_add(T, cap, k, part[g.col[j]], g.ew[j], &used);
+        for (eoff_t j = rs + threadIdx.x; j < re; j += BB_BLOCK)
+            lds_table_add(T, cap, k, part[g.col[j]], g.ew[j], &used);
         __syncthreads();
         int32_t size = shrink_size(used, cap);
         int2* out = conn + conn_off[i];
@@ -841,7 +844,8 @@ __device__ inline void update_move(const DGraph& g, vid_t i, part_t pold, part_t
 // neighbour's table, or, when the tables are rebuilt instead (update_large),
 // the neighbours marked; each move's record for the second pass.
 // ADD == true (second pass): part[v] = new part and the new part's
-// connectivity added (unless rebuilding); workgroup 0 publishes the step.
+// connectivity added (unless rebuilding); workgroup 0 only publishes the
+// step, so that its chain of loads does not add to a chain of move updates.
 // Each kernel loads the deferral flag, the move counts and its first list
 // entry together. G lanes stride the edges of one move.
 template <bool ADD, int G>
@@ -854,8 +858,11 @@ __global__ void __launch_bounds__(256)
     __shared__ int64_t red[256 / WAVE];
     __shared__ unsigned long long ls[ADD ? 1 : LDS_K];
     const int sub = threadIdx.x & (G - 1);
-    const int64_t gstride = ((int64_t)gridDim.x * blockDim.x) / G;
-    const int64_t x0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
+    // the second pass's move workgroups are 1 .. gridDim.x - 1
+    const int64_t blk = ADD ? (int64_t)blockIdx.x - 1 : (int64_t)blockIdx.x;
+    const int64_t nblk = ADD ? (int64_t)gridDim.x - 1 : (int64_t)gridDim.x;
+    const int64_t gstride = (nblk * blockDim.x) / G;
+    const int64_t x0 = ADD && blockIdx.x == 0 ? nmax : (blk * blockDim.x + threadIdx.x) / G;
     const bool have0 = x0 < nmax;
     const vid_t i0 = !ADD && have0 ? moves[x0] : 0;
     MoveRec r0{};
@@ -864,7 +871,10 @@ __global__ void __launch_bounds__(256)
     const int32_t ns = mc[2], nbig = mc[3];
     // the second pass's workgroup 0 publishes the step (the first pass, which
     // completes the results, has finished)
-    if (ADD && blockIdx.x == 0) publish_step(pb, dfr != 0, red);
+    if (ADD && blockIdx.x == 0) {
+        publish_step(pb, dfr != 0, red);
+        return;
+    }
     if (dfr) return;
     const bool rebuild = ns + nbig > g.n / 10;  // update_by_rebuild
     if (!ADD && rebuild && blockIdx.x == 0 && threadIdx.x == 0) mc[6] = 1;  // the tables are to be rebuilt
@@ -960,9 +970,10 @@ __global__ void __launch_bounds__(256)
 }
 
 // update_small for rows of degree > BIG_DEG ("hubs": the workgroup-built tier
-// and the pull_mid rows), one workgroup per row with a moved neighbour: the row's moved neighbours are found by a scan of
-// its adjacency and their weights summed per part in LDS, then each part's
-// total is applied to the table once, with the subtraction phase before the
+// and the pull_mid rows), one workgroup per row with a moved neighbour: the
+// row's moved neighbours are found by a scan of its adjacency and their
+// weights summed per part in LDS, then each part's total is applied to the
+// table once, with the subtraction phase before the
 // addition phase as for the other rows. The resulting table equals the one
 // per-edge pushes leave: a part is reclaimed when its connectivity reaches 0
 // in a table of fewer than k slots (jet_refiner.hpp:806-810), new parts take
@@ -1259,12 +1270,42 @@ __global__ void __launch_bounds__(SEL_BLOCK_MAX)
 // bucket share its part and offsets, so each decides for itself. One vertex
 // per lane: a lane's loads form one dependent chain (measured 0.4 ms less
 // per 128^3 stencil partition than four vertices per lane).
+// FUSED (nbk <= FUSE_SCAN_MAX): `off` holds the bucket sums, not their scan;
+// every workgroup scans them in LDS (one load round trip instead of a scan
+// launch) and workgroup 0 writes the scan to off_out for the ranking and
+// destination kernels.
 constexpr int CL_CHW = WAVE;
-template <bool STRONG>
+constexpr int64_t FUSE_SCAN_MAX = 4096;
+template <bool STRONG, bool FUSED>
 __global__ void __launch_bounds__(LS_WAVES* WAVE)
-    k_classify(DGraph g, const part_t* part, const int32_t* bid, const int64_t* off, const int64_t* sizes,
-               int64_t size_max, int64_t width, vid_t* moves, int32_t* counters, vid_t nmax, vid_t* rlist,
-               int32_t* stamp, int32_t gen) {
+    k_classify(DGraph g, const part_t* part, const int32_t* bid, const int64_t* off_in, int64_t nbk,
+               int64_t* off_out, const int64_t* sizes, int64_t size_max, int64_t width, vid_t* moves,
+               int32_t* counters, vid_t nmax, vid_t* rlist, int32_t* stamp, int32_t gen) {
+    constexpr int BLOCK = LS_WAVES * WAVE;
+    __shared__ int64_t soff[FUSED ? FUSE_SCAN_MAX : 1];
+    __shared__ int64_t sws[BLOCK / WAVE + 1];
+    const int64_t* off = off_in;
+    if (FUSED) {
+        constexpr int PER = (int)(FUSE_SCAN_MAX / BLOCK);
+        int64_t loc[PER];
+        int64_t sum = 0;
+        const int64_t t0 = (int64_t)threadIdx.x * PER;
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const int64_t v = t0 + j < nbk ? off_in[t0 + j] : 0;
+            loc[j] = sum;
+            sum += v;
+        }
+        int64_t tot;
+        const int64_t ex = block_excl_scan<int64_t, BLOCK>(sum, tot, sws);
+#pragma unroll
+        for (int j = 0; j < PER; j++)
+            if (t0 + j < nbk) soff[t0 + j] = ex + loc[j];
+        __syncthreads();
+        if (blockIdx.x == 0)
+            for (int64_t t = threadIdx.x; t < nbk; t += BLOCK) off_out[t] = soff[t];
+        off = soff;
+    }
     __shared__ vid_t sbuf[LS_WAVES * CL_CHW];
     __shared__ vid_t rbuf[LS_WAVES * CL_CHW];
     __shared__ int32_t sh[2 * LS_WAVES + 2];
@@ -1671,6 +1712,7 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
     redraw_ = a.alloc<vid_t>(RD_MAX);
     nbuckets_ = (int64_t)MAX_BUCKETS * p.k * sections_for(n, p.k) + MAX_SECTIONS + 4;
     buckets_ = a.alloc<int64_t>(nbuckets_);
+    boff_ = a.alloc<int64_t>(FUSE_SCAN_MAX);
     ncopies_ = (int32_t)std::max<int64_t>(
         1, std::min<int64_t>(BUCKET_COPIES_MAX, BUCKET_COPIES_BYTES / ((int64_t)sizeof(int64_t) * nbuckets_)));
     bcopies_ = a.alloc<int64_t>(std::max<int64_t>(1, nbuckets_ * (ncopies_ - 1)));
@@ -1762,13 +1804,17 @@ void Refiner::jet_lp(const Level& L, const part_t* part, double fr) {
     // per vertex: cache read 4, destination write 4, lock read 4
     prof_account(PF_LP_SELECT, 12.0 * g.n);
     unsigned ab = (unsigned)std::max<int64_t>(256, ceil_div((int64_t)g.n, (int64_t)AB_WAVES * AB_CHW));
+#define JET_AB(GG)                                                                                          \
+    k_afterburner_small<GG><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, \
+                                                            nmax_, stamp_, gen_)
     switch (group_for_avg(g)) {
-        case 4: k_afterburner_small<4><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_, stamp_, gen_); break;
-        case 8: k_afterburner_small<8><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_, stamp_, gen_); break;
-        case 16: k_afterburner_small<16><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_, stamp_, gen_); break;
-        case 32: k_afterburner_small<32><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_, stamp_, gen_); break;
-        default: k_afterburner_small<64><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_, stamp_, gen_); break;
+        case 4: JET_AB(4); break;
+        case 8: JET_AB(8); break;
+        case 16: JET_AB(16); break;
+        case 32: JET_AB(32); break;
+        default: JET_AB(64); break;
     }
+#undef JET_AB
     if (has_big_)
         k_afterburner_big<<<grid_for((int64_t)g.n * WAVE, 256, 2048), 256, 0, s_>>>(g, cand_, c, part, dest_, pregain_,
                                                                                    lock_, moves_, c, nmax_, stamp_,
@@ -1786,11 +1832,12 @@ void Refiner::rank_flagged(const Level& L, const part_t* part, const int64_t* si
     int32_t* c = cnt();
     int64_t smax = size_max_;
     if (strong)
-        k_rank_small<true><<<1, RS_BLOCK, 0, s_>>>(cand_, c, bid_, part, buckets_, sizes, smax, width, g.vw, g.rowptr,
+        k_rank_small<true><<<1, RS_BLOCK, 0, s_>>>(cand_, c, bid_, part, off_, sizes, smax, width, g.vw, g.rowptr,
                                                   evict_adjust_, score_, moves_, c, nmax_, misc_ + 2, stamp_, gen_);
     else
-        k_rank_small<false><<<1, RS_BLOCK, 0, s_>>>(cand_, c, bid_, part, buckets_, sizes, smax, width, g.vw,
-                                                   g.rowptr, evict_adjust_, score_, moves_, c, nmax_, misc_ + 2, stamp_, gen_);
+        k_rank_small<false><<<1, RS_BLOCK, 0, s_>>>(cand_, c, bid_, part, off_, sizes, smax, width, g.vw,
+                                                   g.rowptr, evict_adjust_, score_, moves_, c, nmax_, misc_ + 2, stamp_,
+                                                   gen_);
     JET_LAUNCH_CHECK();
 }
 
@@ -1812,10 +1859,10 @@ void Refiner::rank_radix(const Level& L, const part_t* part, const int64_t* size
     JET_LAUNCH_CHECK();
     excl_scan_i64_i64(prefix_, prefix_, R, a_, s_);
     if (strong)
-        k_rank_apply<true><<<rb, 256, 0, s_>>>(R, keys, prefix_, part, buckets_, sizes, smax, width, g.vw, g.rowptr,
+        k_rank_apply<true><<<rb, 256, 0, s_>>>(R, keys, prefix_, part, off_, sizes, smax, width, g.vw, g.rowptr,
                                               evict_adjust_, score_, moves_, c, nmax_, stamp_, gen_);
     else
-        k_rank_apply<false><<<rb, 256, 0, s_>>>(R, keys, prefix_, part, buckets_, sizes, smax, width, g.vw, g.rowptr,
+        k_rank_apply<false><<<rb, 256, 0, s_>>>(R, keys, prefix_, part, off_, sizes, smax, width, g.vw, g.rowptr,
                                                evict_adjust_, score_, moves_, c, nmax_, stamp_, gen_);
     JET_LAUNCH_CHECK();
 }
@@ -1846,7 +1893,7 @@ void Refiner::strong_tail(const Level& L, const part_t* part, const int64_t* siz
     int32_t* c = cnt();
     const int32_t k = p_.k;
     int nb = grid_for(g.n, 256, 8192);
-    k_cookie<<<1, CK_BLOCK, 0, s_>>>(k, r.width, buckets_, sizes, r.max_dest, evict_adjust_, evict_start_, acc_,
+    k_cookie<<<1, CK_BLOCK, 0, s_>>>(k, r.width, off_, sizes, r.max_dest, evict_adjust_, evict_start_, acc_,
                                      misc_ + 2);
     k_strong_dest<<<nb, 256, 0, s_>>>(moves_, c, nmax_, part, g.vw, evict_adjust_, evict_start_, k, score_, dest_,
                                       redraw_, c + 4, RD_MAX, misc_ + 2);
@@ -1890,8 +1937,8 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
     const int selb = g.n > SMALL_LEVEL_N ? SEL_BLOCK_MAX : 256;
     if (strong) {
         if (lds)
-            k_strong_select<true><<<nsel, selb, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, acc_, conn_off_, conn_,
-                                                        conn_size_, sections, bid_, bk, bc, copies, nbuckets_);
+            k_strong_select<true><<<nsel, selb, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, acc_, conn_off_,
+                                                         conn_, conn_size_, sections, bid_, bk, bc, copies, nbuckets_);
         else
             k_strong_select<false><<<nsel, selb, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, acc_, conn_off_,
                                                          conn_, conn_size_, sections, bid_, bk, bc, copies, nbuckets_);
@@ -1909,14 +1956,23 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
         k_sum_bucket_copies<<<grid_for(nbk + 1, 256, 2048), 256, 0, s_>>>(buckets_, bcopies_, copies, nbk + 1,
                                                                           nbuckets_);
     JET_LAUNCH_CHECK();
-    excl_scan_i64_i64(buckets_, buckets_, nbk, a_, s_);
+    // a bucket array of up to FUSE_SCAN_MAX sums is scanned by the
+    // classification kernel itself, into boff_
+    const bool fused = nbk <= FUSE_SCAN_MAX;
+    off_ = fused ? boff_ : buckets_;
+    if (!fused) excl_scan_i64_i64(buckets_, buckets_, nbk, a_, s_);
     int cb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_WAVES * CL_CHW));
-    if (strong)
-        k_classify<true><<<cb, LS_WAVES * WAVE, 0, s_>>>(g, part, bid_, buckets_, sizes, smax, width, moves_, c, nmax_,
-                                                         cand_, stamp_, gen_);
-    else
-        k_classify<false><<<cb, LS_WAVES * WAVE, 0, s_>>>(g, part, bid_, buckets_, sizes, smax, width, moves_, c,
-                                                          nmax_, cand_, stamp_, gen_);
+#define JET_CLASSIFY(ST, FU)                                                                                 \
+    k_classify<ST, FU><<<cb, LS_WAVES * WAVE, 0, s_>>>(g, part, bid_, buckets_, nbk, boff_, sizes, smax, width, \
+                                                       moves_, c, nmax_, cand_, stamp_, gen_)
+    if (strong) {
+        if (fused) JET_CLASSIFY(true, true);
+        else JET_CLASSIFY(true, false);
+    } else {
+        if (fused) JET_CLASSIFY(false, true);
+        else JET_CLASSIFY(false, false);
+    }
+#undef JET_CLASSIFY
     JET_LAUNCH_CHECK();
     rank_flagged(L, part, sizes, width, strong);
     if (strong) strong_tail(L, part, sizes, r);
@@ -1955,13 +2011,13 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
     int nbb = grid_for((int64_t)g.n * WAVE, 256, 512);
     for (int add = 0; add < 2; add++) {
 #define JET_UPD(GG)                                                                                                   \
-    if (add) k_update_small<true, GG><<<nbs, 256, 0, s_>>>(g, moves_, mrec_, c, nmax_, part, dest_, conn_off_, conn_,       \
-                                                          conn_size_, dest_cache_, k, misc_ + 1, swap_bit_,      \
-                                                          hub_, hub_dirty_, misc_ + 2, stamp_, gen_, usz, pub_,  \
-                                                          pb);                                                   \
-    else k_update_small<false, GG><<<nbs, 256, 0, s_>>>(g, moves_, mrec_, c, nmax_, part, dest_, conn_off_, conn_,          \
-                                                        conn_size_, dest_cache_, k, misc_ + 1, swap_bit_,        \
-                                                        hub_, hub_dirty_, misc_ + 2, stamp_, gen_, usz, pub_, pb);
+    if (add) k_update_small<true, GG><<<nbs + 1, 256, 0, s_>>>(g, moves_, mrec_, c, nmax_, part, dest_, conn_off_,    \
+                                                              conn_, conn_size_, dest_cache_, k, misc_ + 1,           \
+                                                              swap_bit_, hub_, hub_dirty_, misc_ + 2, stamp_, gen_,   \
+                                                              usz, pub_, pb);                                         \
+    else k_update_small<false, GG><<<nbs, 256, 0, s_>>>(g, moves_, mrec_, c, nmax_, part, dest_, conn_off_, conn_,    \
+                                                        conn_size_, dest_cache_, k, misc_ + 1, swap_bit_, hub_,       \
+                                                        hub_dirty_, misc_ + 2, stamp_, gen_, usz, pub_, pb);
         switch (G) {
             case 4: JET_UPD(4) break;
             case 8: JET_UPD(8) break;
