@@ -749,11 +749,35 @@ struct Publish {
     int64_t* host_out;
     const int32_t* mc;
     int64_t seq;
+    // the refinement loop's choice to keep this step's partition as the best
+    // (jet_refiner.hpp:1112-1124), taken here from the loop's state before
+    // the step so that k_take_copy can copy it before the host has the
+    // results; null when the caller keeps no best partition
+    int32_t* take;
+    int64_t cut_before, best_cut, best_imb, imb_max, opt;
 };
+
+__device__ inline void publish_take(const Publish& pb, int64_t cut_change, int64_t max_size) {
+    const int64_t curr_imb = max_size > pb.opt ? max_size - pb.opt : 0;
+    const int64_t curr_cut = pb.cut_before - cut_change;
+    const bool take = (pb.best_imb > pb.imb_max && curr_imb < pb.best_imb) ||
+                      (curr_cut < pb.best_cut && (curr_imb <= pb.imb_max || curr_imb <= pb.best_imb));
+    *pb.take = take ? 1 : 0;
+}
+
+// The best-partition copy of a kept step (jet_refiner.hpp:1118-1123), run
+// after the update passes and before the host has read the results.
+__global__ void k_take_copy(const int32_t* take, const part_t* cur, part_t* best, vid_t n, const int64_t* cur_sizes,
+                            int64_t* best_sizes, int32_t k) {
+    if (!*take) return;
+    FOR_THREAD(i, n) best[i] = cur[i];
+    FOR_THREAD(t, k) best_sizes[t] = cur_sizes[t];
+}
 
 __device__ inline void publish_step(const Publish& pb, bool deferred, int64_t* red) {
     if (deferred) {
         if (threadIdx.x == 0) {
+            if (pb.take) *pb.take = 0;
             pb.host_out[2] = 1;
             __threadfence_system();
             pb.host_out[5] = pb.seq;  // the host waits for this word
@@ -768,6 +792,7 @@ __device__ inline void publish_step(const Publish& pb, bool deferred, int64_t* r
     c2 = block_reduce_sum<int64_t, 256>(c2, red);
     if (threadIdx.x < PUB_GROUPS) pb.partials[(int64_t)threadIdx.x * PUB_STRIDE] = 0;
     if (threadIdx.x == 0) {
+        if (pb.take) publish_take(pb, c2, mx);
         pb.host_out[0] = c2;
         pb.host_out[1] = mx;
         pb.host_out[2] = 0;
@@ -1343,13 +1368,17 @@ __global__ void __launch_bounds__(LS_WAVES* WAVE)
 }
 
 // Decision for one ranked vertex: pos = weight of same-bucket members with a
-// smaller id.
+// smaller id. Returns 0 (kept), 1 (evicted, degree <= BIG_DEG) or 2 (evicted,
+// larger); the caller appends evicted members to the move list.
 template <bool STRONG>
-__device__ inline void rank_decide(vid_t v, int32_t b, int64_t pos, const part_t* part, const int64_t* off,
-                                   const int64_t* sizes, int64_t size_max, int64_t width, const wgt_t* vw,
-                                   const eoff_t* rowptr, int64_t* evict_adjust, int64_t* score, vid_t* moves,
-                                   int32_t* mcounters, vid_t nmax, int32_t* stamp, int32_t gen) {
-    part_t p = part[v];
+__device__ inline int rank_decide(vid_t v, int32_t b, int64_t pos, const part_t* part, const int64_t* off,
+                                  const int64_t* sizes, int64_t size_max, int64_t width, const wgt_t* vw,
+                                  const eoff_t* rowptr, int64_t* evict_adjust, int64_t* score, int32_t* stamp,
+                                  int32_t gen) {
+    // the member's part, weight and row are loaded together
+    const part_t p = part[v];
+    const int64_t w = vw[v];
+    const eoff_t deg = rowptr[v + 1] - rowptr[v];
     int64_t base = off[b] - off[(int64_t)p * width];
     int64_t limit = sizes[p] - size_max;
     int64_t bs = off[b + 1] - off[b];
@@ -1357,18 +1386,17 @@ __device__ inline void rank_decide(vid_t v, int32_t b, int64_t pos, const part_t
     if (STRONG) {
         if (base < limit && base + pos < limit) {
             ev = true;
-            if (base + pos + vw[v] >= limit) evict_adjust[p] = base + pos + vw[v];
+            if (base + pos + w >= limit) evict_adjust[p] = base + pos + w;
             score[v] = off[b] + pos;
         }
     } else if (base < limit && base + bs > limit) {
         ev = base + pos < limit;
     }
-    if (ev) {
-        stamp[v] = gen;
-        if (rowptr[v + 1] - rowptr[v] > BIG_DEG) moves[nmax - 1 - atomicAdd(&mcounters[3], 1)] = v;
-        else moves[atomicAdd(&mcounters[2], 1)] = v;
-    }
+    if (!ev) return 0;
+    stamp[v] = gen;
+    return deg > BIG_DEG ? 2 : 1;
 }
+
 
 constexpr int RS_CAP = 8192;
 constexpr int RS_BLOCK = 1024;
@@ -1396,25 +1424,53 @@ __global__ void __launch_bounds__(RS_BLOCK)
     }
     uint32_t cap = 1;
     while ((int32_t)cap < R) cap <<= 1;
-    for (uint32_t t = threadIdx.x; t < cap; t += RS_BLOCK) {
-        if ((int32_t)t < R) {
-            const vid_t v = t == threadIdx.x ? m0 : rlist[t];
-            keys[t] = ((uint64_t)(uint32_t)bid[v] << 32) | (uint32_t)v;
-        } else {
-            keys[t] = ~0ull;
+    if (cap <= RS_BLOCK) {
+        // one key per thread: exchanges within a wave are shuffles, only
+        // strides of a wave or more go through LDS (10 barriers instead of
+        // 55 for 1024 keys)
+        const uint32_t t = threadIdx.x;
+        uint64_t key = ~0ull;
+        if ((int32_t)t < R) key = ((uint64_t)(uint32_t)bid[m0] << 32) | (uint32_t)m0;
+        for (uint32_t size = 2; size <= cap; size <<= 1) {
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                uint64_t other;
+                if (stride >= WAVE) {
+                    keys[t] = key;
+                    __syncthreads();
+                    other = keys[t ^ stride];
+                    __syncthreads();
+                } else {
+                    const uint32_t lo = __shfl_xor((uint32_t)key, (int)stride, WAVE);
+                    const uint32_t hi = __shfl_xor((uint32_t)(key >> 32), (int)stride, WAVE);
+                    other = ((uint64_t)hi << 32) | lo;
+                }
+                // ascending blocks keep the smaller key at the lower index
+                const bool lower = (t & stride) == 0, up = (t & size) == 0;
+                key = (lower == up) ? (key < other ? key : other) : (key > other ? key : other);
+            }
         }
-    }
-    for (uint32_t size = 2; size <= cap; size <<= 1) {
-        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-            __syncthreads();
-            for (uint32_t t = threadIdx.x; t < cap / 2; t += RS_BLOCK) {
-                uint32_t lo = 2 * t - (t & (stride - 1));
-                uint32_t hi = lo + stride;
-                bool up = (lo & size) == 0;
-                uint64_t x = keys[lo], y = keys[hi];
-                if ((x > y) == up) {
-                    keys[lo] = y;
-                    keys[hi] = x;
+        keys[t] = key;
+    } else {
+        for (uint32_t t = threadIdx.x; t < cap; t += RS_BLOCK) {
+            if ((int32_t)t < R) {
+                const vid_t v = t == threadIdx.x ? m0 : rlist[t];
+                keys[t] = ((uint64_t)(uint32_t)bid[v] << 32) | (uint32_t)v;
+            } else {
+                keys[t] = ~0ull;
+            }
+        }
+        for (uint32_t size = 2; size <= cap; size <<= 1) {
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                __syncthreads();
+                for (uint32_t t = threadIdx.x; t < cap / 2; t += RS_BLOCK) {
+                    uint32_t lo = 2 * t - (t & (stride - 1));
+                    uint32_t hi = lo + stride;
+                    bool up = (lo & size) == 0;
+                    uint64_t x = keys[lo], y = keys[hi];
+                    if ((x > y) == up) {
+                        keys[lo] = y;
+                        keys[hi] = x;
+                    }
                 }
             }
         }
@@ -1435,17 +1491,20 @@ __global__ void __launch_bounds__(RS_BLOCK)
     for (int j = 0; j < PER; j++) pre[threadIdx.x * PER + j] = ex + loc[j];
     __syncthreads();
     for (int32_t t = threadIdx.x; t < R; t += RS_BLOCK) {
-        uint64_t key = keys[t];
-        int32_t b = (int32_t)(key >> 32);
-        uint64_t lo_key = (uint64_t)(uint32_t)b << 32;
+        const uint64_t key = keys[t];
+        const int32_t b = (int32_t)(key >> 32);
+        const uint64_t lo_key = (uint64_t)(uint32_t)b << 32;
         int32_t lo = 0, hi = t;
         while (lo < hi) {
             int32_t mid = (lo + hi) >> 1;
             if (keys[mid] < lo_key) lo = mid + 1;
             else hi = mid;
         }
-        rank_decide<STRONG>((vid_t)(key & 0xffffffffull), b, pre[t] - pre[lo], part, off, sizes, size_max, width, vw,
-                            rowptr, evict_adjust, score, moves, mcounters, nmax, stamp, gen);
+        const vid_t v = (vid_t)(key & 0xffffffffull);
+        const int code = rank_decide<STRONG>(v, b, pre[t] - pre[lo], part, off, sizes, size_max, width, vw, rowptr,
+                                             evict_adjust, score, stamp, gen);
+        if (code == 2) moves[nmax - 1 - atomicAdd(&mcounters[3], 1)] = v;
+        else if (code == 1) moves[atomicAdd(&mcounters[2], 1)] = v;
     }
 }
 
@@ -1476,8 +1535,11 @@ __global__ void k_rank_apply(int32_t R, const uint64_t* keys, const int64_t* pre
             if (keys[mid] < lo_key) lo = mid + 1;
             else hi = mid;
         }
-        rank_decide<STRONG>((vid_t)(key & 0xffffffffull), b, pre[t] - pre[lo], part, off, sizes, size_max, width, vw,
-                            rowptr, evict_adjust, score, moves, mcounters, nmax, stamp, gen);
+        const vid_t v = (vid_t)(key & 0xffffffffull);
+        const int code = rank_decide<STRONG>(v, b, pre[t] - pre[lo], part, off, sizes, size_max, width, vw, rowptr,
+                                             evict_adjust, score, stamp, gen);
+        if (code == 2) moves[nmax - 1 - atomicAdd(&mcounters[3], 1)] = v;
+        else if (code == 1) moves[atomicAdd(&mcounters[2], 1)] = v;
     }
 }
 
@@ -2004,7 +2066,8 @@ void Refiner::wait_published(volatile int64_t* res, int64_t seq) {
     }
 }
 
-bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_t& cut_change, int64_t& max_size) {
+bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_t& cut_change, int64_t& max_size,
+                            const TakeState* take) {
     DGraph g = L.view();
     const int32_t k = p_.k;
     int32_t* c = cnt();
@@ -2012,7 +2075,15 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
     int G = group_for_avg(g);
     int nbs = grid_for((int64_t)g.n * G, 256, 1024);
     const int64_t seq = ++mailbox_->seq;
-    const Publish pb{pub_, sizes, k, res_dev_, c, seq};
+    Publish pb{pub_, sizes, k, res_dev_, c, seq, nullptr, 0, 0, 0, 0, 0};
+    if (take) {
+        pb.take = misc_ + 3;
+        pb.cut_before = take->cut_before;
+        pb.best_cut = take->best_cut;
+        pb.best_imb = take->best_imb;
+        pb.imb_max = take->imb_max;
+        pb.opt = opt_;
+    }
     int nbb = grid_for((int64_t)g.n * WAVE, 256, 512);
     for (int add = 0; add < 2; add++) {
 #define JET_UPD(GG)                                                                                                   \
@@ -2061,6 +2132,11 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
                                                                    hub_dirty_, part, dest_, conn_off_, conn_,
                                                                    conn_size_, dest_cache_, k, misc_ + 1, misc_ + 2);
         }
+        JET_LAUNCH_CHECK();
+    }
+    if (take) {
+        k_take_copy<<<grid_for(g.n, 256, 2048), 256, 0, s_>>>(misc_ + 3, part, take->best_part, g.n, sizes,
+                                                               take->best_sizes, k);
         JET_LAUNCH_CHECK();
     }
     volatile int64_t* res = res_host_;
@@ -2142,9 +2218,12 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
                 balance_counter++;
             }
             int64_t cc = 0, mx = 0;
-            if (perform_moves(L, cur_part_, curr_sizes_, cc, mx)) {
+            // a kept step's copy to the best partition is made on the device
+            // (k_take_copy) by the same rule as below
+            const TakeState ts{curr_cut, cut_, total_imb_, imb_max, best_part, best_sizes_};
+            if (perform_moves(L, cur_part_, curr_sizes_, cc, mx, &ts)) {
                 rank_deferred(L, cur_part_, curr_sizes_, strong);
-                if (perform_moves(L, cur_part_, curr_sizes_, cc, mx))
+                if (perform_moves(L, cur_part_, curr_sizes_, cc, mx, &ts))
                     throw DeviceError("rebalancing step deferred twice");
             }
             curr_imb = mx > opt_ ? mx - opt_ : 0;
@@ -2161,8 +2240,6 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
                 changed_ = true;
                 cut_ = curr_cut;
                 total_imb_ = curr_imb;
-                copy_i64(best_sizes_, curr_sizes_, k, s_);
-                copy_i32(best_part, cur_part_, g.n, s_);
             }
         }
     }
