@@ -86,15 +86,7 @@ class Refiner {
     void wait_published(volatile int64_t* res, int64_t seq);
     // rebuilds the rows an update_large step marked, if any
     void flush_rebuild(const Level& L);
-    // The refinement loop's state before a step, for the device-side choice
-    // to keep the step's partition (k_take_copy).
-    struct TakeState {
-        int64_t cut_before, best_cut, best_imb, imb_max;
-        part_t* best_part;
-        int64_t* best_sizes;
-    };
-    bool perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_t& cut_change, int64_t& max_size,
-                       const TakeState* take = nullptr);
+    bool perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_t& cut_change, int64_t& max_size);
     // Two counter sets alternate between iterations: the first kernel of an
     // iteration clears the other set, so no fill launch is needed.
     int32_t* cnt() const { return counters_ + 8 * par_; }
@@ -149,7 +141,7 @@ class Refiner {
     int64_t* evict_adjust_;
     int64_t* evict_start_;
     part_t* undersized_;
-    int32_t* misc_;  // [0] undersized count, [1] error bits, [2] ranking deferred, [3] step kept (k_take_copy)
+    int32_t* misc_;  // [0] undersized count, [1] error bits, [2] ranking deferred
     vid_t* redraw_;
     int64_t* buckets_;
     int64_t* boff_;        // the scan of a small bucket array, written by the classification kernel

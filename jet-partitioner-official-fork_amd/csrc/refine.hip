@@ -749,35 +749,11 @@ struct Publish {
     int64_t* host_out;
     const int32_t* mc;
     int64_t seq;
-    // the refinement loop's choice to keep this step's partition as the best
-    // (jet_refiner.hpp:1112-1124), taken here from the loop's state before
-    // the step so that k_take_copy can copy it before the host has the
-    // results; null when the caller keeps no best partition
-    int32_t* take;
-    int64_t cut_before, best_cut, best_imb, imb_max, opt;
 };
-
-__device__ inline void publish_take(const Publish& pb, int64_t cut_change, int64_t max_size) {
-    const int64_t curr_imb = max_size > pb.opt ? max_size - pb.opt : 0;
-    const int64_t curr_cut = pb.cut_before - cut_change;
-    const bool take = (pb.best_imb > pb.imb_max && curr_imb < pb.best_imb) ||
-                      (curr_cut < pb.best_cut && (curr_imb <= pb.imb_max || curr_imb <= pb.best_imb));
-    *pb.take = take ? 1 : 0;
-}
-
-// The best-partition copy of a kept step (jet_refiner.hpp:1118-1123), run
-// after the update passes and before the host has read the results.
-__global__ void k_take_copy(const int32_t* take, const part_t* cur, part_t* best, vid_t n, const int64_t* cur_sizes,
-                            int64_t* best_sizes, int32_t k) {
-    if (!*take) return;
-    FOR_THREAD(i, n) best[i] = cur[i];
-    FOR_THREAD(t, k) best_sizes[t] = cur_sizes[t];
-}
 
 __device__ inline void publish_step(const Publish& pb, bool deferred, int64_t* red) {
     if (deferred) {
         if (threadIdx.x == 0) {
-            if (pb.take) *pb.take = 0;
             pb.host_out[2] = 1;
             __threadfence_system();
             pb.host_out[5] = pb.seq;  // the host waits for this word
@@ -792,7 +768,6 @@ __device__ inline void publish_step(const Publish& pb, bool deferred, int64_t* r
     c2 = block_reduce_sum<int64_t, 256>(c2, red);
     if (threadIdx.x < PUB_GROUPS) pb.partials[(int64_t)threadIdx.x * PUB_STRIDE] = 0;
     if (threadIdx.x == 0) {
-        if (pb.take) publish_take(pb, c2, mx);
         pb.host_out[0] = c2;
         pb.host_out[1] = mx;
         pb.host_out[2] = 0;
@@ -2066,8 +2041,7 @@ void Refiner::wait_published(volatile int64_t* res, int64_t seq) {
     }
 }
 
-bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_t& cut_change, int64_t& max_size,
-                            const TakeState* take) {
+bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_t& cut_change, int64_t& max_size) {
     DGraph g = L.view();
     const int32_t k = p_.k;
     int32_t* c = cnt();
@@ -2075,15 +2049,7 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
     int G = group_for_avg(g);
     int nbs = grid_for((int64_t)g.n * G, 256, 1024);
     const int64_t seq = ++mailbox_->seq;
-    Publish pb{pub_, sizes, k, res_dev_, c, seq, nullptr, 0, 0, 0, 0, 0};
-    if (take) {
-        pb.take = misc_ + 3;
-        pb.cut_before = take->cut_before;
-        pb.best_cut = take->best_cut;
-        pb.best_imb = take->best_imb;
-        pb.imb_max = take->imb_max;
-        pb.opt = opt_;
-    }
+    const Publish pb{pub_, sizes, k, res_dev_, c, seq};
     int nbb = grid_for((int64_t)g.n * WAVE, 256, 512);
     for (int add = 0; add < 2; add++) {
 #define JET_UPD(GG)                                                                                                   \
@@ -2132,11 +2098,6 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
                                                                    hub_dirty_, part, dest_, conn_off_, conn_,
                                                                    conn_size_, dest_cache_, k, misc_ + 1, misc_ + 2);
         }
-        JET_LAUNCH_CHECK();
-    }
-    if (take) {
-        k_take_copy<<<grid_for(g.n, 256, 2048), 256, 0, s_>>>(misc_ + 3, part, take->best_part, g.n, sizes,
-                                                               take->best_sizes, k);
         JET_LAUNCH_CHECK();
     }
     volatile int64_t* res = res_host_;
@@ -2218,12 +2179,9 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
                 balance_counter++;
             }
             int64_t cc = 0, mx = 0;
-            // a kept step's copy to the best partition is made on the device
-            // (k_take_copy) by the same rule as below
-            const TakeState ts{curr_cut, cut_, total_imb_, imb_max, best_part, best_sizes_};
-            if (perform_moves(L, cur_part_, curr_sizes_, cc, mx, &ts)) {
+            if (perform_moves(L, cur_part_, curr_sizes_, cc, mx)) {
                 rank_deferred(L, cur_part_, curr_sizes_, strong);
-                if (perform_moves(L, cur_part_, curr_sizes_, cc, mx, &ts))
+                if (perform_moves(L, cur_part_, curr_sizes_, cc, mx))
                     throw DeviceError("rebalancing step deferred twice");
             }
             curr_imb = mx > opt_ ? mx - opt_ : 0;
@@ -2240,6 +2198,8 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
                 changed_ = true;
                 cut_ = curr_cut;
                 total_imb_ = curr_imb;
+                copy_i64(best_sizes_, curr_sizes_, k, s_);
+                copy_i32(best_part, cur_part_, g.n, s_);
             }
         }
     }
