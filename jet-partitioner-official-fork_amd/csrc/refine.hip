@@ -370,36 +370,42 @@ __device__ inline void for_slots(const int2* tab, int32_t sz, F&& f) {
 
 // ---------------------------------------------------------------- jet_lp
 // jet_refiner.hpp:211-268: destination selection, viable-move filter.
-// Each wave owns a contiguous chunk of vertices; candidates are staged in LDS
-// and published with one atomic per workgroup.
+// Each wave owns a contiguous chunk of CHW vertices (CHW / WAVE passes);
+// candidates are staged in LDS and published with one atomic per workgroup
+// of W waves.
 constexpr int LS_WAVES = 4;
-// levels up to LS_ONE_PASS_WAVES * WAVE vertices run one pass per wave,
-// larger ones LS_MULTI_CHW vertices per wave
+// levels up to LS_ONE_PASS_WAVES * WAVE vertices run one pass per wave in
+// workgroups of LS_WAVES waves; larger ones LS_MULTI_CHW vertices per wave in
+// workgroups of LS_MULTI_WAVES waves: few enough workgroups appending to the
+// candidate counters, and two passes per wave (measured on the 128^3
+// stencil: 12.1 us per launch on average, against 14.2 with eight passes in
+// workgroups of four waves)
 constexpr int LS_ONE_PASS_WAVES = 8192;
-constexpr int LS_MULTI_CHW = 512;
+constexpr int LS_MULTI_CHW = 128;
+constexpr int LS_MULTI_WAVES = 16;
 
-template <int CHW>
-__global__ void __launch_bounds__(LS_WAVES* WAVE)
+template <int CHW, int W>
+__global__ void __launch_bounds__(W* WAVE)
     k_lp_select(DGraph g, const part_t* part, const eoff_t* conn_off, const int2* conn, const int32_t* conn_size,
                 part_t* dest_cache, int32_t* gain_p, int32_t* lock, double fr, part_t* dest, int32_t* pregain,
                 vid_t* cand, int32_t* counters, vid_t nmax, int32_t* next_counters, unsigned long long* pc,
                 unsigned long long* span) {
     prof_span_begin(span);
     constexpr int NPASS = CHW / WAVE;
-    __shared__ vid_t sbuf[LS_WAVES * CHW];
-    __shared__ int32_t sh[2 * LS_WAVES + 2];
-    __shared__ int64_t pred[LS_WAVES];
+    __shared__ vid_t sbuf[W * CHW];
+    __shared__ int32_t sh[2 * W + 2];
+    __shared__ int64_t pred[W];
     // wave-private: the slots (u * WAVE + lane) of the wave's cache misses,
     // and each miss's new destination and gain
-    __shared__ int16_t mslot[LS_WAVES][CHW];
-    __shared__ part_t mbest[LS_WAVES][CHW];
-    __shared__ int32_t mgain[LS_WAVES][CHW];
+    __shared__ int16_t mslot[W][CHW];
+    __shared__ part_t mbest[W][CHW];
+    __shared__ int32_t mgain[W][CHW];
     int64_t n_miss = 0, n_slot = 0, n_cand = 0;
-    StagedLists<LS_WAVES, CHW> st(sbuf);
+    StagedLists<W, CHW> st(sbuf);
     if (blockIdx.x == 0 && threadIdx.x < 8) next_counters[threadIdx.x] = 0;
     const int lane = threadIdx.x & (WAVE - 1);
     const int wv = threadIdx.x / WAVE;
-    const int64_t wbase = ((int64_t)blockIdx.x * LS_WAVES + wv) * CHW;
+    const int64_t wbase = ((int64_t)blockIdx.x * W + wv) * CHW;
     const uint64_t lt = (1ull << lane) - 1ull;
     // 1. the cached destinations and locks of all passes, loaded together
     part_t cached[NPASS];
@@ -490,7 +496,7 @@ __global__ void __launch_bounds__(LS_WAVES* WAVE)
     if (pc) {
         int64_t v3[3] = {n_miss, n_slot, n_cand};
         for (int q = 0; q < 3; q++) {
-            int64_t v = block_reduce_sum<int64_t, LS_WAVES * WAVE>(v3[q], pred);
+            int64_t v = block_reduce_sum<int64_t, W * WAVE>(v3[q], pred);
             if (threadIdx.x == 0 && v) atomicAdd(&pc[q], (unsigned long long)v);
         }
     }
@@ -1826,21 +1832,18 @@ void Refiner::jet_lp(const Level& L, const part_t* part, double fr) {
     DGraph g = L.view();
     int32_t* c = cnt();
     gen_++;  // this step's moves are stamped with a new generation
-    // a pass per wave on levels that fill the GPU with single-pass waves,
-    // eight passes above that: fewer workgroups, so fewer appends to the
-    // candidate counters (measured: 14.2 us per launch against 18.2 with
-    // four passes and 43 with one on the 128^3 stencil)
+    // a pass per wave on levels that fill the GPU with single-pass waves, two
+    // passes per wave in 16-wave workgroups above that (see LS_MULTI_CHW)
     if (g.n <= (int64_t)LS_ONE_PASS_WAVES * WAVE) {
         int nb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_WAVES * WAVE));
-        k_lp_select<WAVE><<<nb, LS_WAVES * WAVE, 0, s_>>>(g, part, conn_off_, conn_, conn_size_, dest_cache_, gain_p_,
-                                                          lock_, fr, dest_, pregain_, cand_, c, nmax_, cnt_next(),
-                                                          prof_counters(PF_LP_SELECT), prof_span(PF_LP_SELECT));
+        k_lp_select<WAVE, LS_WAVES><<<nb, LS_WAVES * WAVE, 0, s_>>>(
+            g, part, conn_off_, conn_, conn_size_, dest_cache_, gain_p_, lock_, fr, dest_, pregain_, cand_, c, nmax_,
+            cnt_next(), prof_counters(PF_LP_SELECT), prof_span(PF_LP_SELECT));
     } else {
-        int nb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_WAVES * LS_MULTI_CHW));
-        k_lp_select<LS_MULTI_CHW><<<nb, LS_WAVES * WAVE, 0, s_>>>(g, part, conn_off_, conn_, conn_size_, dest_cache_,
-                                                                  gain_p_, lock_, fr, dest_, pregain_, cand_, c, nmax_,
-                                                                  cnt_next(), prof_counters(PF_LP_SELECT),
-                                                                  prof_span(PF_LP_SELECT));
+        int nb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_MULTI_WAVES * LS_MULTI_CHW));
+        k_lp_select<LS_MULTI_CHW, LS_MULTI_WAVES><<<nb, LS_MULTI_WAVES * WAVE, 0, s_>>>(
+            g, part, conn_off_, conn_, conn_size_, dest_cache_, gain_p_, lock_, fr, dest_, pregain_, cand_, c, nmax_,
+            cnt_next(), prof_counters(PF_LP_SELECT), prof_span(PF_LP_SELECT));
     }
     JET_LAUNCH_CHECK();
     // per vertex: cache read 4, destination write 4, lock read 4
