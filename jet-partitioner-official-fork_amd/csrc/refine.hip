@@ -45,16 +45,6 @@ __device__ inline int32_t lookup(const int2* tab, int32_t size, part_t target) {
     return 0;
 }
 
-// Appends v (when pred) to the small-degree list growing from the front or
-// the large-degree list growing from the back; wave-aggregated atomics.
-__device__ inline void push_list(vid_t* list, int32_t* counters, int idx_small, vid_t nmax, bool pred, vid_t v,
-                                 eoff_t deg) {
-    int32_t a = wave_append(&counters[idx_small], pred && deg <= BIG_DEG);
-    int32_t b = wave_append(&counters[idx_small + 1], pred && deg > BIG_DEG);
-    if (a >= 0) list[a] = v;
-    if (b >= 0) list[nmax - 1 - b] = v;
-}
-
 __device__ inline vid_t list_at(const vid_t* list, int32_t nsmall, vid_t nmax, int64_t x) {
     return x < nsmall ? list[x] : list[nmax - 1 - (x - nsmall)];
 }
