@@ -363,13 +363,14 @@ __device__ inline void for_slots(const int2* tab, int32_t sz, F&& f) {
 // Each wave owns a contiguous chunk of CHW vertices (CHW / WAVE passes);
 // candidates are staged in LDS and published with one atomic per workgroup
 // of W waves.
-constexpr int LS_WAVES = 4;
-// levels up to LS_ONE_PASS_WAVES * WAVE vertices run one pass per wave in
-// workgroups of LS_WAVES waves; larger ones LS_MULTI_CHW vertices per wave in
-// workgroups of LS_MULTI_WAVES waves: few enough workgroups appending to the
-// candidate counters, and two passes per wave (measured on the 128^3
-// stencil: 12.1 us per launch on average, against 14.2 with eight passes in
-// workgroups of four waves)
+constexpr int LS_WAVES = 8;
+// Levels up to LS_ONE_PASS_WAVES * WAVE vertices run one pass per wave in
+// workgroups of LS_WAVES waves (eight: 0.3 ms less per 128^3 stencil
+// partition than four; sixteen is slower there). Larger levels run
+// LS_MULTI_CHW vertices per wave in workgroups of LS_MULTI_WAVES waves: few
+// enough workgroups appending to the candidate counters, and two passes per
+// wave (measured on the 128^3 stencil: 12.1 us per launch on average,
+// against 14.2 with eight passes in workgroups of four waves).
 constexpr int LS_ONE_PASS_WAVES = 8192;
 constexpr int LS_MULTI_CHW = 128;
 constexpr int LS_MULTI_WAVES = 16;
