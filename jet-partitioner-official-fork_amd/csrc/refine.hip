@@ -1691,15 +1691,23 @@ static inline int32_t sections_for(vid_t n, int32_t k) {
     return sections;
 }
 
-// Lanes per move (afterburner, update passes): the average degree rounded up
-// to a power of two, doubled on levels of up to SMALL_LEVEL_N vertices, whose
-// few moves leave the GPU idle otherwise (measured 0.5 ms less per 128^3
-// stencil partition).
+// Lanes per vertex of a list: the average degree rounded up to a power of
+// two.
 static int group_for_avg(const DGraph& g) {
     double avg = g.n > 0 ? (double)g.m / g.n : 0.0;
-    const int b = avg <= 4 ? 4 : avg <= 8 ? 8 : avg <= 16 ? 16 : avg <= 32 ? 32 : 64;
-    return g.n <= SMALL_LEVEL_N ? std::min(64, 2 * b) : b;
+    return avg <= 4 ? 4 : avg <= 8 ? 8 : avg <= 16 ? 16 : avg <= 32 ? 32 : 64;
 }
+
+// Lanes per candidate in the afterburner: doubled on levels of up to
+// SMALL_LEVEL_N vertices, whose few candidates leave the GPU idle otherwise.
+// Lanes per move in the update passes: doubled on every level, so that a
+// lane's chain of table updates is shorter (each measured 0.5 ms less per
+// 128^3 stencil partition; doubling the afterburner's on large levels was
+// slower).
+static int group_for_afterburner(const DGraph& g) {
+    return g.n <= SMALL_LEVEL_N ? std::min(64, 2 * group_for_avg(g)) : group_for_avg(g);
+}
+static int group_for_update(const DGraph& g) { return std::min(64, 2 * group_for_avg(g)); }
 
 __global__ void __launch_bounds__(256) k_max_degree(DGraph g, int32_t* out) {
     __shared__ int32_t red[256 / WAVE];
@@ -1847,7 +1855,7 @@ void Refiner::jet_lp(const Level& L, const part_t* part, double fr) {
 #define JET_AB(GG)                                                                                          \
     k_afterburner_small<GG><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, \
                                                             nmax_, stamp_, gen_)
-    switch (group_for_avg(g)) {
+    switch (group_for_afterburner(g)) {
         case 4: JET_AB(4); break;
         case 8: JET_AB(8); break;
         case 16: JET_AB(16); break;
@@ -2047,7 +2055,7 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
     const int32_t k = p_.k;
     int32_t* c = cnt();
     unsigned long long* usz = reinterpret_cast<unsigned long long*>(sizes);
-    int G = group_for_avg(g);
+    int G = group_for_update(g);
     int nbs = grid_for((int64_t)g.n * G, 256, 1024);
     const int64_t seq = ++mailbox_->seq;
     const Publish pb{pub_, sizes, k, res_dev_, c, seq};
