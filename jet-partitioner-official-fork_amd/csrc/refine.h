@@ -110,7 +110,7 @@ class Refiner {
     vid_t* cand_;
     vid_t* moves_;
     MoveRec* mrec_;  // per listed move: row, vertex and new part (first pass to second)
-    int32_t* swap_bit_;
+    int32_t* swap_bit_;  // update_large: rows marked for rebuilding (swap_bit, jet_refiner.hpp:747-772)
     uint32_t* sval_;
     uint32_t* vtmp_;
     int64_t* prefix_;
@@ -149,7 +149,7 @@ class Refiner {
     int32_t small_max_ = 0;         // rows of degree <= this take the thread-per-row table build
     bool rebuild_pending_ = false;  // update_large marked rows still to rebuild
     Mailbox* mailbox_ = nullptr;    // the creating thread's mailbox (results and sequence words)
-    int64_t* pub_ = nullptr;      // cut change part 2: PUB_GROUPS partials, 128 bytes apart
+    int64_t* pub_ = nullptr;      // the cut change: PUB_GROUPS partials, 128 bytes apart
     int64_t* bcopies_ = nullptr;  // ncopies_ - 1 more bucket arrays, kept zero between uses
     int32_t ncopies_ = 1;
     int64_t nbuckets_ = 0;
