@@ -1347,14 +1347,9 @@ __global__ void __launch_bounds__(CL_WAVES* WAVE)
 // smaller id. Returns 0 (kept), 1 (evicted, degree <= BIG_DEG) or 2 (evicted,
 // larger); the caller appends evicted members to the move list.
 template <bool STRONG>
-__device__ inline int rank_decide(vid_t v, int32_t b, int64_t pos, const part_t* part, const int64_t* off,
-                                  const int64_t* sizes, int64_t size_max, int64_t width, const wgt_t* vw,
-                                  const eoff_t* rowptr, int64_t* evict_adjust, int64_t* score, int32_t* stamp,
-                                  int32_t gen) {
-    // the member's part, weight and row are loaded together
-    const part_t p = part[v];
-    const int64_t w = vw[v];
-    const eoff_t deg = rowptr[v + 1] - rowptr[v];
+__device__ inline int rank_decide_with(vid_t v, int32_t b, int64_t pos, part_t p, int64_t w, bool big,
+                                       const int64_t* off, const int64_t* sizes, int64_t size_max, int64_t width,
+                                       int64_t* evict_adjust, int64_t* score, int32_t* stamp, int32_t gen) {
     int64_t base = off[b] - off[(int64_t)p * width];
     int64_t limit = sizes[p] - size_max;
     int64_t bs = off[b + 1] - off[b];
@@ -1370,7 +1365,20 @@ __device__ inline int rank_decide(vid_t v, int32_t b, int64_t pos, const part_t*
     }
     if (!ev) return 0;
     stamp[v] = gen;
-    return deg > BIG_DEG ? 2 : 1;
+    return big ? 2 : 1;
+}
+
+template <bool STRONG>
+__device__ inline int rank_decide(vid_t v, int32_t b, int64_t pos, const part_t* part, const int64_t* off,
+                                  const int64_t* sizes, int64_t size_max, int64_t width, const wgt_t* vw,
+                                  const eoff_t* rowptr, int64_t* evict_adjust, int64_t* score, int32_t* stamp,
+                                  int32_t gen) {
+    // the member's part, weight and row are loaded together
+    const part_t p = part[v];
+    const int64_t w = vw[v];
+    const eoff_t deg = rowptr[v + 1] - rowptr[v];
+    return rank_decide_with<STRONG>(v, b, pos, p, w, deg > BIG_DEG, off, sizes, size_max, width, evict_adjust, score,
+                                    stamp, gen);
 }
 
 
@@ -1388,6 +1396,8 @@ __global__ void __launch_bounds__(RS_BLOCK)
     __shared__ uint64_t keys[RS_CAP];
     __shared__ int64_t pre[RS_CAP];
     __shared__ int64_t ws[RS_BLOCK / WAVE + 1];
+    __shared__ uint32_t pay_w[RS_BLOCK];
+    __shared__ uint32_t pay_p[RS_BLOCK];
     // each thread's first member is loaded together with the count
     const vid_t m0 = (vid_t)threadIdx.x < nmax ? rlist[threadIdx.x] : 0;
     const int32_t R = counters[5];
@@ -1401,31 +1411,74 @@ __global__ void __launch_bounds__(RS_BLOCK)
     uint32_t cap = 1;
     while ((int32_t)cap < R) cap <<= 1;
     if (cap <= RS_BLOCK) {
-        // one key per thread: exchanges within a wave are shuffles, only
-        // strides of a wave or more go through LDS (10 barriers instead of
-        // 55 for 1024 keys)
+        // one member per thread, its weight, part and degree class loaded
+        // with its bucket and carried through the sort: exchanges within a
+        // wave are shuffles, only strides of a wave or more go through LDS
+        // (10 barriers instead of 55 for 1024 keys)
         const uint32_t t = threadIdx.x;
         uint64_t key = ~0ull;
-        if ((int32_t)t < R) key = ((uint64_t)(uint32_t)bid[m0] << 32) | (uint32_t)m0;
+        uint32_t pw = 0, pp = 0;  // weight; part | big << 31
+        if ((int32_t)t < R) {
+            const int32_t bb = bid[m0];
+            const wgt_t w0 = vw[m0];
+            const part_t p0 = part[m0];
+            const eoff_t d0 = rowptr[m0 + 1] - rowptr[m0];
+            key = ((uint64_t)(uint32_t)bb << 32) | (uint32_t)m0;
+            pw = (uint32_t)w0;
+            pp = (uint32_t)p0 | (d0 > BIG_DEG ? 0x80000000u : 0u);
+        }
         for (uint32_t size = 2; size <= cap; size <<= 1) {
             for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
                 uint64_t other;
+                uint32_t opw, opp;
                 if (stride >= WAVE) {
                     keys[t] = key;
+                    pay_w[t] = pw;
+                    pay_p[t] = pp;
                     __syncthreads();
                     other = keys[t ^ stride];
+                    opw = pay_w[t ^ stride];
+                    opp = pay_p[t ^ stride];
                     __syncthreads();
                 } else {
                     const uint32_t lo = __shfl_xor((uint32_t)key, (int)stride, WAVE);
                     const uint32_t hi = __shfl_xor((uint32_t)(key >> 32), (int)stride, WAVE);
                     other = ((uint64_t)hi << 32) | lo;
+                    opw = __shfl_xor(pw, (int)stride, WAVE);
+                    opp = __shfl_xor(pp, (int)stride, WAVE);
                 }
                 // ascending blocks keep the smaller key at the lower index
                 const bool lower = (t & stride) == 0, up = (t & size) == 0;
-                key = (lower == up) ? (key < other ? key : other) : (key > other ? key : other);
+                const bool take = (lower == up) ? (other < key) : (other > key);
+                if (take) {
+                    key = other;
+                    pw = opw;
+                    pp = opp;
+                }
             }
         }
         keys[t] = key;
+        int64_t tot;
+        const int64_t ex = block_excl_scan<int64_t, RS_BLOCK>((int32_t)t < R ? (int64_t)(int32_t)pw : 0, tot, ws);
+        pre[t] = ex;
+        __syncthreads();
+        if ((int32_t)t < R) {
+            const int32_t b = (int32_t)(key >> 32);
+            const uint64_t lo_key = (uint64_t)(uint32_t)b << 32;
+            int32_t lo = 0, hi = (int32_t)t;
+            while (lo < hi) {
+                int32_t mid = (lo + hi) >> 1;
+                if (keys[mid] < lo_key) lo = mid + 1;
+                else hi = mid;
+            }
+            const vid_t v = (vid_t)(key & 0xffffffffull);
+            const int code = rank_decide_with<STRONG>(v, b, pre[t] - pre[lo], (part_t)(pp & 0x7fffffffu),
+                                                      (int64_t)(int32_t)pw, (pp >> 31) != 0, off, sizes, size_max,
+                                                      width, evict_adjust, score, stamp, gen);
+            if (code == 2) moves[nmax - 1 - atomicAdd(&mcounters[3], 1)] = v;
+            else if (code == 1) moves[atomicAdd(&mcounters[2], 1)] = v;
+        }
+        return;
     } else {
         for (uint32_t t = threadIdx.x; t < cap; t += RS_BLOCK) {
             if ((int32_t)t < R) {
