@@ -314,7 +314,9 @@ typedef struct jet_job_times_t {
 
 /* workers in [1, 16]; budget_bytes <= 0 means 80% of the device's free memory. */
 int jet_ctx_create(int32_t device, int32_t workers, int64_t budget_bytes, jet_ctx_t** out);
-/* Waits for running jobs, then frees the context (uncollected results are dropped). */
+/* Waits for running jobs, then frees the context (uncollected results are
+ * dropped). No jet_submit, jet_poll or jet_partition_batch call on the same
+ * context may run concurrently with it or start after it. */
 int jet_ctx_destroy(jet_ctx_t* ctx);
 
 /* Queues one job with jet_partition_csr's arguments. The arrays (host or
@@ -326,7 +328,8 @@ int jet_submit(jet_ctx_t* ctx, const jet_config_t* cfg, int64_t n, const int64_t
 
 /* Result of a job: JET_OK (writes *cut_out, stats, times; the ticket is then
  * consumed), the job's error code, or JET_EAGAIN while it is queued or
- * running. wait_ms: 0 polls, > 0 waits up to that long, < 0 waits until done. */
+ * running. wait_ms: 0 polls, > 0 waits up to that long, < 0 waits until done.
+ * One thread at a time may wait for a ticket: a second waiter gets JET_EINVAL. */
 int jet_poll(jet_ctx_t* ctx, int64_t ticket, int32_t wait_ms, int64_t* cut_out, jet_stats_t* stats,
              jet_job_times_t* times);
 

@@ -41,6 +41,9 @@ struct Job {
     uint64_t seed = 0;
     int32_t* part_out = nullptr;
     int64_t footprint = 0;
+    // a thread is waiting in jet_poll for this job; only that thread may
+    // collect (and free) it
+    bool claimed = false;
     // results
     bool done = false;
     int rc = JET_EAGAIN;
@@ -254,11 +257,21 @@ int jet_poll(jet_ctx_t* X, int64_t ticket, int32_t wait_ms, int64_t* cut_out, je
         return JET_EINVAL;
     }
     Job* j = it->second;
+    if (j->claimed) {
+        jet::set_last_error("another thread is already waiting for this ticket");
+        return JET_EINVAL;
+    }
     if (!j->done) {
         if (wait_ms == 0) return JET_EAGAIN;
+        // while this thread sleeps, the job stays in X->jobs and no other
+        // poll may collect it, so `j` stays valid
+        j->claimed = true;
         auto pred = [&] { return j->done; };
+        bool ok = true;
         if (wait_ms < 0) X->cv_done.wait(lk, pred);
-        else if (!X->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), pred)) return JET_EAGAIN;
+        else ok = X->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), pred);
+        j->claimed = false;
+        if (!ok) return JET_EAGAIN;
     }
     int rc = j->rc;
     if (cut_out) *cut_out = j->cut;

@@ -71,7 +71,7 @@ int jet_load_config(const char* path, jet_config_t* cfg) {
 // one line of 1-based neighbours per vertex, with a weight after each
 // neighbour when fmt is 1. Vertex weights (fmt 10/11) and ncon are not
 // supported, as in the reference.
-int jet_load_metis(const char* path, jet_host_graph_t* out) {
+static int load_metis(const char* path, jet_host_graph_t* out) {
     if (!path || !out) return fail(JET_EINVAL, "path and out are required");
     std::memset(out, 0, sizeof(*out));
     std::vector<char> buf;
@@ -88,7 +88,13 @@ int jet_load_metis(const char* path, jet_host_graph_t* out) {
     while (f < fmax && *f != '\n') {
         if (std::isdigit((unsigned char)*f)) {
             uint64_t v = 0;
-            while (f < fmax && std::isdigit((unsigned char)*f)) v = v * 10 + (uint64_t)(*f++ - '0');
+            bool big = false;
+            while (f < fmax && std::isdigit((unsigned char)*f)) {
+                if (v > (UINT64_MAX - 9) / 10) big = true;
+                else v = v * 10 + (uint64_t)(*f - '0');
+                f++;
+            }
+            if (big) return fail(JET_EINVAL, "metis header value out of range");
             if (nh < 4) header[nh] = v;
             nh++;
         } else {
@@ -102,6 +108,8 @@ int jet_load_metis(const char* path, jet_host_graph_t* out) {
     if (fmt != 0 && fmt != 1) return fail(JET_EUNSUPPORTED, "unsupported metis format flags (vertex weights)");
     if (ncon != 0) return fail(JET_EUNSUPPORTED, "unsupported metis ncon");
     if (n < 1 || n > 2147483645LL) return fail(JET_EINVAL, "metis n out of range");
+    // every stored neighbour takes at least two characters of the file
+    if (header[1] > (uint64_t)buf.size()) return fail(JET_EINVAL, "metis m exceeds what the file can hold");
     const bool has_ew = fmt == 1;
     std::vector<int64_t> rp((size_t)n + 1, 0);
     std::vector<int32_t> col, ew;
@@ -120,11 +128,15 @@ int jet_load_metis(const char* path, jet_host_graph_t* out) {
         }
         if (f >= fmax) break;
         int64_t v = 0;
-        while (f < fmax && std::isdigit((unsigned char)*f)) v = v * 10 + (*f++ - '0');
+        while (f < fmax && std::isdigit((unsigned char)*f)) {
+            if (v <= INT32_MAX) v = v * 10 + (*f - '0');  // saturates above INT32_MAX * 10
+            f++;
+        }
         if (!is_value) {
             if (v < 1 || v > n) return fail(JET_EINVAL, "metis neighbour id out of range");
             col.push_back((int32_t)(v - 1));
         } else {
+            if (v > INT32_MAX) return fail(JET_EINVAL, "metis edge weight out of range");
             ew.push_back((int32_t)v);
         }
         if (has_ew) is_value = !is_value;
@@ -146,6 +158,18 @@ int jet_load_metis(const char* path, jet_host_graph_t* out) {
     if (has_ew) std::memcpy(out->ew, ew.data(), sizeof(int32_t) * ew.size());
     out->uniform_ew = has_ew ? 0 : 1;
     return JET_OK;
+}
+
+int jet_load_metis(const char* path, jet_host_graph_t* out) {
+    try {
+        return load_metis(path, out);
+    } catch (const std::bad_alloc&) {
+        if (out) jet_free_graph(out);
+        return fail(JET_ENOMEM, "host allocation failed while reading the metis file");
+    } catch (const std::exception& e) {
+        if (out) jet_free_graph(out);
+        return fail(JET_EINVAL, std::string("metis file: ") + e.what());
+    }
 }
 
 void jet_free_graph(jet_host_graph_t* g) {

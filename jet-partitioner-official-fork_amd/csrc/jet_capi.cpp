@@ -689,6 +689,8 @@ int jet_map_construct(int64_t n, const int32_t* hn, int32_t ptr_space, int32_t* 
         Context& C = context();
         CallScope call(C);
         const vid_t* dh = to_device(C, hn, n, ptr_space);
+        // the claim rounds index stamp[hn[u]] and claim[hn[u]]
+        if (!parts_in_range(n, dh, (int32_t)n, C.scratch, C.stream)) throw InvalidInput("hn entries must be in [0, n)");
         vid_t* map = ptr_space == JET_PTR_DEVICE ? map_out : C.persist.alloc<vid_t>(n);
         *nc_out = map_construct((vid_t)n, dh, map, C.scratch, C.stream);
         from_device(C, map_out, map, n, ptr_space);
@@ -704,6 +706,8 @@ int jet_contract(int64_t n, const int64_t* row_offsets, const int32_t* col, cons
         Level L = stage_graph(C, n, row_offsets, nullptr, col, ew, nullptr, vw, nullptr, ptr_space, false);
         if (nc < 1 || nc > n) throw InvalidInput("nc must be in [1, n]");
         const vid_t* dmap = to_device(C, map, n, ptr_space);
+        // the counting pass adds into per-coarse-vertex arrays at map[i]
+        if (!parts_in_range(n, dmap, nc, C.scratch, C.stream)) throw InvalidInput("map entries must be in [0, nc)");
         Level out;
         contract(L.view(), dmap, nc, C.persist, C.scratch, C.stream, out);
         from_device(C, out_row_offsets, out.rowptr, (int64_t)nc + 1, ptr_space);

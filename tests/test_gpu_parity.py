@@ -344,6 +344,32 @@ def test_part_ids_out_of_range_rejected():
         assert e.value.code == jet_amd.JET_EINVAL
 
 
+def test_map_and_hn_entries_out_of_range_rejected():
+    # jet_contract indexes per-coarse-vertex arrays by map[i] and
+    # jet_map_construct indexes claim arrays by hn[u]: both are range-checked
+    # on the device before any kernel uses them
+    g = graphs.grid2d(16, 16)
+    n = 256
+    mp = (np.arange(n) // 2).astype(np.int32)
+    for bad in (128, -1, 1 << 30):
+        m2 = mp.copy()
+        m2[37] = bad
+        say("contract with map entry", bad)
+        with pytest.raises(jet_amd.JetError) as e:
+            jet_amd.contract(g, m2, 128)
+        assert e.value.code == jet_amd.JET_EINVAL
+    for bad in (n, -5):
+        hn = np.arange(n, dtype=np.int32)[::-1].copy()
+        hn[11] = bad
+        say("map_construct with hn entry", bad)
+        with pytest.raises(jet_amd.JetError) as e:
+            jet_amd.map_construct(hn)
+        assert e.value.code == jet_amd.JET_EINVAL
+    # the checks pass valid inputs through unchanged
+    cg = jet_amd.contract(g, mp, 128)
+    assert len(cg["rowptr"]) == 129
+
+
 def test_cli_jet_and_pstat(tmp_path):
     from test_io_cpu import write_metis
     pkg = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "jet-partitioner-official-fork_amd")

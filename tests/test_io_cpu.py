@@ -54,6 +54,10 @@ def test_metis_isolated_vertices_and_comment_lines(tmp_path):
     ("3 2 0 2\n2\n1\n\n", jet_amd.JET_EUNSUPPORTED),    # ncon
     ("3 2\n2\n1\n", jet_amd.JET_EINVAL),                # too few lines / nonzeros
     ("3 1\n4\n\n\n", jet_amd.JET_EINVAL),               # neighbour out of range
+    ("3 4611686018427387904\n2\n1\n\n", jet_amd.JET_EINVAL),  # m that no reservation can hold
+    ("3 99999999999999999999999\n2\n1\n\n", jet_amd.JET_EINVAL),  # header value overflows 64 bits
+    ("2 1 1\n2 99999999999\n1 99999999999\n", jet_amd.JET_EINVAL),  # edge weight beyond int32
+    ("2 1\n99999999999999999999999999\n1\n", jet_amd.JET_EINVAL),  # neighbour id overflows
 ])
 def test_metis_errors(tmp_path, text, code):
     p = tmp_path / "g.graph"
