@@ -5,13 +5,19 @@
 // direct mapping when the table size is k, linear probing from p % size
 // otherwise, shrink to used + max(used/4, 3), HASH_RECLAIM slots, dense
 // overflow append) so their contents, sizes and zero entries evolve exactly
-// as in the reference. Slot order inside a table is an accident of insertion
-// timing on every parallel backend; every decision that reads a table here is
-// made independent of it: among parts with equal connectivity the lowest part
-// id wins (for direct-mapped tables that is the reference's first-in-slot
-// rule). Rebalancer minibucket positions follow increasing vertex id (a stable
-// sort by bucket id), and the strong rebalancer's destination re-draws run in
-// increasing vertex id.
+// as in the reference. Their slot order is not kept: concurrent updates here
+// place new parts in whichever free slot they reach first. Every decision
+// that reads a table is therefore made independent of slot order: among parts
+// with equal connectivity the lowest part id wins. The reference keeps the
+// first maximum in slot order (jet_refiner.hpp:226-233, :603-611). For
+// direct-mapped rows (table size k) the two rules agree. For hashed rows
+// (size < k) they differ whenever the tied parts' slots are not in id order,
+// e.g. parts 5 and 2 in a 4-slot table sit in slots 1 and 2 and the reference
+// takes 5; such rows are built serially in edge order by the reference when
+// the average degree is below 8, so there its choice is fixed and the lowest-id
+// choice is not one the reference can make. Rebalancer minibucket positions
+// follow increasing vertex id (a stable sort by bucket id), and the strong
+// rebalancer's destination re-draws run in increasing vertex id.
 #include <algorithm>
 #include <chrono>
 #include <cmath>

@@ -21,6 +21,7 @@
 #include "jet_oracle_prims.h"
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
 #include <limits>
@@ -419,6 +420,21 @@ static int32_t optimal_size(int32_t total, int32_t k) { return (total + k - 1) /
 
 // ============================================================== refinement
 
+// Planted mistakes for the mutation tests (tests/test_refsim_refine.py):
+// oracle/Makefile builds one extra library per ORC_MUTANT value, each with one
+// deliberate one-line error; the tests check that the independent Python
+// restatement of the refinement disagrees with every one of them.
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
+// Tie-break between parts of equal connectivity in jet_lp and
+// rebalance_weak: false = lowest part id (the GPU path's rule); true = the
+// first maximum in table-slot order, as the reference's loops keep it
+// (jet_refiner.hpp:226-233, :603-611), with the slot layout of the reference's
+// Serial backend. Used only to measure what the rule costs.
+static bool g_slot_ties = false;
+
 struct RefineData {  // jet_refiner.hpp:100-106
     std::vector<int64_t> part_sizes;
     int64_t total_size = 0;
@@ -511,6 +527,54 @@ class Refiner {
         conn_size[i] = size;
     }
 
+    // jet_refiner.hpp:648-739 (build_row_cdata_large) as the Serial backend
+    // runs it (team size 1): insertion in edge order into the full-capacity
+    // table, then, when it shrinks, a copy of the parts in slot order into the
+    // smaller table. Same content and size as build_row; only the slot layout
+    // differs, and only the slot-order tie-break reads it.
+    void build_row_large(const int32_t* part, int32_t i) {
+        int64_t gs = conn_off[i], ge = conn_off[i + 1];
+        int32_t size = (int32_t)(ge - gs);
+        std::vector<int32_t> se(size, NULL_PART);
+        std::vector<int64_t> sv(size, 0);
+        int32_t used = 0;
+        for (int64_t j = g->rowptr[i]; j < g->rowptr[i + 1]; j++) {
+            int32_t p = part[g->col[j]];
+            int32_t po = p % size;
+            if (size != k) {
+                while (se[po] != NULL_PART && se[po] != p) po = (po + 1) % size;
+            }
+            if (se[po] == NULL_PART) {
+                se[po] = p;
+                used++;
+            }
+            sv[po] += g->ew[j];
+        }
+        int32_t old = size;
+        size = used + std::max(used / 4, 3);
+        for (int64_t j = gs; j < ge; j++) {
+            conn_entry[j] = NULL_PART;
+            conn_val[j] = 0;
+        }
+        if (size < old) {
+            for (int32_t j = 0; j < old; j++) {
+                int32_t p = se[j];
+                if (p <= NULL_PART) continue;
+                int32_t po = p % size;
+                while (conn_entry[gs + po] != NULL_PART) po = (po + 1) % size;
+                conn_entry[gs + po] = p;
+                conn_val[gs + po] = sv[j];
+            }
+        } else {
+            size = old;
+            for (int32_t j = 0; j < old; j++) {
+                conn_entry[gs + j] = se[j];
+                conn_val[gs + j] = sv[j];
+            }
+        }
+        conn_size[i] = size;
+    }
+
     // jet_refiner.hpp:958-1048 (init_conn_data).
     void init_conn(const int32_t* part) {
         int32_t n = g->n;
@@ -525,13 +589,19 @@ class Refiner {
         std::fill(conn_val.begin(), conn_val.begin() + cap, 0);
         std::fill(dest_cache.begin(), dest_cache.begin() + n, NULL_PART);
         std::fill(lock_bit.begin(), lock_bit.begin() + n, 0);
-        for (int32_t i = 0; i < n; i++) build_row(part, i);
+        // the reference's team build for average degree >= 8 (:985)
+        const bool large = g_slot_ties && n > 0 && g->m() / n >= 8;
+        for (int32_t i = 0; i < n; i++) {
+            if (large) build_row_large(part, i);
+            else build_row(part, i);
+        }
     }
 
     // Canonical, slot-order-free argmax: highest connectivity, lowest part id
     // among equals. The reference keeps the first maximum in table-slot order
-    // (jet_refiner.hpp:230-237, :603-611); for direct-mapped rows (table size
-    // == k) slot order is part order, so the two agree there.
+    // (jet_refiner.hpp:226-233, :603-611); for direct-mapped rows (table size
+    // == k) slot order is part order, so the two agree there, and for hashed
+    // rows they can differ (g_slot_ties selects the reference's rule).
     // jet_refiner.hpp:200-323 (jet_lp).
     std::vector<int32_t> jet_lp(const int32_t* part, double filter_ratio) {
         int32_t n = g->n;
@@ -552,7 +622,7 @@ class Refiner {
                 if (jc > b_conn && e != p) {
                     best = e;
                     b_conn = jc;
-                } else if (jc == b_conn && b_conn > 0 && e != p && e < best) {
+                } else if (!g_slot_ties && jc == b_conn && b_conn > 0 && e != p && e < best) {
                     best = e;
                 } else if (jc > 0 && e == p) {
                     p_conn = jc;
@@ -560,7 +630,11 @@ class Refiner {
             }
             save_gp[i] = 0;
             if (best != NO_MOVE) {
+#if ORC_MUTANT == 6
+                if (b_conn > p_conn || ((double)(p_conn - b_conn) < std::floor(filter_ratio * (double)p_conn))) {
+#else
                 if (b_conn >= p_conn || ((double)(p_conn - b_conn) < std::floor(filter_ratio * (double)p_conn))) {
+#endif
                     save_gp[i] = b_conn - p_conn;
                 } else {
                     best = NO_MOVE;
@@ -636,7 +710,11 @@ class Refiner {
             if (sizes[p] < max_dest) undersized[tu++] = p;
         for (int32_t i = 0; i < n; i++) {
             int32_t p = part[i];
+#if ORC_MUTANT == 7
+            if (sizes[p] > smax && (double)vw[i] <= 1.5 * (double)(sizes[p] - opt)) {
+#else
             if (sizes[p] > smax && (double)vw[i] < 1.5 * (double)(sizes[p] - opt)) {
+#endif
                 int32_t best = p;
                 int64_t gain = 0, p_gain = 0;
                 int64_t start = conn_off[i], end = start + conn_size[i];
@@ -647,7 +725,7 @@ class Refiner {
                         if (jg > gain) {
                             best = pj;
                             gain = jg;
-                        } else if (jg == gain && gain > 0 && pj < best) {
+                        } else if (!g_slot_ties && jg == gain && gain > 0 && pj < best) {
                             best = pj;
                         }
                     }
@@ -734,11 +812,20 @@ class Refiner {
                     }
                 }
                 if (tk == 0) tk = 1;
+#if ORC_MUTANT == 2
+                int64_t gain = (int64_t)(tg / tk);
+                (void)p_gain;
+#else
                 int64_t gain = (int64_t)(tg / tk) - p_gain;
+#endif
                 int64_t vwgt = std::min<int64_t>(vw[i], sizes[p] - smax);
                 int32_t gt = gain_bucket(gain, vwgt);
                 if (gt < MAX_BUCKETS) {
+#if ORC_MUTANT == 1
+                    int64_t gid = ((int64_t)MAX_BUCKETS * p + gt) * sections + (i % sections);
+#else
                     int64_t gid = ((int64_t)MAX_BUCKETS * p + gt) * sections + (i % sections) + 1;
+#endif
                     bid[i] = (int32_t)gid;
                     pos[i] = buckets[gid];
                     buckets[gid] += vw[i];
@@ -761,7 +848,11 @@ class Refiner {
             int64_t score = pos[i] + buckets[b] - buckets[begin];
             int64_t limit = sizes[p] - smax;
             if (score < limit) {
+#if ORC_MUTANT == 3
+                if (score + vw[i] > limit) evict_adjust[p] = score + vw[i];
+#else
                 if (score + vw[i] >= limit) evict_adjust[p] = score + vw[i];
+#endif
                 moves.push_back(i);
             }
         }
@@ -770,7 +861,9 @@ class Refiner {
         for (int32_t p = 0; p < k; p++) {
             int64_t add = evict_adjust[p];
             int64_t begin = (int64_t)MAX_BUCKETS * p * sections;
+#if ORC_MUTANT != 4
             if (add == 0) add = buckets[begin + width] - buckets[begin];
+#endif
             evict_adjust[p] = buckets[begin] - acc;
             acc += add;
         }
@@ -782,6 +875,9 @@ class Refiner {
             evict_start[p + 1] = acc2;
         }
         for (int32_t v : moves) save[v] = pos[v] + buckets[bid[v]] - evict_adjust[part[v]];  // (:519-525)
+#if ORC_MUTANT == 5
+        std::reverse(moves.begin(), moves.end());  // re-draws in decreasing id
+#endif
         for (int32_t v : moves) {                                                           // (:526-544)
             int32_t p = 0;
             bool done = false;
@@ -799,6 +895,9 @@ class Refiner {
             }
             if (!done) dest[v] = part[v];
         }
+#if ORC_MUTANT == 5
+        std::reverse(moves.begin(), moves.end());
+#endif
         return moves;
     }
 
@@ -896,7 +995,8 @@ class Refiner {
                 conn_entry[j] = NULL_PART;
                 conn_val[j] = 0;
             }
-            build_row(part, v);
+            if (g_slot_ties) build_row_large(part, v);
+            else build_row(part, v);
             swap_bit[v] = 0;
             dest_cache[v] = NULL_PART;
         }
@@ -915,18 +1015,29 @@ class Refiner {
             part[i] = b;
             dest[i] = p;
         }
+#if ORC_MUTANT == 9
+        if ((int32_t)moves.size() > g->n / 5) update_large(part, moves);
+#else
         if ((int32_t)moves.size() > g->n / 10) update_large(part, moves);
+#endif
         else update_small(part, moves);
         int64_t cc2 = 0;
         for (int32_t i : moves) cc2 += lookup(i, part[i]) - lookup(i, dest[i]);
         int64_t mx = std::numeric_limits<int64_t>::min();
         for (int32_t p = 0; p < k; p++) mx = std::max(mx, st.part_sizes[p]);
         st.total_imb = mx > opt ? mx - opt : 0;
+#if ORC_MUTANT == 8
+        st.cut -= cc1;
+#else
         st.cut -= cc1 + cc2;
+#endif
     }
 
     // jet_refiner.hpp:1050-1142 (jet_refine).
     bool changed = false;  // whether the last refine() replaced the best partition
+    // optional per-iteration record: kind (0 lp, 1 weak, 2 strong), moves,
+    // cut, total_imb, take (0 none, 1 balance, 2 cut), count after the step
+    std::vector<std::array<int64_t, 6>>* trace = nullptr;
 
     LevelRecord refine(const Graph& lg, const int32_t* lvw, int32_t* best_part, bool uniform_ew, double imb, double tol,
                        bool ultra, RefineData& best) {
@@ -966,6 +1077,7 @@ class Refiner {
             int count = 0;
             while (count++ <= 11) {
                 rec.iterations++;
+                const int32_t lp_before = rec.lp_iterations;
                 std::vector<int32_t> moves;
                 if (curr.total_imb <= imb_max) {
                     moves = jet_lp(part.data(), fr);
@@ -977,16 +1089,34 @@ class Refiner {
                     balance_counter++;
                 }
                 perform_moves(part.data(), moves, curr);
+                int take = 0;
                 if (best.total_imb > imb_max && curr.total_imb < best.total_imb) {
                     best = curr;
                     std::copy(part.begin(), part.end(), best_part);
                     count = 0;
                     changed = true;
+                    take = 1;
+#if ORC_MUTANT == 10
+                } else if (curr.cut < best.cut && (curr.total_imb <= imb_max || curr.total_imb < best.total_imb)) {
+#elif ORC_MUTANT == 11
+                } else if (curr.cut <= best.cut && (curr.total_imb <= imb_max || curr.total_imb <= best.total_imb)) {
+#else
                 } else if (curr.cut < best.cut && (curr.total_imb <= imb_max || curr.total_imb <= best.total_imb)) {
+#endif
+#if ORC_MUTANT == 12
+                    count = 0;  // the tolerance ignored
+                    (void)tol;
+#else
                     if ((double)curr.cut < tol * (double)best.cut) count = 0;
+#endif
                     best = curr;
                     std::copy(part.begin(), part.end(), best_part);
                     changed = true;
+                    take = 2;
+                }
+                if (trace) {
+                    int kind = rec.lp_iterations > lp_before ? 0 : (balance_counter <= 2 ? 1 : 2);
+                    trace->push_back({kind, (int64_t)moves.size(), curr.cut, curr.total_imb, take, count});
                 }
             }
         }
@@ -1225,6 +1355,35 @@ int orc_refine_level(int32_t n, const int64_t* rowptr, const int32_t* col, const
     rec_out[7] = r.size_max;
     return 0;
 }
+
+// orc_refine_level plus the per-iteration trace: trace_out holds up to
+// max_iters records of 6 int64 (kind 0 lp / 1 weak / 2 strong, moves, cut2,
+// total_imb, take 0 none / 1 balance / 2 cut, count after the step). Returns
+// the number of iterations.
+int32_t orc_refine_level_trace(int32_t n, const int64_t* rowptr, const int32_t* col, const int32_t* ew,
+                               const int32_t* vw, int32_t k, double imb, double tol, int32_t ultra, int32_t uniform,
+                               int32_t* part_io, int64_t* rec_out, int64_t* trace_out, int32_t max_iters) {
+    Graph g = make_graph(n, rowptr, col, ew);
+    Refiner r(n, conn_capacity(g, k), k);
+    RefineData rfd;
+    std::vector<std::array<int64_t, 6>> trace;
+    r.trace = &trace;
+    LevelRecord rec = r.refine(g, vw, part_io, uniform != 0, imb, tol, ultra != 0, rfd);
+    rec_out[0] = rec.cut2;
+    rec_out[1] = rec.total_imb;
+    rec_out[2] = rec.n;
+    rec_out[3] = rec.m;
+    rec_out[4] = rec.iterations;
+    rec_out[5] = rec.lp_iterations;
+    rec_out[6] = r.opt;
+    rec_out[7] = r.size_max;
+    for (size_t t = 0; t < trace.size() && (int32_t)t < max_iters; t++)
+        for (int c = 0; c < 6; c++) trace_out[6 * t + c] = trace[t][c];
+    return (int32_t)trace.size();
+}
+
+// Selects the tie-break between parts of equal connectivity (see g_slot_ties).
+void orc_set_slot_order_ties(int32_t on) { g_slot_ties = on != 0; }
 
 // Full multilevel partition. levels_out (optional, 8 int64 per level, coarsest
 // first): cut2, total_imb, n, m, iterations, lp_iterations, 0, 0.

@@ -22,12 +22,30 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+_mutants = {}
+
+
+def mutant(idx):
+    """The oracle built with planted mistake ORC_MUTANT=idx (mutation tests only)."""
+    if idx not in _mutants:
+        path = os.path.join(_HERE, "_build", f"libjet_oracle_mut{idx}.so")
+        if not os.path.exists(path):
+            build()
+        _mutants[idx] = _configure(ctypes.CDLL(path))
+    return _mutants[idx]
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
+        _lib = _configure(ctypes.CDLL(_LIB_PATH))
+    return _lib
+
+
+def _configure(L):
+    if True:
         c = ctypes
         L.orc_mix64.restype = c.c_uint64
         L.orc_mix64.argtypes = [c.c_uint64]
@@ -66,8 +84,12 @@ def lib():
                                     c.POINTER(c.c_int32), c.c_int32]
         L.jet_host_initpart.restype = c.c_int
         L.jet_host_initpart.argtypes = [c.c_int32, i64p, i32p, i32p, i32p, c.c_int32, c.c_double, c.c_uint64, i32p]
-        _lib = L
-    return _lib
+        L.orc_refine_level_trace.restype = c.c_int32
+        L.orc_refine_level_trace.argtypes = [c.c_int32, i64p, i32p, i32p, i32p, c.c_int32, c.c_double, c.c_double,
+                                             c.c_int32, c.c_int32, i32p, i64p, i64p, c.c_int32]
+        L.orc_set_slot_order_ties.restype = None
+        L.orc_set_slot_order_ties.argtypes = [c.c_int32]
+    return L
 
 
 def _g(g):
@@ -173,6 +195,25 @@ def refine_level(g, part, k, imb, tol=0.999, ultra=False, uniform=False):
     rec = np.zeros(8, dtype=np.int64)
     lib().orc_refine_level(n, rp, col, ew, _vw(g), k, imb, tol, int(ultra), int(uniform), p, rec)
     return p, rec
+
+
+def refine_level_trace(g, part, k, imb, tol=0.999, ultra=False, uniform=False, L=None, max_iters=4096):
+    """refine_level plus one record per iteration: (kind 0 lp / 1 weak / 2
+    strong, moves, cut2, total_imb, take 0 none / 1 balance / 2 cut, count).
+    L: a library from mutant(), or None for the oracle itself."""
+    n, rp, col, ew = _g(g)
+    p = np.array(part, dtype=np.int32)
+    rec = np.zeros(8, dtype=np.int64)
+    tr = np.zeros((max_iters, 6), dtype=np.int64)
+    it = (L or lib()).orc_refine_level_trace(n, rp, col, ew, _vw(g), k, imb, tol, int(ultra), int(uniform), p, rec,
+                                             tr, max_iters)
+    return p, rec, [tuple(int(x) for x in row) for row in tr[:min(it, max_iters)]]
+
+
+def set_slot_order_ties(on):
+    """Tie-break in jet_lp / rebalance_weak: False = lowest part id (default),
+    True = first maximum in the Serial backend's slot order."""
+    lib().orc_set_slot_order_ties(int(bool(on)))
 
 
 def partition(g, k, imb=1.03, alg=0, seed=0, tol=0.999, ultra=False, uniform=None, repeat_shortcut=False):
