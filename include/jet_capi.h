@@ -179,6 +179,20 @@ int jet_project(int64_t n_fine, const int32_t* map, const int32_t* coarse, int32
 int jet_edge_cut(int64_t n, const int64_t* row_offsets, const int32_t* col, const int32_t* ew, const int32_t* part,
                  int32_t ptr_space, int64_t* cut2_out);
 
+/* jet_edge_cut without any allocation. It has no counterpart in the
+ * reference: it serves callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Every array is device memory (ptr_space JET_PTR_DEVICE implied; ew may be
+ * NULL for unit weights); *cut2_out is host memory. Validation and result are
+ * those of jet_edge_cut. The only device memory it uses besides the arguments
+ * is a fixed 256-byte array that each GPU holds from the library's first use
+ * until the process ends (error word, weight totals, cut accumulator); its
+ * size does not depend on n or m, so no graph is too large for it. Calls on
+ * one device are serialised by a per-device lock; it calls no hipMalloc,
+ * hipFree or allocating routine and runs on the null stream. */
+int jet_edge_cut_noalloc(int64_t n, const int64_t* row_offsets, const int32_t* col, const int32_t* ew,
+                         const int32_t* part, int64_t* cut2_out);
+
 /* part_stat.hpp:146-153: sizes_out[k] = vertex weight per part. */
 int jet_part_sizes(int64_t n, const int32_t* vw, const int32_t* part, int32_t k, int32_t ptr_space,
                    int64_t* sizes_out);
@@ -345,6 +359,15 @@ int jet_partition_batch(jet_ctx_t* ctx, int32_t count, const jet_config_t* cfgs,
                         const int64_t* vtx_offsets, const int64_t* edge_offsets, const int64_t* row_offsets,
                         const int32_t* col, const int32_t* ew, const int32_t* vw, const int32_t* uniform_ew,
                         int32_t ptr_space, int32_t* part_out, int64_t* cuts_out, jet_stats_t* stats);
+
+/* ---- Tuning parameters: launch shapes and path thresholds that change the
+ * speed but never the result (every setting gives the same partition). They
+ * are process-wide and have no counterpart in the reference.
+ * "wg_max_n": levels of up to this many vertices are refined by one
+ * workgroup that runs the whole iteration loop (0 = never; default 16384).
+ * Returns JET_EINVAL for an unknown name or a value out of range. */
+int jet_tuning_set(const char* name, int64_t value);
+int jet_tuning_get(const char* name, int64_t* value);
 
 /* ---- Kernel timing for measurement tools. jet_prof_enable("lp_select")
  * times every later launch of that kernel family on the device's own clock

@@ -51,6 +51,17 @@ __global__ void k_validate(vid_t n, eoff_t m, const vid_t* col, const wgt_t* ew,
     }
 }
 
+void launch_validate_offsets(vid_t n, const eoff_t* rowptr, eoff_t m, int32_t* err, hipStream_t s) {
+    k_validate_offsets<<<grid_for(n, 256, 4096), 256, 0, s>>>(n, rowptr, m, err);
+    JET_LAUNCH_CHECK();
+}
+
+void launch_validate(vid_t n, eoff_t m, const vid_t* col, const wgt_t* ew, const wgt_t* vw, int32_t* err,
+                     unsigned long long* sums, hipStream_t s) {
+    k_validate<<<grid_for(std::max<int64_t>(n, m), 256, 4096), 256, 0, s>>>(n, m, col, ew, vw, err, sums);
+    JET_LAUNCH_CHECK();
+}
+
 int32_t validate_graph(vid_t n, const eoff_t* rowptr, const vid_t* col, const wgt_t* ew, const wgt_t* vw, Arena& a,
                        hipStream_t s) {
     ArenaScope sc(a);
@@ -59,11 +70,9 @@ int32_t validate_graph(vid_t n, const eoff_t* rowptr, const vid_t* col, const wg
     fill_i32(err, 0, 1, s);
     fill_i64(reinterpret_cast<int64_t*>(sums), 0, 2, s);
     eoff_t m = read_scalar(rowptr + n, s);
-    k_validate_offsets<<<grid_for(n, 256, 4096), 256, 0, s>>>(n, rowptr, m, err);
-    JET_LAUNCH_CHECK();
+    launch_validate_offsets(n, rowptr, m, err, s);
     if (read_scalar(err, s)) return 1;
-    k_validate<<<grid_for(std::max<int64_t>(n, m), 256, 4096), 256, 0, s>>>(n, m, col, ew, vw, err, sums);
-    JET_LAUNCH_CHECK();
+    launch_validate(n, m, col, ew, vw, err, sums, s);
     int32_t e = 0;
     unsigned long long h[2];
     JET_HIP(hipMemcpyAsync(&e, err, sizeof(e), hipMemcpyDeviceToHost, s));

@@ -5,6 +5,7 @@
 #include "arena.h"
 #include "contract.h"
 #include "jet_types.h"
+#include "refine_wg.h"
 
 namespace jet {
 
@@ -66,6 +67,8 @@ class Refiner {
     const int64_t* best_sizes_dev() const { return best_sizes_; }
 
   private:
+    // jet_refine on one level by the one-workgroup kernel (small levels)
+    LevelRecord refine_wg(const Level& L, part_t* best_part, const std::vector<double>& temps);
     void init_conn(const Level& L, const part_t* part, const int32_t* mark);
     void build_tiers(const Level& L);
     void jet_lp(const Level& L, const part_t* part, double filter_ratio);
@@ -153,6 +156,9 @@ class Refiner {
     int64_t* bcopies_ = nullptr;  // ncopies_ - 1 more bucket arrays, kept zero between uses
     int32_t ncopies_ = 1;
     int64_t nbuckets_ = 0;
+    WgState* wg_state_ = nullptr;  // the one-workgroup kernel's controller state
+    vid_t* wg_list_ = nullptr;     // its ranked-member list (n)
+    int32_t* wg_runb_ = nullptr;   // its per-bucket running weights (WG_MAX_NBK)
     int64_t* res_host_ = nullptr;  // thread mailbox: (cut change, largest part) of the last iteration
     int64_t* res_dev_ = nullptr;
     // refine_data mirror on the host
@@ -167,6 +173,8 @@ void project(vid_t n_fine, const vid_t* map, const part_t* coarse, part_t* fine,
 
 // part_stat.hpp:77-108 (2 x cut), :146-153.
 int64_t total_cut2(const DGraph& g, const part_t* part, Arena& a, hipStream_t s);
+// Adds the directed cut (g.ew NULL: unit weights) into *out (device).
+void launch_cut2(const DGraph& g, const part_t* part, unsigned long long* out, hipStream_t s);
 void part_sizes(const DGraph& g, const part_t* part, int32_t k, int64_t* sizes, hipStream_t s);
 
 }  // namespace jet

@@ -24,7 +24,9 @@
 
 #include "prims.h"
 #include "prof.h"
+#include "conn_table.h"
 #include "refine.h"
+#include "tuning.h"
 #include "wave.h"
 
 namespace jet {
@@ -37,20 +39,6 @@ constexpr int32_t LDS_K = 2048;         // part-size deltas aggregated in LDS up
 constexpr int32_t HEAVY_DEG_REFINE = 2048;  // rows above this degree are built by a workgroup
 
 // ---------------------------------------------------------------- helpers
-__device__ inline int32_t ld_rx(const int32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ inline int32_t lookup(const int2* tab, int32_t size, part_t target) {
-    for (int32_t q = 0; q < size; q++) {
-        int32_t pi = (target + q) % size;
-        int2 e = tab[pi];
-        if (e.x == target) return e.y;
-        if (e.x == NULL_PART) return 0;
-    }
-    return 0;
-}
-
 __device__ inline vid_t list_at(const vid_t* list, int32_t nsmall, vid_t nmax, int64_t x) {
     return x < nsmall ? list[x] : list[nmax - 1 - (x - nsmall)];
 }
@@ -61,20 +49,24 @@ __global__ void k_cut2(DGraph g, const part_t* part, unsigned long long* out) {
     __shared__ EdgeWalkLds walk;
     int64_t c = 0;
     block_edges<256>(g.rowptr, g.n, g.m, walk, [&](vid_t r, eoff_t j) {
-        if (part[g.col[j]] != part[r]) c += g.ew[j];
+        if (part[g.col[j]] != part[r]) c += g.ew ? g.ew[j] : 1;
     });
     c = block_reduce_sum<int64_t, 256>(c, red);
     if (threadIdx.x == 0 && c) atomicAdd(out, (unsigned long long)c);
+}
+
+void launch_cut2(const DGraph& g, const part_t* part, unsigned long long* out, hipStream_t s) {
+    if (g.n > 0 && g.m > 0) {
+        k_cut2<<<grid_for(g.m, EB_TILE, 8192), 256, 0, s>>>(g, part, out);
+        JET_LAUNCH_CHECK();
+    }
 }
 
 int64_t total_cut2(const DGraph& g, const part_t* part, Arena& a, hipStream_t s) {
     ArenaScope sc(a);
     unsigned long long* out = a.alloc<unsigned long long>(1);
     fill_i64(reinterpret_cast<int64_t*>(out), 0, 1, s);
-    if (g.n > 0 && g.m > 0) {
-        k_cut2<<<grid_for(g.m, EB_TILE, 8192), 256, 0, s>>>(g, part, out);
-        JET_LAUNCH_CHECK();
-    }
+    launch_cut2(g, part, out, s);
     return (int64_t)read_scalar(out, s);
 }
 
@@ -150,13 +142,6 @@ __global__ void __launch_bounds__(256) k_conn_tiers(DGraph g, int32_t k, vid_t* 
         if (b >= 0) big[b] = (vid_t)i;
         if (c >= 0) pull_mid[c] = (vid_t)i;
     }
-}
-
-__device__ inline int32_t shrink_size(int32_t used, int32_t cap) {
-    int32_t q = used / 4;
-    if (q < 3) q = 3;
-    int32_t sz = used + q;
-    return sz < cap ? sz : cap;
 }
 
 constexpr int SB_BLOCK = 256;
@@ -350,20 +335,6 @@ __global__ void __launch_bounds__(BB_BLOCK)
     }
 }
 
-// Table slots in groups of four independent loads; f(slot) sees the slots in
-// order.
-template <class F>
-__device__ inline void for_slots(const int2* tab, int32_t sz, F&& f) {
-    for (int32_t s0 = 0; s0 < sz; s0 += 4) {
-        int2 e[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) e[u] = s0 + u < sz ? tab[s0 + u] : make_int2(NULL_PART, 0);
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-            if (s0 + u < sz) f(e[u]);
-    }
-}
-
 // ---------------------------------------------------------------- jet_lp
 // jet_refiner.hpp:211-268: destination selection, viable-move filter.
 // Each wave owns a contiguous chunk of CHW vertices (CHW / WAVE passes);
@@ -503,26 +474,6 @@ __global__ void __launch_bounds__(W* WAVE)
     }
 }
 
-// jet_refiner.hpp:275-307 (afterburner): gain correction assuming every
-// higher-priority neighbour (larger pre-gain, or equal and smaller id) moves.
-// Non-candidates (destination NO_MOVE) have the lowest priority, as the
-// reference's minimum pre-gain gives them.
-__device__ inline int64_t afterburn_edge(vid_t v, vid_t i, int32_t ig, int32_t w, part_t p, part_t best,
-                                         const part_t* part, const part_t* dest, const int32_t* pregain) {
-    // loaded at once; the pre-gain of a non-candidate is stale and unused
-    const part_t vd = dest[v], vq = part[v];
-    const int32_t vg = pregain[v];
-    if (vd == NO_MOVE) return 0;
-    int64_t ch = 0;
-    if (vg > ig || (vg == ig && v < i)) {
-        if (vd == p) ch -= w;
-        else if (vd == best) ch += w;
-        if (vq == p) ch += w;
-        else if (vq == best) ch -= w;
-    }
-    return ch;
-}
-
 // 16-wave workgroups: each workgroup publishes its moves with one atomic on
 // the move counters, and fewer workgroups contend for them (measured 0.9 ms
 // less per 128^3 stencil partition than four-wave workgroups)
@@ -598,119 +549,6 @@ __global__ void k_afterburner_big(DGraph g, const vid_t* cand, const int32_t* co
 }
 
 // ---------------------------------------------------------------- perform_moves
-__device__ inline int32_t next_slot(int32_t s, int32_t vsz) { return s + 1 == vsz ? 0 : s + 1; }
-
-// update_small subtract (jet_refiner.hpp:785-812). The linear probe for p
-// reads four slots per round trip; during the subtraction pass the slot that
-// holds p does not move (only parts whose count reaches 0 are reclaimed), so
-// the slot found is the one the one-at-a-time probe finds.
-__device__ inline void conn_sub(int2* tab, int32_t vsz, part_t p, int32_t w, int32_t k, int32_t* err) {
-    int32_t po = p % vsz;
-    for (int32_t probed = 0;; probed += 4) {
-        if (probed > vsz) {
-            atomicOr(err, 1);
-            return;
-        }
-        const int32_t a = po, b = next_slot(a, vsz), c = next_slot(b, vsz), d = next_slot(c, vsz);
-        const int32_t xa = ld_rx(&tab[a].x), xb = ld_rx(&tab[b].x), xc = ld_rx(&tab[c].x), xd = ld_rx(&tab[d].x);
-        if (xa == p) break;
-        if (xb == p) {
-            po = b;
-            break;
-        }
-        if (xc == p) {
-            po = c;
-            break;
-        }
-        if (xd == p) {
-            po = d;
-            break;
-        }
-        po = next_slot(d, vsz);
-    }
-    int32_t x = atomicAdd(&tab[po].y, -w);
-    if (vsz < k && x == w) __hip_atomic_store(&tab[po].x, HASH_RECLAIM, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// update_small add (jet_refiner.hpp:815-886).
-__device__ inline void conn_add(int2* tab, int32_t* vsize, int32_t cap, part_t best, int32_t w, int32_t* err) {
-    int32_t vsz = ld_rx(vsize);
-    int32_t po = best % vsz;
-    bool success = false;
-    // the search for best or an empty slot, four slots per round trip (the
-    // values are those a one-at-a-time probe could read: other threads only
-    // fill empty or reclaimed slots meanwhile)
-    for (int32_t q = 0, s0 = po; q < vsz; q += 4) {
-        int32_t sl[4], e[4];
-        sl[0] = s0;
-        for (int j = 1; j < 4; j++) sl[j] = next_slot(sl[j - 1], vsz);
-        for (int j = 0; j < 4; j++) e[j] = ld_rx(&tab[sl[j]].x);
-        bool stop = false;
-        for (int j = 0; j < 4 && !stop; j++) {
-            if (q + j >= vsz) {
-                stop = true;
-            } else if (e[j] == best) {
-                success = true;
-                po = sl[j];
-                stop = true;
-            } else if (e[j] == NULL_PART) {
-                stop = true;
-            }
-        }
-        if (stop) break;
-        s0 = next_slot(sl[3], vsz);
-    }
-    int32_t count = 0;
-    while (!success && count < vsz) {
-        int32_t px = ld_rx(&tab[po].x);
-        while (px != best && px > NULL_PART && count++ < vsz) {
-            po = (po + 1) % vsz;
-            px = ld_rx(&tab[po].x);
-        }
-        if (px == best) {
-            success = true;
-        } else if (px <= NULL_PART) {
-            int32_t old = atomicCAS(&tab[po].x, px, best);
-            if (old == best || old <= NULL_PART) {
-                success = true;
-            } else {
-                po = (po + 1) % vsz;
-                count++;
-            }
-        }
-    }
-    if (!success) {
-        po = vsz;
-        while (!success) {
-            if (po >= cap) {
-                atomicOr(err, 2);
-                return;
-            }
-            int32_t px = ld_rx(&tab[po].x);
-            while (px != best && px > NULL_PART) {
-                po++;
-                if (po >= cap) {
-                    atomicOr(err, 2);
-                    return;
-                }
-                px = ld_rx(&tab[po].x);
-            }
-            if (px == best) {
-                success = true;
-            } else {
-                int32_t py = atomicCAS(&tab[po].x, px, best);
-                if (py == px) {
-                    py = best;
-                    atomicAdd(vsize, 1);
-                }
-                if (py == best) success = true;
-                else po++;
-            }
-        }
-    }
-    atomicAdd(&tab[po].y, w);
-}
-
 // perform_moves' choice between update_small (push each moved vertex's
 // contribution into its neighbours' tables) and update_large (rebuild the
 // tables of every neighbour of a moved vertex): rebuild when more than n/10
@@ -1832,6 +1670,9 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
     fill_i32(stamp_, -1, N, s);
     fill_i32(counters_, 0, 32, s);
     fill_i64(acc_, 0, 8, s);
+    wg_state_ = a.alloc<WgState>(1);
+    wg_list_ = a.alloc<vid_t>(N);
+    wg_runb_ = a.alloc<int32_t>(WG_MAX_NBK + 1);
     pub_ = a.alloc<int64_t>((int64_t)PUB_GROUPS * PUB_STRIDE);
     fill_i64(pub_, 0, (int64_t)PUB_GROUPS * PUB_STRIDE, s);
     // (cut change, largest part) of each iteration land in mailbox slots 8, 9
@@ -2229,6 +2070,14 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
     } else {
         temps.push_back(0.75);
     }
+    {
+        const RebalanceShape wr = rebalance_shape(g.n, false);
+        if (wg_fits(g.n, k, wr.nbk, tuning().wg_max_n) && (int)temps.size() <= WG_MAX_TEMPS) {
+            LevelRecord rec = refine_wg(L, best_part, temps);
+            rec.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            return rec;
+        }
+    }
     LevelRecord rec;
     int balance_counter = 0;
     for (double fr : temps) {
@@ -2279,6 +2128,72 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
     rec.n = g.n;
     rec.m = g.m;
     rec.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return rec;
+}
+
+// The level's iterations in the one-workgroup kernel (csrc/refine_wg.hip),
+// from the tables init_conn built. The kernel stops early only when a step
+// chose update_large: the marked rows are then rebuilt here, by the same
+// multi-workgroup build as flush_rebuild, and the kernel continues.
+LevelRecord Refiner::refine_wg(const Level& L, part_t* best_part, const std::vector<double>& temps) {
+    DGraph g = L.view();
+    const int32_t k = p_.k;
+    const RebalanceShape weak = rebalance_shape(g.n, false), strong = rebalance_shape(g.n, true);
+    WgParams P{};
+    P.g = g;
+    P.k = k;
+    P.sections = weak.sections;
+    P.nbk = (int32_t)weak.nbk;
+    P.opt = opt_;
+    P.size_max = size_max_;
+    P.imb_max = (int64_t)size_max_ - opt_;
+    P.max_dest_weak = weak.max_dest;
+    P.max_dest_strong = strong.max_dest;
+    P.tol = p_.tol;
+    P.ntemps = (int32_t)temps.size();
+    for (size_t i = 0; i < temps.size(); i++) P.temps[i] = temps[i];
+    P.part = cur_part_;
+    P.best_part = best_part;
+    P.conn_off = conn_off_;
+    P.conn = conn_;
+    P.conn_size = conn_size_;
+    P.dest_cache = dest_cache_;
+    P.lock = lock_;
+    P.gain_p = gain_p_;
+    P.pregain = pregain_;
+    P.dest = dest_;
+    P.bid = bid_;
+    P.cand = cand_;
+    P.moves = moves_;
+    P.rlist = wg_list_;
+    P.score = score_;
+    P.mark = swap_bit_;
+    P.runb = wg_runb_;
+    P.curr_sizes = curr_sizes_;
+    P.best_sizes = best_sizes_;
+    P.st = wg_state_;
+    wg_init_state(wg_state_, cut_, total_imb_, s_);
+    WgState h{};
+    for (int launches = 0;; launches++) {
+        if (launches > 100000) throw DeviceError("one-workgroup refinement made no progress");
+        launch_refine_wg(P, s_);
+        JET_HIP(hipMemcpyAsync(&h, wg_state_, sizeof(h), hipMemcpyDeviceToHost, s_));
+        JET_HIP(hipStreamSynchronize(s_));
+        if (h.err) throw DeviceError("connectivity table invariant violated (code " + std::to_string(h.err) + ")");
+        if (h.status == WG_STALLED) throw DeviceError("one-workgroup refinement exceeded its iteration bound");
+        if (h.status != WG_REBUILD) break;
+        init_conn(L, cur_part_, swap_bit_);  // update_large's rebuild of the marked rows
+    }
+    if (h.changed) changed_ = true;
+    cut_ = h.best_cut;
+    total_imb_ = h.best_imb;
+    LevelRecord rec;
+    rec.iterations = h.iterations;
+    rec.lp_iterations = h.lp_iterations;
+    rec.cut2 = cut_;
+    rec.total_imb = total_imb_;
+    rec.n = g.n;
+    rec.m = g.m;
     return rec;
 }
 

@@ -107,6 +107,7 @@ def lib():
             "jet_refine_level": [i64, VP, VP, VP, VP, i32, dbl, dbl, i32, i32, i32, VP, VP],
             "jet_project": [i64, VP, VP, i32, VP],
             "jet_edge_cut": [i64, VP, VP, VP, VP, i32, c.POINTER(i64)],
+            "jet_edge_cut_noalloc": [i64, VP, VP, VP, VP, c.POINTER(i64)],
             "jet_part_sizes": [i64, VP, VP, i32, i32, VP],
             "jet_gain_bucket_batch": [i64, VP, VP, VP],
             "jet_generate_grid2d": [i64, i64, VP, VP],
@@ -131,6 +132,8 @@ def lib():
             "jet_free_hierarchy": [VP],
             "jet_load_coarse_part": [c.c_char_p, i64, VP],
             "jet_uncoarsen_hierarchy": [c.POINTER(Config), VP, VP, VP, c.POINTER(i64), c.POINTER(Stats)],
+            "jet_tuning_set": [c.c_char_p, i64],
+            "jet_tuning_get": [c.c_char_p, c.POINTER(i64)],
             "jet_prof_enable": [c.c_char_p],
             "jet_prof_read": [c.POINTER(i64), c.POINTER(dbl), c.POINTER(dbl)],
         }
@@ -346,6 +349,49 @@ def edge_cut2(g, part):
     part = _c(part, np.int32)
     _check(lib().jet_edge_cut(len(rp) - 1, _p(rp), _p(col), _p(ew), _p(part), PTR_HOST,
                               ctypes.byref(out)))
+    return out.value
+
+
+def tuning_set(name, value):
+    """Sets a result-neutral tuning parameter (jet_tuning_set)."""
+    _check(lib().jet_tuning_set(name.encode(), int(value)))
+
+
+def tuning_get(name):
+    v = ctypes.c_int64(0)
+    _check(lib().jet_tuning_get(name.encode(), ctypes.byref(v)))
+    return v.value
+
+
+class tuned:
+    """Context manager: sets tuning parameters, restores them on exit."""
+
+    def __init__(self, **kv):
+        self.kv = kv
+        self.old = {}
+
+    def __enter__(self):
+        for k, v in self.kv.items():
+            self.old[k] = tuning_get(k)
+            tuning_set(k, v)
+        return self
+
+    def __exit__(self, *exc):
+        for k, v in self.old.items():
+            tuning_set(k, v)
+
+
+def edge_cut2_noalloc(n, rowptr_ptr, col_ptr, ew_ptr, part_ptr):
+    """jet_edge_cut_noalloc on device pointers (ints; ew_ptr may be None)."""
+    out = ctypes.c_int64(0)
+    _check(lib().jet_edge_cut_noalloc(n, rowptr_ptr, col_ptr, ew_ptr, part_ptr, ctypes.byref(out)))
+    return out.value
+
+
+def edge_cut2_device(n, rowptr_ptr, col_ptr, ew_ptr, part_ptr):
+    """jet_edge_cut on device pointers (ints; ew_ptr may be None)."""
+    out = ctypes.c_int64(0)
+    _check(lib().jet_edge_cut(n, rowptr_ptr, col_ptr, ew_ptr, part_ptr, PTR_DEVICE, ctypes.byref(out)))
     return out.value
 
 

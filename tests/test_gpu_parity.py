@@ -171,14 +171,25 @@ def test_refine_step_bit_exact(name, mk, kind, skew):
         assert np.array_equal(mv, emv) and np.array_equal(de, ede), (name, kind, k)
 
 
+# Small levels are refined by one workgroup running the whole iteration loop
+# (wg_max_n); "multi" forces the multi-kernel path on every level.
+PATHS = [("wg", 16384), ("multi", 0)]
+
+
+@pytest.fixture(params=PATHS, ids=[p[0] for p in PATHS])
+def refine_path(request):
+    with jet_amd.tuned(wg_max_n=request.param[1]):
+        yield request.param[0]
+
+
 @pytest.mark.parametrize("name,mk", GRAPHS)
 @pytest.mark.parametrize("skew", [0.0, 1.5])
-def test_refine_level_bit_exact(name, mk, skew):
+def test_refine_level_bit_exact(name, mk, skew, refine_path):
     g = mk()
     n = len(g["rowptr"]) - 1
     for k in (3, 16):
         part = rand_part(n, k, 23, skew)
-        say("refine_level", name, skew, k)
+        say("refine_level", refine_path, name, skew, k)
         p, rec = jet_amd.refine_level(g, part, k, 1.03, uniform=g["uniform"])
         ep, erec = oracle.refine_level(g, part, k, 1.03, uniform=g["uniform"])
         assert np.array_equal(rec, erec), (rec, erec)
@@ -213,11 +224,11 @@ E2E = [
 
 
 @pytest.mark.parametrize("name,mk,k,imb,alg", E2E)
-def test_partition_end_to_end_bit_exact(name, mk, k, imb, alg):
+def test_partition_end_to_end_bit_exact(name, mk, k, imb, alg, refine_path):
     g = mk()
     n = len(g["rowptr"]) - 1
     for seed in (0, 1):
-        say("e2e", name, seed)
+        say("e2e", refine_path, name, seed)
         cfg = jet_amd.Config(num_parts=k, max_imb_ratio=imb, coarsening_alg=alg)
         part, cut, st = jet_amd.partition_csr(g, cfg, seed=seed, stats=True)
         epart, ecut, elv = oracle.partition(g, k, imb, alg=alg, seed=seed)
