@@ -164,6 +164,45 @@ struct Sub {
     int64_t total_w = 0;
 };
 
+// Runs f(t) for t in [0, T): t = 0 on the calling thread, the others on
+// their own threads.
+template <class F>
+static void run_threads(int T, F&& f) {
+    std::vector<std::thread> th;
+    for (int t = 1; t < T; t++) th.emplace_back([&f, t]() { f(t); });
+    f(0);
+    for (auto& x : th) x.join();
+}
+
+// Below this many adjacency entries per thread a whole-graph pass runs on
+// fewer threads.
+constexpr int64_t PAR_EDGES_PER_THREAD = 1 << 20;
+
+// Splits the vertices of s into T contiguous ranges of about equal
+// vertices + adjacency entries; bounds[t]..bounds[t+1] is range t.
+static std::vector<int32_t> vertex_chunks(const Sub& s, int T) {
+    std::vector<int32_t> bounds(T + 1, s.n);
+    bounds[0] = 0;
+    const int64_t total = (int64_t)s.n + s.xadj[s.n];
+    for (int t = 1; t < T; t++) {
+        const int64_t want = total / T * t;
+        int32_t lo = bounds[t - 1], hi = s.n;  // first v with v + xadj[v] >= want
+        while (lo < hi) {
+            const int32_t mid = lo + (hi - lo) / 2;
+            if ((int64_t)mid + s.xadj[mid] < want) lo = mid + 1;
+            else hi = mid;
+        }
+        bounds[t] = lo;
+    }
+    return bounds;
+}
+
+// Threads worth using for a whole-graph pass over s, at most par.
+static int pass_threads(const Sub& s, int par) {
+    const int64_t m = (int64_t)s.adj.size();
+    return (int)std::max<int64_t>(1, std::min<int64_t>(par, m / PAR_EDGES_PER_THREAD));
+}
+
 static inline uint64_t mix(uint64_t z) {
     z += 0x9E3779B97F4A7C15ULL;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
@@ -193,19 +232,39 @@ struct Bisector {
 
     explicit Bisector(const Sub& sub) : s(sub) {}
 
-    void compute() {
+    // ext, inn, side weights and cut from `side`, on up to par threads (each
+    // vertex's sums are its own; the totals are integer sums).
+    void compute(int par = 1) {
         int32_t n = s.n;
-        ext.assign(n, 0);
-        inn.assign(n, 0);
+        ext.resize(n);
+        inn.resize(n);
+        const int T = pass_threads(s, par);
+        const std::vector<int32_t> bounds = vertex_chunks(s, T);
+        std::vector<int64_t> part_w(2 * T, 0), part_cut(T, 0);
+        run_threads(T, [&](int t) {
+            int64_t w0 = 0, w1 = 0, c = 0;
+            for (int32_t v = bounds[t]; v < bounds[t + 1]; v++) {
+                const int8_t sv = side[v];
+                (sv ? w1 : w0) += s.vw[v];
+                int64_t e = 0, i = 0;
+                for (int64_t j = s.xadj[v]; j < s.xadj[v + 1]; j++) {
+                    if (side[s.adj[j]] != sv) e += s.adjw[j];
+                    else i += s.adjw[j];
+                }
+                ext[v] = e;
+                inn[v] = i;
+                c += e;
+            }
+            part_w[2 * t] = w0;
+            part_w[2 * t + 1] = w1;
+            part_cut[t] = c;
+        });
         w[0] = w[1] = 0;
         cut = 0;
-        for (int32_t v = 0; v < n; v++) {
-            w[side[v]] += s.vw[v];
-            for (int64_t j = s.xadj[v]; j < s.xadj[v + 1]; j++) {
-                if (side[s.adj[j]] != side[v]) ext[v] += s.adjw[j];
-                else inn[v] += s.adjw[j];
-            }
-            cut += ext[v];
+        for (int t = 0; t < T; t++) {
+            w[0] += part_w[2 * t];
+            w[1] += part_w[2 * t + 1];
+            cut += part_cut[t];
         }
         cut /= 2;
     }
@@ -610,48 +669,69 @@ static std::vector<int8_t> bisect_multilevel(const Sub& s, double frac, double u
         for (int32_t v = 0; v < fine.n; v++) fs[v] = side[maps[lvl][v]];
         Bisector b(fine);
         b.side = std::move(fs);
-        b.compute();
+        b.compute(par);
         b.fm(maxw, 4, std::max<int32_t>(25, std::min<int32_t>(200, fine.n / 100)));
         side = std::move(b.side);
     }
     return side;
 }
 
-static Sub induced(const Sub& s, const std::vector<int8_t>& side, int8_t which) {
+// The subgraph induced by the vertices with side == which, local ids in
+// increasing order, on up to par threads over contiguous vertex ranges.
+static Sub induced(const Sub& s, const std::vector<int8_t>& side, int8_t which, int par) {
     Sub r;
-    std::vector<int32_t> loc(s.n, -1);
-    int32_t nr = 0;
-    for (int32_t v = 0; v < s.n; v++)
-        if (side[v] == which) loc[v] = nr++;
+    const int T = pass_threads(s, par);
+    const std::vector<int32_t> bounds = vertex_chunks(s, T);
+    std::vector<int32_t> loc(s.n);
+    std::vector<int32_t> first_loc(T + 1, 0);
+    run_threads(T, [&](int t) {  // kept vertices per range
+        int32_t c = 0;
+        for (int32_t v = bounds[t]; v < bounds[t + 1]; v++) c += side[v] == which;
+        first_loc[t + 1] = c;
+    });
+    for (int t = 0; t < T; t++) first_loc[t + 1] += first_loc[t];
+    const int32_t nr = first_loc[T];
     r.n = nr;
     r.gid.resize(nr);
     r.vw.resize(nr);
     r.xadj.resize((size_t)nr + 1);
     r.xadj[0] = 0;
-    for (int32_t v = 0; v < s.n; v++) {  // pass 1: kept edges per kept vertex
-        if (loc[v] < 0) continue;
-        int64_t c = 0;
-        for (int64_t j = s.xadj[v]; j < s.xadj[v + 1]; j++) c += loc[s.adj[j]] >= 0;
-        r.xadj[loc[v] + 1] = c;
-        r.gid[loc[v]] = s.gid[v];
-        r.vw[loc[v]] = s.vw[v];
-        r.total_w += s.vw[v];
-    }
+    std::vector<int64_t> part_w(T, 0);
+    run_threads(T, [&](int t) {  // local ids, then kept edges per kept vertex
+        int32_t at = first_loc[t];
+        for (int32_t v = bounds[t]; v < bounds[t + 1]; v++) loc[v] = side[v] == which ? at++ : -1;
+    });
+    run_threads(T, [&](int t) {
+        int64_t wsum = 0;
+        for (int32_t v = bounds[t]; v < bounds[t + 1]; v++) {
+            if (loc[v] < 0) continue;
+            int64_t c = 0;
+            for (int64_t j = s.xadj[v]; j < s.xadj[v + 1]; j++) c += side[s.adj[j]] == which;
+            r.xadj[loc[v] + 1] = c;
+            r.gid[loc[v]] = s.gid[v];
+            r.vw[loc[v]] = s.vw[v];
+            wsum += s.vw[v];
+        }
+        part_w[t] = wsum;
+    });
+    for (int t = 0; t < T; t++) r.total_w += part_w[t];
     for (int32_t i = 0; i < nr; i++) r.xadj[i + 1] += r.xadj[i];
     r.adj.resize(r.xadj[nr]);
     r.adjw.resize(r.xadj[nr]);
-    for (int32_t v = 0; v < s.n; v++) {  // pass 2: fill
-        if (loc[v] < 0) continue;
-        int64_t o = r.xadj[loc[v]];
-        for (int64_t j = s.xadj[v]; j < s.xadj[v + 1]; j++) {
-            int32_t u = loc[s.adj[j]];
-            if (u >= 0) {
-                r.adj[o] = u;
-                r.adjw[o] = s.adjw[j];
-                o++;
+    run_threads(T, [&](int t) {  // fill
+        for (int32_t v = bounds[t]; v < bounds[t + 1]; v++) {
+            if (loc[v] < 0) continue;
+            int64_t o = r.xadj[loc[v]];
+            for (int64_t j = s.xadj[v]; j < s.xadj[v + 1]; j++) {
+                const int32_t u = loc[s.adj[j]];
+                if (u >= 0) {
+                    r.adj[o] = u;
+                    r.adjw[o] = s.adjw[j];
+                    o++;
+                }
             }
         }
-    }
+    });
     return r;
 }
 
@@ -665,15 +745,8 @@ static void recurse(const Sub& s, int32_t k, int32_t first, double ub, uint64_t 
     const uint64_t bseed = seed + (uint64_t)first * 1315423911ULL;
     std::vector<int8_t> side = s.n > ML_MIN_N ? bisect_multilevel(s, frac, ub, bseed, par)
                                               : bisect_flat(s, frac, ub, bseed, par);
-    Sub a, b;
-    if (par > 1 && (int64_t)s.adj.size() >= PAR_MIN_EDGES) {
-        std::thread th([&]() { b = induced(s, side, 1); });
-        a = induced(s, side, 0);
-        th.join();
-    } else {
-        a = induced(s, side, 0);
-        b = induced(s, side, 1);
-    }
+    Sub a = induced(s, side, 0, par);
+    Sub b = induced(s, side, 1, par);
     side.clear();
     side.shrink_to_fit();
     if (par > 1 && (int64_t)(a.adj.size() + b.adj.size()) >= PAR_MIN_RECURSE_EDGES) {
