@@ -21,6 +21,11 @@ for p in (os.path.join(ROOT, "jet-partitioner-official-fork_amd"), os.path.join(
     if p not in sys.path:
         sys.path.insert(0, p)
 
+# the OpenMP CPU baseline binds its threads to cores; libgomp reads these
+# when the library is loaded
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 import numpy as np  # noqa: E402
 
 import jet_amd  # noqa: E402
@@ -50,6 +55,34 @@ def counted_traffic(kernel, workload):
             if line.startswith("HBM bytes per launch:"):
                 return int(line.split(":")[1])
     return None
+
+
+def cpu_threads():
+    """Threads for the CPU baseline: the CPUs this process may run on, at most
+    16 (one GPU's share of the host)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
+def host_cpu():
+    """Model and core counts of the host, from lscpu."""
+    info = {}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            key, _, val = line.partition(":")
+            key, val = key.strip(), val.strip()
+            if key in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[key] = val
+    except Exception:  # lscpu missing: report what Python knows
+        info["CPU(s)"] = str(os.cpu_count())
+    info["OMP_PROC_BIND"] = os.environ.get("OMP_PROC_BIND")
+    info["OMP_PLACES"] = os.environ.get("OMP_PLACES")
+    return info
 
 
 def copy_bandwidth(torch):
@@ -326,6 +359,9 @@ def main():
                          "the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="timed CPU-baseline runs after one warm-up")
+    ap.add_argument("--tune", action="append", default=[], metavar="NAME=VALUE",
+                    help="set a result-neutral tuning parameter before the run (repeatable)")
     ap.add_argument("--prof", default="lp_select", help="kernel family timed with HIP events")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -333,6 +369,10 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    for kv in args.tune:
+        name, _, val = kv.partition("=")
+        jet_amd.tuning_set(name, int(val))
+        print(f"# tuning {name} = {jet_amd.tuning_get(name)}", file=sys.stderr, flush=True)
     print("# importing torch", file=sys.stderr, flush=True)
     import torch
 
@@ -445,26 +485,34 @@ def main():
     if not args.no_cpu_baseline:
         import oracle
 
-        if "cpu_sample" in wl:
-            sname = wl["cpu_sample"]
-            sg = make_graph(sname)
-            sedges = len(sg["col"]) // 2
-            print(f"# cpu oracle on the {sname} sample ({sedges} edges)", file=sys.stderr, flush=True)
+        # the OpenMP build of the CPU restatement (identical results to the
+        # serial build), on this process's CPU share, bound to cores
+        threads = cpu_threads()
+        OL, threads = oracle.omp_lib(threads)
+        sname = wl.get("cpu_sample")
+        sg = make_graph(sname) if sname else g
+        sedges = len(sg["col"]) // 2
+        what = f"{SAMPLES[sname]} ({sedges} edges)" if sname else f"the same {args.workload} graph, same seed"
+        print(f"# cpu baseline: OpenMP restatement on {threads} threads, "
+              f"{'the ' + sname + ' sample' if sname else 'the full graph'}, 1 warm-up + median of "
+              f"{args.cpu_runs}", file=sys.stderr, flush=True)
+        times = []
+        opart = ocut = None
+        for r in range(args.cpu_runs + 1):
             t1 = time.perf_counter()
-            oracle.partition(sg, wl["k"], wl["imb"], alg=wl["alg"], seed=args.seed)
-            cdt = time.perf_counter() - t1
-            cpu = {"value": round(sedges / cdt, 1), "unit": "edges/s", "cores": 1, "kind": "port",
-                   "sample": f"one full partition of {SAMPLES[sname]} ({sedges} edges), serial oracle ({cdt:.2f}s)"}
-        else:
-            print("# cpu oracle on the full graph", file=sys.stderr, flush=True)
-            t1 = time.perf_counter()
-            opart, ocut, _ = oracle.partition(g, wl["k"], wl["imb"], alg=wl["alg"], seed=args.seed)
-            cdt = time.perf_counter() - t1
-            cpu = {"value": round(edges / cdt, 1), "unit": "edges/s", "cores": 1, "kind": "port",
-                   "sample": f"one full partition of the same {args.workload} graph, same seed, serial oracle "
-                             f"({cdt:.2f}s)"}
+            opart, ocut, _ = oracle.partition(sg, wl["k"], wl["imb"], alg=wl["alg"], seed=args.seed, L=OL)
+            if r > 0:
+                times.append(time.perf_counter() - t1)
+        cdt = float(np.median(times))
+        cpu = {"value": round(sedges / cdt, 1), "unit": "edges/s", "cores": threads, "kind": "port",
+               "build": "oracle/libjet_oracle_omp.so (OpenMP restatement of jet_host; same results as the serial "
+                        "oracle)",
+               "host": host_cpu(),
+               "sample": f"full partitions of {what}: 1 warm-up, then the median of {args.cpu_runs} "
+                         f"({cdt:.2f}s; runs {', '.join(f'{t:.2f}' for t in times)}s)"}
+        if not sname:
             cut_ratio = round(cut / ocut, 6) if ocut else None
-            print(f"# cpu oracle {cdt:.2f}s cut {ocut}; gpu/oracle partitions identical: "
+            print(f"# cpu baseline {cdt:.2f}s cut {ocut}; gpu/oracle partitions identical: "
                   f"{bool(np.array_equal(part, opart))}", file=sys.stderr, flush=True)
 
     out = {

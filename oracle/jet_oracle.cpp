@@ -29,10 +29,70 @@
 #include <unordered_map>
 #include <vector>
 
+#ifdef ORC_OMP
+#include <omp.h>
+#endif
+
 extern "C" int jet_host_initpart(int32_t n, const int64_t* xadj, const int32_t* adj, const int32_t* adjw,
                                  const int32_t* vw, int32_t k, double imb, uint64_t seed, int32_t* part);
 
 namespace orc {
+
+// Loops over independent items. Built with ORC_OMP (oracle/Makefile's
+// libjet_oracle_omp.so, the multi-core CPU baseline) they run on OpenMP
+// threads; every such loop writes only its own item's outputs, and the
+// compactions keep item order, so that build computes exactly what the serial
+// build computes (tests/test_oracle_omp.py checks it).
+template <class F>
+static void par_for(int64_t n, F&& f) {
+#ifdef ORC_OMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int64_t i = 0; i < n; i++) f(i);
+}
+
+template <class F>
+static int64_t par_sum(int64_t n, F&& f) {
+    int64_t s = 0;
+#ifdef ORC_OMP
+#pragma omp parallel for schedule(static) reduction(+ : s)
+#endif
+    for (int64_t i = 0; i < n; i++) s += f(i);
+    return s;
+}
+
+// The items i in [0, n) with flag[i] set, in increasing order (ids[i] when ids
+// is given). Each thread compacts one contiguous range; ranges are joined in
+// order.
+static std::vector<int32_t> par_compact(int64_t n, const uint8_t* flag, const int32_t* ids = nullptr) {
+    std::vector<int32_t> out;
+#ifdef ORC_OMP
+    if (n >= 65536) {
+        std::vector<int64_t> cnt(omp_get_max_threads() + 1, 0);
+#pragma omp parallel
+        {
+            const int t = omp_get_thread_num(), T = omp_get_num_threads();
+            const int64_t lo = n * t / T, hi = n * (t + 1) / T;
+            int64_t c = 0;
+            for (int64_t i = lo; i < hi; i++) c += flag[i] ? 1 : 0;
+            cnt[t + 1] = c;
+#pragma omp barrier
+#pragma omp single
+            {
+                for (int x = 0; x < T; x++) cnt[x + 1] += cnt[x];
+                out.resize(cnt[T]);
+            }
+            int64_t at = cnt[t];
+            for (int64_t i = lo; i < hi; i++)
+                if (flag[i]) out[at++] = ids ? ids[i] : (int32_t)i;
+        }
+        return out;
+    }
+#endif
+    for (int64_t i = 0; i < n; i++)
+        if (flag[i]) out.push_back(ids ? ids[i] : (int32_t)i);
+    return out;
+}
 
 struct Graph {
     int32_t n = 0;
@@ -64,17 +124,16 @@ static std::vector<int32_t> hec_heavy(const Graph& g, const int32_t* vw, bool un
     int32_t n = g.n;
     std::vector<int32_t> hn(n);
     if (uniform) {
-        for (int32_t i = 0; i < n; i++) {
+        par_for(n, [&](int64_t i) {
             uint64_t r = rng(seed, (uint32_t)level, SITE_HEC, 0, (uint32_t)i);
-            int64_t deg = g.deg(i);
+            int64_t deg = g.deg((int32_t)i);
             if (deg > 0) hn[i] = g.col[g.rowptr[i] + (int64_t)(r % (uint64_t)deg)];
             else hn[i] = (int32_t)(r % (uint64_t)n);
-        }
+        });
     } else {
-        int64_t sum = 0;
-        for (int32_t i = 0; i < n; i++) sum += vw[i];
+        int64_t sum = par_sum(n, [&](int64_t i) -> int64_t { return vw[i]; });
         int64_t max_allowed = 6 * sum / n;
-        for (int32_t i = 0; i < n; i++) {
+        par_for(n, [&](int64_t i) {
             int64_t start = g.rowptr[i], end = g.rowptr[i + 1];
             if (end > start && vw[i] < max_allowed) {
                 int64_t best_w = std::numeric_limits<int64_t>::min();
@@ -86,11 +145,11 @@ static std::vector<int32_t> hec_heavy(const Graph& g, const int32_t* vw, bool un
                         loc = j;
                     }
                 }
-                hn[i] = loc >= 0 ? g.col[loc] : i;
+                hn[i] = loc >= 0 ? g.col[loc] : (int32_t)i;
             } else {
-                hn[i] = i;
+                hn[i] = (int32_t)i;
             }
-        }
+        });
     }
     return hn;
 }
@@ -369,31 +428,47 @@ static void contract(const Graph& g, const int32_t* vw, const int32_t* map, int3
     gc.rowptr.assign(nc + 1, 0);
     gc.col.clear();
     gc.ew.clear();
-    std::vector<int64_t> acc(nc, 0);
-    std::vector<uint8_t> seen(nc, 0);
-    std::vector<int32_t> touched;
-    for (int32_t c = 0; c < nc; c++) {
-        touched.clear();
-        for (int64_t x = ms[c]; x < ms[c + 1]; x++) {
-            int32_t i = members[x];
-            for (int64_t j = g.rowptr[i]; j < g.rowptr[i + 1]; j++) {
-                int32_t u = map[g.col[j]];
-                if (u == c) continue;
-                if (!seen[u]) {
-                    seen[u] = 1;
-                    touched.push_back(u);
+    // coarse rows in contiguous ranges of c, each range built into its own
+    // buffers (one range in the serial build), then joined in order
+    int nr = 1;
+#ifdef ORC_OMP
+    if (nc >= 4096) nr = 4 * omp_get_max_threads();
+#endif
+    std::vector<std::vector<int32_t>> rcol(nr), rew(nr);
+    auto build_range = [&](int64_t r) {
+        const int32_t c0 = (int32_t)((int64_t)nc * r / nr), c1 = (int32_t)((int64_t)nc * (r + 1) / nr);
+        std::vector<std::pair<int32_t, int64_t>> es;  // (coarse column, weight) of the row's fine edges
+        std::vector<int32_t>& oc = rcol[r];
+        std::vector<int32_t>& ow = rew[r];
+        for (int32_t c = c0; c < c1; c++) {
+            es.clear();
+            for (int64_t x = ms[c]; x < ms[c + 1]; x++) {
+                int32_t i = members[x];
+                for (int64_t j = g.rowptr[i]; j < g.rowptr[i + 1]; j++) {
+                    int32_t u = map[g.col[j]];
+                    if (u != c) es.emplace_back(u, g.ew[j]);
                 }
-                acc[u] += g.ew[j];
             }
+            std::sort(es.begin(), es.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+            int64_t len = 0;
+            for (size_t x = 0; x < es.size();) {
+                const int32_t u = es[x].first;
+                int64_t w = 0;
+                for (; x < es.size() && es[x].first == u; x++) w += es[x].second;
+                oc.push_back(u);
+                ow.push_back((int32_t)w);
+                len++;
+            }
+            gc.rowptr[c + 1] = len;  // row length; offsets below
         }
-        std::sort(touched.begin(), touched.end());
-        for (int32_t u : touched) {
-            gc.col.push_back(u);
-            gc.ew.push_back((int32_t)acc[u]);
-            acc[u] = 0;
-            seen[u] = 0;
-        }
-        gc.rowptr[c + 1] = (int64_t)gc.col.size();
+    };
+    par_for(nr, build_range);
+    for (int32_t c = 0; c < nc; c++) gc.rowptr[c + 1] += gc.rowptr[c];
+    gc.col.reserve(gc.rowptr[nc]);
+    gc.ew.reserve(gc.rowptr[nc]);
+    for (int r = 0; r < nr; r++) {
+        gc.col.insert(gc.col.end(), rcol[r].begin(), rcol[r].end());
+        gc.ew.insert(gc.ew.end(), rew[r].begin(), rew[r].end());
     }
 }
 
@@ -401,11 +476,12 @@ static void contract(const Graph& g, const int32_t* vw, const int32_t* map, int3
 
 // part_stat.hpp:77-108: sum over directed edges (= 2 x cut).
 static int64_t total_cut(const Graph& g, const int32_t* part) {
-    int64_t cut = 0;
-    for (int32_t i = 0; i < g.n; i++)
+    return par_sum(g.n, [&](int64_t i) -> int64_t {
+        int64_t cut = 0;
         for (int64_t j = g.rowptr[i]; j < g.rowptr[i + 1]; j++)
             if (part[i] != part[g.col[j]]) cut += g.ew[j];
-    return cut;
+        return cut;
+    });
 }
 
 // part_stat.hpp:146-153.
@@ -463,6 +539,7 @@ class Refiner {
     // iteration scratch (jet_refiner.hpp:126-161)
     std::vector<int64_t> pregain, save_gp, save, pos, evict_start, evict_adjust, buckets;
     std::vector<int32_t> dest, bid, undersized, swap_bit;
+    std::vector<uint8_t> flag;  // per-item flags of the compactions
 
     const Graph* g = nullptr;
     const int32_t* vw = nullptr;
@@ -591,10 +668,10 @@ class Refiner {
         std::fill(lock_bit.begin(), lock_bit.begin() + n, 0);
         // the reference's team build for average degree >= 8 (:985)
         const bool large = g_slot_ties && n > 0 && g->m() / n >= 8;
-        for (int32_t i = 0; i < n; i++) {
-            if (large) build_row_large(part, i);
-            else build_row(part, i);
-        }
+        par_for(n, [&](int64_t i) {
+            if (large) build_row_large(part, (int32_t)i);
+            else build_row(part, (int32_t)i);
+        });
     }
 
     // Canonical, slot-order-free argmax: highest connectivity, lowest part id
@@ -605,11 +682,11 @@ class Refiner {
     // jet_refiner.hpp:200-323 (jet_lp).
     std::vector<int32_t> jet_lp(const int32_t* part, double filter_ratio) {
         int32_t n = g->n;
-        for (int32_t i = 0; i < n; i++) {
+        par_for(n, [&](int64_t i) {
             int32_t best = dest_cache[i];
             if (best != NULL_PART) {
                 dest[i] = best;
-                continue;
+                return;
             }
             int32_t p = part[i];
             best = NO_MOVE;
@@ -642,18 +719,20 @@ class Refiner {
             }
             dest_cache[i] = best;
             dest[i] = best;
-        }
-        std::vector<int32_t> cand;
-        for (int32_t i = 0; i < n; i++) {
-            if (dest[i] != NO_MOVE && lock_bit[i] == 0) {
-                cand.push_back(i);
+        });
+        flag.resize(n);
+        par_for(n, [&](int64_t i) {
+            flag[i] = dest[i] != NO_MOVE && lock_bit[i] == 0;
+            if (flag[i]) {
                 pregain[i] = save_gp[i];
             } else {
                 pregain[i] = std::numeric_limits<int64_t>::min();
                 lock_bit[i] = 0;
             }
-        }
-        for (int32_t i : cand) {
+        });
+        std::vector<int32_t> cand = par_compact(n, flag.data());
+        par_for((int64_t)cand.size(), [&](int64_t x) {
+            const int32_t i = cand[x];
             int32_t best = dest[i];
             int32_t p = part[i];
             int64_t ig = pregain[i];
@@ -672,11 +751,9 @@ class Refiner {
                 }
             }
             if (ig + change >= 0) lock_bit[i] = 1;
-        }
-        std::vector<int32_t> moves;
-        for (int32_t i : cand)
-            if (lock_bit[i]) moves.push_back(i);
-        return moves;
+        });
+        par_for((int64_t)cand.size(), [&](int64_t x) { flag[x] = lock_bit[cand[x]] != 0; });
+        return par_compact((int64_t)cand.size(), flag.data(), cand.data());
     }
 
     int32_t sections_for(int32_t n) const {  // jet_refiner.hpp:423-428, :556-561
@@ -708,7 +785,7 @@ class Refiner {
         int32_t tu = 0;
         for (int32_t p = 0; p < k; p++)
             if (sizes[p] < max_dest) undersized[tu++] = p;
-        for (int32_t i = 0; i < n; i++) {
+        par_for(n, [&](int64_t i) {
             int32_t p = part[i];
 #if ORC_MUTANT == 7
             if (sizes[p] > smax && (double)vw[i] <= 1.5 * (double)(sizes[p] - opt)) {
@@ -743,17 +820,21 @@ class Refiner {
             } else {
                 dest[i] = p;
             }
-        }
-        for (int32_t i = 0; i < n; i++) {
+        });
+        par_for(n, [&](int64_t i) {
             int32_t p = part[i];
             bid[i] = -1;
             if (p != dest[i]) {
                 int32_t gt = gain_bucket(save[i], vw[i]);
-                int64_t gid = ((int64_t)MAX_BUCKETS * p + gt) * sections + (i % sections);
-                bid[i] = (int32_t)gid;
-                pos[i] = buckets[gid];
-                buckets[gid] += vw[i];
+                bid[i] = (int32_t)(((int64_t)MAX_BUCKETS * p + gt) * sections + (i % sections));
             }
+        });
+        // positions in increasing id
+        for (int32_t i = 0; i < n; i++) {
+            const int32_t b = bid[i];
+            if (b < 0) continue;
+            pos[i] = buckets[b];
+            buckets[b] += vw[i];
         }
         int64_t run = 0;
         for (size_t b = 0; b < buckets.size(); b++) {
@@ -761,17 +842,18 @@ class Refiner {
             buckets[b] = run;
             run += x;
         }
-        std::vector<int32_t> moves;
-        for (int32_t i = 0; i < n; i++) {
+        flag.resize(n);
+        par_for(n, [&](int64_t i) {
             int32_t b = bid[i];
-            if (b == -1) continue;
+            flag[i] = 0;
+            if (b == -1) return;
             int32_t p = part[i];
             int64_t begin = (int64_t)p * width;
             int64_t score = pos[i] + buckets[b] - buckets[begin];
             int64_t limit = sizes[p] - smax;
-            if (score < limit) moves.push_back(i);
-        }
-        return moves;
+            flag[i] = score < limit;
+        });
+        return par_compact(n, flag.data());
     }
 
     // jet_refiner.hpp:417-546 (rebalance_strong) and :359-412
@@ -913,99 +995,153 @@ class Refiner {
         return 0;
     }
 
-    // jet_refiner.hpp:781-888 (update_small), serial order of moves and edges.
-    void update_small(const int32_t* part, const std::vector<int32_t>& moves) {
-        for (int32_t i : moves) {
-            int32_t p = dest[i];  // old part
-            for (int64_t j = g->rowptr[i]; j < g->rowptr[i + 1]; j++) {
-                int32_t v = g->col[j];
-                int64_t w = g->ew[j];
-                int64_t vs = conn_off[v];
-                int32_t vsz = conn_size[v];
-                int32_t po = p % vsz;
-                while (conn_entry[vs + po] != p) po = (po + 1) % vsz;
-                int64_t x = conn_val[vs + po];
-                conn_val[vs + po] -= w;
-                if (vsz < k && x == w) conn_entry[vs + po] = HASH_RECLAIM;
-            }
-        }
-        for (int32_t i : moves) {
-            int32_t best = part[i];  // new part
-            for (int64_t j = g->rowptr[i]; j < g->rowptr[i + 1]; j++) {
-                int32_t v = g->col[j];
-                int64_t w = g->ew[j];
-                dest_cache[v] = NULL_PART;
-                int64_t vs = conn_off[v];
-                int32_t vsz = conn_size[v];
-                int32_t po = best % vsz;
-                bool success = false;
-                for (int32_t q = 0; q < vsz; q++) {
-                    int32_t pi = (best + q) % vsz;
-                    if (conn_entry[vs + pi] == best) {
-                        success = true;
-                        po = pi;
-                        break;
-                    } else if (conn_entry[vs + pi] == NULL_PART) {
-                        break;
-                    }
-                }
-                int32_t count = 0;
-                while (!success && count < vsz) {
-                    int32_t px = conn_entry[vs + po];
-                    while (px != best && px > NULL_PART && count++ < vsz) {
-                        po = (po + 1) % vsz;
-                        px = conn_entry[vs + po];
-                    }
-                    if (px == best) {
-                        success = true;
-                    } else if (px <= NULL_PART) {
-                        conn_entry[vs + po] = best;
-                        success = true;
-                    }
-                }
-                if (!success) {
-                    po = vsz;
-                    while (!success) {
-                        int32_t px = conn_entry[vs + po];
-                        while (px != best && px > NULL_PART) {
-                            po++;
-                            px = conn_entry[vs + po];
-                        }
-                        if (px == best) {
-                            success = true;
-                        } else {
-                            conn_entry[vs + po] = best;
-                            conn_size[v]++;
-                            success = true;
-                        }
-                    }
-                }
-                conn_val[vs + po] += w;
-            }
-        }
+    // update_small's subtraction of w from part p in row v (jet_refiner.hpp:785-812).
+    void conn_sub(int32_t v, int32_t p, int64_t w) {
+        int64_t vs = conn_off[v];
+        int32_t vsz = conn_size[v];
+        int32_t po = p % vsz;
+        while (conn_entry[vs + po] != p) po = (po + 1) % vsz;
+        int64_t x = conn_val[vs + po];
+        conn_val[vs + po] -= w;
+        if (vsz < k && x == w) conn_entry[vs + po] = HASH_RECLAIM;
     }
+
+    // update_small's addition of w to part best in row v (jet_refiner.hpp:815-886).
+    void conn_add(int32_t v, int32_t best, int64_t w) {
+        dest_cache[v] = NULL_PART;
+        int64_t vs = conn_off[v];
+        int32_t vsz = conn_size[v];
+        int32_t po = best % vsz;
+        bool success = false;
+        for (int32_t q = 0; q < vsz; q++) {
+            int32_t pi = (best + q) % vsz;
+            if (conn_entry[vs + pi] == best) {
+                success = true;
+                po = pi;
+                break;
+            } else if (conn_entry[vs + pi] == NULL_PART) {
+                break;
+            }
+        }
+        int32_t count = 0;
+        while (!success && count < vsz) {
+            int32_t px = conn_entry[vs + po];
+            while (px != best && px > NULL_PART && count++ < vsz) {
+                po = (po + 1) % vsz;
+                px = conn_entry[vs + po];
+            }
+            if (px == best) {
+                success = true;
+            } else if (px <= NULL_PART) {
+                conn_entry[vs + po] = best;
+                success = true;
+            }
+        }
+        if (!success) {
+            po = vsz;
+            while (!success) {
+                int32_t px = conn_entry[vs + po];
+                while (px != best && px > NULL_PART) {
+                    po++;
+                    px = conn_entry[vs + po];
+                }
+                if (px == best) {
+                    success = true;
+                } else {
+                    conn_entry[vs + po] = best;
+                    conn_size[v]++;
+                    success = true;
+                }
+            }
+        }
+        conn_val[vs + po] += w;
+    }
+
+    // jet_refiner.hpp:781-888 (update_small), serial order of moves and edges:
+    // every subtraction, then every addition.
+    void update_small(const int32_t* part, const std::vector<int32_t>& moves) {
+#ifdef ORC_OMP
+        if (moves.size() >= 4096) return update_small_rows(part, moves);
+#endif
+        for (int32_t i : moves)
+            for (int64_t j = g->rowptr[i]; j < g->rowptr[i + 1]; j++) conn_sub(g->col[j], dest[i], g->ew[j]);
+        for (int32_t i : moves)
+            for (int64_t j = g->rowptr[i]; j < g->rowptr[i + 1]; j++) conn_add(g->col[j], part[i], g->ew[j]);
+    }
+
+#ifdef ORC_OMP
+    // The same by rows: a row's operations touch only that row, so each row
+    // applies its own, subtractions first, then additions in the serial order
+    // (move-list position, then edge index), which leaves every row as the
+    // serial loops leave it.
+    std::vector<int32_t> row_cnt;
+    std::vector<int64_t> row_beg;
+    std::vector<std::pair<int32_t, int64_t>> recs;  // (position in the move list, edge index)
+    void update_small_rows(const int32_t* part, const std::vector<int32_t>& moves) {
+        const int32_t n = g->n;
+        const int64_t M = (int64_t)moves.size();
+        row_cnt.resize(n);
+        row_beg.resize(n);
+        par_for(n, [&](int64_t v) { row_cnt[v] = 0; });
+        par_for(M, [&](int64_t x) {
+            const int32_t i = moves[x];
+            for (int64_t j = g->rowptr[i]; j < g->rowptr[i + 1]; j++) __atomic_fetch_add(&row_cnt[g->col[j]], 1, __ATOMIC_RELAXED);
+        });
+        flag.resize(n);
+        par_for(n, [&](int64_t v) { flag[v] = row_cnt[v] > 0; });
+        const std::vector<int32_t> rows = par_compact(n, flag.data());
+        int64_t tot = 0;
+        for (int32_t v : rows) {
+            row_beg[v] = tot;
+            tot += row_cnt[v];
+            row_cnt[v] = 0;  // fill cursor below
+        }
+        recs.resize(tot);
+        par_for(M, [&](int64_t x) {
+            const int32_t i = moves[x];
+            for (int64_t j = g->rowptr[i]; j < g->rowptr[i + 1]; j++) {
+                const int32_t v = g->col[j];
+                recs[row_beg[v] + __atomic_fetch_add(&row_cnt[v], 1, __ATOMIC_RELAXED)] = {(int32_t)x, j};
+            }
+        });
+        par_for((int64_t)rows.size(), [&](int64_t r) {
+            const int32_t v = rows[r];
+            auto* b = recs.data() + row_beg[v];
+            auto* e = b + row_cnt[v];
+            std::sort(b, e);
+            for (auto* q = b; q != e; q++) conn_sub(v, dest[moves[q->first]], g->ew[q->second]);
+            for (auto* q = b; q != e; q++) conn_add(v, part[moves[q->first]], g->ew[q->second]);
+        });
+    }
+#endif
 
     // jet_refiner.hpp:743-777 (update_large).
     void update_large(const int32_t* part, const std::vector<int32_t>& moves) {
-        for (int32_t i : moves)
-            for (int64_t j = g->rowptr[i]; j < g->rowptr[i + 1]; j++) swap_bit[g->col[j]] = 1;
-        for (int32_t v = 0; v < g->n; v++) {
-            if (swap_bit[v] != 1) continue;
+        par_for((int64_t)moves.size(), [&](int64_t x) {
+            const int32_t i = moves[x];
+            for (int64_t j = g->rowptr[i]; j < g->rowptr[i + 1]; j++)
+                __atomic_store_n(&swap_bit[g->col[j]], 1, __ATOMIC_RELAXED);
+        });
+        par_for(g->n, [&](int64_t v) {
+            if (swap_bit[v] != 1) return;
             for (int64_t j = conn_off[v]; j < conn_off[v + 1]; j++) {
                 conn_entry[j] = NULL_PART;
                 conn_val[j] = 0;
             }
-            if (g_slot_ties) build_row_large(part, v);
-            else build_row(part, v);
+            if (g_slot_ties) build_row_large(part, (int32_t)v);
+            else build_row(part, (int32_t)v);
             swap_bit[v] = 0;
             dest_cache[v] = NULL_PART;
-        }
+        });
     }
 
     // jet_refiner.hpp:905-955 (perform_moves).
     void perform_moves(int32_t* part, const std::vector<int32_t>& moves, RefineData& st) {
-        int64_t cc1 = 0;
-        for (int32_t i : moves) cc1 += lookup(i, dest[i]) - lookup(i, part[i]);
+        const int64_t M = (int64_t)moves.size();
+        int64_t cc1 = par_sum(M, [&](int64_t x) -> int64_t {
+            const int32_t i = moves[x];
+            return lookup(i, dest[i]) - lookup(i, part[i]);
+        });
         for (int32_t i : moves) {
             int32_t p = part[i];
             int32_t b = dest[i];
@@ -1021,13 +1157,16 @@ class Refiner {
         if ((int32_t)moves.size() > g->n / 10) update_large(part, moves);
 #endif
         else update_small(part, moves);
-        int64_t cc2 = 0;
-        for (int32_t i : moves) cc2 += lookup(i, part[i]) - lookup(i, dest[i]);
+        int64_t cc2 = par_sum(M, [&](int64_t x) -> int64_t {
+            const int32_t i = moves[x];
+            return lookup(i, part[i]) - lookup(i, dest[i]);
+        });
         int64_t mx = std::numeric_limits<int64_t>::min();
         for (int32_t p = 0; p < k; p++) mx = std::max(mx, st.part_sizes[p]);
         st.total_imb = mx > opt ? mx - opt : 0;
 #if ORC_MUTANT == 8
         st.cut -= cc1;
+        (void)cc2;
 #else
         st.cut -= cc1 + cc2;
 #endif
@@ -1211,7 +1350,7 @@ static int partition(int32_t alg, int32_t k, double imb, double tol, bool ultra,
         if (L > 0) {
             const std::vector<int32_t>& map = levels[L].map;
             std::vector<int32_t> fp(levels[L - 1].g.n);
-            for (size_t i = 0; i < fp.size(); i++) fp[i] = part[map[i]];  // uncoarsen.hpp:82-86
+            par_for((int64_t)fp.size(), [&](int64_t i) { fp[i] = part[map[i]]; });  // uncoarsen.hpp:82-86
             part.swap(fp);
         }
     }
@@ -1384,6 +1523,18 @@ int32_t orc_refine_level_trace(int32_t n, const int64_t* rowptr, const int32_t* 
 
 // Selects the tie-break between parts of equal connectivity (see g_slot_ties).
 void orc_set_slot_order_ties(int32_t on) { g_slot_ties = on != 0; }
+
+// Threads the OpenMP build uses from now on (t > 0), and the count in effect;
+// the serial build always reports 1.
+int32_t orc_threads(int32_t t) {
+#ifdef ORC_OMP
+    if (t > 0) omp_set_num_threads(t);
+    return omp_get_max_threads();
+#else
+    (void)t;
+    return 1;
+#endif
+}
 
 // Full multilevel partition. levels_out (optional, 8 int64 per level, coarsest
 // first): cut2, total_imb, n, m, iterations, lp_iterations, 0, 0.

@@ -12,7 +12,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "libjet_oracle.so")
+_OMP_PATH = os.path.join(_HERE, "_build", "libjet_oracle_omp.so")
 _lib = None
+_omp = None
 
 i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
@@ -42,6 +44,18 @@ def lib():
             build()
         _lib = _configure(ctypes.CDLL(_LIB_PATH))
     return _lib
+
+
+def omp_lib(threads=0):
+    """The same restatement built with OpenMP (identical results, loops over
+    independent items on `threads` threads; 0 keeps the current count): the
+    multi-core CPU baseline. Returns (library, threads in effect)."""
+    global _omp
+    if _omp is None:
+        if not os.path.exists(_OMP_PATH):
+            build()
+        _omp = _configure(ctypes.CDLL(_OMP_PATH))
+    return _omp, _omp.orc_threads(int(threads))
 
 
 def _configure(L):
@@ -89,6 +103,8 @@ def _configure(L):
                                              c.c_int32, c.c_int32, i32p, i64p, i64p, c.c_int32]
         L.orc_set_slot_order_ties.restype = None
         L.orc_set_slot_order_ties.argtypes = [c.c_int32]
+        L.orc_threads.restype = c.c_int32
+        L.orc_threads.argtypes = [c.c_int32]
     return L
 
 
@@ -216,11 +232,12 @@ def set_slot_order_ties(on):
     lib().orc_set_slot_order_ties(int(bool(on)))
 
 
-def partition(g, k, imb=1.03, alg=0, seed=0, tol=0.999, ultra=False, uniform=None, repeat_shortcut=False):
+def partition(g, k, imb=1.03, alg=0, seed=0, tol=0.999, ultra=False, uniform=None, repeat_shortcut=False, L=None):
     """Full multilevel partition; returns (part, cut, level records). With
     repeat_shortcut, the exact shortcuts for a weighted-HEC coarsening fixed
     point are taken (shared repeated levels, no re-refinement of an unchanged
-    repeated level); the default is the literal level loop."""
+    repeated level); the default is the literal level loop. L: another build
+    of the oracle (omp_lib()), or None for the serial one."""
     n, rp, col, ew = _g(g)
     if uniform is None:
         uniform = bool(g.get("uniform", False))
@@ -228,7 +245,7 @@ def partition(g, k, imb=1.03, alg=0, seed=0, tol=0.999, ultra=False, uniform=Non
     cut = ctypes.c_int64(0)
     levels = np.zeros(8 * 256, dtype=np.int64)
     nl = ctypes.c_int32(0)
-    rc = lib().orc_partition(alg, k, imb, tol, int(ultra), n, rp, col, ew, _vw(g), int(uniform), seed, part,
+    rc = (L or lib()).orc_partition(alg, k, imb, tol, int(ultra), n, rp, col, ew, _vw(g), int(uniform), seed, part,
                              ctypes.byref(cut), levels, ctypes.byref(nl), int(repeat_shortcut))
     if rc != 0:
         raise RuntimeError(f"oracle partition failed: {rc}")
