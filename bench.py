@@ -21,11 +21,6 @@ for p in (os.path.join(ROOT, "jet-partitioner-official-fork_amd"), os.path.join(
     if p not in sys.path:
         sys.path.insert(0, p)
 
-# the OpenMP CPU baseline binds its threads to cores; libgomp reads these
-# when the library is loaded
-os.environ.setdefault("OMP_PROC_BIND", "close")
-os.environ.setdefault("OMP_PLACES", "cores")
-
 import numpy as np  # noqa: E402
 
 import jet_amd  # noqa: E402
@@ -82,6 +77,10 @@ def host_cpu():
         info["CPU(s)"] = str(os.cpu_count())
     info["OMP_PROC_BIND"] = os.environ.get("OMP_PROC_BIND")
     info["OMP_PLACES"] = os.environ.get("OMP_PLACES")
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
     return info
 
 
@@ -101,6 +100,36 @@ def copy_bandwidth(torch):
         best = t if best is None else min(best, t)
     del a, b
     return round(2 * (4 << 28) / best / 1e9, 1)
+
+
+def block_partition(name, k):
+    """The geometric block partition of a grid or stencil workload (k split
+    over the axes as evenly as powers of two allow), or None for other
+    workloads: the quality anchor printed beside the edge cut."""
+    dims = {"stencil3d-128": (128, 128, 128), "grid2d-256": (256, 256)}.get(name)
+    if dims is None or k & (k - 1):
+        return None
+    splits = [1] * len(dims)
+    b = 0
+    while (1 << b) < k:
+        splits[b % len(dims)] *= 2
+        b += 1
+    n = int(np.prod(dims))
+    idx = np.arange(n, dtype=np.int64)
+    part = np.zeros(n, dtype=np.int64)
+    stride = 1
+    mult = 1
+    for d, s in zip(dims, splits):
+        coord = (idx // stride) % d
+        part += (coord * s // d) * mult
+        stride *= d
+        mult *= s
+    return part.astype(np.int32)
+
+
+def edge_cut(g, part):
+    rows = np.repeat(np.arange(len(g["rowptr"]) - 1), np.diff(g["rowptr"]))
+    return int(g["ew"][part[rows] != part[g["col"]]].astype(np.int64).sum() // 2)
 
 
 def make_graph(name):
@@ -369,12 +398,14 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    print("# importing torch", file=sys.stderr, flush=True)
+    import torch
+
+    # the library is loaded after torch, so both use the HIP runtime torch loaded
     for kv in args.tune:
         name, _, val = kv.partition("=")
         jet_amd.tuning_set(name, int(val))
         print(f"# tuning {name} = {jet_amd.tuning_get(name)}", file=sys.stderr, flush=True)
-    print("# importing torch", file=sys.stderr, flush=True)
-    import torch
 
     if not torch.cuda.is_available():
         raise SystemExit("no GPU visible")
@@ -474,6 +505,8 @@ def main():
 
     cpu = None
     cut_ratio = None
+    blk = block_partition(args.workload, wl["k"])
+    block_cut = edge_cut(g, blk) if blk is not None else None
     if "golden" in wl:
         # the oracle's partition of this exact graph, recorded once
         with open(os.path.join(ROOT, "tests", "golden", wl["golden"])) as f:
@@ -486,8 +519,14 @@ def main():
         import oracle
 
         # the OpenMP build of the CPU restatement (identical results to the
-        # serial build), on this process's CPU share, bound to cores
+        # serial build), on this process's CPU share, bound to cores. The
+        # binding is set only now: libgomp pins the thread that loads it, so
+        # set before torch (whose own OpenMP runtime loads with it) it would
+        # pin the main thread, and the host threads of every partition call,
+        # to one core.
         threads = cpu_threads()
+        os.environ.setdefault("OMP_PROC_BIND", "close")
+        os.environ.setdefault("OMP_PLACES", "cores")
         OL, threads = oracle.omp_lib(threads)
         sname = wl.get("cpu_sample")
         sg = make_graph(sname) if sname else g
@@ -533,6 +572,8 @@ def main():
                    "k": wl["k"], "imbalance": wl["imb"], "coarsening_alg": wl["alg"], "seed": args.seed},
         "edge_cut": cut,
         "cut_vs_oracle": cut_ratio,
+        "block_partition_cut": block_cut,
+        "cut_vs_block_partition": round(cut / block_cut, 4) if block_cut else None,
         "balanced": balanced,
         "phases_s": {"coarsen": round(st.coarsen_s, 4), "init": round(st.init_partition_s, 4),
                      "refine": round(st.refine_s, 4), "levels": st.num_levels} if st else None,

@@ -172,6 +172,18 @@ int jet_refine_level(int64_t n, const int64_t* row_offsets, const int32_t* col, 
                      int32_t k, double imb, double tol, int32_t ultra, int32_t uniform, int32_t ptr_space,
                      int32_t* part_io, int64_t* rec_out);
 
+/* jet_refine_level with the iteration controller's record of every iteration
+ * (jet_refiner.hpp:1102-1134), for parity tests; no reference counterpart.
+ * trace_out holds max_iters records of 6 int64: step kind (0 jet_lp, 1
+ * rebalance_weak, 2 rebalance_strong), -1, 2*cut and total_imb after the step,
+ * the best-state decision (0 none, 1 better balance, 2 better cut) and the
+ * stall counter after it. *iters_out = the number of iterations (records
+ * beyond max_iters are dropped). The level runs on the multi-kernel path. */
+int jet_refine_level_trace(int64_t n, const int64_t* row_offsets, const int32_t* col, const int32_t* ew,
+                           const int32_t* vw, int32_t k, double imb, double tol, int32_t ultra, int32_t uniform,
+                           int32_t ptr_space, int32_t* part_io, int64_t* rec_out, int64_t* trace_out,
+                           int32_t max_iters, int32_t* iters_out);
+
 /* uncoarsen.hpp:82-86: fine_out[i] = coarse[map[i]]. */
 int jet_project(int64_t n_fine, const int32_t* map, const int32_t* coarse, int32_t ptr_space, int32_t* fine_out);
 
@@ -362,12 +374,28 @@ int jet_partition_batch(jet_ctx_t* ctx, int32_t count, const jet_config_t* cfgs,
 
 /* ---- Tuning parameters: launch shapes and path thresholds that change the
  * speed but never the result (every setting gives the same partition). They
- * are process-wide and have no counterpart in the reference.
+ * are process-wide and have no counterpart in the reference. At the first
+ * context creation the library applies the tuning table of the device's
+ * architecture, tuning/<arch>.txt beside the library (or $JET_TUNING_FILE),
+ * written by the autotuner (jet_amd.autotune); without one the built-in
+ * defaults stay. Names:
  * "wg_max_n": levels of up to this many vertices are refined by one
- * workgroup that runs the whole iteration loop (0 = never; default 16384).
- * Returns JET_EINVAL for an unknown name or a value out of range. */
+ *   workgroup that runs the whole iteration loop (0 = never);
+ * "wg_big_deg": in that kernel, items above this degree take a whole wave;
+ * "small_level_n": levels treated as small (wave-built connectivity rows,
+ *   wider afterburner lane groups, 128-thread rebalancer select workgroups);
+ * "ab_lanes_small", "upd_lanes": lane-group multipliers (1-4) of the
+ *   afterburner on small levels and of the update passes;
+ * "lp_one_pass_n": LP select runs one pass per wave up to this level size;
+ * "ct_wave_max", "ct_block_max": contraction rows of up to this many fine
+ *   entries take the one-wave (<= 128) or one-workgroup (<= 4096) LDS table.
+ * jet_tuning_set / jet_tuning_get return JET_EINVAL for an unknown name or a
+ * value out of range; jet_tuning_load applies a table file ("name value"
+ * lines, '#' comments) and returns JET_EINVAL, applying nothing, when it
+ * cannot be read or holds an unknown name or a value out of range. */
 int jet_tuning_set(const char* name, int64_t value);
 int jet_tuning_get(const char* name, int64_t* value);
+int jet_tuning_load(const char* path);
 
 /* ---- Kernel timing for measurement tools. jet_prof_enable("lp_select")
  * times every later launch of that kernel family on the device's own clock

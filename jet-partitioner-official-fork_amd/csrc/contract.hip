@@ -11,6 +11,10 @@
 // Rows are emitted in increasing neighbour order, which is one of the orders
 // the reference's unordered consolidate can produce; weights are integer sums.
 #include "contract.h"
+
+#include <algorithm>
+
+#include "tuning.h"
 #include "prims.h"
 #include "wave.h"
 
@@ -41,14 +45,15 @@ __global__ void k_members(vid_t n, const vid_t* map, const int32_t* moff, int32_
 }
 
 __global__ void __launch_bounds__(256)
-    k_tiers(vid_t nc, const eoff_t* hoff, vid_t* l1, vid_t* l2, vid_t* l3, int32_t* cnt) {
+    k_tiers(vid_t nc, const eoff_t* hoff, vid_t* l1, vid_t* l2, vid_t* l3, int32_t* cnt, int32_t t1max,
+            int32_t t2max) {
     __shared__ int32_t sh[256 / WAVE + 1];
     FOR_BLOCK_UNIFORM(c, nc) {
         const bool in = c < nc;
         const eoff_t hs = in ? hoff[c + 1] - hoff[c] : 0;
-        int32_t a = block_append<256>(&cnt[0], in && hs <= T1_MAX, sh);
-        int32_t b = block_append<256>(&cnt[1], in && hs > T1_MAX && hs <= T2_MAX, sh);
-        int32_t d = block_append<256>(&cnt[2], in && hs > T2_MAX, sh);
+        int32_t a = block_append<256>(&cnt[0], in && hs <= t1max, sh);
+        int32_t b = block_append<256>(&cnt[1], in && hs > t1max && hs <= t2max, sh);
+        int32_t d = block_append<256>(&cnt[2], in && hs > t2max, sh);
         if (a >= 0) l1[a] = (vid_t)c;
         if (b >= 0) l2[b] = (vid_t)c;
         if (d >= 0) l3[d] = (vid_t)c;
@@ -353,7 +358,10 @@ void contract(const DGraph& g, const vid_t* map, vid_t nc, Arena& persist, Arena
     vid_t* l1 = lists;
     vid_t* l2 = lists + nc;
     vid_t* l3 = lists + 2 * (int64_t)nc;
-    k_tiers<<<grid_for(nc, 256, 8192), 256, 0, s>>>(nc, hoff, l1, l2, l3, cnt);
+    // the tuned tier bounds never exceed T1_MAX / T2_MAX (half of each table)
+    const int32_t t1max = (int32_t)std::min<int64_t>(T1_MAX, tuning().ct_wave_max);
+    const int32_t t2max = (int32_t)std::min<int64_t>(T2_MAX, std::max<int64_t>(t1max, tuning().ct_block_max));
+    k_tiers<<<grid_for(nc, 256, 8192), 256, 0, s>>>(nc, hoff, l1, l2, l3, cnt, t1max, t2max);
     JET_LAUNCH_CHECK();
     int32_t hcnt[4];
     read_i32_words(cnt, 4, hcnt, s);

@@ -26,6 +26,7 @@
 #include <thread>
 #include <vector>
 
+#include <sched.h>
 #include <sys/mman.h>
 
 namespace {
@@ -761,7 +762,12 @@ static void recurse(const Sub& s, int32_t k, int32_t first, double ub, uint64_t 
 }
 
 static int host_threads() {
-    int hw = (int)std::thread::hardware_concurrency();
+    // the CPUs this thread may run on (a pinned process gets no more threads
+    // than it has CPUs), else the machine's count
+    int hw = 0;
+    cpu_set_t cs;
+    if (sched_getaffinity(0, sizeof(cs), &cs) == 0) hw = CPU_COUNT(&cs);
+    if (hw < 1) hw = (int)std::thread::hardware_concurrency();
     if (hw < 1) hw = 1;
     if (const char* e = std::getenv("JET_HOST_THREADS")) {
         int v = std::atoi(e);

@@ -27,6 +27,7 @@
 #include "prims.h"
 #include "prof.h"
 #include "refine.h"
+#include "tuning.h"
 
 extern "C" int jet_host_initpart(int32_t n, const int64_t* xadj, const int32_t* adj, const int32_t* adjw,
                                  const int32_t* vw, int32_t k, double imb, uint64_t seed, int32_t* part);
@@ -49,6 +50,13 @@ Context* context_create(int device) {
     }
     if (device < 0 || device >= cnt) throw InvalidInput("device index out of range");
     JET_HIP(hipSetDevice(device));
+    {
+        // the device's tuning table, once per process
+        hipDeviceProp_t prop{};
+        JET_HIP(hipGetDeviceProperties(&prop, device));
+        std::string arch = prop.gcnArchName;  // e.g. "gfx950:sramecc+:xnack-"
+        tuning_init_once(arch.substr(0, arch.find(':')));
+    }
     Context* c = new Context();
     c->device = device;
     JET_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -771,8 +779,18 @@ int jet_refine_step(int32_t kind, int64_t n, const int64_t* row_offsets, const i
 int jet_refine_level(int64_t n, const int64_t* row_offsets, const int32_t* col, const int32_t* ew, const int32_t* vw,
                      int32_t k, double imb, double tol, int32_t ultra, int32_t uniform, int32_t ptr_space,
                      int32_t* part_io, int64_t* rec_out) {
+    return jet_refine_level_trace(n, row_offsets, col, ew, vw, k, imb, tol, ultra, uniform, ptr_space, part_io, rec_out,
+                                  nullptr, 0, nullptr);
+}
+
+int jet_refine_level_trace(int64_t n, const int64_t* row_offsets, const int32_t* col, const int32_t* ew,
+                           const int32_t* vw, int32_t k, double imb, double tol, int32_t ultra, int32_t uniform,
+                           int32_t ptr_space, int32_t* part_io, int64_t* rec_out, int64_t* trace_out,
+                           int32_t max_iters, int32_t* iters_out) {
     return guard_impl([&] {
         check_k(k);
+        if (iters_out && (max_iters < 0 || (max_iters > 0 && !trace_out)))
+            throw InvalidInput("trace_out must hold max_iters >= 0 records");
         Context& C = context();
         CallScope call(C);
         Level L = stage_graph(C, n, row_offsets, nullptr, col, ew, nullptr, vw, nullptr, ptr_space, uniform != 0);
@@ -788,8 +806,15 @@ int jet_refine_level(int64_t n, const int64_t* row_offsets, const int32_t* col, 
         rp.ultra = ultra != 0;
         int64_t cap = conn_capacity(L.n, L.rowptr, k, C.scratch, C.stream);
         Refiner ref(C.scratch, C.stream, L.n, cap, rp);
+        std::vector<std::array<int64_t, 6>> tr;
+        if (iters_out) ref.trace = &tr;
         LevelRecord rec = ref.refine(L, dp);
         from_device(C, part_io, dp, n, ptr_space);
+        if (iters_out) {
+            *iters_out = (int32_t)tr.size();
+            for (size_t t = 0; t < tr.size() && (int64_t)t < max_iters; t++)
+                for (int c = 0; c < 6; c++) trace_out[6 * t + c] = tr[t][c];
+        }
         rec_out[0] = rec.cut2;
         rec_out[1] = rec.total_imb;
         rec_out[2] = rec.n;

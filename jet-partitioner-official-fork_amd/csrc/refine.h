@@ -1,5 +1,6 @@
 // Jet refinement on the GPU (jet_refiner.hpp and uncoarsen.hpp of the reference).
 #pragma once
+#include <array>
 #include <vector>
 
 #include "arena.h"
@@ -55,6 +56,12 @@ class Refiner {
     // Connectivity tables for `part` (init_conn_data), copied to the host.
     void conn_tables(const Level& L, const part_t* part, std::vector<int64_t>& off, std::vector<int32_t>& slot_part,
                      std::vector<int32_t>& slot_val, std::vector<int32_t>& size);
+
+    // When set, refine() appends one record per iteration: step kind (0 lp,
+    // 1 weak, 2 strong), -1 (move count, not read back), cut2 and imbalance
+    // after the step, best-state decision (0 none, 1 balance, 2 cut) and the
+    // counter after it. A traced level runs on the multi-kernel path.
+    std::vector<std::array<int64_t, 6>>* trace = nullptr;
 
     // Whether the last refine() replaced the best partition (and so the
     // refine_data carried to the next level).

@@ -145,8 +145,6 @@ __global__ void __launch_bounds__(256) k_conn_tiers(DGraph g, int32_t k, vid_t* 
 }
 
 constexpr int SB_BLOCK = 256;
-// Levels up to this many vertices build every table with a wave per row.
-constexpr vid_t SMALL_LEVEL_N = 32768;
 
 __global__ void __launch_bounds__(SB_BLOCK)
     k_conn_build_small(DGraph g, const part_t* part, int32_t k, const eoff_t* conn_off, int2* conn,
@@ -341,14 +339,13 @@ __global__ void __launch_bounds__(BB_BLOCK)
 // candidates are staged in LDS and published with one atomic per workgroup
 // of W waves.
 constexpr int LS_WAVES = 8;
-// Levels up to LS_ONE_PASS_WAVES * WAVE vertices run one pass per wave in
+// Levels up to Tuning::lp_one_pass_n vertices run one pass per wave in
 // workgroups of LS_WAVES waves (eight: 0.3 ms less per 128^3 stencil
 // partition than four; sixteen is slower there). Larger levels run
 // LS_MULTI_CHW vertices per wave in workgroups of LS_MULTI_WAVES waves: few
 // enough workgroups appending to the candidate counters, and two passes per
 // wave (measured on the 128^3 stencil: 12.1 us per launch on average,
 // against 14.2 with eight passes in workgroups of four waves).
-constexpr int LS_ONE_PASS_WAVES = 8192;
 constexpr int LS_MULTI_CHW = 128;
 constexpr int LS_MULTI_WAVES = 16;
 
@@ -1118,14 +1115,33 @@ __global__ void __launch_bounds__(SEL_BLOCK_MAX)
 // every workgroup scans them in LDS (one load round trip instead of a scan
 // launch) and workgroup 0 writes the scan to off_out for the ranking and
 // destination kernels.
+// Whether weak bucket b straddles the eviction limit of a part it holds
+// members of: its owner q = b / width (gain buckets 0..49), and part q - 1 when
+// b is among q's first `sections` minibuckets (the uncapped gain bucket 50 of
+// part q - 1 lands there, jet_refiner.hpp:349-351; for q = k, past the last
+// part, only part k - 1 holds it).
+__device__ inline bool weak_straddle_of(int32_t b, int32_t q, const int64_t* off, const int64_t* sizes,
+                                        int64_t size_max, int64_t width) {
+    const int64_t base = off[b] - off[(int64_t)q * width];
+    const int64_t limit = sizes[q] - size_max;
+    return base < limit && base + (off[b + 1] - off[b]) > limit;
+}
+__device__ inline bool weak_bucket_straddles(int32_t b, int32_t k, int32_t sections, const int64_t* off,
+                                             const int64_t* sizes, int64_t size_max, int64_t width) {
+    const int32_t q = (int32_t)(b / width);
+    if (q < k && weak_straddle_of(b, q, off, sizes, size_max, width)) return true;
+    return q >= 1 && (int64_t)b - (int64_t)q * width < sections &&
+           weak_straddle_of(b, q - 1, off, sizes, size_max, width);
+}
+
 constexpr int CL_CHW = WAVE;
 constexpr int CL_WAVES = 16;  // fewer workgroups appending to the list counters (0.2 ms less than 4)
 constexpr int64_t FUSE_SCAN_MAX = 4096;
 template <bool STRONG, bool FUSED>
 __global__ void __launch_bounds__(CL_WAVES* WAVE)
     k_classify(DGraph g, const part_t* part, const int32_t* bid, const int64_t* off_in, int64_t nbk,
-               int64_t* off_out, const int64_t* sizes, int64_t size_max, int64_t width, vid_t* moves,
-               int32_t* counters, vid_t nmax, vid_t* rlist, int32_t* stamp, int32_t gen) {
+               int64_t* off_out, const int64_t* sizes, int64_t size_max, int64_t width, int32_t k, int32_t sections,
+               vid_t* moves, int32_t* counters, vid_t nmax, vid_t* rlist, int32_t* stamp, int32_t gen) {
     constexpr int BLOCK = CL_WAVES * WAVE;
     __shared__ int64_t soff[FUSED ? FUSE_SCAN_MAX : 1];
     __shared__ int64_t sws[BLOCK / WAVE + 1];
@@ -1175,9 +1191,12 @@ __global__ void __launch_bounds__(CL_WAVES* WAVE)
                 } else if (base + bs <= limit) {
                     ev = true;
                     deg = g.rowptr[i + 1] - g.rowptr[i];
-                } else {
-                    rank = base < limit;
                 }
+                // a weak bucket that straddles its part's limit is ranked
+                // with all of its members: bucket 50 of part q shares its
+                // index with part q + 1's first gain bucket, and a member's
+                // position counts every member of lower id (:363-384)
+                if (!STRONG) rank = weak_bucket_straddles(b, k, sections, off, sizes, size_max, width);
             }
         }
         st.push(ev && deg <= BIG_DEG, ev && deg > BIG_DEG, (vid_t)i);
@@ -1188,7 +1207,7 @@ __global__ void __launch_bounds__(CL_WAVES* WAVE)
 }
 
 // Decision for one ranked vertex: pos = weight of same-bucket members with a
-// smaller id. Returns 0 (kept), 1 (evicted, degree <= BIG_DEG) or 2 (evicted,
+// smaller id (of any part). Returns 0 (kept), 1 (evicted, degree <= BIG_DEG) or 2 (evicted,
 // larger); the caller appends evicted members to the move list.
 template <bool STRONG>
 __device__ inline int rank_decide_with(vid_t v, int32_t b, int64_t pos, part_t p, int64_t w, bool big,
@@ -1602,9 +1621,10 @@ static int group_for_avg(const DGraph& g) {
 // 128^3 stencil partition; doubling the afterburner's on large levels was
 // slower).
 static int group_for_afterburner(const DGraph& g) {
-    return g.n <= SMALL_LEVEL_N ? std::min(64, 2 * group_for_avg(g)) : group_for_avg(g);
+    const int m = g.n <= tuning().small_level_n ? (int)tuning().ab_lanes_small : 1;
+    return std::min(64, m * group_for_avg(g));
 }
-static int group_for_update(const DGraph& g) { return std::min(64, 2 * group_for_avg(g)); }
+static int group_for_update(const DGraph& g) { return std::min(64, (int)tuning().upd_lanes * group_for_avg(g)); }
 
 __global__ void __launch_bounds__(256) k_max_degree(DGraph g, int32_t* out) {
     __shared__ int32_t red[256 / WAVE];
@@ -1687,7 +1707,7 @@ void Refiner::build_tiers(const Level& L) {
     fill_i32(counters_ + 16, 0, 4, s_);
     // on small levels every row with edges takes the wave tier: a thread per
     // row walks its edges one after another, a wave inserts them all at once
-    small_max_ = g.n <= SMALL_LEVEL_N ? 0 : SMALL_DEG;
+    small_max_ = g.n <= tuning().small_level_n ? 0 : SMALL_DEG;
     k_conn_tiers<<<grid_for(g.n, 256, 8192), 256, 0, s_>>>(g, p_.k, tier_mid_, tier_big_, pull_mid_, counters_ + 16,
                                                             hub_, small_max_);
     k_max_degree<<<grid_for(g.n, 256, 512), 256, 0, s_>>>(g, counters_ + 18);
@@ -1737,7 +1757,7 @@ void Refiner::jet_lp(const Level& L, const part_t* part, double fr) {
     gen_++;  // this step's moves are stamped with a new generation
     // a pass per wave on levels that fill the GPU with single-pass waves, two
     // passes per wave in 16-wave workgroups above that (see LS_MULTI_CHW)
-    if (g.n <= (int64_t)LS_ONE_PASS_WAVES * WAVE) {
+    if (g.n <= tuning().lp_one_pass_n) {
         int nb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_WAVES * WAVE));
         k_lp_select<WAVE, LS_WAVES><<<nb, LS_WAVES * WAVE, 0, s_>>>(
             g, part, conn_off_, conn_, conn_size_, dest_cache_, gain_p_, lock_, fr, dest_, pregain_, cand_, c, nmax_,
@@ -1875,7 +1895,7 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
     // levels, one per thread of a 128-thread workgroup): vertices of one
     // oversized part share few buckets, and the serial per-vertex chain of a
     // heavy range stays short
-    int nsel = grid_for(g.n, g.n <= SMALL_LEVEL_N ? 128 : 512, 1024);
+    int nsel = grid_for(g.n, g.n <= tuning().small_level_n ? 128 : 512, 1024);
     unsigned long long* bk = reinterpret_cast<unsigned long long*>(buckets_);
     unsigned long long* bc = reinterpret_cast<unsigned long long*>(bcopies_);
     // about 256 workgroups per copy: fewer workgroups meet on a line without
@@ -1885,7 +1905,7 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
     // 512 threads on large levels (half the serial vertices per thread), 128
     // on small ones (more workgroups over few vertices; 0.3 ms less per 128^3
     // stencil partition than 256)
-    const int selb = g.n > SMALL_LEVEL_N ? SEL_BLOCK_MAX : 128;
+    const int selb = g.n > tuning().small_level_n ? SEL_BLOCK_MAX : 128;
     if (strong) {
         if (lds)
             k_strong_select<true><<<nsel, selb, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, acc_, conn_off_,
@@ -1915,7 +1935,7 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
     int cb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)CL_WAVES * CL_CHW));
 #define JET_CLASSIFY(ST, FU)                                                                                 \
     k_classify<ST, FU><<<cb, CL_WAVES * WAVE, 0, s_>>>(g, part, bid_, buckets_, nbk, boff_, sizes, smax, width, \
-                                                       moves_, c, nmax_, cand_, stamp_, gen_)
+                                                       k, sections, moves_, c, nmax_, cand_, stamp_, gen_)
     if (strong) {
         if (fused) JET_CLASSIFY(true, true);
         else JET_CLASSIFY(true, false);
@@ -2072,7 +2092,7 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
     }
     {
         const RebalanceShape wr = rebalance_shape(g.n, false);
-        if (wg_fits(g.n, k, wr.nbk, tuning().wg_max_n) && (int)temps.size() <= WG_MAX_TEMPS) {
+        if (!trace && wg_fits(g.n, k, wr.nbk, tuning().wg_max_n) && (int)temps.size() <= WG_MAX_TEMPS) {
             LevelRecord rec = refine_wg(L, best_part, temps);
             rec.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             return rec;
@@ -2103,13 +2123,13 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
             }
             curr_imb = mx > opt_ ? mx - opt_ : 0;
             curr_cut -= cc;
-            bool take = false;
+            int take = 0;  // 1: better balance, 2: better cut
             if (total_imb_ > imb_max && curr_imb < total_imb_) {
-                take = true;
+                take = 1;
                 count = 0;
             } else if (curr_cut < cut_ && (curr_imb <= imb_max || curr_imb <= total_imb_)) {
                 if ((double)curr_cut < p_.tol * (double)cut_) count = 0;
-                take = true;
+                take = 2;
             }
             if (take) {
                 changed_ = true;
@@ -2117,6 +2137,10 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
                 total_imb_ = curr_imb;
                 copy_i64(best_sizes_, curr_sizes_, k, s_);
                 copy_i32(best_part, cur_part_, g.n, s_);
+            }
+            if (trace) {
+                const int64_t kind = balance_counter == 0 ? 0 : (strong ? 2 : 1);
+                trace->push_back({kind, -1, curr_cut, curr_imb, take, count});
             }
         }
     }
@@ -2142,6 +2166,7 @@ LevelRecord Refiner::refine_wg(const Level& L, part_t* best_part, const std::vec
     WgParams P{};
     P.g = g;
     P.k = k;
+    P.big_deg = (int32_t)tuning().wg_big_deg;
     P.sections = weak.sections;
     P.nbk = (int32_t)weak.nbk;
     P.opt = opt_;

@@ -24,6 +24,7 @@ constexpr int WG_MAX_TEMPS = 20;
 struct WgParams {
     DGraph g;
     int32_t k, sections;
+    int32_t big_deg;  // items of higher degree are taken by a whole wave
     int32_t nbk;  // minibuckets scanned (the weak layout, the larger of the two)
     int64_t opt, size_max, imb_max, max_dest_weak, max_dest_strong;
     double tol;

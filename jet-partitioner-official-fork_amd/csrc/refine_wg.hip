@@ -26,7 +26,7 @@
 //   rebuild marked; the workgroup then stops after the iteration and the host
 //   rebuilds them with the multi-workgroup build before relaunching.
 //
-// Items of degree > WG_BIG_DEG (afterburner candidates and moves) are taken by
+// Items of degree > P.big_deg (afterburner candidates and moves) are taken by
 // a whole wave instead of one thread.
 #include "refine_wg.h"
 
@@ -39,7 +39,6 @@ namespace jet {
 
 constexpr int WG_BLOCK = 1024;
 constexpr int WG_WAVES = WG_BLOCK / WAVE;
-constexpr int32_t WG_BIG_DEG = 32;
 constexpr int32_t WG_ITER_CAP = 1 << 20;
 
 bool wg_fits(int64_t n, int32_t k, int64_t nbk, int64_t max_n) {
@@ -275,7 +274,7 @@ __global__ void __launch_bounds__(WG_BLOCK) k_refine_wg(WgParams P) {
                 bool big = false;
                 if (cand) {
                     P.pregain[i] = gain;
-                    big = deg_of(g, i) > WG_BIG_DEG;
+                    big = deg_of(g, i) > P.big_deg;
                 }
                 list_push(P.cand, n, &L.cnt[0], &L.cnt[1], cand, big, i);
             }
@@ -408,14 +407,25 @@ __global__ void __launch_bounds__(WG_BLOCK) k_refine_wg(WgParams P) {
             // classification: whole buckets below a part's limit are evicted
             // (weak) or need their members' positions (strong); a weak bucket
             // straddling the limit needs them too
+            // a weak bucket that straddles the limit of a part it holds
+            // members of is ranked with all of its members: bucket 50 of
+            // part q shares its index with part q + 1's first gain bucket,
+            // and a member's position counts every member of lower id
+            auto weak_straddle_of = [&](int32_t b, int32_t q) -> bool {
+                const int64_t base = (int64_t)sbk[b] - sbk[(int64_t)q * width];
+                const int64_t limit = L.sizes[q] - smax;
+                return base < limit && base + ((int64_t)sbk[b + 1] - sbk[b]) > limit;
+            };
             auto needs_rank = [&](vid_t i) -> bool {
                 const int32_t b = P.bid[i];
                 if (b < 0) return false;
-                const part_t p = P.part[i];
-                const int64_t base = (int64_t)sbk[b] - sbk[(int64_t)p * width];
-                const int64_t limit = L.sizes[p] - smax;
-                if (strong) return base < limit;
-                return base < limit && base + ((int64_t)sbk[b + 1] - sbk[b]) > limit;
+                if (strong) {
+                    const part_t p = P.part[i];
+                    return (int64_t)sbk[b] - sbk[(int64_t)p * width] < L.sizes[p] - smax;
+                }
+                const int32_t q = (int32_t)(b / width);
+                if (q < k && weak_straddle_of(b, q)) return true;
+                return q >= 1 && (int64_t)b - (int64_t)q * width < P.sections && weak_straddle_of(b, q - 1);
             };
             if (!strong) {
                 for (vid_t i = t; i < n; i += WG_BLOCK) {
@@ -426,7 +436,7 @@ __global__ void __launch_bounds__(WG_BLOCK) k_refine_wg(WgParams P) {
                         const int64_t base = (int64_t)sbk[b] - sbk[(int64_t)p * width];
                         ev = base + ((int64_t)sbk[b + 1] - sbk[b]) <= L.sizes[p] - smax;
                     }
-                    list_push(P.moves, n, &L.cnt[2], &L.cnt[3], ev, ev && deg_of(g, i) > WG_BIG_DEG, i);
+                    list_push(P.moves, n, &L.cnt[2], &L.cnt[3], ev, ev && deg_of(g, i) > P.big_deg, i);
                 }
             }
             const int32_t R = ordered_compact_ids(n, P.rlist, needs_rank, L.red);
@@ -468,9 +478,13 @@ __global__ void __launch_bounds__(WG_BLOCK) k_refine_wg(WgParams P) {
                     const part_t p = P.part[v];
                     const int64_t base = (int64_t)sbk[b] - sbk[(int64_t)p * width];
                     const int64_t limit = L.sizes[p] - smax;
-                    if (base + pos < limit) {
+                    // a weak member decides here only when its own part's
+                    // limit straddles the bucket (whole buckets were decided
+                    // by the classification)
+                    const bool own = strong || weak_straddle_of(b, p);
+                    if (own && base + pos < limit) {
                         ev = true;
-                        big = deg_of(g, v) > WG_BIG_DEG;
+                        big = deg_of(g, v) > P.big_deg;
                         if (strong) {
                             if (base + pos + (int64_t)w >= limit) L.evict_adjust[p] = base + pos + (int64_t)w;
                             P.score[v] = (int64_t)sbk[b] + pos;

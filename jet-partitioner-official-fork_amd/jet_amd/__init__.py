@@ -105,6 +105,7 @@ def lib():
             "jet_conn_tables": [i64, VP, VP, VP, VP, i32, i32, VP, VP, VP, VP],
             "jet_refine_step": [i32, i64, VP, VP, VP, VP, VP, i32, dbl, dbl, i32, VP, VP, c.POINTER(i32)],
             "jet_refine_level": [i64, VP, VP, VP, VP, i32, dbl, dbl, i32, i32, i32, VP, VP],
+            "jet_refine_level_trace": [i64, VP, VP, VP, VP, i32, dbl, dbl, i32, i32, i32, VP, VP, VP, i32, VP],
             "jet_project": [i64, VP, VP, i32, VP],
             "jet_edge_cut": [i64, VP, VP, VP, VP, i32, c.POINTER(i64)],
             "jet_edge_cut_noalloc": [i64, VP, VP, VP, VP, c.POINTER(i64)],
@@ -134,6 +135,7 @@ def lib():
             "jet_uncoarsen_hierarchy": [c.POINTER(Config), VP, VP, VP, c.POINTER(i64), c.POINTER(Stats)],
             "jet_tuning_set": [c.c_char_p, i64],
             "jet_tuning_get": [c.c_char_p, c.POINTER(i64)],
+            "jet_tuning_load": [c.c_char_p],
             "jet_prof_enable": [c.c_char_p],
             "jet_prof_read": [c.POINTER(i64), c.POINTER(dbl), c.POINTER(dbl)],
         }
@@ -335,6 +337,21 @@ def refine_level(g, part, k, imb, tol=0.999, ultra=False, uniform=False):
     return p, rec
 
 
+def refine_level_trace(g, part, k, imb, tol=0.999, ultra=False, uniform=False, max_iters=4096):
+    """refine_level on the multi-kernel path plus one record per iteration:
+    (kind 0 lp / 1 weak / 2 strong, -1, cut2, total_imb, take 0 none / 1
+    balance / 2 cut, count)."""
+    rp, col, ew, vw = _graph(g)
+    n = len(rp) - 1
+    p = np.array(part, dtype=np.int32)
+    rec = np.zeros(8, dtype=np.int64)
+    tr = np.zeros((max_iters, 6), dtype=np.int64)
+    it = ctypes.c_int32(0)
+    _check(lib().jet_refine_level_trace(n, _p(rp), _p(col), _p(ew), _p(vw), k, imb, tol, int(ultra), int(uniform),
+                                        PTR_HOST, _p(p), _p(rec), _p(tr), max_iters, ctypes.byref(it)))
+    return p, rec, [tuple(int(x) for x in row) for row in tr[:min(it.value, max_iters)]]
+
+
 def project(mp, coarse):
     mp = _c(mp, np.int32)
     out = np.empty(len(mp), dtype=np.int32)
@@ -361,6 +378,21 @@ def tuning_get(name):
     v = ctypes.c_int64(0)
     _check(lib().jet_tuning_get(name.encode(), ctypes.byref(v)))
     return v.value
+
+
+# the result-neutral tuning parameters (include/jet_capi.h)
+TUNING_NAMES = ("wg_max_n", "wg_big_deg", "small_level_n", "ab_lanes_small", "upd_lanes", "lp_one_pass_n",
+                "ct_wave_max", "ct_block_max")
+
+
+def tuning_all():
+    """{name: value} of every tuning parameter."""
+    return {n: tuning_get(n) for n in TUNING_NAMES}
+
+
+def tuning_load(path):
+    """Applies a tuning table file (jet_tuning_load)."""
+    _check(lib().jet_tuning_load(os.fsencode(path)))
 
 
 class tuned:

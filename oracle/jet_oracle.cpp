@@ -842,6 +842,24 @@ class Refiner {
             buckets[b] = run;
             run += x;
         }
+#if ORC_MUTANT == 13
+        // positions counted only among the members whose own part's limit
+        // straddles their bucket (the uncapped bucket 50 of part p shares its
+        // index with part p + 1's first bucket, whose members are left out)
+        std::vector<int64_t> tot(buckets.size(), 0), rpos(buckets.size(), 0);
+        for (int32_t i = 0; i < n; i++)
+            if (bid[i] >= 0) tot[bid[i]] += vw[i];
+        for (int32_t i = 0; i < n; i++) {
+            const int32_t b = bid[i];
+            if (b < 0) continue;
+            const int64_t base = buckets[b] - buckets[(int64_t)part[i] * width];
+            const int64_t limit = sizes[part[i]] - smax;
+            if (base < limit && base + tot[b] > limit) {
+                pos[i] = rpos[b];
+                rpos[b] += vw[i];
+            }
+        }
+#endif
         flag.resize(n);
         par_for(n, [&](int64_t i) {
             int32_t b = bid[i];

@@ -123,3 +123,40 @@ def test_loop_free_gain_bucket_equals_the_loop(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout
     assert int(r.stdout.split()[-1]) > 7_000_000
+
+
+def test_tuning_table_parsing(tmp_path):
+    import jet_amd
+
+    old = jet_amd.tuning_all()
+    try:
+        good = tmp_path / "t.txt"
+        good.write_text("# a table\nupd_lanes 1   # lanes per move\n\nct_wave_max 64\n")
+        jet_amd.tuning_load(str(good))
+        assert jet_amd.tuning_get("upd_lanes") == 1 and jet_amd.tuning_get("ct_wave_max") == 64
+        for text in ("no_such_knob 3\n", "upd_lanes 9\n", "upd_lanes\n", "ct_block_max x\n"):
+            bad = tmp_path / "bad.txt"
+            bad.write_text("upd_lanes 4\n" + text)
+            with pytest.raises(jet_amd.JetError):
+                jet_amd.tuning_load(str(bad))
+            assert jet_amd.tuning_get("upd_lanes") == 1  # nothing applied
+        with pytest.raises(jet_amd.JetError):
+            jet_amd.tuning_load(str(tmp_path / "missing.txt"))
+    finally:
+        for k, v in old.items():
+            jet_amd.tuning_set(k, v)
+
+
+def test_installed_tuning_tables_are_valid():
+    import glob
+
+    import jet_amd
+
+    old = jet_amd.tuning_all()
+    try:
+        tables = glob.glob(os.path.join(os.path.dirname(jet_amd.__file__), "..", "tuning", "*.txt"))
+        for t in tables:
+            jet_amd.tuning_load(t)
+    finally:
+        for k, v in old.items():
+            jet_amd.tuning_set(k, v)

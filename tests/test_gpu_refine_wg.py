@@ -99,3 +99,19 @@ def test_tuning_parameter_api():
     for bad in (("wg_max_n", -1), ("no_such_knob", 1)):
         with pytest.raises(jet_amd.JetError):
             jet_amd.tuning_set(*bad)
+
+
+def test_weak_bucket_50_shared_with_next_part(refine_path):
+    # edge weights up to 60000 give weak-rebalancing gains past the last gain
+    # bucket: the uncapped bucket 50 of part q shares its index with the first
+    # gain bucket of part q + 1, and positions inside it count the members of
+    # both parts; two sections per part (n >= 4096 k)
+    g = graphs.with_weights(graphs.random_graph(9000, 18000, seed=13, wmax=1), seed=8, wmax=60000, vwmax=3)
+    n = len(g["rowptr"]) - 1
+    for k, skew in ((2, 1.0), (2, 0.0), (8, 1.5)):
+        p0 = rand_part(n, k, 5, skew)
+        say("bucket 50", refine_path, k, skew)
+        p, rec = jet_amd.refine_level(g, p0, k, 1.03, uniform=False)
+        ep, erec, tr = oracle.refine_level_trace(g, p0, k, 1.03, uniform=False)
+        assert list(rec) == list(erec), (rec, erec)
+        assert np.array_equal(p, ep)

@@ -38,7 +38,7 @@ BATTERY = [
     ("tol", lambda: graphs.with_weights(graphs.grid2d(24, 20), seed=7, wmax=4), 6, 1.05, 0.95),
 ]
 SKEWS = (0.0, 1.5, 3.0)
-MUTANTS = range(1, 13)
+MUTANTS = range(1, 14)
 
 _cache = {}
 
@@ -82,7 +82,9 @@ def test_planted_mistake_is_detected(mut):
     without its fallback, 5 strong re-draws in decreasing id, 6 LP filter with
     > for >=, 7 weak eligibility with <= for <, 8 cut change without its
     second half, 9 update_large threshold n/5, 10 best-state balance test with
-    < for <=, 11 best-state cut test with <= for <, 12 tolerance ignored."""
+    < for <=, 11 best-state cut test with <= for <, 12 tolerance ignored, 13
+    weak positions counted only among members whose own part straddles their
+    bucket (leaves out the next part's members sharing bucket 50's index)."""
     L = oracle.mutant(mut)
     caught = []
     for name, mk, k, imb, tol in BATTERY:
