@@ -132,6 +132,56 @@ def edge_cut(g, part):
     return int(g["ew"][part[rows] != part[g["col"]]].astype(np.int64).sum() // 2)
 
 
+def kernel_rooflines(step, args, ms_per_step, torch):
+    """Per-family rooflines of the partition: for each kernel family, the
+    same --steps partitions again with every unit of that family bracketed by
+    HIP events on the stream its kernels run on (average unit time), then one
+    partition in counting mode for every family's algorithmic bytes (the
+    computation is deterministic, so its units are the timed ones). Returns
+    (roofline of the family with the most kernel time, all families,
+    end-to-end figure: modeled bytes of all families / partition time)."""
+    fams = {}
+    for fam in jet_amd.PROF_FAMILIES:
+        jet_amd.prof_enable(fam)
+        torch.cuda.synchronize()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        units, tot_ms, _ = jet_amd.prof_read(fam)
+        fams[fam] = {"units": units, "ms": tot_ms}
+    jet_amd.prof_enable("all:count")
+    step()
+    torch.cuda.synchronize()
+    total_bytes = 0.0
+    for fam in jet_amd.PROF_FAMILIES:
+        cu, _, nbytes = jet_amd.prof_read(fam)
+        f = fams[fam]
+        f["units_per_partition"] = cu
+        f["bytes_per_partition"] = round(nbytes)
+        total_bytes += nbytes
+        if cu > 0 and f["units"] == cu * args.steps and f["ms"] > 0:
+            avg_s = f["ms"] / 1e3 / f["units"]
+            f["avg_us"] = round(avg_s * 1e6, 3)
+            f["bytes_per_unit"] = round(nbytes / cu)
+            f["achieved_GBps"] = round(nbytes / cu / avg_s / 1e9, 1)
+            f["frac"] = round(nbytes / cu / avg_s / 8e12, 4)
+            f["ms_per_partition"] = round(f["ms"] / args.steps, 3)
+    jet_amd.prof_enable("none")
+    timed = {k: v for k, v in fams.items() if "frac" in v}
+    roofline = None
+    if timed:
+        top = max(timed, key=lambda k: timed[k]["ms"])
+        f = timed[top]
+        roofline = {"kernel": top, "bound": "hbm", "achieved": f["achieved_GBps"], "peak": 8000.0,
+                    "copy_GBps": copy_bandwidth(torch), "unit": "GB/s", "frac": f["frac"],
+                    "traffic": counted_traffic(top, args.workload),
+                    "launches": f["units"], "avg_us": f["avg_us"], "bytes_per_launch": f["bytes_per_unit"]}
+    e2e = {"modeled_bytes_per_partition": round(total_bytes), "families": list(jet_amd.PROF_FAMILIES),
+           "achieved_GBps": round(total_bytes / (ms_per_step / 1e3) / 1e9, 1),
+           "frac": round(total_bytes / (ms_per_step / 1e3) / 8e12, 5)}
+    return roofline, fams, e2e
+
+
 def make_graph(name):
     if name == "stencil3d-128":
         return jet_amd.stencil3d(128, 128, 128, seed=0, wmax=10)
@@ -391,7 +441,6 @@ def main():
     ap.add_argument("--cpu-runs", type=int, default=5, help="timed CPU-baseline runs after one warm-up")
     ap.add_argument("--tune", action="append", default=[], metavar="NAME=VALUE",
                     help="set a result-neutral tuning parameter before the run (repeatable)")
-    ap.add_argument("--prof", default="lp_select", help="kernel family timed with HIP events")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (each box has one MI355X)",
@@ -447,7 +496,6 @@ def main():
     for w in range(args.warmup):
         step()
         print(f"# warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
-    prof_ok = jet_amd.lib().jet_prof_enable(args.prof.encode()) == 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     cut = None
@@ -465,30 +513,7 @@ def main():
     edges = m // 2
     value = edges * args.steps / dt
 
-    roofline = None
-    if prof_ok:
-        L = jet_amd.lib()
-        launches, tot_ms, nbytes = ctypes.c_int64(0), ctypes.c_double(0), ctypes.c_double(0)
-        ok = L.jet_prof_read(ctypes.byref(launches), ctypes.byref(tot_ms), ctypes.byref(nbytes)) == 0 \
-            and launches.value > 0 and tot_ms.value > 0
-        t_launches, t_ms = launches.value, tot_ms.value
-        # algorithmic bytes: one more (untimed) partition in counting mode;
-        # the computation is deterministic, so its launches are the timed ones
-        if ok and L.jet_prof_enable((args.prof + ":count").encode()) == 0:
-            step()
-            ok = L.jet_prof_read(ctypes.byref(launches), ctypes.byref(tot_ms), ctypes.byref(nbytes)) == 0 \
-                and launches.value > 0
-        L.jet_prof_enable(b"none")
-        if ok:
-            avg_s = t_ms / 1e3 / t_launches
-            per_launch = nbytes.value / launches.value
-            achieved = per_launch / avg_s / 1e9
-            roofline = {"kernel": args.prof, "bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0,
-                        "copy_GBps": copy_bandwidth(torch),
-                        "unit": "GB/s", "frac": round(achieved / 8000.0, 4),
-                        "traffic": counted_traffic(args.prof, args.workload),
-                        "launches": t_launches, "avg_us": round(avg_s * 1e6, 2),
-                        "bytes_per_launch": round(per_launch)}
+    roofline, families, e2e = kernel_rooflines(step, args, ms, torch)
 
     part = d_part.cpu().numpy()
 
@@ -578,6 +603,8 @@ def main():
         "phases_s": {"coarsen": round(st.coarsen_s, 4), "init": round(st.init_partition_s, 4),
                      "refine": round(st.refine_s, 4), "levels": st.num_levels} if st else None,
         "roofline": roofline,
+        "kernel_families": families,
+        "end_to_end": e2e,
         "cpu_baseline": cpu,
         "pcie_inclusive": pcie,
     }

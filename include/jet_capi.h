@@ -397,15 +397,17 @@ int jet_tuning_set(const char* name, int64_t value);
 int jet_tuning_get(const char* name, int64_t* value);
 int jet_tuning_load(const char* path);
 
-/* ---- Kernel timing for measurement tools. jet_prof_enable("lp_select")
- * times every later launch of that kernel family on the device's own clock
- * (earliest workgroup start to latest workgroup end, per launch);
- * "lp_select:count" instead counts the items its algorithmic bytes follow
- * from (a separate run, since counting adds work to the kernel); "none"
- * stops. Families: "lp_select". jet_prof_read returns the launches, their
- * total milliseconds (timing) and their algorithmic bytes (counting). */
+/* ---- Kernel-family timing and algorithmic bytes for measurement tools (no
+ * counterpart in the reference). Families: "lp_select", "afterburner",
+ * "update" (one unit = one pass of update kernels), "weak_select".
+ * jet_prof_enable("<family>") times every later unit of that family with a
+ * pair of HIP events on the stream its kernels run on; "<family>:count" or
+ * "all:count" instead counts the items each family's algorithmic bytes follow
+ * from (a separate run: counting adds work); "none" stops. jet_prof_read
+ * returns, for an enabled family, its units and their total milliseconds
+ * (timing) or its algorithmic bytes (counting). */
 int jet_prof_enable(const char* family);
-int jet_prof_read(int64_t* launches, double* total_ms, double* bytes);
+int jet_prof_read(const char* family, int64_t* units, double* total_ms, double* bytes);
 
 #ifdef __cplusplus
 }

@@ -1002,12 +1002,17 @@ int jet_prof_enable(const char* family) {
     });
 }
 
-int jet_prof_read(int64_t* launches, double* total_ms, double* bytes) {
+int jet_prof_read(const char* family, int64_t* units, double* total_ms, double* bytes) {
     return guard_impl([&] {
+        if (!family || !units || !total_ms || !bytes) throw InvalidInput("family and outputs are required");
+        int fam = -1;
+        for (int f = 0; f < PF_FAMILIES; f++)
+            if (std::strcmp(family, kProfNames[f]) == 0) fam = f;
+        if (fam < 0) throw InvalidInput(std::string("unknown kernel family: ") + family);
         int64_t l = 0;
         double ms = 0, b = 0;
-        if (!prof_read(l, ms, b)) throw InvalidInput("no kernel family enabled");
-        *launches = l;
+        if (!prof_read(fam, l, ms, b)) throw InvalidInput(std::string("family not enabled: ") + family);
+        *units = l;
         *total_ms = ms;
         *bytes = b;
     });

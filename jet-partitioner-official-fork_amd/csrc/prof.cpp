@@ -1,34 +1,99 @@
-// Kernel-family timing and counting (see prof.h).
+// Kernel-family timing (HIP events) and algorithmic-byte counting (prof.h).
 #include "prof.h"
 
+#include <cstring>
 #include <vector>
 
 #include "jet_types.h"
 
 namespace jet {
 
-Prof& prof() {
+const char* const kProfNames[PF_FAMILIES] = {"lp_select", "afterburner", "update", "weak_select"};
+
+namespace {
+
+struct Prof {
+    int timed = -1;                       // family timed with events
+    bool counting[PF_FAMILIES] = {};      // families counted
+    double host_bytes[PF_FAMILIES] = {};  // bytes known on the host
+    int64_t units[PF_FAMILIES] = {};      // units accounted (counting)
+    unsigned long long* dev = nullptr;    // device counters, PROF_COUNTERS per family
+    std::vector<hipEvent_t> ev;           // 2 per timed unit
+    int64_t nev = 0;                      // timed units recorded
+    int dev_id = -1;                      // device the events and counters belong to
+};
+
+Prof& P() {
     static Prof* p = new Prof();
     return *p;
 }
 
-unsigned long long* prof_counters(int fam) { return prof_on(fam) && prof().counting ? prof().dev : nullptr; }
+int family_of(const std::string& name) {
+    for (int f = 0; f < PF_FAMILIES; f++)
+        if (name == kProfNames[f]) return f;
+    return -1;
+}
 
-unsigned long long* prof_span(int fam) {
-    if (!prof_on(fam) || prof().counting) return nullptr;
-    Prof& P = prof();
-    if (P.spans >= PROF_MAX_SPANS) return nullptr;
-    return P.span + 2 * PROF_SPAN_SLOTS * (P.spans++);
+// Algorithmic bytes of one family from its counters, at the declared widths
+// (ordinal 4, offset 8, weight / connectivity / gain 4, part 4).
+double counted_bytes(int fam, const unsigned long long* c) {
+    switch (fam) {
+        case PF_LP_SELECT:
+            // 8 per vertex (cached destination, lock) + 28 per re-evaluated
+            // vertex (offset 8, size 4, part 4, gain 4, destination 4, cache 4)
+            // + 8 per table slot read; c = {evaluated, slots}
+            return 28.0 * (double)c[0] + 8.0 * (double)c[1];
+        case PF_AFTERBURNER:
+            // 16 per candidate edge (column 4, weight 4, neighbour's
+            // destination 4 and pre-gain 4); c = {sum of candidate degrees}
+            return 16.0 * (double)c[0];
+        case PF_UPDATE:
+            // per pass, 24 per move edge (column 4, weight 4, row offset 8,
+            // table slot 8); c = {sum of move degrees over both passes}
+            return 24.0 * (double)c[0];
+        case PF_WEAK_SELECT:
+            // 12 per vertex (part, destination, bucket id) + 16 per vertex of
+            // an oversized part (weight 4, offset 8, size 4) + 8 per table
+            // slot read; c = {evaluated, slots}
+            return 16.0 * (double)c[0] + 8.0 * (double)c[1];
+        default:
+            return 0;
+    }
+}
+
+}  // namespace
+
+unsigned long long* prof_counters(int fam) {
+    Prof& p = P();
+    return fam >= 0 && fam < PF_FAMILIES && p.counting[fam] && p.dev ? p.dev + PROF_COUNTERS * fam : nullptr;
+}
+
+bool prof_counting(int fam) { return fam >= 0 && fam < PF_FAMILIES && P().counting[fam]; }
+
+void prof_begin(int fam, hipStream_t s) {
+    Prof& p = P();
+    if (p.timed != fam || p.nev >= PROF_MAX_UNITS) return;
+    JET_HIP(hipEventRecord(p.ev[2 * p.nev], s));
+}
+
+void prof_end(int fam, hipStream_t s) {
+    Prof& p = P();
+    if (p.timed != fam || p.nev >= PROF_MAX_UNITS) return;
+    JET_HIP(hipEventRecord(p.ev[2 * p.nev + 1], s));
+    p.nev++;
 }
 
 void prof_account(int fam, double host_bytes) {
-    if (!prof_on(fam)) return;
-    Prof& P = prof();
-    P.host_bytes += host_bytes;
-    P.launches++;
+    Prof& p = P();
+    if (fam < 0 || fam >= PF_FAMILIES || !p.counting[fam]) return;
+    p.host_bytes[fam] += host_bytes;
+    p.units[fam]++;
 }
 
+int prof_timed_family() { return P().timed; }
+
 bool prof_enable(const std::string& spec) {
+    Prof& p = P();
     std::string name = spec;
     bool counting = false;
     const std::string suffix = ":count";
@@ -36,73 +101,61 @@ bool prof_enable(const std::string& spec) {
         counting = true;
         name.resize(name.size() - suffix.size());
     }
-    if (name == "none") {
-        prof().family = PF_NONE;
+    int fam = -1;
+    if (name != "none" && !(counting && name == "all")) {
+        fam = family_of(name);
+        if (fam < 0) return false;
+    }
+    JET_HIP(hipDeviceSynchronize());
+    int dev = 0;
+    JET_HIP(hipGetDevice(&dev));
+    if (p.dev_id != dev) {  // counters and events of the current device
+        p.dev = nullptr;
+        p.ev.clear();
+        p.dev_id = dev;
+    }
+    p.timed = -1;
+    std::memset(p.counting, 0, sizeof(p.counting));
+    std::memset(p.host_bytes, 0, sizeof(p.host_bytes));
+    std::memset(p.units, 0, sizeof(p.units));
+    p.nev = 0;
+    if (name == "none") return true;
+    if (counting) {
+        if (!p.dev) JET_HIP(hipMalloc(&p.dev, sizeof(unsigned long long) * PROF_COUNTERS * PF_FAMILIES));
+        JET_HIP(hipMemset(p.dev, 0, sizeof(unsigned long long) * PROF_COUNTERS * PF_FAMILIES));
+        JET_HIP(hipDeviceSynchronize());
+        for (int f = 0; f < PF_FAMILIES; f++) p.counting[f] = fam < 0 || f == fam;
         return true;
     }
-    int fam = PF_NONE;
-    if (name == "lp_select") fam = PF_LP_SELECT;
-    else return false;
-    Prof& P = prof();
-    const size_t words = 2 * PROF_SPAN_SLOTS * (size_t)PROF_MAX_SPANS;
-    if (!P.dev) JET_HIP(hipMalloc(&P.dev, 8 * sizeof(unsigned long long)));
-    if (!P.span) JET_HIP(hipMalloc(&P.span, words * sizeof(unsigned long long)));
-    JET_HIP(hipMemset(P.dev, 0, 8 * sizeof(unsigned long long)));
-    // starts at all-ones (atomic min), ends at zero (atomic max)
-    std::vector<unsigned long long> init(words);
-    for (size_t i = 0; i < words; i++) init[i] = (i / PROF_SPAN_SLOTS) % 2 == 0 ? ~0ull : 0ull;
-    JET_HIP(hipMemcpy(P.span, init.data(), words * sizeof(unsigned long long), hipMemcpyHostToDevice));
-    JET_HIP(hipDeviceSynchronize());
-    P.host_bytes = 0;
-    P.launches = 0;
-    P.spans = 0;
-    P.counting = counting;
-    P.family = fam;
+    if (p.ev.empty()) {
+        p.ev.resize(2 * PROF_MAX_UNITS);
+        for (auto& e : p.ev) JET_HIP(hipEventCreate(&e));
+    }
+    p.timed = fam;
     return true;
 }
 
-bool prof_read(int64_t& launches, double& ms, double& bytes) {
-    Prof& P = prof();
-    if (P.family == PF_NONE) return false;
+bool prof_read(int fam, int64_t& units, double& ms, double& bytes) {
+    Prof& p = P();
     JET_HIP(hipDeviceSynchronize());
-    bytes = P.host_bytes;
-    launches = P.launches;
+    units = 0;
     ms = 0;
-    if (P.counting) {
-        unsigned long long c[8];
-        JET_HIP(hipMemcpy(c, P.dev, sizeof(c), hipMemcpyDeviceToHost));
-        if (P.family == PF_LP_SELECT) {
-            // per cache miss: conn_off 8 + conn_size 4 + part 4 + gain write 4 +
-            // cache write 4; per table slot 8; per candidate: gain read 4,
-            // pre-gain write 4, row offsets 16, list write 4
-            bytes += 24.0 * (double)c[0] + 8.0 * (double)c[1] + 28.0 * (double)c[2];
-        }
+    bytes = 0;
+    if (fam < 0 || fam >= PF_FAMILIES) return false;
+    if (p.counting[fam]) {
+        unsigned long long c[PROF_COUNTERS];
+        JET_HIP(hipMemcpy(c, p.dev + PROF_COUNTERS * fam, sizeof(c), hipMemcpyDeviceToHost));
+        units = p.units[fam];
+        bytes = p.host_bytes[fam] + counted_bytes(fam, c);
         return true;
     }
-    int dev = 0, khz = 0;
-    JET_HIP(hipGetDevice(&dev));
-    JET_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
-    std::vector<unsigned long long> sp(2 * PROF_SPAN_SLOTS * (size_t)P.spans);
-    if (P.spans > 0)
-        JET_HIP(hipMemcpy(sp.data(), P.span, sp.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-    double ticks = 0;
-    int64_t timed = 0;
-    for (int64_t i = 0; i < P.spans; i++) {
-        const unsigned long long* r = sp.data() + 2 * PROF_SPAN_SLOTS * i;
-        unsigned long long s0 = ~0ull, e0 = 0;
-        for (int q = 0; q < PROF_SPAN_SLOTS; q++) {
-            s0 = r[q] < s0 ? r[q] : s0;
-            e0 = r[PROF_SPAN_SLOTS + q] > e0 ? r[PROF_SPAN_SLOTS + q] : e0;
-        }
-        if (s0 != ~0ull && e0 >= s0) {
-            ticks += (double)(e0 - s0);
-            timed++;
-        }
+    if (p.timed != fam) return false;
+    for (int64_t i = 0; i < p.nev; i++) {
+        float t = 0;
+        JET_HIP(hipEventElapsedTime(&t, p.ev[2 * i], p.ev[2 * i + 1]));
+        ms += t;
     }
-    // launches beyond the span records are not timed: report the timed ones
-    if (timed < P.launches && P.launches > 0) bytes *= (double)timed / (double)P.launches;
-    launches = timed;
-    ms = khz > 0 ? ticks / (double)khz : 0.0;
+    units = p.nev;
     return true;
 }
 

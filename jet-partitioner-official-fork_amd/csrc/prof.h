@@ -1,12 +1,17 @@
-// Per-kernel timing for measurement tools. When a kernel family is enabled for
-// timing, each launch of it receives a span record: its workgroups stamp the
-// device's constant-rate wall clock at entry (atomic min) and after their last
-// store (atomic max), spread over PROF_SPAN_SLOTS words each so the atomics
-// stay cheap; the launch's span is the earliest start to the latest end, the
-// kernel's own execution time without host submission gaps. In counting mode
-// the family's launches instead accumulate the item counts from which its
-// algorithmic bytes follow; counting adds work to the kernel, so the two modes
-// are used in separate runs of the same deterministic computation.
+// Per-kernel-family timing and algorithmic-byte counting for measurement
+// tools.
+//
+// Timing mode (one family at a time): every timed unit of the family -- one
+// kernel launch, or for "update" one pass of update kernels -- is bracketed
+// by two HIP events recorded on the stream the kernels are launched on; the
+// family's time is the sum of the event intervals.
+//
+// Counting mode (any set of families, "all:count"): the units accumulate the
+// item counts their algorithmic bytes follow from, per family: sizes the host
+// knows (n, m) directly, data-dependent counts (cache misses and table slots
+// read, candidate and move degrees) through device counters that are null
+// outside counting runs. Counting adds work, so it runs in a separate pass of
+// the same deterministic computation.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -14,44 +19,30 @@
 
 namespace jet {
 
-enum ProfFamily : int { PF_NONE = 0, PF_LP_SELECT = 1 };
+enum ProfFamily : int { PF_LP_SELECT = 0, PF_AFTERBURNER, PF_UPDATE, PF_WEAK_SELECT, PF_FAMILIES };
 
-constexpr int PROF_SPAN_SLOTS = 64;
-constexpr int64_t PROF_MAX_SPANS = 16384;
+// Family names as accepted by prof_enable, indexed by ProfFamily.
+extern const char* const kProfNames[PF_FAMILIES];
 
-struct Prof {
-    int family = PF_NONE;
-    bool counting = false;
-    double host_bytes = 0;               // bytes known on the host (size-dependent terms)
-    int64_t launches = 0;                // launches accounted
-    int64_t spans = 0;                   // span records handed out (<= PROF_MAX_SPANS)
-    unsigned long long* dev = nullptr;   // device item counters (8)
-    unsigned long long* span = nullptr;  // per launch: starts[SLOTS], ends[SLOTS]
-};
+constexpr int64_t PROF_MAX_UNITS = 32768;  // timed units per enable
+constexpr int PROF_COUNTERS = 4;           // device counters per family
 
-Prof& prof();
-inline bool prof_on(int fam) { return prof().family == fam; }
-// Device counters when `fam` is being counted, else nullptr.
+// Device counters of `fam` when it is being counted, else nullptr.
 unsigned long long* prof_counters(int fam);
-// Span record for the next launch of `fam` when it is being timed, else nullptr.
-unsigned long long* prof_span(int fam);
-// Accounts one launch of `fam`.
+// Whether `fam` is being counted (host-side accounting is wanted).
+bool prof_counting(int fam);
+// Brackets one timed unit of `fam` on stream s (no-ops unless fam is timed).
+void prof_begin(int fam, hipStream_t s);
+void prof_end(int fam, hipStream_t s);
+// Accounts one unit of `fam` and the bytes the host knows for it (counting).
 void prof_account(int fam, double host_bytes);
-// Enables a family by name ("lp_select" times it, "lp_select:count" counts
-// its items, "none" stops); returns false for an unknown name.
+// "lp_select" (etc.) times that family; "<family>:count" or "all:count"
+// counts; "none" stops. Returns false for an unknown name.
 bool prof_enable(const std::string& name);
-// Synchronizes, then returns launches, total milliseconds (timing mode) and
-// algorithmic bytes (counting mode; the host-known part in timing mode).
-bool prof_read(int64_t& launches, double& ms, double& bytes);
-
-// Device side: call from every thread at kernel entry, and after the kernel's
-// last store behind a workgroup barrier.
-__device__ inline void prof_span_begin(unsigned long long* sp) {
-    if (sp && threadIdx.x == 0) atomicMin(&sp[blockIdx.x % PROF_SPAN_SLOTS], (unsigned long long)wall_clock64());
-}
-__device__ inline void prof_span_end(unsigned long long* sp) {
-    if (sp && threadIdx.x == 0)
-        atomicMax(&sp[PROF_SPAN_SLOTS + blockIdx.x % PROF_SPAN_SLOTS], (unsigned long long)wall_clock64());
-}
+// Synchronizes, then returns the timed family's units and total milliseconds
+// (timing mode) or, for `fam`, its units and algorithmic bytes (counting).
+bool prof_read(int fam, int64_t& units, double& ms, double& bytes);
+// The family enabled for timing, or -1.
+int prof_timed_family();
 
 }  // namespace jet

@@ -144,6 +144,24 @@ __global__ void __launch_bounds__(256) k_conn_tiers(DGraph g, int32_t k, vid_t* 
     }
 }
 
+// Counting runs of the profiler only: adds the degrees of the items of a
+// list (counters[cs] from the front, counters[cb] from the back of its nmax
+// slots) to out[0]; returns at once when *deferred is set.
+__global__ void __launch_bounds__(256)
+    k_prof_degsum(DGraph g, const vid_t* list, const int32_t* counters, int cs, int cb, vid_t nmax,
+                  const int32_t* deferred, unsigned long long* out) {
+    __shared__ int64_t red[256 / WAVE];
+    if (deferred && *deferred) return;
+    const int64_t ns = counters[cs], nb = counters[cb];
+    int64_t sum = 0;
+    FOR_THREAD(x, ns + nb) {
+        const vid_t v = x < ns ? list[x] : list[nmax - 1 - (x - ns)];
+        sum += g.rowptr[v + 1] - g.rowptr[v];
+    }
+    sum = block_reduce_sum<int64_t, 256>(sum, red);
+    if (threadIdx.x == 0 && sum) atomicAdd(out, (unsigned long long)sum);
+}
+
 constexpr int SB_BLOCK = 256;
 
 __global__ void __launch_bounds__(SB_BLOCK)
@@ -353,9 +371,7 @@ template <int CHW, int W>
 __global__ void __launch_bounds__(W* WAVE)
     k_lp_select(DGraph g, const part_t* part, const eoff_t* conn_off, const int2* conn, const int32_t* conn_size,
                 part_t* dest_cache, int32_t* gain_p, int32_t* lock, double fr, part_t* dest, int32_t* pregain,
-                vid_t* cand, int32_t* counters, vid_t nmax, int32_t* next_counters, unsigned long long* pc,
-                unsigned long long* span) {
-    prof_span_begin(span);
+                vid_t* cand, int32_t* counters, vid_t nmax, int32_t* next_counters, unsigned long long* pc) {
     constexpr int NPASS = CHW / WAVE;
     __shared__ vid_t sbuf[W * CHW];
     __shared__ int32_t sh[2 * W + 2];
@@ -365,7 +381,7 @@ __global__ void __launch_bounds__(W* WAVE)
     __shared__ int16_t mslot[W][CHW];
     __shared__ part_t mbest[W][CHW];
     __shared__ int32_t mgain[W][CHW];
-    int64_t n_miss = 0, n_slot = 0, n_cand = 0;
+    int64_t n_miss = 0, n_slot = 0;
     StagedLists<W, CHW> st(sbuf);
     if (blockIdx.x == 0 && threadIdx.x < 8) next_counters[threadIdx.x] = 0;
     const int lane = threadIdx.x & (WAVE - 1);
@@ -451,23 +467,16 @@ __global__ void __launch_bounds__(W* WAVE)
     for (int u = 0; u < NPASS; u++) {
         const int64_t i = wbase + u * WAVE + lane;
         const eoff_t deg = r1[u] - r0[u];
-        if (is_cand[u]) {
-            n_cand++;
-            pregain[i] = gain[u];
-        }
+        if (is_cand[u]) pregain[i] = gain[u];
         st.push(is_cand[u] && deg <= BIG_DEG, is_cand[u] && deg > BIG_DEG, (vid_t)i);
     }
     st.flush(cand, &counters[0], &counters[1], nmax, sh);
-    if (pc) {
-        int64_t v3[3] = {n_miss, n_slot, n_cand};
-        for (int q = 0; q < 3; q++) {
-            int64_t v = block_reduce_sum<int64_t, W * WAVE>(v3[q], pred);
+    if (pc) {  // counting runs only: evaluated vertices and table slots read
+        int64_t v2[2] = {n_miss, n_slot};
+        for (int q = 0; q < 2; q++) {
+            int64_t v = block_reduce_sum<int64_t, W * WAVE>(v2[q], pred);
             if (threadIdx.x == 0 && v) atomicAdd(&pc[q], (unsigned long long)v);
         }
-    }
-    if (span) {
-        __syncthreads();
-        prof_span_end(span);
     }
 }
 
@@ -1000,7 +1009,8 @@ __global__ void __launch_bounds__(SEL_BLOCK_MAX)
     k_weak_select(DGraph g, const part_t* part, const int64_t* sizes, int32_t k, int64_t size_max, int32_t opt,
                   int64_t max_dest, const part_t* undersized, const int32_t* misc, const eoff_t* conn_off,
                   const int2* conn, const int32_t* conn_size, int32_t sections, part_t* dest, int32_t* bid,
-                  unsigned long long* buckets_base, unsigned long long* extra, int32_t copies, int64_t stride) {
+                  unsigned long long* buckets_base, unsigned long long* extra, int32_t copies, int64_t stride,
+                  unsigned long long* pc) {
     unsigned long long* buckets = bucket_copy(buckets_base, extra, copies, stride);
     __shared__ int32_t bk[BH_SLOTS];
     __shared__ unsigned long long bv[BH_SLOTS];
@@ -1010,6 +1020,7 @@ __global__ void __launch_bounds__(SEL_BLOCK_MAX)
     BucketMap bm{bk, bv};
     bm.init();
     const int32_t tu = misc[0];
+    int64_t n_eval = 0, n_slot = 0;
     FOR_BLOCK_RANGE(i, g.n) {
         const part_t p = part[i];
         part_t d = p;
@@ -1022,6 +1033,8 @@ __global__ void __launch_bounds__(SEL_BLOCK_MAX)
         const int32_t sz = over ? conn_size[i] : 0;
         const part_t spare = over && tu > 0 ? undersized[(uint32_t)i % (uint32_t)tu] : p;
         if (over && (double)w < 1.5 * (double)(sp - opt)) {
+            n_eval++;
+            n_slot += sz;
             part_t best = p;
             int32_t gain = 0, p_gain = 0;
             for_slots(conn + off, sz, [&](int2 e) {
@@ -1053,6 +1066,15 @@ __global__ void __launch_bounds__(SEL_BLOCK_MAX)
         bid[i] = b;
     }
     bm.flush(buckets);
+    if (pc) {  // counting runs only: evaluated vertices and table slots read
+        __shared__ int64_t red[SEL_BLOCK_MAX / WAVE];
+        int64_t v2[2] = {n_eval, n_slot};
+        for (int q = 0; q < 2; q++) {
+            const int64_t v = blockDim.x == SEL_BLOCK_MAX ? block_reduce_sum<int64_t, SEL_BLOCK_MAX>(v2[q], red)
+                                                          : block_reduce_sum<int64_t, 128>(v2[q], red);
+            if (threadIdx.x == 0 && v) atomicAdd(&pc[q], (unsigned long long)v);
+        }
+    }
 }
 
 // jet_refiner.hpp:449-481.
@@ -1757,20 +1779,28 @@ void Refiner::jet_lp(const Level& L, const part_t* part, double fr) {
     gen_++;  // this step's moves are stamped with a new generation
     // a pass per wave on levels that fill the GPU with single-pass waves, two
     // passes per wave in 16-wave workgroups above that (see LS_MULTI_CHW)
+    prof_begin(PF_LP_SELECT, s_);
     if (g.n <= tuning().lp_one_pass_n) {
         int nb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_WAVES * WAVE));
         k_lp_select<WAVE, LS_WAVES><<<nb, LS_WAVES * WAVE, 0, s_>>>(
             g, part, conn_off_, conn_, conn_size_, dest_cache_, gain_p_, lock_, fr, dest_, pregain_, cand_, c, nmax_,
-            cnt_next(), prof_counters(PF_LP_SELECT), prof_span(PF_LP_SELECT));
+            cnt_next(), prof_counters(PF_LP_SELECT));
     } else {
         int nb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_MULTI_WAVES * LS_MULTI_CHW));
         k_lp_select<LS_MULTI_CHW, LS_MULTI_WAVES><<<nb, LS_MULTI_WAVES * WAVE, 0, s_>>>(
             g, part, conn_off_, conn_, conn_size_, dest_cache_, gain_p_, lock_, fr, dest_, pregain_, cand_, c, nmax_,
-            cnt_next(), prof_counters(PF_LP_SELECT), prof_span(PF_LP_SELECT));
+            cnt_next(), prof_counters(PF_LP_SELECT));
     }
+    prof_end(PF_LP_SELECT, s_);
     JET_LAUNCH_CHECK();
-    // per vertex: cache read 4, destination write 4, lock read 4
-    prof_account(PF_LP_SELECT, 12.0 * g.n);
+    // per vertex: cached destination 4, lock 4
+    prof_account(PF_LP_SELECT, 8.0 * g.n);
+    if (unsigned long long* pc = prof_counters(PF_AFTERBURNER)) {
+        // counting runs only: candidate degrees (the list is complete here)
+        k_prof_degsum<<<grid_for(g.n, 256, 1024), 256, 0, s_>>>(g, cand_, c, 0, 1, nmax_, nullptr, pc);
+        prof_account(PF_AFTERBURNER, 0);
+    }
+    prof_begin(PF_AFTERBURNER, s_);
     unsigned ab = (unsigned)std::max<int64_t>(256, ceil_div((int64_t)g.n, (int64_t)AB_WAVES * AB_CHW));
 #define JET_AB(GG)                                                                                          \
     k_afterburner_small<GG><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, \
@@ -1787,6 +1817,7 @@ void Refiner::jet_lp(const Level& L, const part_t* part, double fr) {
         k_afterburner_big<<<grid_for((int64_t)g.n * WAVE, 256, 2048), 256, 0, s_>>>(g, cand_, c, part, dest_, pregain_,
                                                                                    lock_, moves_, c, nmax_, stamp_,
                                                                                    gen_);
+    prof_end(PF_AFTERBURNER, s_);
     JET_LAUNCH_CHECK();
 }
 
@@ -1914,14 +1945,18 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
             k_strong_select<false><<<nsel, selb, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, acc_, conn_off_,
                                                          conn_, conn_size_, sections, bid_, bk, bc, copies, nbuckets_);
     } else {
+        prof_begin(PF_WEAK_SELECT, s_);
         if (lds)
             k_weak_select<true><<<nsel, selb, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, undersized_, misc_,
                                                       conn_off_, conn_, conn_size_, sections, dest_, bid_, bk, bc,
-                                                      copies, nbuckets_);
+                                                      copies, nbuckets_, prof_counters(PF_WEAK_SELECT));
         else
             k_weak_select<false><<<nsel, selb, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, undersized_, misc_,
                                                        conn_off_, conn_, conn_size_, sections, dest_, bid_, bk, bc,
-                                                       copies, nbuckets_);
+                                                       copies, nbuckets_, prof_counters(PF_WEAK_SELECT));
+        prof_end(PF_WEAK_SELECT, s_);
+        // per vertex: part 4, destination 4, bucket id 4
+        prof_account(PF_WEAK_SELECT, 12.0 * g.n);
     }
     if (copies > 1)
         k_sum_bucket_copies<<<grid_for(nbk + 1, 256, 2048), 256, 0, s_>>>(buckets_, bcopies_, copies, nbk + 1,
@@ -1981,6 +2016,13 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
     const Publish pb{pub_, sizes, k, res_dev_, c, seq};
     int nbb = grid_for((int64_t)g.n * WAVE, 256, 512);
     for (int add = 0; add < 2; add++) {
+        if (unsigned long long* pc = prof_counters(PF_UPDATE)) {
+            // counting runs only: move degrees, once per pass (nothing when
+            // the step's ranking was deferred: its kernels then do nothing)
+            k_prof_degsum<<<grid_for(g.n, 256, 1024), 256, 0, s_>>>(g, moves_, c, 2, 3, nmax_, misc_ + 2, pc);
+            prof_account(PF_UPDATE, 0);
+        }
+        prof_begin(PF_UPDATE, s_);
 #define JET_UPD(GG)                                                                                                   \
     if (add) k_update_small<true, GG><<<nbs + 1, 256, 0, s_>>>(g, moves_, mrec_, c, nmax_, part, dest_, conn_off_,    \
                                                               conn_, conn_size_, dest_cache_, k, misc_ + 1,           \
@@ -2027,6 +2069,7 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
                                                                    hub_dirty_, part, dest_, conn_off_, conn_,
                                                                    conn_size_, dest_cache_, k, misc_ + 1, misc_ + 2);
         }
+        prof_end(PF_UPDATE, s_);
         JET_LAUNCH_CHECK();
     }
     volatile int64_t* res = res_host_;

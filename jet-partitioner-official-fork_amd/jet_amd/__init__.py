@@ -137,7 +137,7 @@ def lib():
             "jet_tuning_get": [c.c_char_p, c.POINTER(i64)],
             "jet_tuning_load": [c.c_char_p],
             "jet_prof_enable": [c.c_char_p],
-            "jet_prof_read": [c.POINTER(i64), c.POINTER(dbl), c.POINTER(dbl)],
+            "jet_prof_read": [c.c_char_p, c.POINTER(i64), c.POINTER(dbl), c.POINTER(dbl)],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -378,6 +378,23 @@ def tuning_get(name):
     v = ctypes.c_int64(0)
     _check(lib().jet_tuning_get(name.encode(), ctypes.byref(v)))
     return v.value
+
+
+# kernel families of the profiler (include/jet_capi.h)
+PROF_FAMILIES = ("lp_select", "afterburner", "update", "weak_select")
+
+
+def prof_enable(spec):
+    """Times ("<family>") or counts ("<family>:count", "all:count") a kernel
+    family from now on; "none" stops (jet_prof_enable)."""
+    _check(lib().jet_prof_enable(spec.encode()))
+
+
+def prof_read(family):
+    """(units, total milliseconds, algorithmic bytes) of an enabled family."""
+    u, ms, b = ctypes.c_int64(0), ctypes.c_double(0), ctypes.c_double(0)
+    _check(lib().jet_prof_read(family.encode(), ctypes.byref(u), ctypes.byref(ms), ctypes.byref(b)))
+    return u.value, ms.value, b.value
 
 
 # the result-neutral tuning parameters (include/jet_capi.h)

@@ -13,6 +13,7 @@ Needs a GPU; the workloads are generated on the host and copied to HBM once.
 import argparse
 import hashlib
 import json
+import os
 import sys
 import time
 
@@ -81,28 +82,51 @@ class Workload:
         return float(np.median(ts))
 
 
-def evaluate(wls, reps, expect):
-    """Weighted sum of median partition times; checks every result against
-    `expect` (name -> (cut, digest))."""
-    total = 0.0
+def check(wls, expect):
+    """Runs every workload once and checks its partition against `expect`
+    (name -> (cut, digest))."""
     for w in wls:
-        cut = w.run()  # warm-up, and the result check
-        got = (cut, w.digest())
-        if expect is not None and got != expect[w.name]:
+        got = (w.run(), w.digest())
+        if got != expect[w.name]:
             raise RuntimeError(f"{w.name}: a tuning setting changed the partition ({got[0]} vs "
                                f"{expect[w.name][0]}): {jet_amd.tuning_all()}")
-        total += w.weight * w.time(reps)
-    return total
+
+
+def compare(wls, name, cands, rounds, expect):
+    """Weighted partition time of each candidate value of `name`: the
+    candidates are timed in interleaved rounds (one partition of every
+    workload per candidate per round), so that drift of the device's clocks
+    over the search falls on all of them alike; each workload's median over
+    the rounds is weighted and summed."""
+    import torch
+
+    for v in cands:
+        jet_amd.tuning_set(name, v)
+        check(wls, expect)
+    samples = {v: {w.name: [] for w in wls} for v in cands}
+    for _ in range(rounds):
+        for v in cands:
+            jet_amd.tuning_set(name, v)
+            for w in wls:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                w.run()
+                torch.cuda.synchronize()
+                samples[v][w.name].append(time.perf_counter() - t0)
+    return {v: sum(w.weight * float(np.median(samples[v][w.name])) for w in wls) for v in cands}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True, help="tuning table to write")
-    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--reps", type=int, default=7, help="interleaved timing rounds per comparison")
     ap.add_argument("--sweeps", type=int, default=2)
-    ap.add_argument("--min-gain", type=float, default=0.01, help="relative improvement needed to change a value")
+    ap.add_argument("--min-gain", type=float, default=0.02, help="relative improvement needed to change a value")
     ap.add_argument("--report", help="JSON file for the search record")
     args = ap.parse_args()
+    # the search starts from the built-in defaults: no installed table is
+    # applied when the first context is created
+    os.environ["JET_TUNING_FILE"] = os.devnull
     import torch
 
     if not torch.cuda.is_available():
@@ -118,32 +142,26 @@ def main():
     best = dict(defaults)
     for n, v in best.items():
         jet_amd.tuning_set(n, v)
-    base = evaluate(wls, args.reps, expect)
+    base = compare(wls, "wg_max_n", [best["wg_max_n"]], args.reps, expect)[best["wg_max_n"]]
     say(f"# {arch}: defaults {best}: {1e3 * base:.2f} ms")
     record = {"arch": arch, "defaults": defaults, "default_ms": round(1e3 * base, 3), "steps": []}
-    cur = base
     for sweep in range(args.sweeps):
         changed = False
         for name, cands in SPACE.items():
-            results = {}
+            cands = sorted(set(cands) | {best[name]})
+            results = compare(wls, name, cands, args.reps, expect)
             for v in cands:
-                jet_amd.tuning_set(name, v)
-                results[v] = evaluate(wls, args.reps, expect)
                 record["steps"].append({"sweep": sweep, "name": name, "value": v, "ms": round(1e3 * results[v], 3)})
                 say(f"#   {name}={v}: {1e3 * results[v]:.2f} ms")
-            # re-time the incumbent beside the candidates
-            jet_amd.tuning_set(name, best[name])
-            results[best[name]] = min(results.get(best[name], float("inf")), evaluate(wls, args.reps, expect))
             v_best = min(results, key=results.get)
             if v_best != best[name] and results[v_best] < (1 - args.min_gain) * results[best[name]]:
                 best[name] = v_best
                 changed = True
             jet_amd.tuning_set(name, best[name])
-            cur = results[best[name]]
-            say(f"# {name} -> {best[name]} ({1e3 * cur:.2f} ms)")
+            say(f"# {name} -> {best[name]}")
         if not changed:
             break
-    final = evaluate(wls, args.reps, expect)
+    final = compare(wls, "wg_max_n", [best["wg_max_n"]], args.reps, expect)[best["wg_max_n"]]
     record.update({"tuned": best, "tuned_ms": round(1e3 * final, 3)})
     with open(args.out, "w") as f:
         f.write(f"# tuning table for {arch}: written by jet_amd.autotune\n")
