@@ -223,6 +223,73 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// The rounds of map construction from a short work list on, in one
+// workgroup: the same mark / claim / resolve / next phases as the kernels
+// above, separated by workgroup barriers instead of kernel boundaries, until
+// the list is empty. The list's order does not matter to any phase.
+constexpr int MC_TAIL_BLOCK = 1024;
+constexpr int32_t MC_TAIL_MAX = 8192;  // work lists of at most this many go to the tail kernel
+__global__ void __launch_bounds__(MC_TAIL_BLOCK)
+    k_mc_tail(const vid_t* wl0, int32_t len0, int32_t round0, int swap0, vid_t n, const vid_t* hn, int32_t* stamp,
+              int32_t* claim, vid_t* vcmap, vid_t* wl_a, vid_t* wl_b, int32_t* err) {
+    __shared__ int32_t cnt;
+    const vid_t* wl = wl0;
+    int32_t len = len0;
+    int swap = swap0;
+    for (int32_t round = round0; len > 0; round++) {
+        if (round > 2 * n + 8) {  // each round resolves at least one vertex
+            if (threadIdx.x == 0) atomicOr(err, 1);
+            return;
+        }
+        for (int32_t x = threadIdx.x; x < len; x += MC_TAIL_BLOCK) {
+            const vid_t u = wl[x];
+            stamp[u] = round;
+            claim[u] = ORD_MAX;
+        }
+        if (threadIdx.x == 0) cnt = 0;
+        __syncthreads();
+        if (swap == 0) {
+            for (int32_t x = threadIdx.x; x < len; x += MC_TAIL_BLOCK) {
+                const vid_t u = wl[x];
+                const vid_t v = hn[u];
+                if (u < v && stamp[v] == round && !(v < hn[v])) atomicMin(&claim[v], u);
+            }
+            __syncthreads();
+        }
+        for (int32_t x = threadIdx.x; x < len; x += MC_TAIL_BLOCK) {
+            const vid_t u = wl[x];
+            const vid_t v = hn[u];
+            if (!mc_acting(u, v, swap)) continue;
+            if (v == u) {
+                vcmap[u] = u;
+            } else if (stamp[v] == round) {
+                if (mc_acting(v, hn[v], swap)) {
+                    vcmap[u] = ORD_MAX;
+                } else {
+                    const vid_t cv = swap ? v : claim[v];
+                    vcmap[u] = cv;
+                    vcmap[v] = cv;
+                }
+            } else {
+                vcmap[u] = vcmap[v];
+            }
+        }
+        __syncthreads();
+        vid_t* next = (wl == wl_a) ? wl_b : wl_a;
+        for (int32_t x0 = 0; x0 < len; x0 += MC_TAIL_BLOCK) {
+            const int32_t x = x0 + (int32_t)threadIdx.x;
+            const vid_t u = x < len ? wl[x] : 0;
+            const int32_t at = wave_append(&cnt, x < len && vcmap[u] >= n);
+            if (at >= 0) next[at] = u;
+        }
+        __syncthreads();
+        len = cnt;
+        wl = next;
+        swap ^= 1;
+        __syncthreads();
+    }
+}
+
 __global__ void k_root_flags(const vid_t* vcmap, vid_t n, int32_t* flags) {
     FOR_THREAD(u, n) flags[u] = vcmap[u] == (vid_t)u ? 1 : 0;
 }
@@ -261,6 +328,17 @@ int32_t map_construct(vid_t n, const vid_t* hn, vid_t* map, Arena& a, hipStream_
     int swap = 1;
     for (int32_t round = 0; len > 0; round++) {
         if (round > 2 * n + 8) throw DeviceError("map construction made no progress");
+        if (wl && len <= MC_TAIL_MAX) {
+            // the remaining rounds in one workgroup, with no host read per round
+            int32_t* err = a.alloc<int32_t>(1);
+            fill_i32(err, 0, 1, s);
+            vid_t* other = (wl == wl_a) ? wl_b : wl_a;
+            k_mc_tail<<<1, MC_TAIL_BLOCK, 0, s>>>(wl, len, round, swap, n, hn, stamp, claim, vcmap,
+                                                  const_cast<vid_t*>(wl), other, err);
+            JET_LAUNCH_CHECK();
+            if (read_scalar(err, s)) throw DeviceError("map construction made no progress");
+            break;
+        }
         int blocks = grid_for(len, 256, 8192);
         k_mc_mark<<<blocks, 256, 0, s>>>(wl, len, round, stamp, claim);
         if (swap == 0) k_mc_claim<<<blocks, 256, 0, s>>>(wl, len, round, hn, stamp, claim);

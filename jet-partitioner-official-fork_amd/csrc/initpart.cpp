@@ -29,6 +29,8 @@
 #include <sched.h>
 #include <sys/mman.h>
 
+#include "initpart.h"
+
 namespace {
 
 // Blocks of 2 MiB or more for BigVec: mapped with transparent huge pages
@@ -497,6 +499,10 @@ static std::vector<int8_t> bisect_flat(const Sub& s, double frac, double ub, uin
 // about ML_COARSEST vertices or when a level shrinks by less than 10%.
 constexpr int32_t ML_MIN_N = 2000;
 constexpr int32_t ML_COARSEST = 1000;
+// graphs with at least this many adjacency entries are matched by
+// handshakes on host threads instead of the serial greedy pass
+constexpr int64_t PAR_MATCH_EDGES = int64_t(1) << 21;
+constexpr int HANDSHAKE_ROUNDS = 6;
 
 static void shuffle_ids(std::vector<int32_t>& p, uint64_t seed) {
     for (int32_t i = (int32_t)p.size() - 1; i > 0; i--) {
@@ -505,31 +511,89 @@ static void shuffle_ids(std::vector<int32_t>& p, uint64_t seed) {
     }
 }
 
-// One coarsening level: heavy-edge matching in a seeded order under a vertex
-// weight cap, then pairs of still-unmatched vertices that share their first
-// neighbour (leaves of one hub), then pairs of unmatched isolated vertices.
+// One coarsening level: heavy-edge matching under a vertex weight cap (in a
+// seeded order, or by handshakes on large graphs), then pairs of
+// still-unmatched vertices that share their first neighbour (leaves of one
+// hub), then pairs of unmatched isolated vertices.
 // cmap receives the coarse id of every vertex (ids in order of first member).
+// Heavy-edge matching by handshakes, for large graphs, on up to par threads
+// over contiguous vertex ranges: in each round every unmatched vertex
+// proposes to its unmatched neighbour of heaviest edge (ties: larger seeded
+// key) under the weight cap, and mutual proposals are matched. A vertex's
+// proposal reads only the state at the start of the round, so the result does
+// not depend on the thread count.
+static void handshake_matching(const Sub& s, uint64_t seed, int64_t cap, int par, std::vector<int32_t>& mate) {
+    const int32_t n = s.n;
+    std::vector<int32_t> prop(n, -1);
+    std::vector<uint32_t> key(n);
+    const int T = pass_threads(s, par);
+    const std::vector<int32_t> bounds = vertex_chunks(s, T);
+    run_threads(T, [&](int t) {
+        for (int32_t v = bounds[t]; v < bounds[t + 1]; v++) key[v] = (uint32_t)mix(seed + (uint64_t)v);
+    });
+    for (int round = 0; round < HANDSHAKE_ROUNDS; round++) {
+        std::vector<int64_t> proposed(T, 0);
+        run_threads(T, [&](int t) {
+            int64_t c = 0;
+            for (int32_t v = bounds[t]; v < bounds[t + 1]; v++) {
+                prop[v] = -1;
+                if (mate[v] >= 0) continue;
+                int32_t best = -1;
+                int64_t bw = -1;
+                uint32_t bk = 0;
+                for (int64_t j = s.xadj[v]; j < s.xadj[v + 1]; j++) {
+                    const int32_t u = s.adj[j];
+                    if (u == v || mate[u] >= 0 || (int64_t)s.vw[v] + s.vw[u] > cap) continue;
+                    const int64_t w = s.adjw[j];
+                    if (w > bw || (w == bw && key[u] > bk)) {
+                        bw = w;
+                        bk = key[u];
+                        best = u;
+                    }
+                }
+                prop[v] = best;
+                c += best >= 0;
+            }
+            proposed[t] = c;
+        });
+        int64_t any = 0;
+        for (int64_t c : proposed) any += c;
+        if (any == 0) break;
+        run_threads(T, [&](int t) {  // each vertex writes only its own mate
+            for (int32_t v = bounds[t]; v < bounds[t + 1]; v++) {
+                const int32_t u = prop[v];
+                if (u >= 0 && prop[u] == v) mate[v] = u;
+            }
+        });
+    }
+}
+
 static Sub coarsen_once(const Sub& s, uint64_t seed, int par, std::vector<int32_t>& cmap) {
     const int32_t n = s.n;
     const int64_t cap = std::max<int64_t>(1, (int64_t)(1.5 * (double)s.total_w / ML_COARSEST));
-    std::vector<int32_t> mate(n, -1), order(n);
-    for (int32_t i = 0; i < n; i++) order[i] = i;
-    shuffle_ids(order, seed);
-    for (int32_t v : order) {
-        if (mate[v] >= 0) continue;
-        int32_t best = -1;
-        int64_t bw = -1;
-        for (int64_t j = s.xadj[v]; j < s.xadj[v + 1]; j++) {
-            int32_t u = s.adj[j];
-            if (u == v || mate[u] >= 0 || (int64_t)s.vw[v] + s.vw[u] > cap) continue;
-            if ((int64_t)s.adjw[j] > bw) {
-                bw = s.adjw[j];
-                best = u;
+    std::vector<int32_t> mate(n, -1);
+    if ((int64_t)s.adj.size() >= PAR_MATCH_EDGES) {
+        handshake_matching(s, seed, cap, par, mate);
+    } else {
+        std::vector<int32_t> order(n);
+        for (int32_t i = 0; i < n; i++) order[i] = i;
+        shuffle_ids(order, seed);
+        for (int32_t v : order) {
+            if (mate[v] >= 0) continue;
+            int32_t best = -1;
+            int64_t bw = -1;
+            for (int64_t j = s.xadj[v]; j < s.xadj[v + 1]; j++) {
+                int32_t u = s.adj[j];
+                if (u == v || mate[u] >= 0 || (int64_t)s.vw[v] + s.vw[u] > cap) continue;
+                if ((int64_t)s.adjw[j] > bw) {
+                    bw = s.adjw[j];
+                    best = u;
+                }
             }
-        }
-        if (best >= 0) {
-            mate[v] = best;
-            mate[best] = v;
+            if (best >= 0) {
+                mate[v] = best;
+                mate[best] = v;
+            }
         }
     }
     // leaves sharing a neighbour, and isolated vertices, in increasing id
@@ -777,6 +841,49 @@ static int host_threads() {
 }
 
 }  // namespace
+
+namespace jet_init {
+
+std::vector<HostLevel> init_hierarchy(int32_t n, const int64_t* xadj, const int32_t* adj, const int32_t* adjw,
+                                      const int32_t* vw, int32_t k, uint64_t seed) {
+    std::vector<HostLevel> out;
+    if (n <= 0) return out;
+    Sub cur;
+    cur.n = n;
+    cur.xadj.assign(xadj, xadj + n + 1);
+    const int64_t m = xadj[n];
+    cur.adj.assign(adj, adj + m);
+    if (adjw) cur.adjw.assign(adjw, adjw + m);
+    else cur.adjw.assign(m, 1);
+    if (vw) cur.vw.assign(vw, vw + n);
+    else cur.vw.assign(n, 1);
+    cur.gid.resize(n);
+    for (int32_t v = 0; v < n; v++) {
+        cur.gid[v] = v;
+        cur.total_w += cur.vw[v];
+    }
+    const int32_t target = std::max<int32_t>(ML_COARSEST, 8 * k);
+    const int par = host_threads();
+    for (int lvl = 0; cur.n > target && lvl < 64; lvl++) {
+        std::vector<int32_t> cmap;
+        Sub c = coarsen_once(cur, seed + 0x1A17ULL * (uint64_t)(lvl + 1), par, cmap);
+        if ((double)c.n > 0.9 * (double)cur.n) break;
+        HostLevel L;
+        L.n = c.n;
+        L.xadj.assign(c.xadj.begin(), c.xadj.end());
+        L.adj.assign(c.adj.begin(), c.adj.end());
+        L.adjw.assign(c.adjw.begin(), c.adjw.end());
+        L.vw.assign(c.vw.begin(), c.vw.end());
+        L.map = std::move(cmap);
+        out.push_back(std::move(L));
+        cur = std::move(c);
+    }
+    cur = Sub();
+    block_pool().trim();
+    return out;
+}
+
+}  // namespace jet_init
 
 // Returns 0 on success, -1 on invalid arguments.
 extern "C" int jet_host_initpart(int32_t n, const int64_t* xadj, const int32_t* adj, const int32_t* adjw,

@@ -29,8 +29,7 @@
 #include "refine.h"
 #include "tuning.h"
 
-extern "C" int jet_host_initpart(int32_t n, const int64_t* xadj, const int32_t* adj, const int32_t* adjw,
-                                 const int32_t* vw, int32_t k, double imb, uint64_t seed, int32_t* part);
+#include "initpart.h"
 
 namespace jet {
 
@@ -402,6 +401,75 @@ static void uncoarsen_levels(Context& C, const jet_config_t& cfg, const std::vec
     R.t_refine = secs(t_start, clk::now());
 }
 
+// The initial partition of a large coarsest graph `cs`, multilevel k-way
+// (initpart.h): the host coarsens it further, partitions the smallest level
+// by recursive bisection, and each level of that hierarchy is refined on the
+// device with Jet refinement (its own refine_data) before its partition is
+// projected to the next finer one; the partition of `cs` itself is left to
+// the uncoarsening that follows. Levels live in `persist`, the refinement
+// state in `scratch`.
+static void initial_partition_multilevel(Context& C, const jet_config_t& cfg, const Level& cs,
+                                         const std::vector<int64_t>& h_rp, const std::vector<int32_t>& h_col,
+                                         const std::vector<int32_t>& h_ew, const std::vector<int32_t>& h_vw,
+                                         uint64_t seed, part_t* cs_part) {
+    hipStream_t s = C.stream;
+    const int32_t k = cfg.num_parts;
+    std::vector<jet_init::HostLevel> H =
+        jet_init::init_hierarchy(cs.n, h_rp.data(), h_col.data(), h_ew.data(), h_vw.data(), k, seed);
+    std::vector<int32_t> hp;
+    if (H.empty()) {  // no coarser level: the recursive bisection alone
+        hp.resize(cs.n);
+        if (jet_host_initpart(cs.n, h_rp.data(), h_col.data(), h_ew.data(), h_vw.data(), k, cfg.max_imb_ratio, seed,
+                              hp.data()) != 0)
+            throw InitFailure("initial partitioner could not partition the coarsest graph");
+        JET_HIP(hipMemcpyAsync(cs_part, hp.data(), sizeof(int32_t) * cs.n, hipMemcpyHostToDevice, s));
+        return;
+    }
+    const jet_init::HostLevel& last = H.back();
+    hp.resize(last.n);
+    if (jet_host_initpart(last.n, last.xadj.data(), last.adj.data(), last.adjw.data(), last.vw.data(), k,
+                          cfg.max_imb_ratio, seed, hp.data()) != 0)
+        throw InitFailure("initial partitioner could not partition the coarsest graph");
+    // the hierarchy's levels and maps on the device
+    std::vector<Level> D(H.size());
+    for (size_t i = 0; i < H.size(); i++) {
+        const jet_init::HostLevel& h = H[i];
+        Level& d = D[i];
+        d.n = h.n;
+        d.m = h.xadj[h.n];
+        d.rowptr = C.persist.alloc<eoff_t>((int64_t)h.n + 1);
+        d.col = C.persist.alloc<vid_t>(std::max<int64_t>(d.m, 1));
+        d.ew = C.persist.alloc<wgt_t>(std::max<int64_t>(d.m, 1));
+        d.vw = C.persist.alloc<wgt_t>(std::max<int64_t>(h.n, 1));
+        const int64_t nf = i == 0 ? cs.n : H[i - 1].n;
+        d.map = C.persist.alloc<vid_t>(std::max<int64_t>(nf, 1));
+        d.uniform = false;
+        JET_HIP(hipMemcpyAsync(d.rowptr, h.xadj.data(), sizeof(int64_t) * (h.n + 1), hipMemcpyHostToDevice, s));
+        if (d.m > 0) {
+            JET_HIP(hipMemcpyAsync(d.col, h.adj.data(), sizeof(int32_t) * d.m, hipMemcpyHostToDevice, s));
+            JET_HIP(hipMemcpyAsync(d.ew, h.adjw.data(), sizeof(int32_t) * d.m, hipMemcpyHostToDevice, s));
+        }
+        JET_HIP(hipMemcpyAsync(d.vw, h.vw.data(), sizeof(int32_t) * h.n, hipMemcpyHostToDevice, s));
+        JET_HIP(hipMemcpyAsync(d.map, h.map.data(), sizeof(int32_t) * nf, hipMemcpyHostToDevice, s));
+    }
+    RefineParams rp;
+    rp.k = k;
+    rp.imb = cfg.max_imb_ratio;
+    rp.tol = cfg.refine_tolerance;
+    rp.ultra = cfg.ultra_settings != 0;
+    ArenaScope rscope(C.scratch);
+    int64_t cap = conn_capacity(D[0].n, D[0].rowptr, k, C.scratch, s);
+    Refiner ref(C.scratch, s, D[0].n, cap, rp);
+    part_t* cur = C.persist.alloc<part_t>(last.n);
+    JET_HIP(hipMemcpyAsync(cur, hp.data(), sizeof(int32_t) * last.n, hipMemcpyHostToDevice, s));
+    for (size_t L = D.size(); L-- > 0;) {
+        ref.refine(D[L], cur);
+        part_t* fp = L == 0 ? cs_part : C.persist.alloc<part_t>(H[L - 1].n);
+        project(L == 0 ? cs.n : H[L - 1].n, D[L].map, cur, fp, s);
+        cur = fp;
+    }
+}
+
 // partitioner.hpp:61-125. part_out: device buffer of fine.n entries.
 static AttemptResult partition_once(Context& C, const jet_config_t& cfg, const Level& fine, uint64_t seed,
                                     part_t* part_out, const DumpWidths* dump) {
@@ -475,11 +543,15 @@ static AttemptResult partition_once(Context& C, const jet_config_t& cfg, const L
     }
     JET_HIP(hipMemcpyAsync(h_vw.data(), cs.vw, sizeof(int32_t) * cs.n, hipMemcpyDeviceToHost, s));
     JET_HIP(hipStreamSynchronize(s));
-    if (jet_host_initpart(cs.n, h_rp.data(), h_col.data(), h_ew.data(), h_vw.data(), k, cfg.max_imb_ratio, seed,
-                          h_part.data()) != 0)
-        throw InitFailure("initial partitioner could not partition the coarsest graph");
     part_t* cur = (levels.size() == 1) ? part_out : C.persist.alloc<part_t>(cs.n);
-    JET_HIP(hipMemcpyAsync(cur, h_part.data(), sizeof(int32_t) * cs.n, hipMemcpyHostToDevice, s));
+    if (jet_init::wants_hierarchy(cs.n, cs.m, k)) {
+        initial_partition_multilevel(C, cfg, cs, h_rp, h_col, h_ew, h_vw, seed, cur);
+    } else {
+        if (jet_host_initpart(cs.n, h_rp.data(), h_col.data(), h_ew.data(), h_vw.data(), k, cfg.max_imb_ratio, seed,
+                              h_part.data()) != 0)
+            throw InitFailure("initial partitioner could not partition the coarsest graph");
+        JET_HIP(hipMemcpyAsync(cur, h_part.data(), sizeof(int32_t) * cs.n, hipMemcpyHostToDevice, s));
+    }
     JET_HIP(hipStreamSynchronize(s));
     auto t2 = clk::now();
     R.t_init = secs(t1, t2);

@@ -139,6 +139,12 @@ class Refiner {
     int32_t gen_ = 0;
     vid_t* pull_mid_;  // mid-tier rows of degree > BIG_DEG
     int32_t ntier_mid_ = 0, ntier_big_ = 0, npull_mid_ = 0;
+    // hub rows above the wide threshold, and their per-part update totals
+    // (nwide_ x k, zero between passes)
+    vid_t* wide_;
+    int32_t nwide_ = 0;
+    int32_t* gdelta_ = nullptr;
+    int64_t gdelta_cap_ = 0;
     bool has_big_ = false;
     // small device state; counter set layout: [0] small candidates, [1] big
     // candidates, [2] small moves, [3] big moves, [4] re-draws, [5] ranked
@@ -152,7 +158,8 @@ class Refiner {
     int64_t* evict_start_;
     part_t* undersized_;
     int32_t* misc_;  // [0] undersized count, [1] error bits, [2] ranking deferred
-    vid_t* redraw_;
+    vid_t* redraw_;    // strong re-draws (n)
+    int32_t* rd_mark_; // marks of a long re-draw list (n, zero between steps)
     int64_t* buckets_;
     int64_t* boff_;        // the scan of a small bucket array, written by the classification kernel
     const int64_t* off_ = nullptr;  // the current rebalance step's scanned buckets (buckets_ or boff_)

@@ -125,7 +125,8 @@ __global__ void k_conn_caps(DGraph g, int32_t k, int32_t* caps) {
 // the rows whose update_small changes are pulled (degree > BIG_DEG): big-tier
 // rows and pull_mid rows; hub[] flags both kinds.
 __global__ void __launch_bounds__(256) k_conn_tiers(DGraph g, int32_t k, vid_t* mid, vid_t* big, vid_t* pull_mid,
-                                                    int32_t* cnt, uint8_t* hub, int32_t small_max) {
+                                                    vid_t* wide, int32_t* cnt, uint8_t* hub, int32_t small_max,
+                                                    int64_t wide_deg) {
     __shared__ int32_t sh[256 / WAVE + 1];
     FOR_BLOCK_UNIFORM(i, g.n) {
         const bool in = i < g.n;
@@ -134,13 +135,17 @@ __global__ void __launch_bounds__(256) k_conn_tiers(DGraph g, int32_t k, vid_t* 
         bool is_mid = d > small_max && d <= HEAVY_DEG_REFINE && cap <= WAVE_TABLE_MAX;
         bool is_big = d > SMALL_DEG && !is_mid;
         bool is_pmid = is_mid && d > BIG_DEG;
-        if (in) hub[i] = (is_big || is_pmid) ? 1 : 0;
+        bool is_wide = is_big && d > wide_deg;
+        // hub: 1 = updated by k_update_hubs, 2 = by k_wide_scan / k_wide_apply
+        if (in) hub[i] = is_wide ? 2 : (is_big || is_pmid) ? 1 : 0;
         int32_t a = block_append<256>(&cnt[0], is_mid, sh);
         int32_t b = block_append<256>(&cnt[1], is_big, sh);
         int32_t c = block_append<256>(&cnt[3], is_pmid, sh);
+        int32_t e = block_append<256>(&cnt[4], is_wide, sh);
         if (a >= 0) mid[a] = (vid_t)i;
         if (b >= 0) big[b] = (vid_t)i;
         if (c >= 0) pull_mid[c] = (vid_t)i;
+        if (e >= 0) wide[e] = (vid_t)i;
     }
 }
 
@@ -639,14 +644,16 @@ struct MoveRec {
     int32_t unused;
 };
 
-// One move in either pass; LANES lanes (sub = lane index) stride its edges.
-template <bool ADD, int LANES>
+// One move in either pass, or one slice of it: the edges rs + sub, rs + sub +
+// stride, ... before re.
+template <bool ADD>
 __device__ inline void update_move(const DGraph& g, vid_t i, part_t pold, part_t pnew, eoff_t rs, eoff_t re, int sub,
+                                   int stride,
                                    bool rebuild, const part_t* part, const part_t* dest, const eoff_t* conn_off,
                                    int2* conn, int32_t* conn_size, part_t* dest_cache, int32_t k, int32_t* err,
                                    int32_t* mark, const uint8_t* hub, uint8_t* hub_dirty, const int32_t* stamp,
                                    int32_t gen, int64_t& cc) {
-    for (eoff_t j = rs + sub; j < re; j += LANES) {
+    for (eoff_t j = rs + sub; j < re; j += stride) {
         const vid_t v = g.col[j];
         const int32_t w = g.ew[j];
         if (!ADD) {
@@ -739,7 +746,7 @@ __global__ void __launch_bounds__(256)
             const MoveRec r = x == x0 ? r0 : rec[x];
             if (sub == 0) part[r.v] = r.pnew;
             if (rebuild) continue;
-            update_move<true, G>(g, r.v, 0, r.pnew, r.rs, r.rs + r.deg, sub, false, part, dest, conn_off, conn,
+            update_move<true>(g, r.v, 0, r.pnew, r.rs, r.rs + r.deg, sub, G, false, part, dest, conn_off, conn,
                                  conn_size, dest_cache, k, err, mark, hub, hub_dirty, stamp, gen, cc);
         } else {
             const vid_t i = x == x0 ? i0 : moves[x];
@@ -757,7 +764,7 @@ __global__ void __launch_bounds__(256)
                     atomicAdd(&sizes[pnew], w);
                 }
             }
-            update_move<false, G>(g, i, pold, pnew, rs, re, sub, rebuild, part, dest, conn_off, conn, conn_size,
+            update_move<false>(g, i, pold, pnew, rs, re, sub, G, rebuild, part, dest, conn_off, conn, conn_size,
                                   dest_cache, k, err, mark, hub, hub_dirty, stamp, gen, cc);
         }
     }
@@ -779,11 +786,15 @@ __global__ void __launch_bounds__(256)
                  const part_t* dest, const eoff_t* conn_off, int2* conn, int32_t* conn_size, part_t* dest_cache,
                  int32_t k, int32_t* err, int32_t* mark, const uint8_t* hub, uint8_t* hub_dirty,
                  const int32_t* deferred, const int32_t* stamp, int32_t gen, unsigned long long* sizes,
-                 int64_t* partials) {
+                 int64_t* partials, int32_t slices) {
     __shared__ int64_t red[256 / WAVE];
     const int lane = threadIdx.x & (WAVE - 1);
+    // units: (move x, slice q) for q < slices; slice q of a move takes its
+    // edges q * WAVE + lane, stride slices * WAVE, so a hub's edges are
+    // spread over `slices` waves; slice 0 does the move's bookkeeping
     const int64_t wstride = ((int64_t)gridDim.x * blockDim.x) / WAVE;
-    const int64_t x0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
+    const int64_t u0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
+    const int64_t x0 = u0 / slices;
     const bool have0 = x0 < nmax;
     const vid_t i0 = !ADD && have0 ? moves[nmax - 1 - x0] : 0;
     MoveRec r0{};
@@ -792,28 +803,31 @@ __global__ void __launch_bounds__(256)
     const int32_t ns = mc[2], nb = mc[3];
     if (dfr) return;
     const bool rebuild = ns + nb > g.n / 10;  // update_by_rebuild
+    const int STRIDE = slices * WAVE;
     int64_t cc = 0;
-    for (int64_t x = x0; x < nb; x += wstride) {
+    for (int64_t u = u0; u < (int64_t)nb * slices; u += wstride) {
+        const int64_t x = u / slices;
+        const int q = (int)(u % slices);
         const int64_t at = nmax - 1 - x;
         if (ADD) {
             const MoveRec r = x == x0 ? r0 : rec[at];
-            if (lane == 0) part[r.v] = r.pnew;
+            if (q == 0 && lane == 0) part[r.v] = r.pnew;
             if (rebuild) continue;
-            update_move<true, WAVE>(g, r.v, 0, r.pnew, r.rs, r.rs + r.deg, lane, false, part, dest, conn_off, conn,
-                                    conn_size, dest_cache, k, err, mark, hub, hub_dirty, stamp, gen, cc);
+            update_move<true>(g, r.v, 0, r.pnew, r.rs, r.rs + r.deg, q * WAVE + lane, STRIDE, false, part, dest,
+                              conn_off, conn, conn_size, dest_cache, k, err, mark, hub, hub_dirty, stamp, gen, cc);
         } else {
             const vid_t i = x == x0 ? i0 : moves[at];
             const part_t pold = part[i], pnew = dest[i];
             const eoff_t rs = g.rowptr[i], re = g.rowptr[i + 1];
-            if (lane == 0) {
+            if (q == 0 && lane == 0) {
                 dest_cache[i] = NULL_PART;
                 rec[at] = MoveRec{rs, (int32_t)(re - rs), i, pnew, 0};
                 const unsigned long long w = (unsigned long long)g.vw[i];
                 atomicAdd(&sizes[pold], (unsigned long long)(-(long long)w));
                 atomicAdd(&sizes[pnew], w);
             }
-            update_move<false, WAVE>(g, i, pold, pnew, rs, re, lane, rebuild, part, dest, conn_off, conn,
-                                     conn_size, dest_cache, k, err, mark, hub, hub_dirty, stamp, gen, cc);
+            update_move<false>(g, i, pold, pnew, rs, re, q * WAVE + lane, STRIDE, rebuild, part, dest, conn_off,
+                               conn, conn_size, dest_cache, k, err, mark, hub, hub_dirty, stamp, gen, cc);
         }
     }
     if (!ADD) add_cut_part2(partials, cc, red);
@@ -832,13 +846,14 @@ __global__ void __launch_bounds__(256)
 template <bool ADD, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
     k_update_hubs(DGraph g, const vid_t* list, int32_t nlist, const int32_t* mc, const int32_t* stamp, int32_t gen,
-                  uint8_t* hub_dirty, const part_t* part, const part_t* dest, const eoff_t* conn_off, int2* conn,
-                  int32_t* conn_size, part_t* dest_cache, int32_t k, int32_t* err, const int32_t* deferred) {
+                  uint8_t* hub_dirty, const uint8_t* hub, const part_t* part, const part_t* dest,
+                  const eoff_t* conn_off, int2* conn, int32_t* conn_size, part_t* dest_cache, int32_t k, int32_t* err,
+                  const int32_t* deferred) {
     if (*deferred || update_by_rebuild(mc, g.n)) return;
     extern __shared__ int32_t delta[];  // k
     for (int32_t r = blockIdx.x; r < nlist; r += gridDim.x) {
         const vid_t u = list[r];
-        if (!hub_dirty[u]) continue;
+        if (!hub_dirty[u] || hub[u] == 2) continue;  // wide rows: k_wide_scan / k_wide_apply
         for (int32_t p = threadIdx.x; p < k; p += BLOCK) delta[p] = 0;
         __syncthreads();
         for (eoff_t j = g.rowptr[u] + threadIdx.x; j < g.rowptr[u + 1]; j += BLOCK) {
@@ -864,6 +879,66 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
+
+// The hub rows of degree above the wide threshold ("wide rows"): their scan
+// for moved neighbours is split into `slices` parts per row, each summing its
+// parts' weights in LDS and adding them to the row's k global totals
+// (gdelta, zero between passes); k_wide_apply then applies each row's totals
+// to its table exactly as k_update_hubs applies its LDS totals, and clears
+// them.
+template <bool ADD>
+__global__ void __launch_bounds__(256)
+    k_wide_scan(DGraph g, const vid_t* wide, int32_t nwide, int32_t slices, const int32_t* mc, const int32_t* stamp,
+                int32_t gen, const uint8_t* hub_dirty, const part_t* part, const part_t* dest, int32_t* gdelta,
+                int32_t k, const int32_t* deferred) {
+    if (*deferred || update_by_rebuild(mc, g.n)) return;
+    extern __shared__ int32_t delta[];  // k
+    const int64_t units = (int64_t)nwide * slices;
+    for (int64_t x = blockIdx.x; x < units; x += gridDim.x) {
+        const int32_t r = (int32_t)(x / slices), q = (int32_t)(x % slices);
+        const vid_t u = wide[r];
+        if (!hub_dirty[u]) continue;
+        for (int32_t p = threadIdx.x; p < k; p += blockDim.x) delta[p] = 0;
+        __syncthreads();
+        const eoff_t re = g.rowptr[u + 1];
+        for (eoff_t j = g.rowptr[u] + (eoff_t)q * blockDim.x + threadIdx.x; j < re; j += (eoff_t)slices * blockDim.x) {
+            const vid_t v = g.col[j];
+            if (stamp[v] == gen) atomicAdd(&delta[ADD ? dest[v] : part[v]], g.ew[j]);
+        }
+        __syncthreads();
+        for (int32_t p = threadIdx.x; p < k; p += blockDim.x)
+            if (delta[p]) atomicAdd(&gdelta[(int64_t)r * k + p], delta[p]);
+        __syncthreads();
+    }
+}
+
+template <bool ADD>
+__global__ void __launch_bounds__(256)
+    k_wide_apply(const vid_t* wide, int32_t nwide, const int32_t* mc, vid_t n, uint8_t* hub_dirty,
+                 const eoff_t* conn_off, int2* conn, int32_t* conn_size, part_t* dest_cache, int32_t* gdelta,
+                 int32_t k, int32_t* err, const int32_t* deferred) {
+    if (*deferred || update_by_rebuild(mc, n)) return;
+    for (int32_t r = blockIdx.x; r < nwide; r += gridDim.x) {
+        const vid_t u = wide[r];
+        if (!hub_dirty[u]) continue;
+        int2* tab = conn + conn_off[u];
+        const int32_t cap = (int32_t)(conn_off[u + 1] - conn_off[u]);
+        const int32_t vsz = conn_size[u];
+        int32_t* gd = gdelta + (int64_t)r * k;
+        for (int32_t p = threadIdx.x; p < k; p += blockDim.x) {
+            const int32_t w = gd[p];
+            if (w == 0) continue;
+            gd[p] = 0;
+            if (ADD) conn_add(tab, &conn_size[u], cap, p, w, err);
+            else conn_sub(tab, vsz, p, w, k, err);
+        }
+        if (ADD && threadIdx.x == 0) {
+            dest_cache[u] = NULL_PART;
+            hub_dirty[u] = 0;
+        }
+        __syncthreads();
+    }
+}
 
 // ---------------------------------------------------------------- rebalancing
 // Rebalancer minibuckets (jet_refiner.hpp:359-412, :449-481, :632-643).
@@ -1532,59 +1607,92 @@ __global__ void k_strong_dest(const vid_t* moves, const int32_t* mc, vid_t nmax,
 // Re-draws in increasing vertex id, one after another (the counter starts at
 // the total eviction weight). The serial walk reads only LDS: the chunk
 // starts (k + 1 <= RD_ES_MAX) and, a batch of 256 at a time, the re-drawn
-// vertices' scores and weights are staged there first.
+// vertices' scores and weights are staged there first. Up to lds_max (at
+// most RD_MAX) re-drawn vertices are sorted by id in LDS; a longer list is
+// put in id order in place
+// by marking its vertices in `mark` (n entries, all zero between launches)
+// and compacting the marks over all vertex ids, clearing them as it goes.
 constexpr int RD_MAX = 8192;
 constexpr int RD_ES_MAX = 2049;
+constexpr int RD_SCAN = 16;  // marks per thread per compaction pass
 __global__ void __launch_bounds__(256) k_strong_redraw(vid_t* redraw, const int32_t* rcount, const part_t* part,
                                                        const wgt_t* vw, const int64_t* evict_start, int32_t k,
-                                                       const int64_t* score, part_t* dest, int64_t* acc, int32_t* err,
+                                                       const int64_t* score, part_t* dest, int64_t* acc,
+                                                       int32_t* mark, vid_t n, int32_t lds_max,
                                                        const int32_t* deferred) {
     __shared__ vid_t ids[RD_MAX];
     __shared__ int64_t es_lds[RD_ES_MAX];
     __shared__ int64_t bscore[256];
     __shared__ wgt_t bw[256];
+    __shared__ int64_t red[256 / WAVE + 1];
     if (*deferred) return;
-    int32_t r = rcount[0];
-    if (r > RD_MAX) {
-        if (threadIdx.x == 0) atomicOr(err, 4);
-        return;
-    }
+    const int32_t r = rcount[0];
     if (r == 0) return;
     const bool es_in_lds = k + 1 <= RD_ES_MAX;
     if (es_in_lds)
         for (int32_t t = threadIdx.x; t <= k; t += blockDim.x) es_lds[t] = evict_start[t];
     const int64_t* es = es_in_lds ? es_lds : evict_start;
-    uint32_t cap = 1;
-    while ((int32_t)cap < r) cap <<= 1;
-    for (uint32_t t = threadIdx.x; t < cap; t += blockDim.x) ids[t] = (int32_t)t < r ? redraw[t] : ORD_MAX;
-    for (uint32_t size = 2; size <= cap; size <<= 1) {
-        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-            __syncthreads();
-            for (uint32_t t = threadIdx.x; t < cap / 2; t += blockDim.x) {
-                uint32_t lo = 2 * t - (t & (stride - 1));
-                uint32_t hi = lo + stride;
-                bool up = (lo & size) == 0;
-                vid_t a = ids[lo], b = ids[hi];
-                if ((a > b) == up) {
-                    ids[lo] = b;
-                    ids[hi] = a;
+    const vid_t* idp = ids;
+    if (r <= lds_max) {
+        uint32_t cap = 1;
+        while ((int32_t)cap < r) cap <<= 1;
+        for (uint32_t t = threadIdx.x; t < cap; t += blockDim.x) ids[t] = (int32_t)t < r ? redraw[t] : ORD_MAX;
+        for (uint32_t size = 2; size <= cap; size <<= 1) {
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                __syncthreads();
+                for (uint32_t t = threadIdx.x; t < cap / 2; t += blockDim.x) {
+                    uint32_t lo = 2 * t - (t & (stride - 1));
+                    uint32_t hi = lo + stride;
+                    bool up = (lo & size) == 0;
+                    vid_t a = ids[lo], b = ids[hi];
+                    if ((a > b) == up) {
+                        ids[lo] = b;
+                        ids[hi] = a;
+                    }
                 }
             }
         }
+    } else {
+        // the list's vertices marked, then written back in increasing id
+        for (int32_t x = threadIdx.x; x < r; x += blockDim.x) mark[redraw[x]] = 1;
+        __threadfence_block();
+        __syncthreads();
+        int64_t out = 0;
+        for (int64_t v0 = 0; v0 < n; v0 += 256 * RD_SCAN) {
+            const int64_t t0 = v0 + (int64_t)threadIdx.x * RD_SCAN;
+            int32_t f[RD_SCAN];
+            int64_t c = 0;
+#pragma unroll
+            for (int j = 0; j < RD_SCAN; j++) {
+                f[j] = t0 + j < n ? mark[t0 + j] : 0;
+                c += f[j];
+            }
+            int64_t tot;
+            int64_t at = out + block_excl_scan<int64_t, 256>(c, tot, red);
+#pragma unroll
+            for (int j = 0; j < RD_SCAN; j++)
+                if (f[j]) {
+                    redraw[at++] = (vid_t)(t0 + j);
+                    mark[t0 + j] = 0;
+                }
+            out += tot;
+        }
+        __threadfence_block();
+        idp = redraw;
     }
     __syncthreads();
     int64_t counter = acc[3];  // carried by thread 0
     for (int32_t base = 0; base < r; base += 256) {
         const int32_t nb = r - base < 256 ? r - base : 256;
         if ((int32_t)threadIdx.x < nb) {
-            const vid_t v = ids[base + threadIdx.x];
+            const vid_t v = idp[base + threadIdx.x];
             bscore[threadIdx.x] = score[v];
             bw[threadIdx.x] = vw[v];
         }
         __syncthreads();
         if (threadIdx.x == 0) {
             for (int32_t x = 0; x < nb; x++) {
-                const vid_t v = ids[base + x];
+                const vid_t v = idp[base + x];
                 const wgt_t wv = bw[x];
                 int64_t sc = bscore[x];
                 int32_t p = 0;
@@ -1687,6 +1795,7 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
     cur_part_ = a.alloc<part_t>(N);
     tier_mid_ = a.alloc<vid_t>(N);
     tier_big_ = a.alloc<vid_t>(N);
+    wide_ = a.alloc<vid_t>(N);
     hub_ = a.alloc<uint8_t>(N);
     hub_dirty_ = a.alloc<uint8_t>(N);
     pull_mid_ = a.alloc<vid_t>(N);
@@ -1699,7 +1808,8 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
     evict_start_ = a.alloc<int64_t>(p.k + 1);
     undersized_ = a.alloc<part_t>(p.k);
     misc_ = a.alloc<int32_t>(8);
-    redraw_ = a.alloc<vid_t>(RD_MAX);
+    redraw_ = a.alloc<vid_t>(N);
+    rd_mark_ = a.alloc<int32_t>(N);
     nbuckets_ = (int64_t)MAX_BUCKETS * p.k * sections_for(n, p.k) + MAX_SECTIONS + 4;
     buckets_ = a.alloc<int64_t>(nbuckets_);
     boff_ = a.alloc<int64_t>(FUSE_SCAN_MAX);
@@ -1708,6 +1818,7 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
     bcopies_ = a.alloc<int64_t>(std::max<int64_t>(1, nbuckets_ * (ncopies_ - 1)));
     if (ncopies_ > 1) fill_i64(bcopies_, 0, nbuckets_ * (ncopies_ - 1), s);
     fill_i32(swap_bit_, 0, N, s);
+    fill_i32(rd_mark_, 0, N, s);
     JET_HIP(hipMemsetAsync(hub_dirty_, 0, (size_t)N, s));
     fill_i32(stamp_, -1, N, s);
     fill_i32(counters_, 0, 32, s);
@@ -1726,20 +1837,29 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
 
 void Refiner::build_tiers(const Level& L) {
     DGraph g = L.view();
-    fill_i32(counters_ + 16, 0, 4, s_);
+    fill_i32(counters_ + 16, 0, 5, s_);
     // on small levels every row with edges takes the wave tier: a thread per
     // row walks its edges one after another, a wave inserts them all at once
     small_max_ = g.n <= tuning().small_level_n ? 0 : SMALL_DEG;
-    k_conn_tiers<<<grid_for(g.n, 256, 8192), 256, 0, s_>>>(g, p_.k, tier_mid_, tier_big_, pull_mid_, counters_ + 16,
-                                                            hub_, small_max_);
+    k_conn_tiers<<<grid_for(g.n, 256, 8192), 256, 0, s_>>>(g, p_.k, tier_mid_, tier_big_, pull_mid_, wide_,
+                                                            counters_ + 16, hub_, small_max_,
+                                                            tuning().hub_wide_deg);
     k_max_degree<<<grid_for(g.n, 256, 512), 256, 0, s_>>>(g, counters_ + 18);
     JET_LAUNCH_CHECK();
-    int32_t c[4];
-    read_i32_words(counters_ + 16, 4, c, s_);
+    int32_t c[5];
+    read_i32_words(counters_ + 16, 5, c, s_);
     ntier_mid_ = c[0];
     ntier_big_ = c[1];
     has_big_ = c[2] > BIG_DEG;
     npull_mid_ = c[3];
+    nwide_ = c[4];
+    // the wide rows' per-part totals: zero, and kept zero by k_wide_apply
+    const int64_t need = (int64_t)nwide_ * p_.k;
+    if (need > gdelta_cap_) {
+        gdelta_cap_ = std::max<int64_t>(need, 2 * gdelta_cap_);
+        gdelta_ = a_.alloc<int32_t>(gdelta_cap_);
+        fill_i32(gdelta_, 0, gdelta_cap_, s_);
+    }
 }
 
 void Refiner::init_conn(const Level& L, const part_t* part, const int32_t* mark) {
@@ -1895,9 +2015,10 @@ void Refiner::strong_tail(const Level& L, const part_t* part, const int64_t* siz
     k_cookie<<<1, CK_BLOCK, 0, s_>>>(k, r.width, off_, sizes, r.max_dest, evict_adjust_, evict_start_, acc_,
                                      misc_ + 2);
     k_strong_dest<<<nb, 256, 0, s_>>>(moves_, c, nmax_, part, g.vw, evict_adjust_, evict_start_, k, score_, dest_,
-                                      redraw_, c + 4, RD_MAX, misc_ + 2);
-    k_strong_redraw<<<1, 256, 0, s_>>>(redraw_, c + 4, part, g.vw, evict_start_, k, score_, dest_, acc_, misc_ + 1,
-                                       misc_ + 2);
+                                      redraw_, c + 4, nmax_, misc_ + 2);
+    const int32_t lds_max = (int32_t)std::min<int64_t>(RD_MAX, tuning().redraw_lds_max);
+    k_strong_redraw<<<1, 256, 0, s_>>>(redraw_, c + 4, part, g.vw, evict_start_, k, score_, dest_, acc_, rd_mark_,
+                                       g.n, lds_max, misc_ + 2);
     JET_LAUNCH_CHECK();
 }
 
@@ -2040,12 +2161,13 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
         }
 #undef JET_UPD
         if (has_big_) {
+            const int32_t slices = (int32_t)tuning().upd_big_slices;
             if (add) k_update_big<true><<<nbb, 256, 0, s_>>>(g, moves_, mrec_, c, nmax_, part, dest_, conn_off_, conn_,
                                                              conn_size_, dest_cache_, k, misc_ + 1, swap_bit_, hub_,
-                                                             hub_dirty_, misc_ + 2, stamp_, gen_, usz, pub_);
+                                                             hub_dirty_, misc_ + 2, stamp_, gen_, usz, pub_, slices);
             else k_update_big<false><<<nbb, 256, 0, s_>>>(g, moves_, mrec_, c, nmax_, part, dest_, conn_off_, conn_,
                                                            conn_size_, dest_cache_, k, misc_ + 1, swap_bit_, hub_,
-                                                           hub_dirty_, misc_ + 2, stamp_, gen_, usz, pub_);
+                                                           hub_dirty_, misc_ + 2, stamp_, gen_, usz, pub_, slices);
         }
         // pulled rows: big-tier rows by a 256-thread workgroup, rows of
         // degree (BIG_DEG, HEAVY_DEG_REFINE] by a single-wave workgroup
@@ -2053,21 +2175,38 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
         if (ntier_big_ > 0) {
             const int hb = std::min(ntier_big_, 2048);
             if (add) k_update_hubs<true, 256><<<hb, 256, lds, s_>>>(g, tier_big_, ntier_big_, c, stamp_, gen_,
-                                                                    hub_dirty_, part, dest_, conn_off_, conn_,
+                                                                    hub_dirty_, hub_, part, dest_, conn_off_, conn_,
                                                                     conn_size_, dest_cache_, k, misc_ + 1, misc_ + 2);
             else k_update_hubs<false, 256><<<hb, 256, lds, s_>>>(g, tier_big_, ntier_big_, c, stamp_, gen_,
-                                                                 hub_dirty_, part, dest_, conn_off_, conn_,
+                                                                 hub_dirty_, hub_, part, dest_, conn_off_, conn_,
                                                                  conn_size_, dest_cache_, k, misc_ + 1, misc_ + 2);
         }
         if (npull_mid_ > 0) {
             const int hb = std::min(npull_mid_, 8192);
             if (add) k_update_hubs<true, WAVE><<<hb, WAVE, lds, s_>>>(g, pull_mid_, npull_mid_, c, stamp_, gen_,
-                                                                      hub_dirty_, part, dest_, conn_off_, conn_,
+                                                                      hub_dirty_, hub_, part, dest_, conn_off_, conn_,
                                                                       conn_size_, dest_cache_, k, misc_ + 1,
                                                                       misc_ + 2);
             else k_update_hubs<false, WAVE><<<hb, WAVE, lds, s_>>>(g, pull_mid_, npull_mid_, c, stamp_, gen_,
-                                                                   hub_dirty_, part, dest_, conn_off_, conn_,
+                                                                   hub_dirty_, hub_, part, dest_, conn_off_, conn_,
                                                                    conn_size_, dest_cache_, k, misc_ + 1, misc_ + 2);
+        }
+        if (nwide_ > 0) {
+            const int32_t slices = (int32_t)tuning().hub_wide_slices;
+            const size_t lds = sizeof(int32_t) * (size_t)k;
+            const int sb = (int)std::min<int64_t>((int64_t)nwide_ * slices, 4096);
+            const int ab = std::min(nwide_, 2048);
+            if (add) {
+                k_wide_scan<true><<<sb, 256, lds, s_>>>(g, wide_, nwide_, slices, c, stamp_, gen_, hub_dirty_, part,
+                                                        dest_, gdelta_, k, misc_ + 2);
+                k_wide_apply<true><<<ab, 256, 0, s_>>>(wide_, nwide_, c, g.n, hub_dirty_, conn_off_, conn_,
+                                                       conn_size_, dest_cache_, gdelta_, k, misc_ + 1, misc_ + 2);
+            } else {
+                k_wide_scan<false><<<sb, 256, lds, s_>>>(g, wide_, nwide_, slices, c, stamp_, gen_, hub_dirty_, part,
+                                                         dest_, gdelta_, k, misc_ + 2);
+                k_wide_apply<false><<<ab, 256, 0, s_>>>(wide_, nwide_, c, g.n, hub_dirty_, conn_off_, conn_,
+                                                        conn_size_, dest_cache_, gdelta_, k, misc_ + 1, misc_ + 2);
+            }
         }
         prof_end(PF_UPDATE, s_);
         JET_LAUNCH_CHECK();
@@ -2083,6 +2222,16 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
     rebuild_pending_ = res[3] != 0;
     par_ ^= 1;
     return false;
+}
+
+// The refinement kernels' error bits, as text.
+static std::string refine_error_text(int32_t err) {
+    std::string m = "refinement invariant violated:";
+    if (err & 1) m += " a moved vertex's old part was missing from a neighbour's connectivity row;";
+    if (err & 2) m += " a connectivity row ran out of slots;";
+    if (err & 8) m += " a strong re-draw walked past its bound;";
+    if (err & ~11) m += " error bits " + std::to_string(err) + ";";
+    return m;
 }
 
 void Refiner::flush_rebuild(const Level& L) {
@@ -2189,7 +2338,7 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
     }
     flush_rebuild(L);
     int32_t err = read_scalar(misc_ + 1, s_);
-    if (err) throw DeviceError("connectivity table invariant violated (code " + std::to_string(err) + ")");
+    if (err) throw DeviceError(refine_error_text(err));
     rec.cut2 = cut_;
     rec.total_imb = total_imb_;
     rec.n = g.n;
@@ -2247,7 +2396,7 @@ LevelRecord Refiner::refine_wg(const Level& L, part_t* best_part, const std::vec
         launch_refine_wg(P, s_);
         JET_HIP(hipMemcpyAsync(&h, wg_state_, sizeof(h), hipMemcpyDeviceToHost, s_));
         JET_HIP(hipStreamSynchronize(s_));
-        if (h.err) throw DeviceError("connectivity table invariant violated (code " + std::to_string(h.err) + ")");
+        if (h.err) throw DeviceError(refine_error_text(h.err));
         if (h.status == WG_STALLED) throw DeviceError("one-workgroup refinement exceeded its iteration bound");
         if (h.status != WG_REBUILD) break;
         init_conn(L, cur_part_, swap_bit_);  // update_large's rebuild of the marked rows
@@ -2306,7 +2455,7 @@ void Refiner::single_step(const Level& L, const part_t* part, int kind, double f
     }
     JET_HIP(hipStreamSynchronize(s_));
     int32_t err = read_scalar(misc_ + 1, s_);
-    if (err) throw DeviceError("refinement invariant violated (code " + std::to_string(err) + ")");
+    if (err) throw DeviceError(refine_error_text(err));
     std::vector<int32_t> idx(total);
     for (int32_t t = 0; t < total; t++) idx[t] = t;
     std::sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) { return v[x] < v[y]; });

@@ -35,9 +35,13 @@ const Field kFields[] = {
     {"small_level_n", &Tuning::small_level_n, 0, 1 << 30},
     {"ab_lanes_small", &Tuning::ab_lanes_small, 1, 4},
     {"upd_lanes", &Tuning::upd_lanes, 1, 4},
+    {"upd_big_slices", &Tuning::upd_big_slices, 1, 64},
+    {"hub_wide_deg", &Tuning::hub_wide_deg, 2048, int64_t(1) << 40},
+    {"hub_wide_slices", &Tuning::hub_wide_slices, 1, 256},
     {"lp_one_pass_n", &Tuning::lp_one_pass_n, 0, int64_t(1) << 31},
     {"ct_wave_max", &Tuning::ct_wave_max, 1, 128},
     {"ct_block_max", &Tuning::ct_block_max, 1, 4096},
+    {"redraw_lds_max", &Tuning::redraw_lds_max, 0, 8192},
 };
 std::mutex g_mu;
 // fields set through jet_tuning_set: the table applied at context creation

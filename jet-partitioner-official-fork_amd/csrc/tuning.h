@@ -23,6 +23,12 @@ struct Tuning {
     // lanes per move in the update passes: upd_lanes times the level's lane
     // group (from its average degree), at most a wave
     int64_t upd_lanes = 2;
+    // moves of degree above 64 are spread over this many waves each
+    int64_t upd_big_slices = 16;
+    // hub rows of higher degree take their updates through a scan split
+    // into hub_wide_slices workgroups per row and per-part global totals
+    int64_t hub_wide_deg = 4096;
+    int64_t hub_wide_slices = 16;
     // levels of up to this many vertices run LP select with one pass per wave
     int64_t lp_one_pass_n = 524288;
     // contraction: coarse rows of up to ct_wave_max fine entries take the
@@ -30,6 +36,9 @@ struct Tuning {
     // table (8192 slots), longer rows the radix spill path
     int64_t ct_wave_max = 128;
     int64_t ct_block_max = 4096;
+    // strong rebalancing: up to this many re-drawn vertices are put in id
+    // order in LDS, longer lists in global memory
+    int64_t redraw_lds_max = 8192;
 };
 
 Tuning& tuning();

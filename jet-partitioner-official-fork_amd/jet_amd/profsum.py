@@ -3,6 +3,8 @@
 kept under profiles/.
 
     python -m jet_amd.profsum <run_results.db> --partitions N [--title T]
+    python -m jet_amd.profsum --traffic --family F --fetch <db> --write <db> \
+        --copy-fetch <db> --copy-write <db>
 
 Families group the template instances of one kernel; the profiler families of
 bench.py ("lp_select", "afterburner", "update", "weak_select") are the
@@ -41,7 +43,53 @@ def load(db_path):
     return [(n, s, e) for n, s, e in rows]
 
 
+def counter_rows(db_path, counter):
+    db = sqlite3.connect(db_path)
+    return db.execute("select kernel_name, value from counters_collection where counter_name = ? order by dispatch_id",
+                      (counter,)).fetchall()
+
+
+def copy_scale(db_path, counter, nbytes):
+    """nbytes / the counter's bytes for the plain copy kernel (the elementwise
+    add of bench.py --calibrate-copy, 1 GiB read and 1 GiB written)."""
+    vals = [v for n, v in counter_rows(db_path, counter) if "add" in n]
+    if not vals:
+        sys.exit(f"no copy dispatches in {db_path}")
+    return nbytes / (1024.0 * float(sorted(vals)[len(vals) // 2]))
+
+
+def traffic(args):
+    """HBM bytes per dispatch of a family, from one FETCH_SIZE and one
+    WRITE_SIZE pass, each scaled by the same counter over the plain copy."""
+    fs = copy_scale(args.copy_fetch, "FETCH_SIZE", 1 << 30)
+    ws = copy_scale(args.copy_write, "WRITE_SIZE", 1 << 30)
+    out = []
+    for counter, db_path, scale in (("FETCH_SIZE", args.fetch, fs), ("WRITE_SIZE", args.write, ws)):
+        vals = [v for n, v in counter_rows(db_path, counter) if family(n) == args.family]
+        if not vals:
+            sys.exit(f"no {args.family} dispatches in {db_path}")
+        kib = sum(vals) / len(vals)
+        out.append((counter, len(vals), kib, scale, kib * 1024.0 * scale))
+    print(f"# HBM traffic of the {args.family} family")
+    print("")
+    print("| counter | dispatches | KiB per dispatch (raw) | scale from the 1 GiB copy | bytes per dispatch |")
+    print("|---|---:|---:|---:|---:|")
+    for counter, nd, kib, scale, b in out:
+        print(f"| {counter} | {nd} | {kib:.1f} | {scale:.3f} | {b:.0f} |")
+    print("")
+    print(f"HBM bytes per launch: {round(out[0][4] + out[1][4])}")
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--traffic":
+        ap = argparse.ArgumentParser()
+        ap.add_argument("--traffic", action="store_true")
+        ap.add_argument("--family", required=True)
+        ap.add_argument("--fetch", required=True, help="results db of a --pmc FETCH_SIZE pass")
+        ap.add_argument("--write", required=True, help="results db of a --pmc WRITE_SIZE pass")
+        ap.add_argument("--copy-fetch", required=True, help="FETCH_SIZE pass over bench.py --calibrate-copy")
+        ap.add_argument("--copy-write", required=True, help="WRITE_SIZE pass over bench.py --calibrate-copy")
+        return traffic(ap.parse_args())
     ap = argparse.ArgumentParser()
     ap.add_argument("db")
     ap.add_argument("--partitions", type=int, required=True, help="partitions the traced command ran")

@@ -33,8 +33,9 @@
 #include <omp.h>
 #endif
 
-extern "C" int jet_host_initpart(int32_t n, const int64_t* xadj, const int32_t* adj, const int32_t* adjw,
-                                 const int32_t* vw, int32_t k, double imb, uint64_t seed, int32_t* part);
+// the host initial partitioner shared with the GPU path
+// (jet-partitioner-official-fork_amd/csrc/initpart.{h,cpp})
+#include "initpart.h"
 
 namespace orc {
 
@@ -1298,6 +1299,40 @@ struct Level {
     bool uniform = false;
 };
 
+// The initial partition of a large coarsest graph, multilevel k-way
+// (initpart.h, the same scheme as the GPU path's): the shared host code
+// coarsens it further and partitions the smallest level by recursive
+// bisection; each level of that hierarchy is then refined with jet_refine
+// (its own refine_data) and its partition projected to the next finer one.
+// The coarsest graph's own refinement is left to the uncoarsening.
+static int initial_partition_multilevel(const Graph& cs, const int32_t* csvw, int32_t k, double imb, double tol,
+                                        bool ultra, uint64_t seed, std::vector<int32_t>& part) {
+    std::vector<jet_init::HostLevel> H = jet_init::init_hierarchy(cs.n, cs.rowptr.data(), cs.col.data(),
+                                                                  cs.ew.data(), csvw, k, seed);
+    if (H.empty())
+        return jet_host_initpart(cs.n, cs.rowptr.data(), cs.col.data(), cs.ew.data(), csvw, k, imb, seed,
+                                 part.data());
+    const jet_init::HostLevel& last = H.back();
+    std::vector<int32_t> cur(last.n);
+    int rc = jet_host_initpart(last.n, last.xadj.data(), last.adj.data(), last.adjw.data(), last.vw.data(), k, imb,
+                               seed, cur.data());
+    if (rc != 0) return rc;
+    std::vector<Graph> G(H.size());
+    for (size_t i = 0; i < H.size(); i++)
+        G[i] = make_graph(H[i].n, H[i].xadj.data(), H[i].adj.data(), H[i].adjw.data());
+    Refiner ref(G[0].n, conn_capacity(G[0], k), k);
+    RefineData rd;
+    for (size_t L = H.size(); L-- > 0;) {
+        ref.refine(G[L], H[L].vw.data(), cur.data(), false, imb, tol, ultra, rd);
+        const int32_t nf = L == 0 ? cs.n : H[L - 1].n;
+        std::vector<int32_t> fp(nf);
+        for (int32_t i = 0; i < nf; i++) fp[i] = cur[H[L].map[i]];  // uncoarsen.hpp:82-86
+        cur.swap(fp);
+    }
+    part.swap(cur);
+    return 0;
+}
+
 // partitioner.hpp:61-125 with contract.hpp:494-533 and uncoarsen.hpp:88-158.
 //
 // With `repeat_shortcut`, two exact shortcuts are taken, for graphs whose
@@ -1344,8 +1379,10 @@ static int partition(int32_t alg, int32_t k, double imb, double tol, bool ultra,
     }
     const Level& cs = levels.back();
     std::vector<int32_t> part(cs.g.n);
-    int rc = jet_host_initpart(cs.g.n, cs.g.rowptr.data(), cs.g.col.data(), cs.g.ew.data(), cs.vw.data(), k, imb,
-                               seed, part.data());
+    int rc = jet_init::wants_hierarchy(cs.g.n, cs.g.m(), k)
+                 ? initial_partition_multilevel(cs.g, cs.vw.data(), k, imb, tol, ultra, seed, part)
+                 : jet_host_initpart(cs.g.n, cs.g.rowptr.data(), cs.g.col.data(), cs.g.ew.data(), cs.vw.data(), k,
+                                     imb, seed, part.data());
     if (rc != 0) return rc;
     Refiner ref(fine.n, conn_capacity(fine, k), k);
     RefineData rfd;

@@ -27,7 +27,9 @@ def main():
     g["uniform"] = True
     print(f"generated in {time.time() - t:.1f}s", flush=True)
     t = time.time()
-    part, cut, levels = oracle.partition(g, 256, 1.10, alg=1, seed=0, repeat_shortcut=True)
+    # the OpenMP build of the oracle: the same results as the serial build
+    # (tests/test_oracle_omp.py), in a fraction of the time
+    part, cut, levels = oracle.partition(g, 256, 1.10, alg=1, seed=0, repeat_shortcut=True, L=oracle.omp_lib()[0])
     print(f"oracle partition in {time.time() - t:.1f}s, cut {cut}, {len(levels)} levels", flush=True)
     out = {
         "workload": "rmat-22 ef16 (.57,.19,.19) seed 0; k=256 imb 1.10 HEC seed 0",

@@ -115,3 +115,27 @@ def test_weak_bucket_50_shared_with_next_part(refine_path):
         ep, erec, tr = oracle.refine_level_trace(g, p0, k, 1.03, uniform=False)
         assert list(rec) == list(erec), (rec, erec)
         assert np.array_equal(p, ep)
+
+
+@pytest.mark.parametrize("name", ["vheavy", "grid-300-vw25-k64"])
+def test_long_redraw_list_path(name, refine_path):
+    # redraw_lds_max=0 sends every strong re-draw list through the path for
+    # lists longer than the LDS holds (marks over all vertex ids, compacted in
+    # id order in global memory); results must not change
+    if name == "vheavy":
+        (_, mk, k, imb, tol), = [c for c in BATTERY if c[0] == "vheavy"]
+        g = mk()
+        starts = [rand_part(len(g["rowptr"]) - 1, k, 6, s) for s in SKEWS]
+    else:
+        g = graphs.with_weights(graphs.grid2d(300, 300), seed=3, wmax=5, vwmax=25)
+        k, imb, tol = 64, 1.01, 0.999
+        starts = [rand_part(len(g["rowptr"]) - 1, k, 7, 2.0)]
+    for p0 in starts:
+        ep, erec, tr = oracle.refine_level_trace(g, p0, k, imb, tol=tol, uniform=False)
+        assert any(t[0] == 2 for t in tr)
+        for lds_max in (0, 8192):
+            say("long re-draw list", refine_path, name, lds_max)
+            with jet_amd.tuned(redraw_lds_max=lds_max):
+                p, rec = jet_amd.refine_level(g, p0, k, imb, tol=tol, uniform=False)
+            assert list(rec) == list(erec)
+            assert np.array_equal(p, ep)
