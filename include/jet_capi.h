@@ -388,9 +388,6 @@ int jet_partition_batch(jet_ctx_t* ctx, int32_t count, const jet_config_t* cfgs,
  *   afterburner on small levels and of the update passes;
  * "upd_big_slices": waves (1-64) that share the edges of one move of degree
  *   above 64 in the update passes;
- * "hub_wide_deg", "hub_wide_slices": hub rows of higher degree (>= 2048)
- *   take their table updates through a scan split over that many (1-256)
- *   workgroups per row;
  * "lp_one_pass_n": LP select runs one pass per wave up to this level size;
  * "ct_wave_max", "ct_block_max": contraction rows of up to this many fine
  *   entries take the one-wave (<= 128) or one-workgroup (<= 4096) LDS table;

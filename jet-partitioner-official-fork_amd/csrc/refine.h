@@ -139,12 +139,6 @@ class Refiner {
     int32_t gen_ = 0;
     vid_t* pull_mid_;  // mid-tier rows of degree > BIG_DEG
     int32_t ntier_mid_ = 0, ntier_big_ = 0, npull_mid_ = 0;
-    // hub rows above the wide threshold, and their per-part update totals
-    // (nwide_ x k, zero between passes)
-    vid_t* wide_;
-    int32_t nwide_ = 0;
-    int32_t* gdelta_ = nullptr;
-    int64_t gdelta_cap_ = 0;
     bool has_big_ = false;
     // small device state; counter set layout: [0] small candidates, [1] big
     // candidates, [2] small moves, [3] big moves, [4] re-draws, [5] ranked

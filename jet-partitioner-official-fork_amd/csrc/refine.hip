@@ -125,8 +125,7 @@ __global__ void k_conn_caps(DGraph g, int32_t k, int32_t* caps) {
 // the rows whose update_small changes are pulled (degree > BIG_DEG): big-tier
 // rows and pull_mid rows; hub[] flags both kinds.
 __global__ void __launch_bounds__(256) k_conn_tiers(DGraph g, int32_t k, vid_t* mid, vid_t* big, vid_t* pull_mid,
-                                                    vid_t* wide, int32_t* cnt, uint8_t* hub, int32_t small_max,
-                                                    int64_t wide_deg) {
+                                                    int32_t* cnt, uint8_t* hub, int32_t small_max) {
     __shared__ int32_t sh[256 / WAVE + 1];
     FOR_BLOCK_UNIFORM(i, g.n) {
         const bool in = i < g.n;
@@ -135,17 +134,13 @@ __global__ void __launch_bounds__(256) k_conn_tiers(DGraph g, int32_t k, vid_t* 
         bool is_mid = d > small_max && d <= HEAVY_DEG_REFINE && cap <= WAVE_TABLE_MAX;
         bool is_big = d > SMALL_DEG && !is_mid;
         bool is_pmid = is_mid && d > BIG_DEG;
-        bool is_wide = is_big && d > wide_deg;
-        // hub: 1 = updated by k_update_hubs, 2 = by k_wide_scan / k_wide_apply
-        if (in) hub[i] = is_wide ? 2 : (is_big || is_pmid) ? 1 : 0;
+        if (in) hub[i] = (is_big || is_pmid) ? 1 : 0;
         int32_t a = block_append<256>(&cnt[0], is_mid, sh);
         int32_t b = block_append<256>(&cnt[1], is_big, sh);
         int32_t c = block_append<256>(&cnt[3], is_pmid, sh);
-        int32_t e = block_append<256>(&cnt[4], is_wide, sh);
         if (a >= 0) mid[a] = (vid_t)i;
         if (b >= 0) big[b] = (vid_t)i;
         if (c >= 0) pull_mid[c] = (vid_t)i;
-        if (e >= 0) wide[e] = (vid_t)i;
     }
 }
 
@@ -853,7 +848,7 @@ __global__ void __launch_bounds__(BLOCK)
     extern __shared__ int32_t delta[];  // k
     for (int32_t r = blockIdx.x; r < nlist; r += gridDim.x) {
         const vid_t u = list[r];
-        if (!hub_dirty[u] || hub[u] == 2) continue;  // wide rows: k_wide_scan / k_wide_apply
+        if (!hub_dirty[u]) continue;
         for (int32_t p = threadIdx.x; p < k; p += BLOCK) delta[p] = 0;
         __syncthreads();
         for (eoff_t j = g.rowptr[u] + threadIdx.x; j < g.rowptr[u + 1]; j += BLOCK) {
@@ -879,66 +874,6 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
-
-// The hub rows of degree above the wide threshold ("wide rows"): their scan
-// for moved neighbours is split into `slices` parts per row, each summing its
-// parts' weights in LDS and adding them to the row's k global totals
-// (gdelta, zero between passes); k_wide_apply then applies each row's totals
-// to its table exactly as k_update_hubs applies its LDS totals, and clears
-// them.
-template <bool ADD>
-__global__ void __launch_bounds__(256)
-    k_wide_scan(DGraph g, const vid_t* wide, int32_t nwide, int32_t slices, const int32_t* mc, const int32_t* stamp,
-                int32_t gen, const uint8_t* hub_dirty, const part_t* part, const part_t* dest, int32_t* gdelta,
-                int32_t k, const int32_t* deferred) {
-    if (*deferred || update_by_rebuild(mc, g.n)) return;
-    extern __shared__ int32_t delta[];  // k
-    const int64_t units = (int64_t)nwide * slices;
-    for (int64_t x = blockIdx.x; x < units; x += gridDim.x) {
-        const int32_t r = (int32_t)(x / slices), q = (int32_t)(x % slices);
-        const vid_t u = wide[r];
-        if (!hub_dirty[u]) continue;
-        for (int32_t p = threadIdx.x; p < k; p += blockDim.x) delta[p] = 0;
-        __syncthreads();
-        const eoff_t re = g.rowptr[u + 1];
-        for (eoff_t j = g.rowptr[u] + (eoff_t)q * blockDim.x + threadIdx.x; j < re; j += (eoff_t)slices * blockDim.x) {
-            const vid_t v = g.col[j];
-            if (stamp[v] == gen) atomicAdd(&delta[ADD ? dest[v] : part[v]], g.ew[j]);
-        }
-        __syncthreads();
-        for (int32_t p = threadIdx.x; p < k; p += blockDim.x)
-            if (delta[p]) atomicAdd(&gdelta[(int64_t)r * k + p], delta[p]);
-        __syncthreads();
-    }
-}
-
-template <bool ADD>
-__global__ void __launch_bounds__(256)
-    k_wide_apply(const vid_t* wide, int32_t nwide, const int32_t* mc, vid_t n, uint8_t* hub_dirty,
-                 const eoff_t* conn_off, int2* conn, int32_t* conn_size, part_t* dest_cache, int32_t* gdelta,
-                 int32_t k, int32_t* err, const int32_t* deferred) {
-    if (*deferred || update_by_rebuild(mc, n)) return;
-    for (int32_t r = blockIdx.x; r < nwide; r += gridDim.x) {
-        const vid_t u = wide[r];
-        if (!hub_dirty[u]) continue;
-        int2* tab = conn + conn_off[u];
-        const int32_t cap = (int32_t)(conn_off[u + 1] - conn_off[u]);
-        const int32_t vsz = conn_size[u];
-        int32_t* gd = gdelta + (int64_t)r * k;
-        for (int32_t p = threadIdx.x; p < k; p += blockDim.x) {
-            const int32_t w = gd[p];
-            if (w == 0) continue;
-            gd[p] = 0;
-            if (ADD) conn_add(tab, &conn_size[u], cap, p, w, err);
-            else conn_sub(tab, vsz, p, w, k, err);
-        }
-        if (ADD && threadIdx.x == 0) {
-            dest_cache[u] = NULL_PART;
-            hub_dirty[u] = 0;
-        }
-        __syncthreads();
-    }
-}
 
 // ---------------------------------------------------------------- rebalancing
 // Rebalancer minibuckets (jet_refiner.hpp:359-412, :449-481, :632-643).
@@ -1795,7 +1730,6 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
     cur_part_ = a.alloc<part_t>(N);
     tier_mid_ = a.alloc<vid_t>(N);
     tier_big_ = a.alloc<vid_t>(N);
-    wide_ = a.alloc<vid_t>(N);
     hub_ = a.alloc<uint8_t>(N);
     hub_dirty_ = a.alloc<uint8_t>(N);
     pull_mid_ = a.alloc<vid_t>(N);
@@ -1837,29 +1771,20 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
 
 void Refiner::build_tiers(const Level& L) {
     DGraph g = L.view();
-    fill_i32(counters_ + 16, 0, 5, s_);
+    fill_i32(counters_ + 16, 0, 4, s_);
     // on small levels every row with edges takes the wave tier: a thread per
     // row walks its edges one after another, a wave inserts them all at once
     small_max_ = g.n <= tuning().small_level_n ? 0 : SMALL_DEG;
-    k_conn_tiers<<<grid_for(g.n, 256, 8192), 256, 0, s_>>>(g, p_.k, tier_mid_, tier_big_, pull_mid_, wide_,
-                                                            counters_ + 16, hub_, small_max_,
-                                                            tuning().hub_wide_deg);
+    k_conn_tiers<<<grid_for(g.n, 256, 8192), 256, 0, s_>>>(g, p_.k, tier_mid_, tier_big_, pull_mid_,
+                                                            counters_ + 16, hub_, small_max_);
     k_max_degree<<<grid_for(g.n, 256, 512), 256, 0, s_>>>(g, counters_ + 18);
     JET_LAUNCH_CHECK();
-    int32_t c[5];
-    read_i32_words(counters_ + 16, 5, c, s_);
+    int32_t c[4];
+    read_i32_words(counters_ + 16, 4, c, s_);
     ntier_mid_ = c[0];
     ntier_big_ = c[1];
     has_big_ = c[2] > BIG_DEG;
     npull_mid_ = c[3];
-    nwide_ = c[4];
-    // the wide rows' per-part totals: zero, and kept zero by k_wide_apply
-    const int64_t need = (int64_t)nwide_ * p_.k;
-    if (need > gdelta_cap_) {
-        gdelta_cap_ = std::max<int64_t>(need, 2 * gdelta_cap_);
-        gdelta_ = a_.alloc<int32_t>(gdelta_cap_);
-        fill_i32(gdelta_, 0, gdelta_cap_, s_);
-    }
 }
 
 void Refiner::init_conn(const Level& L, const part_t* part, const int32_t* mark) {
@@ -2190,23 +2115,6 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
             else k_update_hubs<false, WAVE><<<hb, WAVE, lds, s_>>>(g, pull_mid_, npull_mid_, c, stamp_, gen_,
                                                                    hub_dirty_, hub_, part, dest_, conn_off_, conn_,
                                                                    conn_size_, dest_cache_, k, misc_ + 1, misc_ + 2);
-        }
-        if (nwide_ > 0) {
-            const int32_t slices = (int32_t)tuning().hub_wide_slices;
-            const size_t lds = sizeof(int32_t) * (size_t)k;
-            const int sb = (int)std::min<int64_t>((int64_t)nwide_ * slices, 4096);
-            const int ab = std::min(nwide_, 2048);
-            if (add) {
-                k_wide_scan<true><<<sb, 256, lds, s_>>>(g, wide_, nwide_, slices, c, stamp_, gen_, hub_dirty_, part,
-                                                        dest_, gdelta_, k, misc_ + 2);
-                k_wide_apply<true><<<ab, 256, 0, s_>>>(wide_, nwide_, c, g.n, hub_dirty_, conn_off_, conn_,
-                                                       conn_size_, dest_cache_, gdelta_, k, misc_ + 1, misc_ + 2);
-            } else {
-                k_wide_scan<false><<<sb, 256, lds, s_>>>(g, wide_, nwide_, slices, c, stamp_, gen_, hub_dirty_, part,
-                                                         dest_, gdelta_, k, misc_ + 2);
-                k_wide_apply<false><<<ab, 256, 0, s_>>>(wide_, nwide_, c, g.n, hub_dirty_, conn_off_, conn_,
-                                                        conn_size_, dest_cache_, gdelta_, k, misc_ + 1, misc_ + 2);
-            }
         }
         prof_end(PF_UPDATE, s_);
         JET_LAUNCH_CHECK();

@@ -36,8 +36,6 @@ const Field kFields[] = {
     {"ab_lanes_small", &Tuning::ab_lanes_small, 1, 4},
     {"upd_lanes", &Tuning::upd_lanes, 1, 4},
     {"upd_big_slices", &Tuning::upd_big_slices, 1, 64},
-    {"hub_wide_deg", &Tuning::hub_wide_deg, 2048, int64_t(1) << 40},
-    {"hub_wide_slices", &Tuning::hub_wide_slices, 1, 256},
     {"lp_one_pass_n", &Tuning::lp_one_pass_n, 0, int64_t(1) << 31},
     {"ct_wave_max", &Tuning::ct_wave_max, 1, 128},
     {"ct_block_max", &Tuning::ct_block_max, 1, 4096},

@@ -25,10 +25,6 @@ struct Tuning {
     int64_t upd_lanes = 2;
     // moves of degree above 64 are spread over this many waves each
     int64_t upd_big_slices = 16;
-    // hub rows of higher degree take their updates through a scan split
-    // into hub_wide_slices workgroups per row and per-part global totals
-    int64_t hub_wide_deg = 4096;
-    int64_t hub_wide_slices = 16;
     // levels of up to this many vertices run LP select with one pass per wave
     int64_t lp_one_pass_n = 524288;
     // contraction: coarse rows of up to ct_wave_max fine entries take the

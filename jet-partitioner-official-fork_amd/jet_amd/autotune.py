@@ -27,8 +27,6 @@ SPACE = {
     "ab_lanes_small": [1, 2, 4],
     "upd_lanes": [1, 2, 4],
     "upd_big_slices": [1, 4, 16, 64],
-    "hub_wide_deg": [4096, 16384, 1 << 40],
-    "hub_wide_slices": [4, 16, 64],
     "lp_one_pass_n": [131072, 524288, 2097152],
     "ct_wave_max": [32, 64, 128],
     "ct_block_max": [1024, 4096],

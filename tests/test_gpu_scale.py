@@ -56,3 +56,19 @@ def test_rmat22_k256_equals_golden():
             [r[0], r[2], r[3], r[4], r[5]], L
     sizes = np.bincount(part, minlength=256)
     assert sizes.max() <= int(((len(part) + 255) // 256) * 1.10)
+
+
+def test_rmat21_k256_same_partition_every_run():
+    """Three partitions of R-MAT scale 21 (k=256) in one process are identical:
+    the multilevel initial partition refines host-built levels with thousands
+    of hub rows, and the later calls reuse the device memory of the first."""
+    say("rmat-21 k=256 HEC: generating")
+    g = jet_amd.rmat(21, 16, seed=0)
+    cfg = jet_amd.Config(num_parts=256, max_imb_ratio=1.10, coarsening_alg=1)
+    runs = []
+    for rep in range(3):
+        part, cut = jet_amd.partition_csr(g, cfg, seed=0, uniform=True)
+        digest = hashlib.sha256(np.ascontiguousarray(part, dtype=np.int32).tobytes()).hexdigest()
+        say("run", rep, "cut", cut, digest[:16])
+        runs.append((cut, digest))
+    assert runs[0] == runs[1] == runs[2]
