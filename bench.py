@@ -134,12 +134,14 @@ def edge_cut(g, part):
 
 def kernel_rooflines(step, args, ms_per_step, torch):
     """Per-family rooflines of the partition: for each kernel family, the
-    same --steps partitions again with every unit of that family bracketed by
-    HIP events on the stream its kernels run on (average unit time), then one
-    partition in counting mode for every family's algorithmic bytes (the
-    computation is deterministic, so its units are the timed ones). Returns
-    (roofline of the family with the most kernel time, all families,
-    end-to-end figure: modeled bytes of all families / partition time)."""
+    same --steps partitions again with every launch of that family made with
+    its own start and end HIP events (hipExtLaunchKernelGGL on the refinement
+    stream: the interval is the kernel's execution, as in a profiler's
+    dispatch record), then one partition in counting mode for every family's
+    algorithmic bytes (the computation is deterministic, so its launches are
+    the timed ones). Returns (roofline of the family with the most kernel
+    time, all families, end-to-end figure: modeled bytes of all families /
+    partition time)."""
     fams = {}
     for fam in jet_amd.PROF_FAMILIES:
         jet_amd.prof_enable(fam)

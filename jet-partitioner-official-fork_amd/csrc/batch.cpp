@@ -9,7 +9,9 @@
 // arenas: nothing is shared between jobs, so a job's result is the same alone
 // and in any mix.
 #include <hip/hip_runtime.h>
+#include <sched.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -21,6 +23,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "initpart.h"
 #include "jet_capi.h"
 #include "jet_types.h"
 
@@ -83,6 +86,14 @@ int64_t footprint_of(int64_t n, int64_t m) { return 128 * n + 48 * m + (int64_t(
 
 void worker_loop(jet_ctx* X, jet::Context* E) {
     (void)hipSetDevice(X->device);
+    // the workers' initial partitions share the host's CPUs
+    {
+        int cpus = 0;
+        cpu_set_t cs;
+        if (sched_getaffinity(0, sizeof(cs), &cs) == 0) cpus = CPU_COUNT(&cs);
+        if (cpus < 1) cpus = (int)std::thread::hardware_concurrency();
+        jet_init::set_thread_budget(std::max(1, cpus / std::max<int>(1, (int)X->engines.size())));
+    }
     std::unique_lock<std::mutex> lk(X->mu);
     while (true) {
         // FIFO admission: the head job starts once it fits beside the running

@@ -825,6 +825,10 @@ static void recurse(const Sub& s, int32_t k, int32_t first, double ub, uint64_t 
     }
 }
 
+// Threads of this host thread's partitions (0: no cap), set by the batched
+// layer so that concurrent jobs share the host's CPUs.
+thread_local int t_thread_budget = 0;
+
 static int host_threads() {
     // the CPUs this thread may run on (a pinned process gets no more threads
     // than it has CPUs), else the machine's count
@@ -837,12 +841,15 @@ static int host_threads() {
         int v = std::atoi(e);
         if (v >= 1) hw = v;
     }
+    if (t_thread_budget > 0) hw = std::min(hw, t_thread_budget);
     return std::min(hw, JET_HOST_THREADS_MAX);
 }
 
 }  // namespace
 
 namespace jet_init {
+
+void set_thread_budget(int threads) { t_thread_budget = threads > 0 ? threads : 0; }
 
 std::vector<HostLevel> init_hierarchy(int32_t n, const int64_t* xadj, const int32_t* adj, const int32_t* adjw,
                                       const int32_t* vw, int32_t k, uint64_t seed) {

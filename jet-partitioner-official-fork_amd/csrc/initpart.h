@@ -28,6 +28,10 @@ inline bool wants_hierarchy(int32_t n, int64_t m, int32_t k) {
     return m >= INIT_ML_EDGES && (int64_t)n > 8 * (int64_t)k;
 }
 
+// Caps the host threads that the initial partitioner uses for calls made on
+// the calling thread (0: no cap). Results do not depend on the thread count.
+void set_thread_budget(int threads);
+
 // The levels coarser than the graph (first = one level coarser), down to
 // about max(1000, 8 k) vertices or until a level shrinks by less than 10%.
 std::vector<HostLevel> init_hierarchy(int32_t n, const int64_t* xadj, const int32_t* adj, const int32_t* adjw,

@@ -154,6 +154,37 @@ __global__ void __launch_bounds__(SB) k_scan_down(const Tin* in, Tout* out, int6
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = (Tout)(b0 + total);
 }
 
+// Arrays of up to SC_MAX items (the rebalancers' bucket sums on large
+// levels): one workgroup, each thread a contiguous chunk of at most SC_PER
+// items loaded at once, one workgroup scan of the chunk totals, then the
+// chunk written. One launch instead of three; in place is safe, as every
+// thread reads its own chunk before writing it and the chunks are disjoint.
+constexpr int SC_BLOCK = 1024;
+constexpr int SC_PER = 32;
+constexpr int64_t SC_MAX = (int64_t)SC_BLOCK * SC_PER;
+
+template <class Tin, class Tout>
+__global__ void __launch_bounds__(SC_BLOCK) k_scan_chunked(const Tin* in, Tout* out, int64_t n) {
+    __shared__ int64_t ws[SC_BLOCK / WAVE + 1];
+    const int64_t per = (n + SC_BLOCK - 1) / SC_BLOCK;  // <= SC_PER
+    const int64_t b = (int64_t)threadIdx.x * per;
+    int64_t v[SC_PER];
+    int64_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < SC_PER; j++) {
+        v[j] = j < per && b + j < n ? (int64_t)in[b + j] : 0;
+        sum += v[j];
+    }
+    int64_t total;
+    int64_t carry = block_excl_scan<int64_t, SC_BLOCK>(sum, total, ws);
+#pragma unroll
+    for (int j = 0; j < SC_PER; j++) {
+        if (j < per && b + j < n) out[b + j] = (Tout)carry;
+        carry += v[j];
+    }
+    if (threadIdx.x == 0) out[n] = (Tout)total;
+}
+
 template <class Tin, class Tout>
 static void excl_scan_impl(const Tin* in, Tout* out, int64_t n, Arena& a, hipStream_t s) {
     if (n <= 0) {
@@ -164,6 +195,11 @@ static void excl_scan_impl(const Tin* in, Tout* out, int64_t n, Arena& a, hipStr
     int64_t nb = ceil_div(n, STILE);
     if (nb == 1) {
         k_scan_down<Tin, Tout><<<1, SB, 0, s>>>(in, out, n, nullptr);
+        JET_LAUNCH_CHECK();
+        return;
+    }
+    if (n <= SC_MAX) {
+        k_scan_chunked<Tin, Tout><<<1, SC_BLOCK, 0, s>>>(in, out, n);
         JET_LAUNCH_CHECK();
         return;
     }

@@ -16,10 +16,10 @@ struct Prof {
     int timed = -1;                       // family timed with events
     bool counting[PF_FAMILIES] = {};      // families counted
     double host_bytes[PF_FAMILIES] = {};  // bytes known on the host
-    int64_t units[PF_FAMILIES] = {};      // units accounted (counting)
+    int64_t units[PF_FAMILIES] = {};      // launches counted (counting)
     unsigned long long* dev = nullptr;    // device counters, PROF_COUNTERS per family
-    std::vector<hipEvent_t> ev;           // 2 per timed unit
-    int64_t nev = 0;                      // timed units recorded
+    std::vector<hipEvent_t> ev;           // 2 per timed launch
+    int64_t nev = 0;                      // timed launches made
     int dev_id = -1;                      // device the events and counters belong to
 };
 
@@ -70,24 +70,22 @@ unsigned long long* prof_counters(int fam) {
 
 bool prof_counting(int fam) { return fam >= 0 && fam < PF_FAMILIES && P().counting[fam]; }
 
-void prof_begin(int fam, hipStream_t s) {
+ProfEvents prof_launch(int fam) {
     Prof& p = P();
-    if (p.timed != fam || p.nev >= PROF_MAX_UNITS) return;
-    JET_HIP(hipEventRecord(p.ev[2 * p.nev], s));
-}
-
-void prof_end(int fam, hipStream_t s) {
-    Prof& p = P();
-    if (p.timed != fam || p.nev >= PROF_MAX_UNITS) return;
-    JET_HIP(hipEventRecord(p.ev[2 * p.nev + 1], s));
+    ProfEvents e;
+    if (fam < 0 || fam >= PF_FAMILIES) return e;
+    if (p.counting[fam]) p.units[fam]++;
+    if (p.timed != fam || p.nev >= PROF_MAX_UNITS) return e;
+    e.start = p.ev[2 * p.nev];
+    e.stop = p.ev[2 * p.nev + 1];
     p.nev++;
+    return e;
 }
 
-void prof_account(int fam, double host_bytes) {
+void prof_bytes(int fam, double host_bytes) {
     Prof& p = P();
     if (fam < 0 || fam >= PF_FAMILIES || !p.counting[fam]) return;
     p.host_bytes[fam] += host_bytes;
-    p.units[fam]++;
 }
 
 int prof_timed_family() { return P().timed; }

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <sched.h>
 
 #include "prims.h"
 #include "prof.h"
@@ -841,9 +842,8 @@ __global__ void __launch_bounds__(256)
 template <bool ADD, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
     k_update_hubs(DGraph g, const vid_t* list, int32_t nlist, const int32_t* mc, const int32_t* stamp, int32_t gen,
-                  uint8_t* hub_dirty, const uint8_t* hub, const part_t* part, const part_t* dest,
-                  const eoff_t* conn_off, int2* conn, int32_t* conn_size, part_t* dest_cache, int32_t k, int32_t* err,
-                  const int32_t* deferred) {
+                  uint8_t* hub_dirty, const part_t* part, const part_t* dest, const eoff_t* conn_off, int2* conn,
+                  int32_t* conn_size, part_t* dest_cache, int32_t k, int32_t* err, const int32_t* deferred) {
     if (*deferred || update_by_rebuild(mc, g.n)) return;
     extern __shared__ int32_t delta[];  // k
     for (int32_t r = blockIdx.x; r < nlist; r += gridDim.x) {
@@ -1824,32 +1824,28 @@ void Refiner::jet_lp(const Level& L, const part_t* part, double fr) {
     gen_++;  // this step's moves are stamped with a new generation
     // a pass per wave on levels that fill the GPU with single-pass waves, two
     // passes per wave in 16-wave workgroups above that (see LS_MULTI_CHW)
-    prof_begin(PF_LP_SELECT, s_);
     if (g.n <= tuning().lp_one_pass_n) {
         int nb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_WAVES * WAVE));
-        k_lp_select<WAVE, LS_WAVES><<<nb, LS_WAVES * WAVE, 0, s_>>>(
-            g, part, conn_off_, conn_, conn_size_, dest_cache_, gain_p_, lock_, fr, dest_, pregain_, cand_, c, nmax_,
-            cnt_next(), prof_counters(PF_LP_SELECT));
+        family_launch(PF_LP_SELECT, k_lp_select<WAVE, LS_WAVES>, dim3(nb), dim3(LS_WAVES * WAVE), 0, s_, g, part,
+                      conn_off_, conn_, conn_size_, dest_cache_, gain_p_, lock_, fr, dest_, pregain_, cand_, c, nmax_,
+                      cnt_next(), prof_counters(PF_LP_SELECT));
     } else {
         int nb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_MULTI_WAVES * LS_MULTI_CHW));
-        k_lp_select<LS_MULTI_CHW, LS_MULTI_WAVES><<<nb, LS_MULTI_WAVES * WAVE, 0, s_>>>(
-            g, part, conn_off_, conn_, conn_size_, dest_cache_, gain_p_, lock_, fr, dest_, pregain_, cand_, c, nmax_,
-            cnt_next(), prof_counters(PF_LP_SELECT));
+        family_launch(PF_LP_SELECT, k_lp_select<LS_MULTI_CHW, LS_MULTI_WAVES>, dim3(nb), dim3(LS_MULTI_WAVES * WAVE),
+                      0, s_, g, part, conn_off_, conn_, conn_size_, dest_cache_, gain_p_, lock_, fr, dest_, pregain_,
+                      cand_, c, nmax_, cnt_next(), prof_counters(PF_LP_SELECT));
     }
-    prof_end(PF_LP_SELECT, s_);
     JET_LAUNCH_CHECK();
     // per vertex: cached destination 4, lock 4
-    prof_account(PF_LP_SELECT, 8.0 * g.n);
+    prof_bytes(PF_LP_SELECT, 8.0 * g.n);
     if (unsigned long long* pc = prof_counters(PF_AFTERBURNER)) {
         // counting runs only: candidate degrees (the list is complete here)
         k_prof_degsum<<<grid_for(g.n, 256, 1024), 256, 0, s_>>>(g, cand_, c, 0, 1, nmax_, nullptr, pc);
-        prof_account(PF_AFTERBURNER, 0);
     }
-    prof_begin(PF_AFTERBURNER, s_);
     unsigned ab = (unsigned)std::max<int64_t>(256, ceil_div((int64_t)g.n, (int64_t)AB_WAVES * AB_CHW));
-#define JET_AB(GG)                                                                                          \
-    k_afterburner_small<GG><<<ab, AB_WAVES * WAVE, 0, s_>>>(g, cand_, c, part, dest_, pregain_, lock_, moves_, c, \
-                                                            nmax_, stamp_, gen_)
+#define JET_AB(GG)                                                                                                    \
+    family_launch(PF_AFTERBURNER, k_afterburner_small<GG>, dim3(ab), dim3(AB_WAVES * WAVE), 0, s_, g, cand_, c, part, \
+                  dest_, pregain_, lock_, moves_, c, nmax_, stamp_, gen_)
     switch (group_for_afterburner(g)) {
         case 4: JET_AB(4); break;
         case 8: JET_AB(8); break;
@@ -1859,10 +1855,8 @@ void Refiner::jet_lp(const Level& L, const part_t* part, double fr) {
     }
 #undef JET_AB
     if (has_big_)
-        k_afterburner_big<<<grid_for((int64_t)g.n * WAVE, 256, 2048), 256, 0, s_>>>(g, cand_, c, part, dest_, pregain_,
-                                                                                   lock_, moves_, c, nmax_, stamp_,
-                                                                                   gen_);
-    prof_end(PF_AFTERBURNER, s_);
+        family_launch(PF_AFTERBURNER, k_afterburner_big, dim3(grid_for((int64_t)g.n * WAVE, 256, 2048)), dim3(256), 0,
+                      s_, g, cand_, c, part, dest_, pregain_, lock_, moves_, c, nmax_, stamp_, gen_);
     JET_LAUNCH_CHECK();
 }
 
@@ -1991,18 +1985,11 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
             k_strong_select<false><<<nsel, selb, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, acc_, conn_off_,
                                                          conn_, conn_size_, sections, bid_, bk, bc, copies, nbuckets_);
     } else {
-        prof_begin(PF_WEAK_SELECT, s_);
-        if (lds)
-            k_weak_select<true><<<nsel, selb, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, undersized_, misc_,
-                                                      conn_off_, conn_, conn_size_, sections, dest_, bid_, bk, bc,
-                                                      copies, nbuckets_, prof_counters(PF_WEAK_SELECT));
-        else
-            k_weak_select<false><<<nsel, selb, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, undersized_, misc_,
-                                                       conn_off_, conn_, conn_size_, sections, dest_, bid_, bk, bc,
-                                                       copies, nbuckets_, prof_counters(PF_WEAK_SELECT));
-        prof_end(PF_WEAK_SELECT, s_);
+        family_launch(PF_WEAK_SELECT, lds ? k_weak_select<true> : k_weak_select<false>, dim3(nsel), dim3(selb), 0,
+                      s_, g, part, sizes, k, smax, opt_, max_dest, undersized_, misc_, conn_off_, conn_, conn_size_,
+                      sections, dest_, bid_, bk, bc, copies, nbuckets_, prof_counters(PF_WEAK_SELECT));
         // per vertex: part 4, destination 4, bucket id 4
-        prof_account(PF_WEAK_SELECT, 12.0 * g.n);
+        prof_bytes(PF_WEAK_SELECT, 12.0 * g.n);
     }
     if (copies > 1)
         k_sum_bucket_copies<<<grid_for(nbk + 1, 256, 2048), 256, 0, s_>>>(buckets_, bcopies_, copies, nbk + 1,
@@ -2038,6 +2025,10 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
 void Refiner::wait_published(volatile int64_t* res, int64_t seq) {
     for (uint64_t spins = 1;; spins++) {
         if (res[5] == seq) return;
+        __builtin_ia32_pause();
+        // a wait longer than about half a millisecond (the device is busy with
+        // other work, e.g. concurrent jobs) gives the CPU to other threads
+        if (spins >= (1u << 14) && spins % 64 == 0) sched_yield();
         if (spins % 4096 == 0) {
             hipError_t e = hipStreamQuery(s_);
             if (e == hipSuccess) {
@@ -2066,17 +2057,14 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
             // counting runs only: move degrees, once per pass (nothing when
             // the step's ranking was deferred: its kernels then do nothing)
             k_prof_degsum<<<grid_for(g.n, 256, 1024), 256, 0, s_>>>(g, moves_, c, 2, 3, nmax_, misc_ + 2, pc);
-            prof_account(PF_UPDATE, 0);
         }
-        prof_begin(PF_UPDATE, s_);
 #define JET_UPD(GG)                                                                                                   \
-    if (add) k_update_small<true, GG><<<nbs + 1, 256, 0, s_>>>(g, moves_, mrec_, c, nmax_, part, dest_, conn_off_,    \
-                                                              conn_, conn_size_, dest_cache_, k, misc_ + 1,           \
-                                                              swap_bit_, hub_, hub_dirty_, misc_ + 2, stamp_, gen_,   \
-                                                              usz, pub_, pb);                                         \
-    else k_update_small<false, GG><<<nbs, 256, 0, s_>>>(g, moves_, mrec_, c, nmax_, part, dest_, conn_off_, conn_,    \
-                                                        conn_size_, dest_cache_, k, misc_ + 1, swap_bit_, hub_,       \
-                                                        hub_dirty_, misc_ + 2, stamp_, gen_, usz, pub_, pb);
+    if (add) family_launch(PF_UPDATE, k_update_small<true, GG>, dim3(nbs + 1), dim3(256), 0, s_, g, moves_, mrec_, c, \
+                           nmax_, part, dest_, conn_off_, conn_, conn_size_, dest_cache_, k, misc_ + 1, swap_bit_,    \
+                           hub_, hub_dirty_, misc_ + 2, stamp_, gen_, usz, pub_, pb);                                 \
+    else family_launch(PF_UPDATE, k_update_small<false, GG>, dim3(nbs), dim3(256), 0, s_, g, moves_, mrec_, c, nmax_, \
+                       part, dest_, conn_off_, conn_, conn_size_, dest_cache_, k, misc_ + 1, swap_bit_, hub_,         \
+                       hub_dirty_, misc_ + 2, stamp_, gen_, usz, pub_, pb);
         switch (G) {
             case 4: JET_UPD(4) break;
             case 8: JET_UPD(8) break;
@@ -2087,36 +2075,25 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
 #undef JET_UPD
         if (has_big_) {
             const int32_t slices = (int32_t)tuning().upd_big_slices;
-            if (add) k_update_big<true><<<nbb, 256, 0, s_>>>(g, moves_, mrec_, c, nmax_, part, dest_, conn_off_, conn_,
-                                                             conn_size_, dest_cache_, k, misc_ + 1, swap_bit_, hub_,
-                                                             hub_dirty_, misc_ + 2, stamp_, gen_, usz, pub_, slices);
-            else k_update_big<false><<<nbb, 256, 0, s_>>>(g, moves_, mrec_, c, nmax_, part, dest_, conn_off_, conn_,
-                                                           conn_size_, dest_cache_, k, misc_ + 1, swap_bit_, hub_,
-                                                           hub_dirty_, misc_ + 2, stamp_, gen_, usz, pub_, slices);
+            family_launch(PF_UPDATE, add ? k_update_big<true> : k_update_big<false>, dim3(nbb), dim3(256), 0, s_, g,
+                          moves_, mrec_, c, nmax_, part, dest_, conn_off_, conn_, conn_size_, dest_cache_, k,
+                          misc_ + 1, swap_bit_, hub_, hub_dirty_, misc_ + 2, stamp_, gen_, usz, pub_, slices);
         }
         // pulled rows: big-tier rows by a 256-thread workgroup, rows of
         // degree (BIG_DEG, HEAVY_DEG_REFINE] by a single-wave workgroup
-        const size_t lds = sizeof(int32_t) * (size_t)k;
+        const uint32_t lds = (uint32_t)(sizeof(int32_t) * (size_t)k);
         if (ntier_big_ > 0) {
             const int hb = std::min(ntier_big_, 2048);
-            if (add) k_update_hubs<true, 256><<<hb, 256, lds, s_>>>(g, tier_big_, ntier_big_, c, stamp_, gen_,
-                                                                    hub_dirty_, hub_, part, dest_, conn_off_, conn_,
-                                                                    conn_size_, dest_cache_, k, misc_ + 1, misc_ + 2);
-            else k_update_hubs<false, 256><<<hb, 256, lds, s_>>>(g, tier_big_, ntier_big_, c, stamp_, gen_,
-                                                                 hub_dirty_, hub_, part, dest_, conn_off_, conn_,
-                                                                 conn_size_, dest_cache_, k, misc_ + 1, misc_ + 2);
+            family_launch(PF_UPDATE, add ? k_update_hubs<true, 256> : k_update_hubs<false, 256>, dim3(hb), dim3(256),
+                          lds, s_, g, tier_big_, ntier_big_, c, stamp_, gen_, hub_dirty_, part, dest_, conn_off_,
+                          conn_, conn_size_, dest_cache_, k, misc_ + 1, misc_ + 2);
         }
         if (npull_mid_ > 0) {
             const int hb = std::min(npull_mid_, 8192);
-            if (add) k_update_hubs<true, WAVE><<<hb, WAVE, lds, s_>>>(g, pull_mid_, npull_mid_, c, stamp_, gen_,
-                                                                      hub_dirty_, hub_, part, dest_, conn_off_, conn_,
-                                                                      conn_size_, dest_cache_, k, misc_ + 1,
-                                                                      misc_ + 2);
-            else k_update_hubs<false, WAVE><<<hb, WAVE, lds, s_>>>(g, pull_mid_, npull_mid_, c, stamp_, gen_,
-                                                                   hub_dirty_, hub_, part, dest_, conn_off_, conn_,
-                                                                   conn_size_, dest_cache_, k, misc_ + 1, misc_ + 2);
+            family_launch(PF_UPDATE, add ? k_update_hubs<true, WAVE> : k_update_hubs<false, WAVE>, dim3(hb),
+                          dim3(WAVE), lds, s_, g, pull_mid_, npull_mid_, c, stamp_, gen_, hub_dirty_, part, dest_,
+                          conn_off_, conn_, conn_size_, dest_cache_, k, misc_ + 1, misc_ + 2);
         }
-        prof_end(PF_UPDATE, s_);
         JET_LAUNCH_CHECK();
     }
     volatile int64_t* res = res_host_;
