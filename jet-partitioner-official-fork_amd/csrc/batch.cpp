@@ -26,6 +26,7 @@
 #include "initpart.h"
 #include "jet_capi.h"
 #include "jet_types.h"
+#include "tuning.h"
 
 namespace {
 
@@ -94,6 +95,7 @@ void worker_loop(jet_ctx* X, jet::Context* E) {
         if (cpus < 1) cpus = (int)std::thread::hardware_concurrency();
         jet_init::set_thread_budget(std::max(1, cpus / std::max<int>(1, (int)X->engines.size())));
     }
+    jet::set_batch_thread(true);
     std::unique_lock<std::mutex> lk(X->mu);
     while (true) {
         // FIFO admission: the head job starts once it fits beside the running

@@ -2169,7 +2169,7 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
     }
     {
         const RebalanceShape wr = rebalance_shape(g.n, false);
-        if (!trace && wg_fits(g.n, k, wr.nbk, tuning().wg_max_n) && (int)temps.size() <= WG_MAX_TEMPS) {
+        if (!trace && wg_fits(g.n, k, wr.nbk, wg_threshold()) && (int)temps.size() <= WG_MAX_TEMPS) {
             LevelRecord rec = refine_wg(L, best_part, temps);
             rec.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             return rec;

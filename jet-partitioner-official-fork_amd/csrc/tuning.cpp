@@ -22,6 +22,14 @@ Tuning& tuning() {
 }
 
 namespace {
+thread_local bool t_batch_thread = false;
+}
+
+void set_batch_thread(bool on) { t_batch_thread = on; }
+
+int64_t wg_threshold() { return t_batch_thread ? tuning().batch_wg_max_n : tuning().wg_max_n; }
+
+namespace {
 struct Field {
     const char* name;
     int64_t Tuning::*member;
@@ -31,6 +39,7 @@ struct Field {
 // within the limits it is built for (LDS table loads of at most one half)
 const Field kFields[] = {
     {"wg_max_n", &Tuning::wg_max_n, 0, 1 << 20},
+    {"batch_wg_max_n", &Tuning::batch_wg_max_n, 0, 1 << 20},
     {"wg_big_deg", &Tuning::wg_big_deg, 0, 1 << 30},
     {"small_level_n", &Tuning::small_level_n, 0, 1 << 30},
     {"ab_lanes_small", &Tuning::ab_lanes_small, 1, 4},

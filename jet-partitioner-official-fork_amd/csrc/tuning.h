@@ -12,6 +12,10 @@ struct Tuning {
     // levels of up to this many vertices are refined by the one-workgroup
     // kernel (0: never)
     int64_t wg_max_n = 0;
+    // the same threshold for jobs of the batched and served layer, whose
+    // many concurrent small jobs are bound by kernel launches more than by
+    // kernel time (the one-workgroup kernel runs a level in one launch)
+    int64_t batch_wg_max_n = 0;
     // in that kernel, candidates and moves of higher degree are taken by a
     // whole wave instead of one thread
     int64_t wg_big_deg = 32;
@@ -38,6 +42,11 @@ struct Tuning {
 };
 
 Tuning& tuning();
+// Marks the calling thread as a worker of the batched and served layer.
+void set_batch_thread(bool on);
+// The one-workgroup refinement threshold for the calling thread: wg_max_n,
+// or batch_wg_max_n on batch worker threads.
+int64_t wg_threshold();
 
 // Applies a tuning table (lines "name value"; '#' starts a comment). Returns
 // the number of settings applied, or -1 when the file cannot be read or holds
