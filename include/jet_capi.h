@@ -327,7 +327,7 @@ int jet_cut_heatmap(int64_t n, const int64_t* row_offsets, const int32_t* col, c
  * private state, so its result equals the same call made alone. The
  * reference has no batched or served mode: these entry points have no
  * counterpart in it and serve callers that partition many graphs. */
-#define JET_MAX_WORKERS 16
+#define JET_MAX_WORKERS 64
 
 typedef struct jet_ctx jet_ctx_t;
 
@@ -338,7 +338,7 @@ typedef struct jet_job_times_t {
     double end_s;
 } jet_job_times_t;
 
-/* workers in [1, 16]; budget_bytes <= 0 means 80% of the device's free memory. */
+/* workers in [1, JET_MAX_WORKERS]; budget_bytes <= 0 means 80% of the device's free memory. */
 int jet_ctx_create(int32_t device, int32_t workers, int64_t budget_bytes, jet_ctx_t** out);
 /* Waits for running jobs, then frees the context (uncollected results are
  * dropped). No jet_submit, jet_poll or jet_partition_batch call on the same
@@ -405,13 +405,15 @@ int jet_tuning_load(const char* path);
 
 /* ---- Kernel-family timing and algorithmic bytes for measurement tools (no
  * counterpart in the reference). Families: "lp_select", "afterburner",
- * "update" (one unit = one pass of update kernels), "weak_select".
- * jet_prof_enable("<family>") times every later unit of that family with a
- * pair of HIP events on the stream its kernels run on; "<family>:count" or
- * "all:count" instead counts the items each family's algorithmic bytes follow
- * from (a separate run: counting adds work); "none" stops. jet_prof_read
- * returns, for an enabled family, its units and their total milliseconds
- * (timing) or its algorithmic bytes (counting). */
+ * "update", "weak_select"; a unit is one kernel launch of the family.
+ * jet_prof_enable("<family>") times every later launch of that family with a
+ * pair of HIP events that the launch records at its kernel's start and end
+ * (hipExtLaunchKernelGGL), so the time is the kernels' execution;
+ * "<family>:count" or "all:count" instead counts the launches and the items
+ * each family's algorithmic bytes follow from (a separate run: counting adds
+ * work); "none" stops. jet_prof_read returns, for an enabled family, its
+ * launches and their total milliseconds (timing) or its launches and
+ * algorithmic bytes (counting). */
 int jet_prof_enable(const char* family);
 int jet_prof_read(const char* family, int64_t* units, double* total_ms, double* bytes);
 

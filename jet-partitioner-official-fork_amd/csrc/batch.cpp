@@ -12,11 +12,13 @@
 #include <sched.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <mutex>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -24,6 +26,7 @@
 
 #include "engine.h"
 #include "initpart.h"
+#include "refine_wg.h"
 #include "jet_capi.h"
 #include "jet_types.h"
 #include "tuning.h"
@@ -57,6 +60,112 @@ struct Job {
     double t_submit = 0, t_start = 0, t_end = 0;
 };
 
+// Gathers the one-workgroup level runs of a context's concurrent jobs into
+// graph-tagged launches (k_refine_wg_multi): a run waits until every running
+// job has a run pending, the batch is full, or COLLECT_WAIT has passed, and
+// whichever waiting job finds the batch ready launches it on the collector's
+// own stream for all of them. Each workgroup reads only its own level's
+// arrays, so a job's result does not depend on what it was batched with.
+class Collector final : public jet::WgCollector {
+  public:
+    static constexpr int MAX_BATCH = 1024;
+    static constexpr auto COLLECT_WAIT = std::chrono::microseconds(300);
+
+    explicit Collector(const std::atomic<int>* running) : running_(running) {}
+    ~Collector() override {
+        if (d_ps_) (void)hipFree(d_ps_);
+        if (d_st_) (void)hipFree(d_st_);
+        if (h_ps_) (void)hipHostFree(h_ps_);
+        if (h_st_) (void)hipHostFree(h_st_);
+        if (stream_) (void)hipStreamDestroy(stream_);
+    }
+
+    void run(const jet::WgParams& p, hipStream_t job_stream, jet::WgState& out) override {
+        JET_HIP(hipStreamSynchronize(job_stream));  // the level's tables and state
+        Req r{&p, &out};
+        std::unique_lock<std::mutex> lk(mu_);
+        pending_.push_back(&r);
+        cv_.notify_all();
+        const auto deadline = clk::now() + COLLECT_WAIT;
+        while (!r.done) {
+            const bool ready = (int)pending_.size() >= running_->load() || (int)pending_.size() >= MAX_BATCH ||
+                               clk::now() >= deadline;
+            if (!launching_ && !pending_.empty() && ready) {
+                launching_ = true;
+                std::vector<Req*> batch;
+                batch.swap(pending_);
+                lk.unlock();
+                std::string err;
+                try {
+                    launch(batch);
+                } catch (const std::exception& e) {
+                    err = e.what();
+                }
+                lk.lock();
+                launching_ = false;
+                for (Req* q : batch) {
+                    q->err = err;
+                    q->done = true;
+                }
+                cv_.notify_all();
+            } else if (launching_ || clk::now() >= deadline) {
+                cv_.wait(lk);
+            } else {
+                cv_.wait_until(lk, deadline);
+            }
+        }
+        if (!r.err.empty()) throw jet::DeviceError(r.err);
+    }
+
+  private:
+    struct Req {
+        const jet::WgParams* p;
+        jet::WgState* out;
+        bool done = false;
+        std::string err;
+    };
+
+    void launch(const std::vector<Req*>& batch) {
+        const int n = (int)batch.size();
+        if (!stream_) JET_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        if (n > cap_) {
+            if (d_ps_) JET_HIP(hipFree(d_ps_));
+            if (d_st_) JET_HIP(hipFree(d_st_));
+            if (h_ps_) JET_HIP(hipHostFree(h_ps_));
+            if (h_st_) JET_HIP(hipHostFree(h_st_));
+            d_ps_ = nullptr;
+            d_st_ = nullptr;
+            h_ps_ = nullptr;
+            h_st_ = nullptr;
+            cap_ = 0;
+            const int c = std::max(n, 64);
+            JET_HIP(hipMalloc(&d_ps_, sizeof(jet::WgParams) * c));
+            JET_HIP(hipMalloc(&d_st_, sizeof(jet::WgState) * c));
+            JET_HIP(hipHostMalloc(&h_ps_, sizeof(jet::WgParams) * c));
+            JET_HIP(hipHostMalloc(&h_st_, sizeof(jet::WgState) * c));
+            cap_ = c;
+        }
+        for (int i = 0; i < n; i++) h_ps_[i] = *batch[i]->p;
+        JET_HIP(hipMemcpyAsync(d_ps_, h_ps_, sizeof(jet::WgParams) * n, hipMemcpyHostToDevice, stream_));
+        jet::launch_refine_wg_multi(h_ps_, d_ps_, d_st_, n, stream_);
+        JET_HIP(hipMemcpyAsync(h_st_, d_st_, sizeof(jet::WgState) * n, hipMemcpyDeviceToHost, stream_));
+        JET_HIP(hipStreamSynchronize(stream_));
+        for (int i = 0; i < n; i++) *batch[i]->out = h_st_[i];
+    }
+
+    const std::atomic<int>* running_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::vector<Req*> pending_;
+    bool launching_ = false;
+    hipStream_t stream_ = nullptr;
+    jet::WgParams* d_ps_ = nullptr;
+    jet::WgState* d_st_ = nullptr;
+    jet::WgParams* h_ps_ = nullptr;
+    jet::WgState* h_st_ = nullptr;
+    int cap_ = 0;
+};
+
 }  // namespace
 
 struct jet_ctx {
@@ -72,6 +181,8 @@ struct jet_ctx {
     int64_t next_ticket = 1;
     int64_t in_use = 0;
     int running = 0;
+    std::atomic<int> running_jobs{0};  // `running`, readable without mu
+    Collector collector{&running_jobs};
     bool stop = false;
 
     double now() const { return std::chrono::duration<double>(clk::now() - origin).count(); }
@@ -96,6 +207,7 @@ void worker_loop(jet_ctx* X, jet::Context* E) {
         jet_init::set_thread_budget(std::max(1, cpus / std::max<int>(1, (int)X->engines.size())));
     }
     jet::set_batch_thread(true);
+    jet::set_wg_collector(&X->collector);
     std::unique_lock<std::mutex> lk(X->mu);
     while (true) {
         // FIFO admission: the head job starts once it fits beside the running
@@ -112,6 +224,7 @@ void worker_loop(jet_ctx* X, jet::Context* E) {
         X->queue.pop_front();
         X->in_use += j->footprint;
         X->running++;
+        X->running_jobs.store(X->running);
         j->t_start = X->now();
         lk.unlock();
         int64_t cut = 0;
@@ -128,6 +241,7 @@ void worker_loop(jet_ctx* X, jet::Context* E) {
         j->done = true;
         X->in_use -= j->footprint;
         X->running--;
+        X->running_jobs.store(X->running);
         X->cv_done.notify_all();
         X->cv_work.notify_all();
     }
@@ -188,7 +302,7 @@ int jet_ctx_create(int32_t device, int32_t workers, int64_t budget_bytes, jet_ct
     }
     *out = nullptr;
     if (workers < 1 || workers > JET_MAX_WORKERS) {
-        jet::set_last_error("workers must be in [1, 16]");
+        jet::set_last_error("workers must be in [1, 64]");
         return JET_EINVAL;
     }
     jet_ctx* X = new jet_ctx();

@@ -2244,6 +2244,7 @@ LevelRecord Refiner::refine_wg(const Level& L, part_t* best_part, const std::vec
     P.g = g;
     P.k = k;
     P.big_deg = (int32_t)tuning().wg_big_deg;
+    P.lanes = group_for_avg(g);  // the average degree rounded up to a power of two (4-64)
     P.sections = weak.sections;
     P.nbk = (int32_t)weak.nbk;
     P.opt = opt_;
@@ -2278,9 +2279,13 @@ LevelRecord Refiner::refine_wg(const Level& L, part_t* best_part, const std::vec
     WgState h{};
     for (int launches = 0;; launches++) {
         if (launches > 100000) throw DeviceError("one-workgroup refinement made no progress");
-        launch_refine_wg(P, s_);
-        JET_HIP(hipMemcpyAsync(&h, wg_state_, sizeof(h), hipMemcpyDeviceToHost, s_));
-        JET_HIP(hipStreamSynchronize(s_));
+        if (WgCollector* col = wg_collector()) {
+            col->run(P, s_, h);  // graph-tagged with the levels of other jobs
+        } else {
+            launch_refine_wg(P, s_);
+            JET_HIP(hipMemcpyAsync(&h, wg_state_, sizeof(h), hipMemcpyDeviceToHost, s_));
+            JET_HIP(hipStreamSynchronize(s_));
+        }
         if (h.err) throw DeviceError(refine_error_text(h.err));
         if (h.status == WG_STALLED) throw DeviceError("one-workgroup refinement exceeded its iteration bound");
         if (h.status != WG_REBUILD) break;

@@ -25,6 +25,7 @@ struct WgParams {
     DGraph g;
     int32_t k, sections;
     int32_t big_deg;  // items of higher degree are taken by a whole wave
+    int32_t lanes;    // lanes per smaller item: a power of two up to a wave
     int32_t nbk;  // minibuckets scanned (the weak layout, the larger of the two)
     int64_t opt, size_max, imb_max, max_dest_weak, max_dest_strong;
     double tol;
@@ -62,5 +63,23 @@ bool wg_fits(int64_t n, int32_t k, int64_t nbk, int64_t max_n);
 void wg_init_state(WgState* st, int64_t cut, int64_t imb, hipStream_t s);
 // One launch of the kernel on stream s.
 void launch_refine_wg(const WgParams& p, hipStream_t s);
+// Gathers the one-workgroup level runs of concurrent jobs into graph-tagged
+// launches (the batched and served layer has one per context). run() first
+// waits for the job's stream (the level's tables and state are built there),
+// and returns the level's final state once its workgroup has finished.
+class WgCollector {
+  public:
+    virtual ~WgCollector() = default;
+    virtual void run(const WgParams& p, hipStream_t job_stream, WgState& out) = 0;
+};
+// The collector of the calling thread's level runs (none: each run is its own
+// launch on the job's stream).
+void set_wg_collector(WgCollector* c);
+WgCollector* wg_collector();
+
+// One graph-tagged launch: workgroup i runs the level of dev_ps[i] (a device
+// copy of host_ps[i]) and writes its final state to dev_states[i].
+void launch_refine_wg_multi(const WgParams* host_ps, const WgParams* dev_ps, WgState* dev_states, int count,
+                            hipStream_t s);
 
 }  // namespace jet

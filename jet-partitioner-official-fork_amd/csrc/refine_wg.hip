@@ -30,6 +30,7 @@
 // a whole wave instead of one thread.
 #include "refine_wg.h"
 
+#include <algorithm>
 #include <mutex>
 
 #include "conn_table.h"
@@ -40,6 +41,13 @@ namespace jet {
 constexpr int WG_BLOCK = 1024;
 constexpr int WG_WAVES = WG_BLOCK / WAVE;
 constexpr int32_t WG_ITER_CAP = 1 << 20;
+
+namespace {
+thread_local WgCollector* t_collector = nullptr;
+}
+
+void set_wg_collector(WgCollector* c) { t_collector = c; }
+WgCollector* wg_collector() { return t_collector; }
 
 bool wg_fits(int64_t n, int32_t k, int64_t nbk, int64_t max_n) {
     return n >= 1 && n <= max_n && k >= 1 && k <= WG_MAX_K && nbk <= WG_MAX_NBK;
@@ -76,17 +84,31 @@ __device__ inline void list_push(vid_t* list, vid_t n, int32_t* cnt_small, int32
 }
 
 // Sum of f over the edges of the listed items: small items (front, ns of
-// them) one thread each, big items (back, nb of them) one wave each;
-// item(i, rs, re, sub, lanes) returns the item's partial sum. Returns each
-// thread's share.
+// them) G lanes each (a power of two up to a wave: the lanes of a group split
+// the item's edges, so its loads are in flight together), big items (back,
+// nb of them) one wave each; item(i, sub, lanes) returns the lane's partial
+// sum. Returns each thread's share.
 template <class F>
-__device__ inline int64_t over_items(const vid_t* list, vid_t n, int32_t ns, int32_t nb, F&& item) {
+__device__ inline int64_t over_items(const vid_t* list, vid_t n, int32_t ns, int32_t nb, int G, F&& item) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int wv = threadIdx.x / WAVE;
+    const int gsub = threadIdx.x & (G - 1);
+    const int32_t gid = threadIdx.x / G, ngroups = WG_BLOCK / G;
     int64_t acc = 0;
-    for (int32_t x = threadIdx.x; x < ns; x += WG_BLOCK) acc += item(list[x], 0, 1);
+    for (int32_t x = gid; x < ns; x += ngroups) acc += item(list[x], gsub, G);
     for (int32_t x = wv; x < nb; x += WG_WAVES) acc += item(list[n - 1 - x], lane, WAVE);
     return acc;
+}
+
+// Sum of v over the G lanes of each aligned group (G a power of two up to a
+// wave), in every lane of the group.
+__device__ inline int64_t group_sum(int64_t v, int G) {
+    for (int o = 1; o < G; o <<= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)(uint64_t)v, o, WAVE);
+        const uint32_t hi = __shfl_xor((uint32_t)((uint64_t)v >> 32), o, WAVE);
+        v += (int64_t)(((uint64_t)hi << 32) | lo);
+    }
+    return v;
 }
 
 // Exclusive scan of a[0..len) in place (LDS), whole workgroup; returns the total.
@@ -184,7 +206,9 @@ __device__ inline int32_t walk_chunks(const int64_t* es, int32_t lo, int32_t hi,
 
 }  // namespace
 
-__global__ void __launch_bounds__(WG_BLOCK) k_refine_wg(WgParams P) {
+// The level loop of one workgroup over the level P describes; the final
+// controller state goes to *P.st and, when mirror is set, to *mirror.
+__device__ __forceinline__ void refine_wg_body(const WgParams& P, WgState* mirror) {
     extern __shared__ int32_t sbk[];  // minibucket sums, then their scan
     __shared__ WgShared L;
     const DGraph g = P.g;
@@ -290,11 +314,22 @@ __global__ void __launch_bounds__(WG_BLOCK) k_refine_wg(WgParams P) {
                     change += afterburn_edge(g.col[j], i, ig, g.ew[j], p, best, P.part, P.dest, P.pregain);
                 return change;
             };
-            for (int32_t x = t; x < ns; x += WG_BLOCK) {
-                const vid_t i = P.cand[x];
-                const bool keep = (int64_t)P.pregain[i] + burn(i, 0, 1) >= 0;
-                if (keep) P.lock[i] = 1;
-                list_push(P.moves, n, &L.cnt[2], &L.cnt[3], keep, false, i);
+            {
+                // small candidates: a group of P.lanes lanes each (the whole
+                // workgroup takes the same number of rounds, so the group
+                // sums and the list appends see every lane)
+                const int G = P.lanes;
+                const int gsub = t & (G - 1);
+                const int32_t gid = t / G, ngroups = WG_BLOCK / G;
+                for (int32_t x0 = 0; x0 < ns; x0 += ngroups) {
+                    const int32_t x = x0 + gid;
+                    const bool have = x < ns;
+                    const vid_t i = have ? P.cand[x] : 0;
+                    const int64_t change = group_sum(have ? burn(i, gsub, G) : 0, G);
+                    const bool keep = have && gsub == 0 && (int64_t)P.pregain[i] + change >= 0;
+                    if (keep) P.lock[i] = 1;
+                    list_push(P.moves, n, &L.cnt[2], &L.cnt[3], keep, false, i);
+                }
             }
             for (int32_t x = wv; x < nb; x += WG_WAVES) {
                 const vid_t i = P.cand[n - 1 - x];
@@ -592,7 +627,7 @@ __global__ void __launch_bounds__(WG_BLOCK) k_refine_wg(WgParams P) {
             }
             return c;
         };
-        int64_t cc = over_items(P.moves, n, ms, mb, [&](vid_t v, int sub, int lanes) -> int64_t {
+        int64_t cc = over_items(P.moves, n, ms, mb, P.lanes, [&](vid_t v, int sub, int lanes) -> int64_t {
             const part_t pold = P.part[v], pnew = P.dest[v];
             if (sub == 0) {
                 P.dest_cache[v] = NULL_PART;
@@ -613,7 +648,7 @@ __global__ void __launch_bounds__(WG_BLOCK) k_refine_wg(WgParams P) {
         __syncthreads();
         // the cut change after the moves; update_small's subtractions
         // (:785-812) or the rows to rebuild marked (update_large, :743-756)
-        cc += over_items(P.moves, n, ms, mb, [&](vid_t v, int sub, int lanes) -> int64_t {
+        cc += over_items(P.moves, n, ms, mb, P.lanes, [&](vid_t v, int sub, int lanes) -> int64_t {
             const part_t pnew = P.part[v], pold = P.dest[v];
             int64_t c = 0;
             const eoff_t re = g.rowptr[v + 1];
@@ -630,7 +665,7 @@ __global__ void __launch_bounds__(WG_BLOCK) k_refine_wg(WgParams P) {
         __syncthreads();
         if (!rebuild) {
             // update_small's additions (:815-886)
-            over_items(P.moves, n, ms, mb, [&](vid_t v, int sub, int lanes) -> int64_t {
+            over_items(P.moves, n, ms, mb, P.lanes, [&](vid_t v, int sub, int lanes) -> int64_t {
                 const part_t pnew = P.part[v];
                 const eoff_t re = g.rowptr[v + 1];
                 for (eoff_t j = g.rowptr[v] + sub; j < re; j += lanes) {
@@ -691,7 +726,17 @@ __global__ void __launch_bounds__(WG_BLOCK) k_refine_wg(WgParams P) {
     if (t == 0) {
         L.S.err = ld_rx(err);  // table errors the steps reported (atomics, not cached)
         *P.st = L.S;
+        if (mirror) *mirror = L.S;
     }
+}
+
+__global__ void __launch_bounds__(WG_BLOCK) k_refine_wg(WgParams P) { refine_wg_body(P, nullptr); }
+
+// Graph-tagged: workgroup b runs the level that ps[b] describes (levels of
+// different graphs, each with its own arrays), and copies its final state to
+// states[b].
+__global__ void __launch_bounds__(WG_BLOCK) k_refine_wg_multi(const WgParams* ps, WgState* states) {
+    refine_wg_body(ps[blockIdx.x], states + blockIdx.x);
 }
 
 __global__ void k_wg_init(WgState* st, int64_t cut, int64_t imb) {
@@ -706,24 +751,45 @@ void wg_init_state(WgState* st, int64_t cut, int64_t imb, hipStream_t s) {
     JET_LAUNCH_CHECK();
 }
 
-void launch_refine_wg(const WgParams& p, hipStream_t s) {
-    // the bucket array may take the kernel past 64 KB of LDS (gfx950 has 160
-    // KB per workgroup): allowed once per device
-    static std::once_flag once[64];
-    static hipError_t attr_rc[64];
+// The dynamic LDS of the bucket array may take the kernels past 64 KB
+// (gfx950 has 160 KB per workgroup): allowed once per device and kernel.
+static void allow_wg_lds(const void* kernel, int which) {
+    static std::once_flag once[2][64];
+    static hipError_t attr_rc[2][64];
     int dev = 0;
     JET_HIP(hipGetDevice(&dev));
     if (dev < 0 || dev >= 64) throw DeviceError("device index beyond 64");
-    std::call_once(once[dev], [&] {
-        attr_rc[dev] = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_refine_wg),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)(sizeof(int32_t) * (WG_MAX_NBK + 1)));
+    std::call_once(once[which][dev], [&] {
+        attr_rc[which][dev] = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                  (int)(sizeof(int32_t) * (WG_MAX_NBK + 1)));
     });
-    JET_HIP(attr_rc[dev]);
+    JET_HIP(attr_rc[which][dev]);
+}
+
+static void check_wg_params(const WgParams& p) {
     if (p.nbk < 1 || p.nbk > WG_MAX_NBK || p.k < 1 || p.k > WG_MAX_K || p.ntemps > WG_MAX_TEMPS)
         throw DeviceError("one-workgroup refinement: level outside the kernel's limits");
+}
+
+void launch_refine_wg(const WgParams& p, hipStream_t s) {
+    allow_wg_lds(reinterpret_cast<const void*>(&k_refine_wg), 0);
+    check_wg_params(p);
     const size_t dyn = sizeof(int32_t) * (size_t)(p.nbk + 1);
     k_refine_wg<<<1, WG_BLOCK, dyn, s>>>(p);
+    JET_LAUNCH_CHECK();
+}
+
+void launch_refine_wg_multi(const WgParams* host_ps, const WgParams* dev_ps, WgState* dev_states, int count,
+                            hipStream_t s) {
+    if (count <= 0) return;
+    allow_wg_lds(reinterpret_cast<const void*>(&k_refine_wg_multi), 1);
+    int32_t nbk = 1;
+    for (int i = 0; i < count; i++) {
+        check_wg_params(host_ps[i]);
+        nbk = std::max(nbk, host_ps[i].nbk);
+    }
+    const size_t dyn = sizeof(int32_t) * (size_t)(nbk + 1);
+    k_refine_wg_multi<<<count, WG_BLOCK, dyn, s>>>(dev_ps, dev_states);
     JET_LAUNCH_CHECK();
 }
 
