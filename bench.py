@@ -430,7 +430,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="stencil3d-128", choices=sorted(WORKLOADS) + ["batch", "serve"])
     ap.add_argument("--graphs", type=int, default=512, help="batch: number of graphs")
-    ap.add_argument("--workers", type=int, default=4, help="batch/serve: concurrent GPU workers")
+    ap.add_argument("--workers", type=int, default=None,
+                    help="batch/serve: concurrent GPU workers (default 16)")
     ap.add_argument("--requests", type=int, default=400, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrival rate per second")
     ap.add_argument("--pool", type=int, default=16, help="serve: distinct graphs requests draw from")
@@ -469,8 +470,12 @@ def main():
         print(json.dumps({"copy_bytes_read": 4 << 28, "copy_bytes_written": 4 << 28, "launches": 3}), flush=True)
         return
     if args.workload == "batch":
+        if args.workers is None:
+            args.workers = 16
         return run_batch(args)
     if args.workload == "serve":
+        if args.workers is None:
+            args.workers = 16
         return run_serve(args)
     wl = WORKLOADS[args.workload]
     t_gen = time.time()

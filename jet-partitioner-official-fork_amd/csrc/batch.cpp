@@ -206,8 +206,10 @@ void worker_loop(jet_ctx* X, jet::Context* E) {
         if (cpus < 1) cpus = (int)std::thread::hardware_concurrency();
         jet_init::set_thread_budget(std::max(1, cpus / std::max<int>(1, (int)X->engines.size())));
     }
-    jet::set_batch_thread(true);
-    jet::set_wg_collector(&X->collector);
+    if ((int)X->engines.size() >= jet::BATCH_WG_MIN_WORKERS) {
+        jet::set_batch_thread(true);
+        jet::set_wg_collector(&X->collector);
+    }
     std::unique_lock<std::mutex> lk(X->mu);
     while (true) {
         // FIFO admission: the head job starts once it fits beside the running

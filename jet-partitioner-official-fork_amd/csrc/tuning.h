@@ -12,10 +12,11 @@ struct Tuning {
     // levels of up to this many vertices are refined by the one-workgroup
     // kernel (0: never)
     int64_t wg_max_n = 0;
-    // the same threshold for jobs of the batched and served layer, whose
-    // many concurrent small jobs are bound by kernel launches more than by
-    // kernel time (the one-workgroup kernel runs a level in one launch)
-    int64_t batch_wg_max_n = 0;
+    // the same threshold for jobs of the batched and served layer on contexts
+    // of at least BATCH_WG_MIN_WORKERS workers, whose many concurrent small
+    // jobs are bound by kernel launches more than by kernel time (their
+    // one-workgroup levels are launched together, one launch per round)
+    int64_t batch_wg_max_n = 2048;
     // in that kernel, candidates and moves of higher degree are taken by a
     // whole wave instead of one thread
     int64_t wg_big_deg = 32;
@@ -42,6 +43,9 @@ struct Tuning {
 };
 
 Tuning& tuning();
+// Contexts with fewer workers gather too few levels per launch for the
+// one-workgroup kernel to pay: their jobs refine as single calls do.
+constexpr int BATCH_WG_MIN_WORKERS = 8;
 // Marks the calling thread as a worker of the batched and served layer.
 void set_batch_thread(bool on);
 // The one-workgroup refinement threshold for the calling thread: wg_max_n,

@@ -49,6 +49,22 @@ def test_batch_equals_single_calls():
         assert np.array_equal(parts[i], p), i
 
 
+@pytest.mark.parametrize("wg_max", [2048, 16384])
+def test_graph_tagged_levels_equal_single_calls(wg_max):
+    """16 workers: the levels of up to wg_max vertices of all running jobs are
+    refined together by the graph-tagged one-workgroup kernel; every result
+    must still equal the plain call's (which uses the multi-kernel path)."""
+    jobs = mixed_jobs(40, seed=3)
+    alone = [jet_amd.partition_csr(g, cfg, seed=s) for g, cfg, s in jobs]
+    say(f"batch: 40 graphs, 16 workers, batch_wg_max_n {wg_max}")
+    with jet_amd.tuned(batch_wg_max_n=wg_max):
+        with jet_amd.Context(workers=16) as ctx:
+            parts, cuts = ctx.batch([j[0] for j in jobs], [j[1] for j in jobs], [j[2] for j in jobs])
+    for i, (p, c) in enumerate(alone):
+        assert cuts[i] == c, i
+        assert np.array_equal(parts[i], p), i
+
+
 def test_batch_matches_oracle():
     jobs = mixed_jobs(6, seed=2, max_n=4000)
     with jet_amd.Context(workers=3) as ctx:
