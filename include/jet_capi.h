@@ -383,7 +383,7 @@ int jet_partition_batch(jet_ctx_t* ctx, int32_t count, const jet_config_t* cfgs,
  *   workgroup that runs the whole iteration loop (0 = never);
  * "batch_wg_max_n": the same threshold for jobs of jet_partition_batch and
  *   jet_submit on contexts of at least 8 workers, whose one-workgroup levels
- *   are launched together (default 2048);
+ *   are launched together (0 = never, the default);
  * "wg_big_deg": in that kernel, items above this degree take a whole wave;
  * "small_level_n": levels treated as small (wave-built connectivity rows,
  *   wider afterburner lane groups, 128-thread rebalancer select workgroups);

@@ -16,7 +16,7 @@ struct Tuning {
     // of at least BATCH_WG_MIN_WORKERS workers, whose many concurrent small
     // jobs are bound by kernel launches more than by kernel time (their
     // one-workgroup levels are launched together, one launch per round)
-    int64_t batch_wg_max_n = 2048;
+    int64_t batch_wg_max_n = 0;
     // in that kernel, candidates and moves of higher degree are taken by a
     // whole wave instead of one thread
     int64_t wg_big_deg = 32;
