@@ -96,7 +96,10 @@ class Refiner {
     void wait_published(volatile int64_t* res, int64_t seq);
     // rebuilds the rows an update_large step marked, if any
     void flush_rebuild(const Level& L);
-    bool perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_t& cut_change, int64_t& max_size);
+    // next_strong >= 0: the second pass also prepares the start of a next
+    // rebalancing step of that kind (strong 1, weak 0), so it needs no reset
+    bool perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_t& cut_change, int64_t& max_size,
+                       int next_strong = -1);
     // Two counter sets alternate between iterations: the first kernel of an
     // iteration clears the other set, so no fill launch is needed.
     int32_t* cnt() const { return counters_ + 8 * par_; }
@@ -137,6 +140,8 @@ class Refiner {
     // producers write it; every selection step takes a new generation)
     int32_t* stamp_;
     int32_t gen_ = 0;
+    // the last update pass prepared a rebalancing step of kind prep_strong_
+    bool prep_ready_ = false, prep_strong_ = false;
     vid_t* pull_mid_;  // mid-tier rows of degree > BIG_DEG
     int32_t ntier_mid_ = 0, ntier_big_ = 0, npull_mid_ = 0;
     bool has_big_ = false;

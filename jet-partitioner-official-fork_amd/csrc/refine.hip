@@ -593,6 +593,51 @@ __device__ inline void add_cut_part2(int64_t* partials, int64_t c2, int64_t* red
 // ticket (a ticket in a many-workgroup kernel needs an agent-scope release
 // fence per workgroup, which writes back L2), and the kernel boundary before
 // it orders the other kernels' atomics.
+// The state a rebalancing step starts from, besides zeroed bucket sums: the
+// undersized parts in increasing id (weak, jet_refiner.hpp:578-591) or the
+// largest receiver capacity and cleared eviction adjustments (strong,
+// :440-448), from the part sizes before the step. One workgroup of 256.
+__device__ inline void rebalance_prep_block(const int64_t* sizes, int32_t k, int64_t max_dest, bool strong,
+                                            part_t* undersized, int32_t* misc, int64_t* acc, int64_t* evict_adjust,
+                                            int64_t* red, int32_t* ws) {
+    if (strong) {
+        int64_t mx = INT64_MIN;
+        for (int t = threadIdx.x; t < k; t += blockDim.x) {
+            int64_t sz = sizes[t];
+            if (sz < max_dest && max_dest - sz > mx) mx = max_dest - sz;
+            evict_adjust[t] = 0;
+        }
+        mx = block_reduce_max<int64_t, 256>(mx, red);
+        if (threadIdx.x == 0) acc[3] = mx;
+    } else {
+        int32_t base = 0;
+        for (int32_t p0 = 0; p0 < k; p0 += 256) {
+            int32_t p = p0 + threadIdx.x;
+            int32_t f = (p < k && sizes[p] < max_dest) ? 1 : 0;
+            int32_t tot;
+            int32_t ex = block_excl_scan<int32_t, 256>(f, tot, ws);
+            if (f) undersized[base + ex] = p;
+            base += tot;
+        }
+        if (threadIdx.x == 0) misc[0] = base;
+    }
+}
+
+// What the second update pass prepares for the next step in case it is a
+// rebalancing step of the given kind (buckets == nullptr: nothing): its
+// bucket sums zeroed (nb entries) and rebalance_prep_block's state, so that
+// the step needs no reset launch.
+struct RebalPrep {
+    int64_t* buckets;
+    int64_t nb;
+    int64_t max_dest;
+    int32_t strong;
+    part_t* undersized;
+    int32_t* misc;
+    int64_t* acc;
+    int64_t* evict_adjust;
+};
+
 struct Publish {
     int64_t* partials;
     const int64_t* sizes;
@@ -707,8 +752,9 @@ __global__ void __launch_bounds__(256)
                    const part_t* dest, const eoff_t* conn_off, int2* conn, int32_t* conn_size, part_t* dest_cache,
                    int32_t k, int32_t* err, int32_t* mark, const uint8_t* hub, uint8_t* hub_dirty,
                    const int32_t* deferred, const int32_t* stamp, int32_t gen, unsigned long long* sizes,
-                   int64_t* partials, Publish pb) {
+                   int64_t* partials, Publish pb, RebalPrep prep) {
     __shared__ int64_t red[256 / WAVE];
+    __shared__ int32_t ws[ADD ? 256 / WAVE + 1 : 1];
     __shared__ unsigned long long ls[ADD ? 1 : LDS_K];
     const int sub = threadIdx.x & (G - 1);
     // the second pass's move workgroups are 1 .. gridDim.x - 1
@@ -726,9 +772,18 @@ __global__ void __launch_bounds__(256)
     // completes the results, has finished)
     if (ADD && blockIdx.x == 0) {
         publish_step(pb, dfr != 0, red);
+        // then the next rebalancing step's start (its kernels run after this one)
+        if (!dfr && prep.buckets)
+            rebalance_prep_block(pb.sizes, pb.k, prep.max_dest, prep.strong != 0, prep.undersized, prep.misc,
+                                 prep.acc, prep.evict_adjust, red, ws);
         return;
     }
     if (dfr) return;
+    if (ADD && prep.buckets) {
+        // the move workgroups zero the next rebalancing step's bucket sums (the
+        // current step's readers have all run before the first pass)
+        for (int64_t i = (blk * blockDim.x) + threadIdx.x; i < prep.nb; i += nblk * blockDim.x) prep.buckets[i] = 0;
+    }
     const bool rebuild = ns + nbig > g.n / 10;  // update_by_rebuild
     if (!ADD && rebuild && blockIdx.x == 0 && threadIdx.x == 0) mc[6] = 1;  // the tables are to be rebuilt
     const bool lds = !ADD && k <= LDS_K;
@@ -897,27 +952,7 @@ __global__ void __launch_bounds__(256)
     if (threadIdx.x < 8) next_counters[threadIdx.x] = 0;
     __shared__ int32_t ws[256 / WAVE + 1];
     __shared__ int64_t red[256 / WAVE];
-    if (strong) {
-        int64_t mx = INT64_MIN;
-        for (int t = threadIdx.x; t < k; t += blockDim.x) {
-            int64_t sz = sizes[t];
-            if (sz < max_dest && max_dest - sz > mx) mx = max_dest - sz;
-            evict_adjust[t] = 0;
-        }
-        mx = block_reduce_max<int64_t, 256>(mx, red);
-        if (threadIdx.x == 0) acc[3] = mx;
-    } else {
-        int32_t base = 0;
-        for (int32_t p0 = 0; p0 < k; p0 += 256) {
-            int32_t p = p0 + threadIdx.x;
-            int32_t f = (p < k && sizes[p] < max_dest) ? 1 : 0;
-            int32_t tot;
-            int32_t ex = block_excl_scan<int32_t, 256>(f, tot, ws);
-            if (f) undersized[base + ex] = p;
-            base += tot;
-        }
-        if (threadIdx.x == 0) misc[0] = base;
-    }
+    rebalance_prep_block(sizes, k, max_dest, strong != 0, undersized, misc, acc, evict_adjust, red, ws);
 }
 
 // Per-workgroup LDS map of bucket sums (a workgroup takes a contiguous range
@@ -1020,7 +1055,8 @@ __global__ void __launch_bounds__(SEL_BLOCK_MAX)
                   int64_t max_dest, const part_t* undersized, const int32_t* misc, const eoff_t* conn_off,
                   const int2* conn, const int32_t* conn_size, int32_t sections, part_t* dest, int32_t* bid,
                   unsigned long long* buckets_base, unsigned long long* extra, int32_t copies, int64_t stride,
-                  unsigned long long* pc) {
+                  unsigned long long* pc, int32_t* next_counters) {
+    if (blockIdx.x == 0 && threadIdx.x < 8) next_counters[threadIdx.x] = 0;
     unsigned long long* buckets = bucket_copy(buckets_base, extra, copies, stride);
     __shared__ int32_t bk[BH_SLOTS];
     __shared__ unsigned long long bv[BH_SLOTS];
@@ -1093,7 +1129,8 @@ __global__ void __launch_bounds__(SEL_BLOCK_MAX)
     k_strong_select(DGraph g, const part_t* part, const int64_t* sizes, int32_t k, int64_t size_max, int32_t opt,
                     int64_t max_dest, const int64_t* acc, const eoff_t* conn_off, const int2* conn,
                     const int32_t* conn_size, int32_t sections, int32_t* bid, unsigned long long* buckets_base,
-                    unsigned long long* extra, int32_t copies, int64_t stride) {
+                    unsigned long long* extra, int32_t copies, int64_t stride, int32_t* next_counters) {
+    if (blockIdx.x == 0 && threadIdx.x < 8) next_counters[threadIdx.x] = 0;
     unsigned long long* buckets = bucket_copy(buckets_base, extra, copies, stride);
     __shared__ int32_t bk[BH_SLOTS];
     __shared__ unsigned long long bv[BH_SLOTS];
@@ -1959,9 +1996,13 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
     const int64_t width = r.width, nbk = r.nbk, max_dest = r.max_dest;
     const int64_t smax = size_max_;
     gen_++;  // this step's moves are stamped with a new generation
-    k_reset_buckets<<<grid_for(nbk + 1, 256, 2048), 256, 0, s_>>>(buckets_, nbk + 1, cnt_next(), sizes, k,
-                                                                    max_dest, strong ? 1 : 0, undersized_, misc_, acc_,
-                                                                    evict_adjust_);
+    // the previous step's second update pass prepared this kind of step (bucket
+    // sums zeroed, undersized parts or capacity), else a reset launch does
+    if (!(prep_ready_ && prep_strong_ == strong))
+        k_reset_buckets<<<grid_for(nbk + 1, 256, 2048), 256, 0, s_>>>(buckets_, nbk + 1, cnt_next(), sizes, k,
+                                                                        max_dest, strong ? 1 : 0, undersized_, misc_,
+                                                                        acc_, evict_adjust_);
+    prep_ready_ = false;
     // contiguous ranges of at least 512 vertices per workgroup (128 on small
     // levels, one per thread of a 128-thread workgroup): vertices of one
     // oversized part share few buckets, and the serial per-vertex chain of a
@@ -1980,14 +2021,16 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
     if (strong) {
         if (lds)
             k_strong_select<true><<<nsel, selb, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, acc_, conn_off_,
-                                                         conn_, conn_size_, sections, bid_, bk, bc, copies, nbuckets_);
+                                                         conn_, conn_size_, sections, bid_, bk, bc, copies, nbuckets_,
+                                                         cnt_next());
         else
             k_strong_select<false><<<nsel, selb, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, acc_, conn_off_,
-                                                         conn_, conn_size_, sections, bid_, bk, bc, copies, nbuckets_);
+                                                          conn_, conn_size_, sections, bid_, bk, bc, copies, nbuckets_,
+                                                          cnt_next());
     } else {
         family_launch(PF_WEAK_SELECT, lds ? k_weak_select<true> : k_weak_select<false>, dim3(nsel), dim3(selb), 0,
                       s_, g, part, sizes, k, smax, opt_, max_dest, undersized_, misc_, conn_off_, conn_, conn_size_,
-                      sections, dest_, bid_, bk, bc, copies, nbuckets_, prof_counters(PF_WEAK_SELECT));
+                      sections, dest_, bid_, bk, bc, copies, nbuckets_, prof_counters(PF_WEAK_SELECT), cnt_next());
         // per vertex: part 4, destination 4, bucket id 4
         prof_bytes(PF_WEAK_SELECT, 12.0 * g.n);
     }
@@ -2042,7 +2085,8 @@ void Refiner::wait_published(volatile int64_t* res, int64_t seq) {
     }
 }
 
-bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_t& cut_change, int64_t& max_size) {
+bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_t& cut_change, int64_t& max_size,
+                             int next_strong) {
     DGraph g = L.view();
     const int32_t k = p_.k;
     int32_t* c = cnt();
@@ -2051,6 +2095,15 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
     int nbs = grid_for((int64_t)g.n * G, 256, 1024);
     const int64_t seq = ++mailbox_->seq;
     const Publish pb{pub_, sizes, k, res_dev_, c, seq};
+    // the next step's start in case it is a rebalancing step of kind
+    // next_strong (the controller knows the kind before this step's result)
+    RebalPrep prep{};
+    if (next_strong >= 0) {
+        const RebalanceShape nr = rebalance_shape(g.n, next_strong != 0);
+        prep = RebalPrep{buckets_, nr.nbk + 1, nr.max_dest, next_strong, undersized_, misc_, acc_, evict_adjust_};
+    }
+    const RebalPrep no_prep{};
+    prep_ready_ = false;
     int nbb = grid_for((int64_t)g.n * WAVE, 256, 512);
     for (int add = 0; add < 2; add++) {
         if (unsigned long long* pc = prof_counters(PF_UPDATE)) {
@@ -2061,10 +2114,10 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
 #define JET_UPD(GG)                                                                                                   \
     if (add) family_launch(PF_UPDATE, k_update_small<true, GG>, dim3(nbs + 1), dim3(256), 0, s_, g, moves_, mrec_, c, \
                            nmax_, part, dest_, conn_off_, conn_, conn_size_, dest_cache_, k, misc_ + 1, swap_bit_,    \
-                           hub_, hub_dirty_, misc_ + 2, stamp_, gen_, usz, pub_, pb);                                 \
+                           hub_, hub_dirty_, misc_ + 2, stamp_, gen_, usz, pub_, pb, prep);                           \
     else family_launch(PF_UPDATE, k_update_small<false, GG>, dim3(nbs), dim3(256), 0, s_, g, moves_, mrec_, c, nmax_, \
                        part, dest_, conn_off_, conn_, conn_size_, dest_cache_, k, misc_ + 1, swap_bit_, hub_,         \
-                       hub_dirty_, misc_ + 2, stamp_, gen_, usz, pub_, pb);
+                       hub_dirty_, misc_ + 2, stamp_, gen_, usz, pub_, pb, no_prep);
         switch (G) {
             case 4: JET_UPD(4) break;
             case 8: JET_UPD(8) break;
@@ -2101,6 +2154,10 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
     if (res[2]) return true;  // ranking deferred: nothing applied, counters kept
     cut_change = res[0];
     max_size = res[1];
+    if (next_strong >= 0) {
+        prep_ready_ = true;
+        prep_strong_ = next_strong != 0;
+    }
     // update_large: the marked rows are rebuilt before the tables are next
     // read (rebuild_pending), so a step that pushed its updates launches no
     // rebuild kernels
@@ -2128,6 +2185,7 @@ void Refiner::flush_rebuild(const Level& L) {
 LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
     auto t0 = std::chrono::steady_clock::now();
     changed_ = false;
+    prep_ready_ = false;
     DGraph g = L.view();
     const int32_t k = p_.k;
     fill_i32(misc_, 0, 8, s_);
@@ -2193,9 +2251,12 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
                 balance_counter++;
             }
             int64_t cc = 0, mx = 0;
-            if (perform_moves(L, cur_part_, curr_sizes_, cc, mx)) {
+            // a rebalancing step after this one is strong iff two rebalancing
+            // steps precede it (balance_counter as updated for this step)
+            const int next_strong = balance_counter >= 2 ? 1 : 0;
+            if (perform_moves(L, cur_part_, curr_sizes_, cc, mx, next_strong)) {
                 rank_deferred(L, cur_part_, curr_sizes_, strong);
-                if (perform_moves(L, cur_part_, curr_sizes_, cc, mx))
+                if (perform_moves(L, cur_part_, curr_sizes_, cc, mx, next_strong))
                     throw DeviceError("rebalancing step deferred twice");
             }
             curr_imb = mx > opt_ ? mx - opt_ : 0;
@@ -2221,6 +2282,7 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
             }
         }
     }
+    prep_ready_ = false;
     flush_rebuild(L);
     int32_t err = read_scalar(misc_ + 1, s_);
     if (err) throw DeviceError(refine_error_text(err));
