@@ -90,7 +90,9 @@ class Refiner {
     void rank_flagged(const Level& L, const part_t* part, const int64_t* sizes, int64_t width, bool strong);
     void rank_radix(const Level& L, const part_t* part, const int64_t* sizes, int64_t width, int key_bits,
                     bool strong);
-    void strong_tail(const Level& L, const part_t* part, const int64_t* sizes, const RebalanceShape& r);
+    // cookie_done: the ranking workgroup already ran the cookie cutter
+    void strong_tail(const Level& L, const part_t* part, const int64_t* sizes, const RebalanceShape& r,
+                     bool cookie_done);
     void rank_deferred(const Level& L, const part_t* part, const int64_t* sizes, bool strong);
     // Returns true when the step's ranking was deferred (nothing applied).
     void wait_published(volatile int64_t* res, int64_t seq);

@@ -101,6 +101,15 @@ __global__ void k_project(vid_t n, const vid_t* map, const part_t* coarse, part_
     FOR_THREAD(i, n) fine[i] = coarse[map[i]];
 }
 
+// The controller's best state (jet_refiner.hpp:1117-1132): the partition
+// and the part sizes copied in one launch.
+__global__ void k_copy_best(part_t* best_part, const part_t* part, vid_t n, int64_t* best_sizes,
+                            const int64_t* sizes, int32_t k) {
+    FOR_THREAD(i, n) best_part[i] = part[i];
+    if (blockIdx.x == 0)
+        for (int32_t p = threadIdx.x; p < k; p += blockDim.x) best_sizes[p] = sizes[p];
+}
+
 void project(vid_t n_fine, const vid_t* map, const part_t* coarse, part_t* fine, hipStream_t s) {
     if (n_fine <= 0) return;
     k_project<<<grid_for(n_fine, 256, 16384), 256, 0, s>>>(n_fine, map, coarse, fine);
@@ -1314,6 +1323,50 @@ __device__ inline int rank_decide(vid_t v, int32_t b, int64_t pos, const part_t*
 }
 
 
+// The cookie cutter (jet_refiner.hpp:491-518) by one workgroup of BLOCK
+// threads: two scans over the parts give each part's eviction adjustment and
+// the destination chunk starts; the re-draw counter starts after every
+// eviction.
+template <int BLOCK>
+__device__ inline void cookie_block(int32_t k, int64_t width, const int64_t* off, const int64_t* sizes,
+                                    int64_t max_dest, int64_t* evict_adjust, int64_t* evict_start, int64_t* acc,
+                                    int64_t* ws) {
+    int64_t run = 0, run2 = 0;
+    for (int32_t p0 = 0; p0 < k; p0 += BLOCK) {
+        int32_t p = p0 + threadIdx.x;
+        int64_t add = 0, cap = 0, beg = 0;
+        if (p < k) {
+            int64_t begin = (int64_t)p * width;
+            beg = off[begin];
+            add = evict_adjust[p];
+            if (add == 0) add = off[begin + width] - beg;
+            if (max_dest > sizes[p]) cap = max_dest - sizes[p];
+        }
+        int64_t t1, t2;
+        int64_t e1 = block_excl_scan<int64_t, BLOCK>(add, t1, ws);
+        int64_t e2 = block_excl_scan<int64_t, BLOCK>(cap, t2, ws);
+        if (p < k) {
+            evict_adjust[p] = beg - (run + e1);
+            evict_start[p + 1] = run2 + e2 + cap;
+        }
+        run += t1;
+        run2 += t2;
+    }
+    if (threadIdx.x == 0) {
+        acc[3] = run;
+        evict_start[0] = 0;
+    }
+}
+
+// What a strong ranking workgroup also does when it ranks every member
+// itself: the cookie cutter (so the step needs no cookie launch).
+struct CookieArgs {
+    int32_t k;  // 0: not here
+    int64_t max_dest;
+    int64_t* evict_start;
+    int64_t* acc;
+};
+
 constexpr int RS_CAP = 8192;
 constexpr int RS_BLOCK = 1024;
 
@@ -1324,7 +1377,7 @@ __global__ void __launch_bounds__(RS_BLOCK)
     k_rank_small(const vid_t* rlist, const int32_t* counters, const int32_t* bid, const part_t* part,
                  const int64_t* off, const int64_t* sizes, int64_t size_max, int64_t width, const wgt_t* vw,
                  const eoff_t* rowptr, int64_t* evict_adjust, int64_t* score, vid_t* moves, int32_t* mcounters,
-                 vid_t nmax, int32_t* deferred, int32_t* stamp, int32_t gen) {
+                 vid_t nmax, int32_t* deferred, int32_t* stamp, int32_t gen, CookieArgs ck) {
     __shared__ uint64_t keys[RS_CAP];
     __shared__ int64_t pre[RS_CAP];
     __shared__ int64_t ws[RS_BLOCK / WAVE + 1];
@@ -1410,6 +1463,10 @@ __global__ void __launch_bounds__(RS_BLOCK)
             if (code == 2) moves[nmax - 1 - atomicAdd(&mcounters[3], 1)] = v;
             else if (code == 1) moves[atomicAdd(&mcounters[2], 1)] = v;
         }
+        if (STRONG && ck.k > 0) {
+            __syncthreads();  // every member's eviction adjustment written
+            cookie_block<RS_BLOCK>(ck.k, width, off, sizes, ck.max_dest, evict_adjust, ck.evict_start, ck.acc, ws);
+        }
         return;
     } else {
         for (uint32_t t = threadIdx.x; t < cap; t += RS_BLOCK) {
@@ -1467,6 +1524,10 @@ __global__ void __launch_bounds__(RS_BLOCK)
         if (code == 2) moves[nmax - 1 - atomicAdd(&mcounters[3], 1)] = v;
         else if (code == 1) moves[atomicAdd(&mcounters[2], 1)] = v;
     }
+    if (STRONG && ck.k > 0) {
+        __syncthreads();  // every member's eviction adjustment written
+        cookie_block<RS_BLOCK>(ck.k, width, off, sizes, ck.max_dest, evict_adjust, ck.evict_start, ck.acc, ws);
+    }
 }
 
 __global__ void k_rank_keys(const vid_t* rlist, int32_t R, const int32_t* bid, uint64_t* keys, uint32_t* vals) {
@@ -1513,31 +1574,7 @@ __global__ void __launch_bounds__(CK_BLOCK)
              int64_t* evict_adjust, int64_t* evict_start, int64_t* acc, const int32_t* deferred) {
     if (*deferred) return;
     __shared__ int64_t ws[CK_BLOCK / WAVE + 1];
-    int64_t run = 0, run2 = 0;
-    for (int32_t p0 = 0; p0 < k; p0 += CK_BLOCK) {
-        int32_t p = p0 + threadIdx.x;
-        int64_t add = 0, cap = 0, beg = 0;
-        if (p < k) {
-            int64_t begin = (int64_t)p * width;
-            beg = off[begin];
-            add = evict_adjust[p];
-            if (add == 0) add = off[begin + width] - beg;
-            if (max_dest > sizes[p]) cap = max_dest - sizes[p];
-        }
-        int64_t t1, t2;
-        int64_t e1 = block_excl_scan<int64_t, CK_BLOCK>(add, t1, ws);
-        int64_t e2 = block_excl_scan<int64_t, CK_BLOCK>(cap, t2, ws);
-        if (p < k) {
-            evict_adjust[p] = beg - (run + e1);
-            evict_start[p + 1] = run2 + e2 + cap;
-        }
-        run += t1;
-        run2 += t2;
-    }
-    if (threadIdx.x == 0) {
-        acc[3] = run;  // re-draw counter starts after every eviction
-        evict_start[0] = 0;
-    }
+    cookie_block<CK_BLOCK>(k, width, off, sizes, max_dest, evict_adjust, evict_start, acc, ws);
 }
 
 // Last index q in [lo, k] with evict_start[q] <= s (evict_start is non-decreasing).
@@ -1906,13 +1943,17 @@ void Refiner::rank_flagged(const Level& L, const part_t* part, const int64_t* si
     DGraph g = L.view();
     int32_t* c = cnt();
     int64_t smax = size_max_;
-    if (strong)
+    if (strong) {
+        // the ranking workgroup also runs the cookie cutter (unless it defers)
+        const CookieArgs ck{p_.k, rebalance_shape(g.n, true).max_dest, evict_start_, acc_};
         k_rank_small<true><<<1, RS_BLOCK, 0, s_>>>(cand_, c, bid_, part, off_, sizes, smax, width, g.vw, g.rowptr,
-                                                  evict_adjust_, score_, moves_, c, nmax_, misc_ + 2, stamp_, gen_);
-    else
+                                                  evict_adjust_, score_, moves_, c, nmax_, misc_ + 2, stamp_, gen_,
+                                                  ck);
+    } else {
         k_rank_small<false><<<1, RS_BLOCK, 0, s_>>>(cand_, c, bid_, part, off_, sizes, smax, width, g.vw,
                                                    g.rowptr, evict_adjust_, score_, moves_, c, nmax_, misc_ + 2, stamp_,
-                                                   gen_);
+                                                   gen_, CookieArgs{});
+    }
     JET_LAUNCH_CHECK();
 }
 
@@ -1963,13 +2004,15 @@ Refiner::RebalanceShape Refiner::rebalance_shape(vid_t n, bool strong) const {
 }
 
 // rebalance_strong's destination step (jet_refiner.hpp:480-544), after ranking.
-void Refiner::strong_tail(const Level& L, const part_t* part, const int64_t* sizes, const RebalanceShape& r) {
+void Refiner::strong_tail(const Level& L, const part_t* part, const int64_t* sizes, const RebalanceShape& r,
+                          bool cookie_done) {
     DGraph g = L.view();
     int32_t* c = cnt();
     const int32_t k = p_.k;
     int nb = grid_for(g.n, 256, 8192);
-    k_cookie<<<1, CK_BLOCK, 0, s_>>>(k, r.width, off_, sizes, r.max_dest, evict_adjust_, evict_start_, acc_,
-                                     misc_ + 2);
+    if (!cookie_done)
+        k_cookie<<<1, CK_BLOCK, 0, s_>>>(k, r.width, off_, sizes, r.max_dest, evict_adjust_, evict_start_, acc_,
+                                         misc_ + 2);
     k_strong_dest<<<nb, 256, 0, s_>>>(moves_, c, nmax_, part, g.vw, evict_adjust_, evict_start_, k, score_, dest_,
                                       redraw_, c + 4, nmax_, misc_ + 2);
     const int32_t lds_max = (int32_t)std::min<int64_t>(RD_MAX, tuning().redraw_lds_max);
@@ -1984,7 +2027,7 @@ void Refiner::rank_deferred(const Level& L, const part_t* part, const int64_t* s
     fill_i32(misc_ + 2, 0, 1, s_);
     RebalanceShape r = rebalance_shape(L.n, strong);
     rank_radix(L, part, sizes, r.width, r.key_bits, strong);
-    if (strong) strong_tail(L, part, sizes, r);
+    if (strong) strong_tail(L, part, sizes, r, false);
 }
 
 void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes, bool strong) {
@@ -2057,7 +2100,9 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
 #undef JET_CLASSIFY
     JET_LAUNCH_CHECK();
     rank_flagged(L, part, sizes, width, strong);
-    if (strong) strong_tail(L, part, sizes, r);
+    // the ranking workgroup ran the cookie cutter (when it deferred, every
+    // later kernel of the step does nothing and rank_deferred runs it)
+    if (strong) strong_tail(L, part, sizes, r, true);
 }
 
 // Waits for the step's sequence word in the mapped mailbox: a poll of host
@@ -2273,8 +2318,10 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
                 changed_ = true;
                 cut_ = curr_cut;
                 total_imb_ = curr_imb;
-                copy_i64(best_sizes_, curr_sizes_, k, s_);
-                copy_i32(best_part, cur_part_, g.n, s_);
+                // the best state: part sizes and partition in one launch
+                k_copy_best<<<grid_for(g.n, 256, 8192), 256, 0, s_>>>(best_part, cur_part_, g.n, best_sizes_,
+                                                                     curr_sizes_, k);
+                JET_LAUNCH_CHECK();
             }
             if (trace) {
                 const int64_t kind = balance_counter == 0 ? 0 : (strong ? 2 : 1);
