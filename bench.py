@@ -136,8 +136,8 @@ def kernel_rooflines(step, args, ms_per_step, torch):
     """Per-family rooflines of the partition: for each kernel family, the
     same --steps partitions again with every launch of that family made with
     its own start and end HIP events (hipExtLaunchKernelGGL on the refinement
-    stream: the interval is the kernel's execution, as in a profiler's
-    dispatch record), then one partition in counting mode for every family's
+    stream, bounded below by a marker event recorded before the launch: the
+    interval is the kernel's execution, as in a profiler's dispatch record), then one partition in counting mode for every family's
     algorithmic bytes (the computation is deterministic, so its launches are
     the timed ones). Returns (roofline of the family with the most kernel
     time, all families, end-to-end figure: modeled bytes of all families /

@@ -409,7 +409,9 @@ int jet_tuning_load(const char* path);
  * "update", "weak_select"; a unit is one kernel launch of the family.
  * jet_prof_enable("<family>") times every later launch of that family with a
  * pair of HIP events that the launch records at its kernel's start and end
- * (hipExtLaunchKernelGGL), so the time is the kernels' execution;
+ * (hipExtLaunchKernelGGL), each interval starting no earlier than a marker
+ * event recorded just before the launch, so the time is the kernels'
+ * execution;
  * "<family>:count" or "all:count" instead counts the launches and the items
  * each family's algorithmic bytes follow from (a separate run: counting adds
  * work); "none" stops. jet_prof_read returns, for an enabled family, its

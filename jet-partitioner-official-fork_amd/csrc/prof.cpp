@@ -18,7 +18,7 @@ struct Prof {
     double host_bytes[PF_FAMILIES] = {};  // bytes known on the host
     int64_t units[PF_FAMILIES] = {};      // launches counted (counting)
     unsigned long long* dev = nullptr;    // device counters, PROF_COUNTERS per family
-    std::vector<hipEvent_t> ev;           // 2 per timed launch
+    std::vector<hipEvent_t> ev;           // 3 per timed launch: marker, start, end
     int64_t nev = 0;                      // timed launches made
     int dev_id = -1;                      // device the events and counters belong to
 };
@@ -70,14 +70,18 @@ unsigned long long* prof_counters(int fam) {
 
 bool prof_counting(int fam) { return fam >= 0 && fam < PF_FAMILIES && P().counting[fam]; }
 
-ProfEvents prof_launch(int fam) {
+ProfEvents prof_launch(int fam, hipStream_t s) {
     Prof& p = P();
     ProfEvents e;
     if (fam < 0 || fam >= PF_FAMILIES) return e;
     if (p.counting[fam]) p.units[fam]++;
     if (p.timed != fam || p.nev >= PROF_MAX_UNITS) return e;
-    e.start = p.ev[2 * p.nev];
-    e.stop = p.ev[2 * p.nev + 1];
+    // a marker first: it completes when the work queued before the launch
+    // has finished, which bounds the kernel's start from below when the
+    // launch's own start stamp is taken while that work still runs
+    JET_HIP(hipEventRecord(p.ev[3 * p.nev], s));
+    e.start = p.ev[3 * p.nev + 1];
+    e.stop = p.ev[3 * p.nev + 2];
     p.nev++;
     return e;
 }
@@ -126,7 +130,7 @@ bool prof_enable(const std::string& spec) {
         return true;
     }
     if (p.ev.empty()) {
-        p.ev.resize(2 * PROF_MAX_UNITS);
+        p.ev.resize(3 * PROF_MAX_UNITS);
         for (auto& e : p.ev) JET_HIP(hipEventCreate(&e));
     }
     p.timed = fam;
@@ -149,9 +153,11 @@ bool prof_read(int fam, int64_t& units, double& ms, double& bytes) {
     }
     if (p.timed != fam) return false;
     for (int64_t i = 0; i < p.nev; i++) {
-        float t = 0;
-        JET_HIP(hipEventElapsedTime(&t, p.ev[2 * i], p.ev[2 * i + 1]));
-        ms += t;
+        // from the later of the marker and the launch's start stamp
+        float t_start = 0, t_marker = 0;
+        JET_HIP(hipEventElapsedTime(&t_start, p.ev[3 * i + 1], p.ev[3 * i + 2]));
+        JET_HIP(hipEventElapsedTime(&t_marker, p.ev[3 * i], p.ev[3 * i + 2]));
+        ms += t_start < t_marker ? t_start : t_marker;
     }
     units = p.nev;
     return true;

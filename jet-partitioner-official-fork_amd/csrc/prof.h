@@ -3,9 +3,13 @@
 //
 // Timing mode (one family at a time): every kernel launch of the family is
 // made with hipExtLaunchKernelGGL and a pair of HIP events that the launch
-// itself records at the kernel's start and end, so an interval is the
-// kernel's execution alone (what a profiler's dispatch record measures), not
-// the launch gap before it; the family's time is the sum over its launches.
+// itself records at the kernel's start and end, after a marker event on the
+// same stream; an interval runs from the later of the marker and the start
+// stamp to the end stamp, so it is the kernel's execution alone (what a
+// profiler's dispatch record measures), neither the launch gap before it
+// nor, for a launch queued behind a running kernel (whose start stamp can be
+// taken early), the wait for that kernel. The family's time is the sum over
+// its launches.
 //
 // Counting mode (any set of families, "all:count"): the launches are counted
 // and the item counts their algorithmic bytes follow from are accumulated,
@@ -35,12 +39,13 @@ constexpr int PROF_COUNTERS = 4;           // device counters per family
 unsigned long long* prof_counters(int fam);
 // Whether `fam` is being counted (host-side accounting is wanted).
 bool prof_counting(int fam);
-// Start and end events for one launch of `fam`: a fresh pair when fam is
-// timed, null events otherwise; in counting mode the launch is counted.
+// Start and end events for one launch of `fam` on stream s: a fresh pair
+// when fam is timed (after a marker event recorded on s), null events
+// otherwise; in counting mode the launch is counted.
 struct ProfEvents {
     hipEvent_t start = nullptr, stop = nullptr;
 };
-ProfEvents prof_launch(int fam);
+ProfEvents prof_launch(int fam, hipStream_t s);
 // Adds bytes the host knows to `fam` (counting mode; no-op otherwise).
 void prof_bytes(int fam, double host_bytes);
 // "lp_select" (etc.) times that family; "<family>:count" or "all:count"
@@ -58,7 +63,7 @@ int prof_timed_family();
 template <typename... P, typename... A>
 inline void family_launch(int fam, void (*kernel)(P...), dim3 grid, dim3 block, uint32_t shm, hipStream_t s,
                           A&&... args) {
-    const ProfEvents e = prof_launch(fam);
+    const ProfEvents e = prof_launch(fam, s);
     hipExtLaunchKernelGGL(kernel, grid, block, shm, s, e.start, e.stop, 0, std::forward<A>(args)...);
 }
 
