@@ -395,7 +395,9 @@ int jet_partition_batch(jet_ctx_t* ctx, int32_t count, const jet_config_t* cfgs,
  * "ct_wave_max", "ct_block_max": contraction rows of up to this many fine
  *   entries take the one-wave (<= 128) or one-workgroup (<= 4096) LDS table;
  * "redraw_lds_max": strong rebalancing puts up to this many (<= 8192)
- *   re-drawn vertices in id order in LDS, longer lists in global memory.
+ *   re-drawn vertices in id order in LDS, longer lists in global memory;
+ * "sel_max_blocks": the rebalancers' select kernels run at most this many
+ *   workgroups (64-65536), each over a contiguous range of vertices.
  * jet_tuning_set / jet_tuning_get return JET_EINVAL for an unknown name or a
  * value out of range; jet_tuning_load applies a table file ("name value"
  * lines, '#' comments) and returns JET_EINVAL, applying nothing, when it

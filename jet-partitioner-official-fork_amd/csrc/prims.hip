@@ -154,35 +154,113 @@ __global__ void __launch_bounds__(SB) k_scan_down(const Tin* in, Tout* out, int6
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = (Tout)(b0 + total);
 }
 
-// Arrays of up to SC_MAX items (the rebalancers' bucket sums on large
-// levels): one workgroup, each thread a contiguous chunk of at most SC_PER
-// items loaded at once, one workgroup scan of the chunk totals, then the
-// chunk written. One launch instead of three; in place is safe, as every
-// thread reads its own chunk before writing it and the chunks are disjoint.
+// Arrays of up to SC_MAX items (the rebalancers' bucket sums and the
+// table offsets of mid-sized levels): one workgroup walks the array in tiles
+// of SC_TILE items. A tile is loaded coalesced into LDS (padded against bank
+// conflicts, 132 KB of gfx950's 160 KB), each thread scans SC_PER consecutive
+// items, one workgroup scan of the thread totals, then the tile is stored
+// coalesced. One launch instead of three; in place is safe, as a tile is
+// read whole before it is written and later tiles lie beyond it.
 constexpr int SC_BLOCK = 1024;
-constexpr int SC_PER = 32;
-constexpr int64_t SC_MAX = (int64_t)SC_BLOCK * SC_PER;
+constexpr int SC_PER = 16;
+constexpr int SC_TILE = SC_BLOCK * SC_PER;
+constexpr int64_t SC_MAX = 2 * (int64_t)SC_TILE;
 
 template <class Tin, class Tout>
 __global__ void __launch_bounds__(SC_BLOCK) k_scan_chunked(const Tin* in, Tout* out, int64_t n) {
+    __shared__ int64_t tile[SC_TILE + SC_TILE / 32];
     __shared__ int64_t ws[SC_BLOCK / WAVE + 1];
-    const int64_t per = (n + SC_BLOCK - 1) / SC_BLOCK;  // <= SC_PER
-    const int64_t b = (int64_t)threadIdx.x * per;
-    int64_t v[SC_PER];
+    int64_t carry_in = 0;
+    for (int64_t base = 0; base < n; base += SC_TILE) {
+#pragma unroll
+        for (int i = 0; i < SC_PER; i++) {
+            const int li = i * SC_BLOCK + threadIdx.x;
+            const int64_t idx = base + li;
+            tile[pad_idx(li)] = idx < n ? (int64_t)in[idx] : 0;
+        }
+        __syncthreads();
+        int64_t sum = 0;
+#pragma unroll
+        for (int j = 0; j < SC_PER; j++) sum += tile[pad_idx(threadIdx.x * SC_PER + j)];
+        int64_t total;
+        int64_t carry = carry_in + block_excl_scan<int64_t, SC_BLOCK>(sum, total, ws);
+#pragma unroll
+        for (int j = 0; j < SC_PER; j++) {
+            const int li = pad_idx(threadIdx.x * SC_PER + j);
+            const int64_t v = tile[li];
+            tile[li] = carry;
+            carry += v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < SC_PER; i++) {
+            const int li = i * SC_BLOCK + threadIdx.x;
+            const int64_t idx = base + li;
+            if (idx < n) out[idx] = (Tout)tile[pad_idx(li)];
+        }
+        carry_in += total;
+        __syncthreads();  // the tile and ws are reused by the next tile
+    }
+    if (threadIdx.x == 0) out[n] = (Tout)carry_in;
+}
+
+// Out-of-place arrays of up to SC_MAX items in one launch and without waits
+// between workgroups: workgroup b scans tile b (SP_TILE items, coalesced
+// through LDS) and sums every item before the tile itself (at most SC_MAX
+// items, read from L2 by all its threads at once). Not for in == out: a
+// workgroup would read a tile that another has already overwritten.
+constexpr int SP_BLOCK = 1024;
+constexpr int SP_PER = 4;
+constexpr int SP_TILE = SP_BLOCK * SP_PER;
+
+template <class Tin, class Tout>
+__global__ void __launch_bounds__(SP_BLOCK) k_scan_prefix_tiles(const Tin* __restrict__ in, Tout* __restrict__ out,
+                                                                int64_t n) {
+    __shared__ int64_t tile[SP_TILE + SP_TILE / 32];
+    __shared__ int64_t ws[SP_BLOCK / WAVE + 1];
+    __shared__ int64_t red[SP_BLOCK / WAVE];
+    const int64_t base = (int64_t)blockIdx.x * SP_TILE;
+    int64_t v[SP_PER];
+#pragma unroll
+    for (int i = 0; i < SP_PER; i++) {
+        const int64_t idx = base + i * SP_BLOCK + threadIdx.x;
+        v[i] = idx < n ? (int64_t)in[idx] : 0;
+    }
+    int64_t before = 0;
+    for (int64_t j0 = 0; j0 < base; j0 += 8 * SP_BLOCK) {
+        int64_t t[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int64_t idx = j0 + u * SP_BLOCK + threadIdx.x;
+            t[u] = idx < base ? (int64_t)in[idx] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) before += t[u];
+    }
+    before = block_reduce_sum<int64_t, SP_BLOCK>(before, red);
+#pragma unroll
+    for (int i = 0; i < SP_PER; i++) tile[pad_idx(i * SP_BLOCK + threadIdx.x)] = v[i];
+    __syncthreads();
     int64_t sum = 0;
 #pragma unroll
-    for (int j = 0; j < SC_PER; j++) {
-        v[j] = j < per && b + j < n ? (int64_t)in[b + j] : 0;
-        sum += v[j];
-    }
+    for (int j = 0; j < SP_PER; j++) sum += tile[pad_idx(threadIdx.x * SP_PER + j)];
     int64_t total;
-    int64_t carry = block_excl_scan<int64_t, SC_BLOCK>(sum, total, ws);
+    int64_t carry = before + block_excl_scan<int64_t, SP_BLOCK>(sum, total, ws);
 #pragma unroll
-    for (int j = 0; j < SC_PER; j++) {
-        if (j < per && b + j < n) out[b + j] = (Tout)carry;
-        carry += v[j];
+    for (int j = 0; j < SP_PER; j++) {
+        const int li = pad_idx(threadIdx.x * SP_PER + j);
+        const int64_t x = tile[li];
+        tile[li] = carry;
+        carry += x;
     }
-    if (threadIdx.x == 0) out[n] = (Tout)total;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < SP_PER; i++) {
+        const int li = i * SP_BLOCK + threadIdx.x;
+        const int64_t idx = base + li;
+        if (idx < n) out[idx] = (Tout)tile[pad_idx(li)];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = (Tout)(before + total);
 }
 
 template <class Tin, class Tout>
@@ -195,6 +273,11 @@ static void excl_scan_impl(const Tin* in, Tout* out, int64_t n, Arena& a, hipStr
     int64_t nb = ceil_div(n, STILE);
     if (nb == 1) {
         k_scan_down<Tin, Tout><<<1, SB, 0, s>>>(in, out, n, nullptr);
+        JET_LAUNCH_CHECK();
+        return;
+    }
+    if (n <= SC_MAX && (const void*)in != (const void*)out) {
+        k_scan_prefix_tiles<Tin, Tout><<<(unsigned)ceil_div(n, (int64_t)SP_TILE), SP_BLOCK, 0, s>>>(in, out, n);
         JET_LAUNCH_CHECK();
         return;
     }

@@ -162,8 +162,8 @@ class Refiner {
     vid_t* redraw_;    // strong re-draws (n)
     int32_t* rd_mark_; // marks of a long re-draw list (n, zero between steps)
     int64_t* buckets_;
-    int64_t* boff_;        // the scan of a small bucket array, written by the classification kernel
-    const int64_t* off_ = nullptr;  // the current rebalance step's scanned buckets (buckets_ or boff_)
+    int64_t* boff_;        // the scanned bucket array (by the classification kernel when it is small)
+    const int64_t* off_ = nullptr;  // the current rebalance step's scanned buckets (boff_)
     int32_t small_max_ = 0;         // rows of degree <= this take the thread-per-row table build
     bool rebuild_pending_ = false;  // update_large marked rows still to rebuild
     Mailbox* mailbox_ = nullptr;    // the creating thread's mailbox (results and sequence words)

@@ -1076,49 +1076,76 @@ __global__ void __launch_bounds__(SEL_BLOCK_MAX)
     bm.init();
     const int32_t tu = misc[0];
     int64_t n_eval = 0, n_slot = 0;
-    FOR_BLOCK_RANGE(i, g.n) {
-        const part_t p = part[i];
-        part_t d = p;
-        int32_t b = -1;
-        const int64_t sp = ps[p];
-        // the loads a vertex of an oversized part needs are issued together
-        const bool over = sp > size_max;
-        const wgt_t w = over ? g.vw[i] : 0;
-        const eoff_t off = over ? conn_off[i] : 0;
-        const int32_t sz = over ? conn_size[i] : 0;
-        const part_t spare = over && tu > 0 ? undersized[(uint32_t)i % (uint32_t)tu] : p;
-        if (over && (double)w < 1.5 * (double)(sp - opt)) {
-            n_eval++;
-            n_slot += sz;
-            part_t best = p;
-            int32_t gain = 0, p_gain = 0;
-            for_slots(conn + off, sz, [&](int2 e) {
-                if (e.x > NULL_PART && ps[e.x] < max_dest) {
-                    if (e.y > gain) {
-                        best = e.x;
-                        gain = e.y;
-                    } else if (e.y == gain && gain > 0 && e.x < best) {
-                        best = e.x;
-                    }
-                }
-                if (e.x == p) p_gain = e.y;
-            });
-            int64_t sv = 0;
-            if (gain > 0) {
-                d = best;
-                sv = (int64_t)gain - p_gain;
-            } else if (tu > 0) {
-                d = spare;
-                sv = -(int64_t)p_gain;
-            }
-            if (d != p) {
-                int32_t gt = gain_bucket_fast(sv, w);
-                b = (MAX_BUCKETS * p + gt) * sections + (int32_t)((uint32_t)i % (uint32_t)sections);
-                bm.add(b, (unsigned long long)w, buckets);
-            }
+    // one contiguous range per workgroup (as FOR_BLOCK_RANGE), WS_V of a
+    // thread's vertices per pass with their part and table-header loads
+    // issued together: the dependent chain part -> header -> slots is paid
+    // once per pass, not once per vertex
+    constexpr int WS_V = 4;
+    const int64_t per_block = ((g.n + gridDim.x - 1) / gridDim.x + blockDim.x - 1) / blockDim.x * blockDim.x;
+    const int64_t lo = (int64_t)blockIdx.x * per_block;
+    const int64_t hi = min((int64_t)g.n, lo + per_block);
+    for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (int64_t)WS_V * blockDim.x) {
+        part_t p[WS_V], spare[WS_V];
+        wgt_t w[WS_V];
+        eoff_t off[WS_V];
+        int32_t sz[WS_V];
+        int64_t sp[WS_V];
+        bool ev[WS_V];
+#pragma unroll
+        for (int u = 0; u < WS_V; u++) {
+            const int64_t i = i0 + (int64_t)u * blockDim.x;
+            p[u] = i < hi ? part[i] : 0;
         }
-        dest[i] = d;
-        bid[i] = b;
+#pragma unroll
+        for (int u = 0; u < WS_V; u++) {
+            const int64_t i = i0 + (int64_t)u * blockDim.x;
+            sp[u] = ps[p[u]];
+            const bool over = i < hi && sp[u] > size_max;
+            w[u] = over ? g.vw[i] : 0;
+            off[u] = over ? conn_off[i] : 0;
+            sz[u] = over ? conn_size[i] : 0;
+            spare[u] = over && tu > 0 ? undersized[(uint32_t)i % (uint32_t)tu] : p[u];
+            ev[u] = over && (double)w[u] < 1.5 * (double)(sp[u] - opt);
+        }
+#pragma unroll
+        for (int u = 0; u < WS_V; u++) {
+            const int64_t i = i0 + (int64_t)u * blockDim.x;
+            if (i >= hi) continue;
+            part_t d = p[u];
+            int32_t b = -1;
+            if (ev[u]) {
+                n_eval++;
+                n_slot += sz[u];
+                part_t best = p[u];
+                int32_t gain = 0, p_gain = 0;
+                for_slots(conn + off[u], sz[u], [&](int2 e) {
+                    if (e.x > NULL_PART && ps[e.x] < max_dest) {
+                        if (e.y > gain) {
+                            best = e.x;
+                            gain = e.y;
+                        } else if (e.y == gain && gain > 0 && e.x < best) {
+                            best = e.x;
+                        }
+                    }
+                    if (e.x == p[u]) p_gain = e.y;
+                });
+                int64_t sv = 0;
+                if (gain > 0) {
+                    d = best;
+                    sv = (int64_t)gain - p_gain;
+                } else if (tu > 0) {
+                    d = spare[u];
+                    sv = -(int64_t)p_gain;
+                }
+                if (d != p[u]) {
+                    int32_t gt = gain_bucket_fast(sv, w[u]);
+                    b = (MAX_BUCKETS * p[u] + gt) * sections + (int32_t)((uint32_t)i % (uint32_t)sections);
+                    bm.add(b, (unsigned long long)w[u], buckets);
+                }
+            }
+            dest[i] = d;
+            bid[i] = b;
+        }
     }
     bm.flush(buckets);
     if (pc) {  // counting runs only: evaluated vertices and table slots read
@@ -1820,7 +1847,7 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
     rd_mark_ = a.alloc<int32_t>(N);
     nbuckets_ = (int64_t)MAX_BUCKETS * p.k * sections_for(n, p.k) + MAX_SECTIONS + 4;
     buckets_ = a.alloc<int64_t>(nbuckets_);
-    boff_ = a.alloc<int64_t>(FUSE_SCAN_MAX);
+    boff_ = a.alloc<int64_t>(std::max<int64_t>(FUSE_SCAN_MAX, nbuckets_));
     ncopies_ = (int32_t)std::max<int64_t>(
         1, std::min<int64_t>(BUCKET_COPIES_MAX, BUCKET_COPIES_BYTES / ((int64_t)sizeof(int64_t) * nbuckets_)));
     bcopies_ = a.alloc<int64_t>(std::max<int64_t>(1, nbuckets_ * (ncopies_ - 1)));
@@ -2050,7 +2077,7 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
     // levels, one per thread of a 128-thread workgroup): vertices of one
     // oversized part share few buckets, and the serial per-vertex chain of a
     // heavy range stays short
-    int nsel = grid_for(g.n, g.n <= tuning().small_level_n ? 128 : 512, 1024);
+    int nsel = grid_for(g.n, g.n <= tuning().small_level_n ? 128 : 512, (int)tuning().sel_max_blocks);
     unsigned long long* bk = reinterpret_cast<unsigned long long*>(buckets_);
     unsigned long long* bc = reinterpret_cast<unsigned long long*>(bcopies_);
     // about 256 workgroups per copy: fewer workgroups meet on a line without
@@ -2081,15 +2108,17 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
         k_sum_bucket_copies<<<grid_for(nbk + 1, 256, 2048), 256, 0, s_>>>(buckets_, bcopies_, copies, nbk + 1,
                                                                           nbuckets_);
     JET_LAUNCH_CHECK();
-    // a bucket array of up to FUSE_SCAN_MAX sums is scanned by the
-    // classification kernel itself, into boff_
+    // the scanned buckets go to boff_: a bucket array of up to FUSE_SCAN_MAX
+    // sums is scanned by the classification kernel itself, a longer one by a
+    // scan launch (out of place, so its workgroups need not wait on each other)
     const bool fused = nbk <= FUSE_SCAN_MAX;
-    off_ = fused ? boff_ : buckets_;
-    if (!fused) excl_scan_i64_i64(buckets_, buckets_, nbk, a_, s_);
+    off_ = boff_;
+    if (!fused) excl_scan_i64_i64(buckets_, boff_, nbk, a_, s_);
     int cb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)CL_WAVES * CL_CHW));
 #define JET_CLASSIFY(ST, FU)                                                                                 \
-    k_classify<ST, FU><<<cb, CL_WAVES * WAVE, 0, s_>>>(g, part, bid_, buckets_, nbk, boff_, sizes, smax, width, \
-                                                       k, sections, moves_, c, nmax_, cand_, stamp_, gen_)
+    k_classify<ST, FU><<<cb, CL_WAVES * WAVE, 0, s_>>>(g, part, bid_, FU ? buckets_ : boff_, nbk, boff_, sizes,    \
+                                                       smax, width, k, sections, moves_, c, nmax_, cand_, stamp_,  \
+                                                       gen_)
     if (strong) {
         if (fused) JET_CLASSIFY(true, true);
         else JET_CLASSIFY(true, false);

@@ -40,6 +40,9 @@ struct Tuning {
     // strong rebalancing: up to this many re-drawn vertices are put in id
     // order in LDS, longer lists in global memory
     int64_t redraw_lds_max = 8192;
+    // rebalancers' select kernels: at most this many workgroups (each a
+    // contiguous range of vertices)
+    int64_t sel_max_blocks = 1024;
 };
 
 Tuning& tuning();

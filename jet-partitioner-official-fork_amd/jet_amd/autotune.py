@@ -33,6 +33,7 @@ SPACE = {
     "wg_max_n": [0, 1024, 4096, 16384],
     "wg_big_deg": [0, 8, 32],
     "redraw_lds_max": [0, 8192],
+    "sel_max_blocks": [512, 1024, 2048, 4096, 8192],
 }
 
 # (name, builder, k, imbalance, coarsening algorithm, weight of its time)

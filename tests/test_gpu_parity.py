@@ -152,6 +152,21 @@ def test_conn_tables_large_k(k):
         assert got[i] == list(zip(ep[eoff[i]:eoff[i + 1]].tolist(), ev[eoff[i]:eoff[i + 1]].tolist())), i
 
 
+@pytest.mark.parametrize("n", [4096, 4097, 16384, 16385, 30011, 32768, 32769, 70001])
+def test_conn_offsets_are_capacity_prefix_sums(n):
+    """Table offsets = exclusive prefix sums of min(degree, k), across the
+    scan's one-tile, single-workgroup multi-tile and multi-workgroup sizes."""
+    rs = np.random.default_rng(n)
+    m = 3 * n
+    g = graphs.from_edges(n, rs.integers(0, n, m), rs.integers(0, n, m), w=rs.integers(1, 9, m))
+    k = 5
+    part = rs.integers(0, k, n).astype(np.int32)
+    say("conn-offsets", n)
+    off, _, _, _ = jet_amd.conn_tables(g, part, k)
+    exp = np.concatenate([[0], np.cumsum(np.minimum(np.diff(g["rowptr"]), k))])
+    assert np.array_equal(off, exp)
+
+
 def rand_part(n, k, seed, skew=0.0):
     rs = np.random.default_rng(seed)
     pr = np.exp(-skew * np.arange(k))
