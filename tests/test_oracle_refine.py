@@ -252,10 +252,10 @@ def test_initpart_balance_and_quality():
 def test_initpart_independent_of_host_threads(monkeypatch):
     """The bisection tree and the trials of one bisection may run on several
     host threads; the partition must be the same for any thread count."""
-    g = graphs.rmat(14, 16, seed=2)
-    outs = []
-    for th in ("1", "3", "16"):
-        monkeypatch.setenv("JET_HOST_THREADS", th)
-        outs.append(oracle.initpart(g, 16, 1.05, seed=7))
-    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
-    assert len(np.unique(outs[0])) == 16
+    for g, k in ((graphs.rmat(14, 16, seed=2), 16), (graphs.stencil3d(9, 8, 8, seed=3), 64)):
+        outs = []
+        for th in ("1", "3", "16"):
+            monkeypatch.setenv("JET_HOST_THREADS", th)
+            outs.append(oracle.initpart(g, k, 1.05, seed=7))
+        assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+        assert len(np.unique(outs[0])) == k
