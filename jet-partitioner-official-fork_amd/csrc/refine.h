@@ -168,6 +168,7 @@ class Refiner {
     bool rebuild_pending_ = false;  // update_large marked rows still to rebuild
     Mailbox* mailbox_ = nullptr;    // the creating thread's mailbox (results and sequence words)
     int64_t* pub_ = nullptr;      // the cut change: PUB_GROUPS partials, 128 bytes apart
+    int64_t* szc_ = nullptr;      // the first update pass's part-size deltas: up to 16 copies of k
     int64_t* bcopies_ = nullptr;  // ncopies_ - 1 more bucket arrays, kept zero between uses
     int32_t ncopies_ = 1;
     int64_t nbuckets_ = 0;

@@ -647,14 +647,40 @@ struct RebalPrep {
     int64_t* evict_adjust;
 };
 
+// The first update pass's workgroups add their part-size deltas into one of
+// `ncopies` copies of a k-entry array (workgroup b into copy b mod ncopies),
+// so that fewer workgroups' atomics meet on one address; the second pass's
+// publishing workgroup folds the copies into the sizes and clears them.
+constexpr int SIZE_COPIES_MAX = 16;
+
 struct Publish {
     int64_t* partials;
-    const int64_t* sizes;
+    int64_t* sizes;
     int32_t k;
     int64_t* host_out;
     const int32_t* mc;
     int64_t seq;
+    unsigned long long* copies;  // ncopies * k deltas, zero between steps
+    int32_t ncopies;
 };
+
+__device__ inline void fold_size_copies(const Publish& pb) {
+    if (pb.ncopies <= 0) return;
+    for (int t = threadIdx.x; t < pb.k; t += blockDim.x) {
+        // every copy's load in flight at once, then the clears
+        unsigned long long v[SIZE_COPIES_MAX];
+#pragma unroll
+        for (int c = 0; c < SIZE_COPIES_MAX; c++) v[c] = c < pb.ncopies ? pb.copies[(int64_t)c * pb.k + t] : 0;
+        unsigned long long d = 0;
+#pragma unroll
+        for (int c = 0; c < SIZE_COPIES_MAX; c++) {
+            d += v[c];
+            if (v[c]) pb.copies[(int64_t)c * pb.k + t] = 0;
+        }
+        if (d) pb.sizes[t] += (int64_t)d;
+    }
+    __syncthreads();
+}
 
 __device__ inline void publish_step(const Publish& pb, bool deferred, int64_t* red) {
     if (deferred) {
@@ -780,6 +806,9 @@ __global__ void __launch_bounds__(256)
     // the second pass's workgroup 0 publishes the step (the first pass, which
     // completes the results, has finished)
     if (ADD && blockIdx.x == 0) {
+        // the first pass's size deltas (every copy complete: that pass has
+        // finished); no other kernel of this pass touches the sizes
+        fold_size_copies(pb);
         publish_step(pb, dfr != 0, red);
         // then the next rebalancing step's start (its kernels run after this one)
         if (!dfr && prep.buckets)
@@ -832,8 +861,10 @@ __global__ void __launch_bounds__(256)
         add_cut_part2(partials, cc, red);
         if (lds) {
             __syncthreads();
+            unsigned long long* dst =
+                pb.ncopies > 0 ? pb.copies + (int64_t)(blockIdx.x % (uint32_t)pb.ncopies) * k : sizes;
             for (int t = threadIdx.x; t < k; t += blockDim.x)
-                if (ls[t]) atomicAdd(&sizes[t], ls[t]);
+                if (ls[t]) atomicAdd(&dst[t], ls[t]);
         }
     }
 }
@@ -1863,6 +1894,8 @@ Refiner::Refiner(Arena& a, hipStream_t s, vid_t n, int64_t cap, const RefinePara
     wg_runb_ = a.alloc<int32_t>(WG_MAX_NBK + 1);
     pub_ = a.alloc<int64_t>((int64_t)PUB_GROUPS * PUB_STRIDE);
     fill_i64(pub_, 0, (int64_t)PUB_GROUPS * PUB_STRIDE, s);
+    szc_ = a.alloc<int64_t>((int64_t)SIZE_COPIES_MAX * p.k);
+    fill_i64(szc_, 0, (int64_t)SIZE_COPIES_MAX * p.k, s);
     // (cut change, largest part) of each iteration land in mailbox slots 8, 9
     Mailbox& mb = thread_mailbox();
     mailbox_ = &mb;
@@ -2168,7 +2201,10 @@ bool Refiner::perform_moves(const Level& L, part_t* part, int64_t* sizes, int64_
     int G = group_for_update(g);
     int nbs = grid_for((int64_t)g.n * G, 256, 1024);
     const int64_t seq = ++mailbox_->seq;
-    const Publish pb{pub_, sizes, k, res_dev_, c, seq};
+    // size-delta copies for the LDS-aggregated deltas (k <= LDS_K): about
+    // 4096 entries in all
+    const int32_t nsc = k <= LDS_K ? std::max(1, std::min(SIZE_COPIES_MAX, 4096 / k)) : 0;
+    const Publish pb{pub_, sizes, k, res_dev_, c, seq, reinterpret_cast<unsigned long long*>(szc_), nsc};
     // the next step's start in case it is a rebalancing step of kind
     // next_strong (the controller knows the kind before this step's result)
     RebalPrep prep{};
