@@ -397,7 +397,9 @@ int jet_partition_batch(jet_ctx_t* ctx, int32_t count, const jet_config_t* cfgs,
  * "redraw_lds_max": strong rebalancing puts up to this many (<= 8192)
  *   re-drawn vertices in id order in LDS, longer lists in global memory;
  * "sel_max_blocks": the rebalancers' select kernels run at most this many
- *   workgroups (64-65536), each over a contiguous range of vertices.
+ *   workgroups (64-65536), each over a contiguous range of vertices;
+ * "sel_copy_wgs": and flush their bucket sums into one copy of the bucket
+ *   array per this many workgroups (1-65536; at most 16 copies).
  * jet_tuning_set / jet_tuning_get return JET_EINVAL for an unknown name or a
  * value out of range; jet_tuning_load applies a table file ("name value"
  * lines, '#' comments) and returns JET_EINVAL, applying nothing, when it

@@ -2113,9 +2113,9 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
     int nsel = grid_for(g.n, g.n <= tuning().small_level_n ? 128 : 512, (int)tuning().sel_max_blocks);
     unsigned long long* bk = reinterpret_cast<unsigned long long*>(buckets_);
     unsigned long long* bc = reinterpret_cast<unsigned long long*>(bcopies_);
-    // about 256 workgroups per copy: fewer workgroups meet on a line without
-    // copies, and a copy costs a pass of the fold
-    const int32_t copies = std::max(1, std::min(ncopies_, nsel / 256));
+    // one copy per sel_copy_wgs workgroups (256 by default): fewer
+    // workgroups meet on a line, and a copy costs a pass of the fold
+    const int32_t copies = std::max(1, std::min<int32_t>(ncopies_, (int32_t)(nsel / tuning().sel_copy_wgs)));
     const bool lds = k <= SEL_LDS_K;
     // 512 threads on large levels (half the serial vertices per thread), 128
     // on small ones (more workgroups over few vertices; 0.3 ms less per 128^3

@@ -50,6 +50,7 @@ const Field kFields[] = {
     {"ct_block_max", &Tuning::ct_block_max, 1, 4096},
     {"redraw_lds_max", &Tuning::redraw_lds_max, 0, 8192},
     {"sel_max_blocks", &Tuning::sel_max_blocks, 64, 65536},
+    {"sel_copy_wgs", &Tuning::sel_copy_wgs, 1, 65536},
 };
 std::mutex g_mu;
 // fields set through jet_tuning_set: the table applied at context creation

@@ -43,6 +43,9 @@ struct Tuning {
     // rebalancers' select kernels: at most this many workgroups (each a
     // contiguous range of vertices)
     int64_t sel_max_blocks = 1024;
+    // and flush their bucket maps into one copy of the bucket array per this
+    // many workgroups (at most 16 copies)
+    int64_t sel_copy_wgs = 256;
 };
 
 Tuning& tuning();

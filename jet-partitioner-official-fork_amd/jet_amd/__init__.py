@@ -400,7 +400,7 @@ def prof_read(family):
 # the result-neutral tuning parameters (include/jet_capi.h)
 TUNING_NAMES = ("wg_max_n", "batch_wg_max_n", "wg_big_deg", "small_level_n", "ab_lanes_small", "upd_lanes", "upd_big_slices",
                 "lp_one_pass_n", "ct_wave_max", "ct_block_max", "redraw_lds_max",
-                "sel_max_blocks")
+                "sel_max_blocks", "sel_copy_wgs")
 
 
 def tuning_all():

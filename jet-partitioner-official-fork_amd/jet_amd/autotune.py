@@ -34,6 +34,7 @@ SPACE = {
     "wg_big_deg": [0, 8, 32],
     "redraw_lds_max": [0, 8192],
     "sel_max_blocks": [512, 1024, 2048, 4096, 8192],
+    "sel_copy_wgs": [32, 64, 128, 256],
 }
 
 # (name, builder, k, imbalance, coarsening algorithm, weight of its time)
