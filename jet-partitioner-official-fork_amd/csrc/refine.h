@@ -29,6 +29,7 @@ struct RefineParams {
 };
 
 struct MoveRec;
+struct BestCopy;
 
 // Holds the refiner's scratch (sized by the finest level, jet_refiner.hpp:134-181)
 // and the refine_data persisting across levels (jet_refiner.hpp:100-106).
@@ -144,6 +145,11 @@ class Refiner {
     int32_t gen_ = 0;
     // the last update pass prepared a rebalancing step of kind prep_strong_
     bool prep_ready_ = false, prep_strong_ = false;
+    // a best-state copy (of cur_part_ and curr_sizes_ into best_out_ and
+    // best_sizes_) is due: the next selection kernel or the level's end makes it
+    bool best_due_ = false;
+    part_t* best_out_ = nullptr;
+    BestCopy take_best_copy();
     vid_t* pull_mid_;  // mid-tier rows of degree > BIG_DEG
     int32_t ntier_mid_ = 0, ntier_big_ = 0, npull_mid_ = 0;
     bool has_big_ = false;

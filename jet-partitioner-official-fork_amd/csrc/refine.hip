@@ -101,6 +101,19 @@ __global__ void k_project(vid_t n, const vid_t* map, const part_t* coarse, part_
     FOR_THREAD(i, n) fine[i] = coarse[map[i]];
 }
 
+// A best-state copy due from the previous step, made by the next step's
+// selection kernel as it reads every vertex's part (part_out null: none due).
+struct BestCopy {
+    part_t* part_out;
+    int64_t* sizes_out;
+    const int64_t* sizes;
+    int32_t k;
+    __device__ void sizes_block0() const {
+        if (part_out && blockIdx.x == 0)
+            for (int32_t p = threadIdx.x; p < k; p += blockDim.x) sizes_out[p] = sizes[p];
+    }
+};
+
 // The controller's best state (jet_refiner.hpp:1117-1132): the partition
 // and the part sizes copied in one launch.
 __global__ void k_copy_best(part_t* best_part, const part_t* part, vid_t n, int64_t* best_sizes,
@@ -381,7 +394,8 @@ template <int CHW, int W>
 __global__ void __launch_bounds__(W* WAVE)
     k_lp_select(DGraph g, const part_t* part, const eoff_t* conn_off, const int2* conn, const int32_t* conn_size,
                 part_t* dest_cache, int32_t* gain_p, int32_t* lock, double fr, part_t* dest, int32_t* pregain,
-                vid_t* cand, int32_t* counters, vid_t nmax, int32_t* next_counters, unsigned long long* pc) {
+                vid_t* cand, int32_t* counters, vid_t nmax, int32_t* next_counters, unsigned long long* pc,
+                BestCopy bc) {
     constexpr int NPASS = CHW / WAVE;
     __shared__ vid_t sbuf[W * CHW];
     __shared__ int32_t sh[2 * W + 2];
@@ -406,6 +420,14 @@ __global__ void __launch_bounds__(W* WAVE)
         const int64_t i = wbase + u * WAVE + lane;
         cached[u] = i < g.n ? dest_cache[i] : NO_MOVE;
         locks[u] = i < g.n ? lock[i] : 0;
+    }
+    if (bc.part_out) {  // the previous step's best state, with the loads above
+#pragma unroll
+        for (int u = 0; u < NPASS; u++) {
+            const int64_t i = wbase + u * WAVE + lane;
+            if (i < g.n) bc.part_out[i] = part[i];
+        }
+        bc.sizes_block0();
     }
     // 2. the wave's misses, compacted, so that every lane evaluates one
     // table at a time instead of each pass waiting on its own few misses
@@ -1095,8 +1117,9 @@ __global__ void __launch_bounds__(SEL_BLOCK_MAX)
                   int64_t max_dest, const part_t* undersized, const int32_t* misc, const eoff_t* conn_off,
                   const int2* conn, const int32_t* conn_size, int32_t sections, part_t* dest, int32_t* bid,
                   unsigned long long* buckets_base, unsigned long long* extra, int32_t copies, int64_t stride,
-                  unsigned long long* pc, int32_t* next_counters) {
+                  unsigned long long* pc, int32_t* next_counters, BestCopy bc) {
     if (blockIdx.x == 0 && threadIdx.x < 8) next_counters[threadIdx.x] = 0;
+    bc.sizes_block0();
     unsigned long long* buckets = bucket_copy(buckets_base, extra, copies, stride);
     __shared__ int32_t bk[BH_SLOTS];
     __shared__ unsigned long long bv[BH_SLOTS];
@@ -1126,6 +1149,7 @@ __global__ void __launch_bounds__(SEL_BLOCK_MAX)
         for (int u = 0; u < WS_V; u++) {
             const int64_t i = i0 + (int64_t)u * blockDim.x;
             p[u] = i < hi ? part[i] : 0;
+            if (bc.part_out && i < hi) bc.part_out[i] = p[u];
         }
 #pragma unroll
         for (int u = 0; u < WS_V; u++) {
@@ -1196,8 +1220,10 @@ __global__ void __launch_bounds__(SEL_BLOCK_MAX)
     k_strong_select(DGraph g, const part_t* part, const int64_t* sizes, int32_t k, int64_t size_max, int32_t opt,
                     int64_t max_dest, const int64_t* acc, const eoff_t* conn_off, const int2* conn,
                     const int32_t* conn_size, int32_t sections, int32_t* bid, unsigned long long* buckets_base,
-                    unsigned long long* extra, int32_t copies, int64_t stride, int32_t* next_counters) {
+                    unsigned long long* extra, int32_t copies, int64_t stride, int32_t* next_counters,
+                    BestCopy bc) {
     if (blockIdx.x == 0 && threadIdx.x < 8) next_counters[threadIdx.x] = 0;
+    bc.sizes_block0();
     unsigned long long* buckets = bucket_copy(buckets_base, extra, copies, stride);
     __shared__ int32_t bk[BH_SLOTS];
     __shared__ unsigned long long bv[BH_SLOTS];
@@ -1209,6 +1235,7 @@ __global__ void __launch_bounds__(SEL_BLOCK_MAX)
     const int64_t max_vwgt = acc[3];
     FOR_BLOCK_RANGE(i, g.n) {
         const part_t p = part[i];
+        if (bc.part_out) bc.part_out[i] = p;
         int32_t b = -1;
         const int64_t sp = ps[p];
         const bool over = sp > size_max;
@@ -1952,26 +1979,37 @@ void Refiner::init_conn(const Level& L, const part_t* part, const int32_t* mark)
     }
 }
 
+// The best-state copy due from the previous step, for the next selection
+// kernel to make (none when not due); the due flag is cleared.
+BestCopy Refiner::take_best_copy() {
+    BestCopy b{nullptr, nullptr, nullptr, 0};
+    if (best_due_) b = BestCopy{best_out_, best_sizes_, curr_sizes_, p_.k};
+    best_due_ = false;
+    return b;
+}
+
 void Refiner::jet_lp(const Level& L, const part_t* part, double fr) {
     DGraph g = L.view();
     int32_t* c = cnt();
     gen_++;  // this step's moves are stamped with a new generation
+    const BestCopy bcp = take_best_copy();
     // a pass per wave on levels that fill the GPU with single-pass waves, two
     // passes per wave in 16-wave workgroups above that (see LS_MULTI_CHW)
     if (g.n <= tuning().lp_one_pass_n) {
         int nb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_WAVES * WAVE));
         family_launch(PF_LP_SELECT, k_lp_select<WAVE, LS_WAVES>, dim3(nb), dim3(LS_WAVES * WAVE), 0, s_, g, part,
                       conn_off_, conn_, conn_size_, dest_cache_, gain_p_, lock_, fr, dest_, pregain_, cand_, c, nmax_,
-                      cnt_next(), prof_counters(PF_LP_SELECT));
+                      cnt_next(), prof_counters(PF_LP_SELECT), bcp);
     } else {
         int nb = (int)std::max<int64_t>(1, ceil_div(g.n, (int64_t)LS_MULTI_WAVES * LS_MULTI_CHW));
         family_launch(PF_LP_SELECT, k_lp_select<LS_MULTI_CHW, LS_MULTI_WAVES>, dim3(nb), dim3(LS_MULTI_WAVES * WAVE),
                       0, s_, g, part, conn_off_, conn_, conn_size_, dest_cache_, gain_p_, lock_, fr, dest_, pregain_,
-                      cand_, c, nmax_, cnt_next(), prof_counters(PF_LP_SELECT));
+                      cand_, c, nmax_, cnt_next(), prof_counters(PF_LP_SELECT), bcp);
     }
     JET_LAUNCH_CHECK();
-    // per vertex: cached destination 4, lock 4
-    prof_bytes(PF_LP_SELECT, 8.0 * g.n);
+    // per vertex: cached destination 4, lock 4 (and part 4, best copy 4 when
+    // the previous step's best state rides along)
+    prof_bytes(PF_LP_SELECT, (bcp.part_out ? 16.0 : 8.0) * g.n);
     if (unsigned long long* pc = prof_counters(PF_AFTERBURNER)) {
         // counting runs only: candidate degrees (the list is complete here)
         k_prof_degsum<<<grid_for(g.n, 256, 1024), 256, 0, s_>>>(g, cand_, c, 0, 1, nmax_, nullptr, pc);
@@ -2106,6 +2144,7 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
                                                                         max_dest, strong ? 1 : 0, undersized_, misc_,
                                                                         acc_, evict_adjust_);
     prep_ready_ = false;
+    const BestCopy bcp = take_best_copy();
     // contiguous ranges of at least 512 vertices per workgroup (128 on small
     // levels, one per thread of a 128-thread workgroup): vertices of one
     // oversized part share few buckets, and the serial per-vertex chain of a
@@ -2125,17 +2164,17 @@ void Refiner::rebalance(const Level& L, const part_t* part, const int64_t* sizes
         if (lds)
             k_strong_select<true><<<nsel, selb, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, acc_, conn_off_,
                                                          conn_, conn_size_, sections, bid_, bk, bc, copies, nbuckets_,
-                                                         cnt_next());
+                                                         cnt_next(), bcp);
         else
             k_strong_select<false><<<nsel, selb, 0, s_>>>(g, part, sizes, k, smax, opt_, max_dest, acc_, conn_off_,
                                                           conn_, conn_size_, sections, bid_, bk, bc, copies, nbuckets_,
-                                                          cnt_next());
+                                                          cnt_next(), bcp);
     } else {
         family_launch(PF_WEAK_SELECT, lds ? k_weak_select<true> : k_weak_select<false>, dim3(nsel), dim3(selb), 0,
                       s_, g, part, sizes, k, smax, opt_, max_dest, undersized_, misc_, conn_off_, conn_, conn_size_,
-                      sections, dest_, bid_, bk, bc, copies, nbuckets_, prof_counters(PF_WEAK_SELECT), cnt_next());
+                      sections, dest_, bid_, bk, bc, copies, nbuckets_, prof_counters(PF_WEAK_SELECT), cnt_next(), bcp);
         // per vertex: part 4, destination 4, bucket id 4
-        prof_bytes(PF_WEAK_SELECT, 12.0 * g.n);
+        prof_bytes(PF_WEAK_SELECT, (bcp.part_out ? 16.0 : 12.0) * g.n);
     }
     if (copies > 1)
         k_sum_bucket_copies<<<grid_for(nbk + 1, 256, 2048), 256, 0, s_>>>(buckets_, bcopies_, copies, nbk + 1,
@@ -2296,6 +2335,7 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
     auto t0 = std::chrono::steady_clock::now();
     changed_ = false;
     prep_ready_ = false;
+    best_due_ = false;
     DGraph g = L.view();
     const int32_t k = p_.k;
     fill_i32(misc_, 0, 8, s_);
@@ -2383,10 +2423,10 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
                 changed_ = true;
                 cut_ = curr_cut;
                 total_imb_ = curr_imb;
-                // the best state: part sizes and partition in one launch
-                k_copy_best<<<grid_for(g.n, 256, 8192), 256, 0, s_>>>(best_part, cur_part_, g.n, best_sizes_,
-                                                                     curr_sizes_, k);
-                JET_LAUNCH_CHECK();
+                // the best state (partition and part sizes) is copied by the
+                // next step's selection kernel, or at the end of the level
+                best_due_ = true;
+                best_out_ = best_part;
             }
             if (trace) {
                 const int64_t kind = balance_counter == 0 ? 0 : (strong ? 2 : 1);
@@ -2395,6 +2435,11 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
         }
     }
     prep_ready_ = false;
+    if (best_due_) {
+        k_copy_best<<<grid_for(g.n, 256, 8192), 256, 0, s_>>>(best_part, cur_part_, g.n, best_sizes_, curr_sizes_, k);
+        JET_LAUNCH_CHECK();
+        best_due_ = false;
+    }
     flush_rebuild(L);
     int32_t err = read_scalar(misc_ + 1, s_);
     if (err) throw DeviceError(refine_error_text(err));
@@ -2482,6 +2527,7 @@ void Refiner::single_step(const Level& L, const part_t* part, int kind, double f
                           std::vector<int32_t>& dest) {
     DGraph g = L.view();
     const int32_t k = p_.k;
+    best_due_ = false;
     fill_i32(misc_, 0, 8, s_);
     fill_i32(counters_, 0, 16, s_);
     par_ = 0;
