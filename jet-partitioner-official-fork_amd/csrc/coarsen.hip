@@ -166,7 +166,11 @@ void hec_heavy_neighbours(const DGraph& g, bool uniform, uint64_t seed, int32_t 
 // ---------------------------------------------------------------- map construction
 __device__ inline bool mc_acting(vid_t u, vid_t h, int swap) { return (((u < h) ? 1 : 0) ^ swap) != 0; }
 
-__global__ void k_mc_mark(const vid_t* wl, int32_t len, int32_t round, int32_t* stamp, int32_t* claim) {
+// Also clears the round's work-list counter (k_mc_next appends to it; the
+// host has read the previous round's count before this launch).
+__global__ void k_mc_mark(const vid_t* wl, int32_t len, int32_t round, int32_t* stamp, int32_t* claim,
+                          int32_t* cnt) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *cnt = 0;
     FOR_THREAD(x, len) {
         vid_t u = wl ? wl[x] : (vid_t)x;
         stamp[u] = round;
@@ -233,6 +237,7 @@ __global__ void __launch_bounds__(MC_TAIL_BLOCK)
     k_mc_tail(const vid_t* wl0, int32_t len0, int32_t round0, int swap0, vid_t n, const vid_t* hn, int32_t* stamp,
               int32_t* claim, vid_t* vcmap, vid_t* wl_a, vid_t* wl_b, int32_t* err) {
     __shared__ int32_t cnt;
+    if (threadIdx.x == 0) *err = 0;  // only thread 0 writes it
     const vid_t* wl = wl0;
     int32_t len = len0;
     int swap = swap0;
@@ -330,8 +335,7 @@ int32_t map_construct(vid_t n, const vid_t* hn, vid_t* map, Arena& a, hipStream_
         if (round > 2 * n + 8) throw DeviceError("map construction made no progress");
         if (wl && len <= MC_TAIL_MAX) {
             // the remaining rounds in one workgroup, with no host read per round
-            int32_t* err = a.alloc<int32_t>(1);
-            fill_i32(err, 0, 1, s);
+            int32_t* err = a.alloc<int32_t>(1);  // cleared by the kernel
             vid_t* other = (wl == wl_a) ? wl_b : wl_a;
             k_mc_tail<<<1, MC_TAIL_BLOCK, 0, s>>>(wl, len, round, swap, n, hn, stamp, claim, vcmap,
                                                   const_cast<vid_t*>(wl), other, err);
@@ -340,11 +344,10 @@ int32_t map_construct(vid_t n, const vid_t* hn, vid_t* map, Arena& a, hipStream_
             break;
         }
         int blocks = grid_for(len, 256, 8192);
-        k_mc_mark<<<blocks, 256, 0, s>>>(wl, len, round, stamp, claim);
+        k_mc_mark<<<blocks, 256, 0, s>>>(wl, len, round, stamp, claim, cnt);
         if (swap == 0) k_mc_claim<<<blocks, 256, 0, s>>>(wl, len, round, hn, stamp, claim);
         k_mc_resolve<<<blocks, 256, 0, s>>>(wl, len, round, swap, hn, stamp, claim, vcmap);
         vid_t* next = (wl == wl_a) ? wl_b : wl_a;
-        fill_i32(cnt, 0, 1, s);
         k_mc_next<<<blocks, 256, 0, s>>>(wl, len, n, vcmap, next, cnt);
         JET_LAUNCH_CHECK();
         len = read_scalar(cnt, s);

@@ -8,6 +8,8 @@
 namespace jet {
 
 void fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
+// Two fills in one launch: p1[0, n1) = v1 and p2[0, n2) = v2.
+void fill2_i32(int32_t* p1, int32_t v1, int64_t n1, int32_t* p2, int32_t v2, int64_t n2, hipStream_t s);
 void fill_i64(int64_t* p, int64_t v, int64_t n, hipStream_t s);
 void iota_i32(int32_t* p, int64_t n, hipStream_t s);
 void copy_i32(int32_t* dst, const int32_t* src, int64_t n, hipStream_t s);

@@ -1,6 +1,7 @@
 // Device-wide primitives for the Jet hot path on gfx950.
 #include "prims.h"
 
+#include <algorithm>
 #include <stdexcept>
 #include "wave.h"
 
@@ -10,6 +11,14 @@ namespace jet {
 __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
+}
+
+__global__ void k_fill2_i32(int32_t* p1, int32_t v1, int64_t n1, int32_t* p2, int32_t v2, int64_t n2) {
+    const int64_t n = n1 > n2 ? n1 : n2;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (i < n1) p1[i] = v1;
+        if (i < n2) p2[i] = v2;
+    }
 }
 __global__ void k_fill_i64(int64_t* p, int64_t v, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -79,6 +88,12 @@ void copy_i64(int64_t* d, const int64_t* src, int64_t n, hipStream_t s) {
 void fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s) {
     if (n <= 0) return;
     k_fill_i32<<<grid_for(n, 256, 8192), 256, 0, s>>>(p, v, n);
+    JET_LAUNCH_CHECK();
+}
+void fill2_i32(int32_t* p1, int32_t v1, int64_t n1, int32_t* p2, int32_t v2, int64_t n2, hipStream_t s) {
+    const int64_t n = std::max<int64_t>(n1, n2);
+    if (n <= 0) return;
+    k_fill2_i32<<<grid_for(n, 256, 8192), 256, 0, s>>>(p1, v1, n1, p2, v2, n2);
     JET_LAUNCH_CHECK();
 }
 void fill_i64(int64_t* p, int64_t v, int64_t n, hipStream_t s) {

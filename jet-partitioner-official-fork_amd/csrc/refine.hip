@@ -1957,8 +1957,7 @@ void Refiner::init_conn(const Level& L, const part_t* part, const int32_t* mark)
         k_conn_caps<<<grid_for(g.n, 256, 8192), 256, 0, s_>>>(g, p_.k, caps);
         JET_LAUNCH_CHECK();
         excl_scan_i32_i64(caps, conn_off_, g.n, a_, s_);
-        fill_i32(dest_cache_, NULL_PART, g.n, s_);
-        fill_i32(lock_, 0, g.n, s_);
+        fill2_i32(dest_cache_, NULL_PART, g.n, lock_, 0, g.n, s_);
     }
     k_conn_build_small<<<grid_for(g.n, SB_BLOCK, 8192), SB_BLOCK, 0, s_>>>(g, part, p_.k, conn_off_, conn_,
                                                                           conn_size_, mk, dest_cache_, nullptr,
@@ -2338,8 +2337,7 @@ LevelRecord Refiner::refine(const Level& L, part_t* best_part) {
     best_due_ = false;
     DGraph g = L.view();
     const int32_t k = p_.k;
-    fill_i32(misc_, 0, 8, s_);
-    fill_i32(counters_, 0, 16, s_);
+    fill2_i32(misc_, 0, 8, counters_, 0, 16, s_);
     par_ = 0;
     if (!init_) {
         cut_ = total_cut2(g, best_part, a_, s_);
